@@ -1,0 +1,100 @@
+// Shared device/host helpers for the FANS MI355X kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace fans {
+
+template <typename T>
+struct cplx {
+    T x, y;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> mk(T a, T b)
+{
+    return cplx<T>{a, b};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b)
+{
+    return {a.x + b.x, a.y + b.y};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b)
+{
+    return {a.x - b.x, a.y - b.y};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b)
+{
+    return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s)
+{
+    return {a.x * s, a.y * s};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> conjg(cplx<T> a)
+{
+    return {a.x, -a.y};
+}
+// multiply by -i (forward) or +i (inverse)
+template <typename T, bool INV>
+__host__ __device__ __forceinline__ cplx<T> mul_mi(cplx<T> a)
+{
+    return INV ? cplx<T>{-a.y, a.x} : cplx<T>{a.y, -a.x};
+}
+
+// 64-lane wavefront reductions (deterministic butterfly order)
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block-wide reduction of NV doubles; result valid in thread 0.  `sm` must hold
+// NV * (blockDim.x / 64) doubles.  is_max selects max instead of sum per slot.
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], const bool (&is_max)[NV], double *sm)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        v[k] = is_max[k] ? wave_max(v[k]) : wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            sm[k * nw + wid] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double a = sm[k * nw];
+            for (int w = 1; w < nw; ++w)
+                a = is_max[k] ? fmax(a, sm[k * nw + w]) : a + sm[k * nw + w];
+            v[k] = a;
+        }
+    }
+}
+
+}  // namespace fans

@@ -1,0 +1,1903 @@
+// libfans_gpu.so: MI355X (gfx950) implementation of the FANS per-iteration
+// solver path behind the C ABI of include/fans_gpu.h.
+//
+// Host side of one load case (the reference's MaterialManager + Solver pair):
+// device buffers, FFT plans, Green operator, and the solver loops of
+// include/solverCG.h (linear fast path with device-resident scalars; nonlinear
+// path with the secant line search), include/solverFP.h and the mixed
+// strain/stress controller of include/mixedBCs.h.
+#include "fans_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "element_kernels.h"
+#include "fft_kernels.h"
+#include "green_kernels.h"
+#include "vec_kernels.h"
+
+using namespace fans;
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// kernel families for the timing report
+enum {
+    KF_FFT_Z_FWD = 0,
+    KF_FFT_Y_FWD,
+    KF_FFT_X_GREEN,
+    KF_FFT_Y_INV,
+    KF_FFT_Z_INV,
+    KF_ASSEMBLE_LIN,
+    KF_ASSEMBLE_RES,
+    KF_CG_UPDATE,
+    KF_REDUCE,
+    KF_VECTOR,
+    KF_ELEMENT,
+    KF_COUNT
+};
+const char *kFamilyNames[KF_COUNT] = {"fft_z_r2c",    "fft_y_fwd",    "fft_x_green", "fft_y_inv",
+                                      "fft_z_c2r",    "assemble_lin", "assemble_res", "cg_update",
+                                      "reduce",       "vector",       "element_pass"};
+
+int nparams(int kind)
+{
+    switch (kind) {
+    case FANS_MODEL_THERMAL_ISO: return 1;
+    case FANS_MODEL_THERMAL_TRI: return 6;
+    case FANS_MODEL_ELASTIC_ISO: return 2;
+    case FANS_MODEL_ELASTIC_TRI: return 21;
+    case FANS_MODEL_J2_NEW: return 4;
+    case FANS_MODEL_PSEUDO_PLASTIC: return 4;
+    default: return -1;
+    }
+}
+bool model_linear(int kind) { return kind >= 1 && kind <= 4; }
+
+// ---- host twiddles (long double, then rounded) ------------------------------
+template <typename T>
+std::vector<cplx<T>> twiddles(int n)
+{
+    std::vector<cplx<T>> w(n);
+    const long double    pi = 3.14159265358979323846264338327950288L;
+    for (int k = 0; k < n; ++k) {
+        long double th = -2.0L * pi * (long double) k / (long double) n;
+        w[k]           = cplx<T>{(T) cosl(th), (T) sinl(th)};
+    }
+    return w;
+}
+
+bool make_len(int n, int tws, FftLen &L)
+{
+    L.n   = n;
+    L.tws = tws;
+    L.nst = 0;
+    int m = n;
+    for (int r : {5, 3})
+        while (m % r == 0) {
+            if (L.nst >= 14)
+                return false;
+            L.rad[L.nst++] = r;
+            m /= r;
+        }
+    while (m % 8 == 0) {
+        L.rad[L.nst++] = 8;
+        m /= 8;
+    }
+    while (m % 4 == 0) {
+        L.rad[L.nst++] = 4;
+        m /= 4;
+    }
+    while (m % 2 == 0) {
+        L.rad[L.nst++] = 2;
+        m /= 2;
+    }
+    return m == 1;
+}
+
+template <typename T>
+struct Fft3D {
+    int           nx = 0, ny = 0, nz = 0, h = 0, M = 0, Mc = 0;
+    FftLen        Lz{}, Ly{}, Lx{};
+    cplx<T>      *twz = nullptr, *twy = nullptr, *twx = nullptr;
+    int           lpb = 1, bc = 8, bk = 2;
+    size_t        smz = 0, smy = 0, smx = 0;
+    int64_t       nblk_z = 0;
+    bool          ready  = false;
+
+    hipError_t init(int nx_, int ny_, int nz_, int h_)
+    {
+        nx = nx_, ny = ny_, nz = nz_, h = h_;
+        M  = nz / 2;
+        Mc = M + 1;
+        if (!make_len(M, 2, Lz) || !make_len(ny, 1, Ly) || !make_len(nx, 1, Lx))
+            return hipErrorInvalidValue;
+        auto up = [](cplx<T> *&dst, const std::vector<cplx<T>> &v) {
+            hipError_t e = hipMalloc(&dst, sizeof(cplx<T>) * v.size());
+            if (e != hipSuccess)
+                return e;
+            return hipMemcpy(dst, v.data(), sizeof(cplx<T>) * v.size(), hipMemcpyHostToDevice);
+        };
+        hipError_t e;
+        if ((e = up(twz, twiddles<T>(nz))) != hipSuccess)
+            return e;
+        if ((e = up(twy, twiddles<T>(ny))) != hipSuccess)
+            return e;
+        if ((e = up(twx, twiddles<T>(nx))) != hipSuccess)
+            return e;
+        // LDS budget per workgroup: 60 KiB for the two ping-pong buffers
+        const size_t cb     = sizeof(cplx<T>);
+        const size_t budget = 60 * 1024;
+        lpb                 = std::max(1, (int) (budget / (2 * cb * M * h)));
+        lpb                 = std::min(lpb, 16);
+        smz                 = 2 * cb * M * h * lpb;
+        const int Kc        = Mc * h;
+        bc                  = std::max(1, (int) (budget / (2 * cb * ny)));
+        bc                  = std::min(bc, Kc);
+        bc                  = std::min(bc, 32);
+        smy                 = 2 * cb * ny * bc;
+        bk                  = std::max(1, (int) (budget / (2 * cb * nx * h)));
+        bk                  = std::min(bk, Mc);
+        smx                 = 2 * cb * nx * h * bk;
+        nblk_z              = ((int64_t) nx * ny + lpb - 1) / lpb;
+        ready               = true;
+        return hipSuccess;
+    }
+    void release()
+    {
+        (void) hipFree(twz);
+        (void) hipFree(twy);
+        (void) hipFree(twx);
+        twz = twy = twx = nullptr;
+        ready           = false;
+    }
+};
+
+}  // namespace
+
+struct fans_gpu_ctx {
+    fans_gpu_desc d{};
+    std::string   err;
+    int           nx = 0, ny = 0, nz = 0, h = 0, nstr = 0, ngp = 0;
+    int64_t       N = 0, Nh = 0, Nf = 0;
+    hipStream_t   stream = nullptr;
+    int           tol_type = 0;  // current error type (tangent switches it, solver.h:749)
+    double        tol      = 0;
+
+    // materials
+    bool                        have_ms = false, have_mat = false;
+    int                         nphases = 0;
+    std::vector<int>            phase_kind;
+    bool                        all_linear = true, has_j2 = false;
+    double                      kref[36]{};
+    double                      keref[576]{};
+    std::vector<Phase<double>>  ph64;
+    std::vector<Phase<float>>   ph32;
+
+    // device buffers
+    uint8_t       *ms = nullptr;
+    Phase<double> *dph64 = nullptr;
+    Phase<float>  *dph32 = nullptr;
+    double        *u = nullptr, *u_prev = nullptr, *r = nullptr, *s = nullptr, *dA = nullptr, *dB = nullptr,
+           *rnew = nullptr;
+    cplx<double> *spec64 = nullptr;
+    double       *G64 = nullptr, *keref_d = nullptr;
+    float        *G32 = nullptr;
+    double       *js = nullptr;
+    double       *g0d = nullptr;
+    float        *g0f = nullptr;
+    double       *partial = nullptr;
+    int64_t       partial_len = 0;
+    double       *sc = nullptr;
+    int          *is = nullptr;
+    double       *hist = nullptr;
+    // FP32 inner solve (mixed form)
+    float        *r32 = nullptr, *s32 = nullptr, *dA32 = nullptr, *dB32 = nullptr, *rnew32 = nullptr, *x32 = nullptr;
+    cplx<float>  *spec32 = nullptr;
+    Fft3D<double> fft64;
+    Fft3D<float>  fft32;
+
+    // launch geometry
+    int     cx = 16, nty = 0, ntz = 0, nblk_asm = 0, nblk_vec = 0, nblk_el = 0;
+    double  g0[9]{};
+    Geom    geom{};
+
+    // mixed BC (mixedBCs.h)
+    bool                mixed_active = false;
+    std::vector<int>    idxE, idxF;
+    std::vector<double> FE, PF, M;
+    int                 nsteps = 0, step = 0;
+    double              gvec[9]{}, gvec_prev[9]{};
+    double              alpha_warm = 0.1;
+
+    // results
+    int                 iter = 0, inner_iters = 0, outer_iters = 0;
+    std::vector<double> err_hist;
+    double              solve_ms = 0;
+
+    // timing
+    bool                                                timing = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+    std::vector<hipEvent_t>                             ev_pool;
+    double                                              fam_ms[KF_COUNT]{};
+    int                                                 fam_n[KF_COUNT]{};
+};
+
+namespace {
+
+int fail(fans_gpu_ctx *c, int code, const char *fmt, ...)
+{
+    char    buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c)
+        c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                                      \
+    do {                                                                                                       \
+        hipError_t e_ = (expr);                                                                                \
+        if (e_ != hipSuccess)                                                                                  \
+            return fail(ctx, FANS_GPU_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                                             \
+    } while (0)
+
+hipEvent_t get_event(fans_gpu_ctx *c)
+{
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void) hipEventCreate(&e);
+    return e;
+}
+
+// optional per-launch event bracketing (timing mode)
+struct Timed {
+    fans_gpu_ctx *c;
+    int           fam;
+    hipEvent_t    a = nullptr;
+    Timed(fans_gpu_ctx *c_, int f) : c(c_), fam(f)
+    {
+        if (c->timing) {
+            a = get_event(c);
+            (void) hipEventRecord(a, c->stream);
+        }
+    }
+    ~Timed()
+    {
+        if (c->timing) {
+            hipEvent_t b = get_event(c);
+            (void) hipEventRecord(b, c->stream);
+            c->ev_used.push_back({fam, {a, b}});
+        }
+    }
+};
+
+void harvest_timing(fans_gpu_ctx *c)
+{
+    for (auto &p : c->ev_used) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+            c->fam_ms[p.first] += ms;
+            c->fam_n[p.first] += 1;
+        }
+        c->ev_pool.push_back(p.second.first);
+        c->ev_pool.push_back(p.second.second);
+    }
+    c->ev_used.clear();
+}
+
+int grid_vec(int64_t n) { return (int) std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, 2048)); }
+
+// ---------------------------------------------------------------------------
+// FFT convolution: out = G * r  with the requested epilogue
+// ---------------------------------------------------------------------------
+template <typename T>
+int convolve_dev(fans_gpu_ctx *c, Fft3D<T> &P, const T *rin, cplx<T> *spec, const T *Gp, int epi, EpiArgs<T> ep,
+                 const int *done)
+{
+    const int     h = c->h, Kc = P.Mc * h;
+    const int64_t nlines = (int64_t) c->nx * c->ny;
+    {
+        Timed t(c, KF_FFT_Z_FWD);
+        if (h == 1)
+            hipLaunchKernelGGL((k_fft_z_r2c<T, 1>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, rin, spec, P.Lz,
+                               P.twz, c->nz, nlines, P.lpb, done);
+        else
+            hipLaunchKernelGGL((k_fft_z_r2c<T, 3>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, rin, spec, P.Lz,
+                               P.twz, c->nz, nlines, P.lpb, done);
+    }
+    {
+        Timed t(c, KF_FFT_Y_FWD);
+        hipLaunchKernelGGL((k_fft_strided<T, false>), dim3((Kc + P.bc - 1) / P.bc, c->nx), dim3(kThreads), P.smy,
+                           c->stream, spec, P.Ly, P.twy, (int64_t) Kc, Kc, (int64_t) c->ny * Kc, P.bc, done);
+    }
+    {
+        Timed t(c, KF_FFT_X_GREEN);
+        dim3  g((P.Mc + P.bk - 1) / P.bk, c->ny);
+        if (h == 1)
+            hipLaunchKernelGGL((k_fft_x_green<T, 1>), g, dim3(kThreads), P.smx, c->stream, spec, P.Lx, P.twx, Gp,
+                               c->ny, P.Mc, P.bk, done);
+        else
+            hipLaunchKernelGGL((k_fft_x_green<T, 3>), g, dim3(kThreads), P.smx, c->stream, spec, P.Lx, P.twx, Gp,
+                               c->ny, P.Mc, P.bk, done);
+    }
+    {
+        Timed t(c, KF_FFT_Y_INV);
+        hipLaunchKernelGGL((k_fft_strided<T, true>), dim3((Kc + P.bc - 1) / P.bc, c->nx), dim3(kThreads), P.smy,
+                           c->stream, spec, P.Ly, P.twy, (int64_t) Kc, Kc, (int64_t) c->ny * Kc, P.bc, done);
+    }
+    {
+        Timed t(c, KF_FFT_Z_INV);
+#define ZINV(HH, EE)                                                                                              \
+    hipLaunchKernelGGL((k_fft_z_c2r<T, HH, EE>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, spec, P.Lz, \
+                       P.twz, c->nz, nlines, P.lpb, ep, done)
+        if (h == 1) {
+            if (epi == EPI_STORE)
+                ZINV(1, EPI_STORE);
+            else if (epi == EPI_CG)
+                ZINV(1, EPI_CG);
+            else
+                ZINV(1, EPI_FP);
+        } else {
+            if (epi == EPI_STORE)
+                ZINV(3, EPI_STORE);
+            else if (epi == EPI_CG)
+                ZINV(3, EPI_CG);
+            else
+                ZINV(3, EPI_FP);
+        }
+#undef ZINV
+    }
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// assembly launches
+// ---------------------------------------------------------------------------
+template <typename T>
+AsmArgs<T> asm_args(fans_gpu_ctx *c)
+{
+    AsmArgs<T> a{};
+    a.nx      = c->nx;
+    a.ny      = c->ny;
+    a.nz      = c->nz;
+    a.cx      = c->cx;
+    a.nty     = c->nty;
+    a.ntz     = c->ntz;
+    a.geom    = c->geom;
+    a.ms      = c->ms;
+    a.js      = J2State{c->js, c->N};
+    a.partial = c->partial;
+    a.measure = c->d.measure;
+    a.done    = nullptr;
+    return a;
+}
+
+template <typename T, int MODE>
+int launch_assemble(fans_gpu_ctx *c, AsmArgs<T> a)
+{
+    Timed t(c, MODE == ASM_CGLIN ? KF_ASSEMBLE_LIN : KF_ASSEMBLE_RES);
+#define ASM(HH, GG) \
+    hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE>), dim3(c->nblk_asm), dim3(kThreads), 0, c->stream, a)
+    if (c->h == 1) {
+        if (c->ngp == 8)
+            ASM(1, 8);
+        else
+            ASM(1, 1);
+    } else {
+        if (c->ngp == 8)
+            ASM(3, 8);
+        else
+            ASM(3, 1);
+    }
+#undef ASM
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+// r = R(u) (material path with g0); partial slot 0 = norm(r), slot 1 = <r, dotv>
+int residual64(fans_gpu_ctx *c, const double *uin, double *rout, const double *dotv)
+{
+    AsmArgs<double> a = asm_args<double>(c);
+    a.phases          = c->dph64;
+    a.in              = uin;
+    a.out             = rout;
+    a.dotv            = dotv;
+    a.g0              = c->g0d;
+    return launch_assemble<double, ASM_RESID>(c, a);
+}
+
+int reduce(fans_gpu_ctx *c, const double *partial, int nblk, int nq, int stage)
+{
+    Timed t(c, KF_REDUCE);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, c->stream, partial, nblk, nq, stage, c->sc, c->is,
+                       c->hist);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int read_scalars(fans_gpu_ctx *c, double *sc, int *is)
+{
+    if (sc)
+        HIPCHK(c, hipMemcpyAsync(sc, c->sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
+    if (is)
+        HIPCHK(c, hipMemcpyAsync(is, c->is, sizeof(int) * IS_COUNT, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int init_scalars(fans_gpu_ctx *c, double tol, int err_type, int n_it)
+{
+    double sc[SC_COUNT] = {0};
+    int    is[IS_COUNT] = {0};
+    sc[SC_TOL]          = tol;
+    sc[SC_DELTA]        = 1.0;
+    is[IS_NIT]          = n_it;
+    is[IS_ERRTYPE]      = err_type;
+    is[IS_MEASURE]      = c->d.measure;
+    HIPCHK(c, hipMemcpyAsync(c->sc, sc, sizeof(sc), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->is, is, sizeof(is), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+template <typename T>
+int fill(fans_gpu_ctx *c, T *p, T v, int64_t n)
+{
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid_vec(n)), dim3(kThreads), 0, c->stream, p, v, n);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int upload_g0(fans_gpu_ctx *c)
+{
+    float gf[9];
+    for (int i = 0; i < 9; ++i)
+        gf[i] = (float) c->g0[i];
+    HIPCHK(c, hipMemcpyAsync(c->g0d, c->g0, sizeof(double) * 9, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->g0f, gf, sizeof(float) * 9, hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+// Solver::get_homogenized_stress (solver.h:707-737)
+int homog_stress(fans_gpu_ctx *c, double *out)
+{
+    ElemArgs<double> a{};
+    a.nx      = c->nx;
+    a.ny      = c->ny;
+    a.nz      = c->nz;
+    a.geom    = c->geom;
+    a.ms      = c->ms;
+    a.phases  = c->dph64;
+    a.js      = J2State{c->js, c->N};
+    a.u       = c->u;
+    a.g0      = c->g0d;
+    a.partial = c->partial;
+    {
+        Timed t(c, KF_ELEMENT);
+#define EP(HH, GG)                                                                                              \
+    hipLaunchKernelGGL((k_element_pass<double, HH, GG, ELEM_HOMOG>), dim3(c->nblk_el), dim3(kThreads), 0, \
+                       c->stream, a)
+        if (c->h == 1) {
+            if (c->ngp == 8)
+                EP(1, 8);
+            else
+                EP(1, 1);
+        } else {
+            if (c->ngp == 8)
+                EP(3, 8);
+            else
+                EP(3, 1);
+        }
+#undef EP
+    }
+    HIPCHK(c, hipGetLastError());
+    int rc = reduce(c, c->partial, c->nblk_el, c->nstr, RED_SUM);
+    if (rc)
+        return rc;
+    double sc[SC_COUNT];
+    if ((rc = read_scalars(c, sc, nullptr)))
+        return rc;
+    for (int k = 0; k < c->nstr; ++k)
+        out[k] = sc[SC_RES0 + k] / (double) c->N;
+    return 0;
+}
+
+// commit the J2 state at the current u, g0 (J2PlasticityNew.h:37-41)
+int commit_state(fans_gpu_ctx *c)
+{
+    if (!c->has_j2)
+        return 0;
+    ElemArgs<double> a{};
+    a.nx     = c->nx;
+    a.ny     = c->ny;
+    a.nz     = c->nz;
+    a.geom   = c->geom;
+    a.ms     = c->ms;
+    a.phases = c->dph64;
+    a.js     = J2State{c->js, c->N};
+    a.u      = c->u;
+    a.g0     = c->g0d;
+    Timed t(c, KF_ELEMENT);
+    if (c->ngp == 8)
+        hipLaunchKernelGGL((k_element_pass<double, 3, 8, ELEM_COMMIT>), dim3(c->nblk_el), dim3(kThreads), 0,
+                           c->stream, a);
+    else
+        hipLaunchKernelGGL((k_element_pass<double, 3, 1, ELEM_COMMIT>), dim3(c->nblk_el), dim3(kThreads), 0,
+                           c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// mixed boundary conditions (mixedBCs.h:150-227)
+// ---------------------------------------------------------------------------
+int mixed_update(fans_gpu_ctx *c)
+{
+    if (!c->mixed_active)
+        return 0;
+    double Pbar[9];
+    int    rc = homog_stress(c, Pbar);
+    if (rc)
+        return rc;
+    const int nF = (int) c->idxF.size();
+    if (nF) {
+        std::vector<double> rhs(nF), dE(nF, 0.0);
+        for (int i = 0; i < nF; ++i)
+            rhs[i] = c->PF[(size_t) c->step * nF + i] - Pbar[c->idxF[i]];
+        for (int i = 0; i < nF; ++i)
+            for (int j = 0; j < nF; ++j)
+                dE[i] += c->M[i * nF + j] * rhs[j];
+        for (int i = 0; i < nF; ++i)
+            c->gvec[c->idxF[i]] += dE[i];
+    }
+    for (int i = 0; i < c->nstr; ++i)
+        c->g0[i] = c->gvec[i];
+    return upload_g0(c);
+}
+
+// symmetric pseudo-inverse of a small SPD block (Jacobi, host)
+void host_sym_pinv(int n, const double *A, double *P)
+{
+    double a[81], V[81];
+    std::memcpy(a, A, sizeof(double) * n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            V[i * n + j] = i == j;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                tot += a[i * n + j] * a[i * n + j];
+                if (i != j)
+                    off += a[i * n + j] * a[i * n + j];
+            }
+        if (off <= 1e-32 * tot || off == 0.0)
+            break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = a[p * n + q];
+                if (apq == 0.0)
+                    continue;
+                double theta = (a[q * n + q] - a[p * n + p]) / (2 * apq);
+                double t     = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                double cs = 1 / std::sqrt(t * t + 1), sn = t * cs;
+                for (int k = 0; k < n; ++k) {
+                    double akp = a[k * n + p], akq = a[k * n + q];
+                    a[k * n + p] = cs * akp - sn * akq;
+                    a[k * n + q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double apk = a[p * n + k], aqk = a[q * n + k];
+                    a[p * n + k] = cs * apk - sn * aqk;
+                    a[q * n + k] = sn * apk + cs * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = cs * vkp - sn * vkq;
+                    V[k * n + q] = sn * vkp + cs * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double acc = 0;
+            for (int k = 0; k < n; ++k)
+                if (std::fabs(a[k * n + k]) > 1e-300)
+                    acc += V[i * n + k] * V[j * n + k] / a[k * n + k];
+            P[i * n + j] = acc;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// solver loops
+// ---------------------------------------------------------------------------
+
+// Linear CG fast path (solverCG.h:84-114, islinear && !mixed); every scalar
+// lives on the device, the host only polls the done flag once per chunk.
+template <typename T, typename TU>
+int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext, T *rnew, TU *u, cplx<T> *spec,
+                   const T *Gp, const Phase<T> *ph, int *iters_out)
+{
+    const int *done = c->is + IS_DONE;
+    int        chunk = 4, rc;
+    int        is[IS_COUNT];
+    for (;;) {
+        for (int k = 0; k < chunk; ++k) {
+            EpiArgs<T> ep{};
+            ep.out     = s;
+            ep.r       = r;
+            ep.partial = c->partial;
+            if ((rc = convolve_dev<T>(c, P, r, spec, Gp, EPI_CG, ep, done)))
+                return rc;
+            if ((rc = reduce(c, c->partial, (int) P.nblk_z, 1, RED_DELTA)))
+                return rc;
+            AsmArgs<T> a = asm_args<T>(c);
+            a.phases     = ph;
+            a.in         = s;
+            a.din        = dcur;
+            a.dout       = dnext;
+            a.out        = rnew;
+            a.scal       = c->sc;  // SC_BETA == 0
+            a.done       = done;
+            if ((rc = launch_assemble<T, ASM_CGLIN>(c, a)))
+                return rc;
+            if ((rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ALPHA)))
+                return rc;
+            {
+                Timed t(c, KF_CG_UPDATE);
+                hipLaunchKernelGGL((k_cg_update<T, TU>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, r, rnew, u,
+                                   dnext, s, c->Nh, c->sc, c->is, c->partial);
+                HIPCHK(c, hipGetLastError());
+            }
+            if ((rc = reduce(c, c->partial, c->nblk_vec, 2, RED_UPDATE)))
+                return rc;
+            std::swap(dcur, dnext);
+        }
+        if ((rc = read_scalars(c, nullptr, is)))
+            return rc;
+        if (is[IS_DONE])
+            break;
+        chunk = std::min(chunk * 2, 16);
+    }
+    if (iters_out)
+        *iters_out = is[IS_ITER];
+    return 0;
+}
+
+int fetch_history(fans_gpu_ctx *c, int iters)
+{
+    c->err_hist.assign(iters + 1, 0.0);
+    HIPCHK(c, hipMemcpyAsync(c->err_hist.data(), c->hist, sizeof(double) * (iters + 1), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int solve_cg_fp64(fans_gpu_ctx *c)
+{
+    int rc;
+    if ((rc = init_scalars(c, c->tol, c->tol_type, c->d.n_it)))
+        return rc;
+    c->alpha_warm = 0.1;
+    if ((rc = fill<double>(c, c->s, 0.0, c->Nh)) || (rc = fill<double>(c, c->dA, 0.0, c->Nh)))
+        return rc;
+    if ((rc = residual64(c, c->u, c->r, nullptr)))
+        return rc;
+    if ((rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_INIT)))
+        return rc;
+    const bool linear = c->all_linear && !c->mixed_active;
+    if (linear) {
+        double *dcur = c->dA, *dnext = c->dB;
+        int     it   = 0;
+        if ((rc = cg_linear_loop<double, double>(c, c->fft64, c->r, c->s, dcur, dnext, c->rnew, c->u, c->spec64,
+                                                 c->G64, c->dph64, &it)))
+            return rc;
+        c->iter = it;
+        return fetch_history(c, it);
+    }
+    // nonlinear CG with the secant line search (solverCG.h:84-160)
+    int    is[IS_COUNT];
+    double sc[SC_COUNT];
+    if ((rc = read_scalars(c, nullptr, is)))
+        return rc;
+    double *d = c->dA;
+    while (!is[IS_DONE]) {
+        // deltamid = <r, s>
+        hipLaunchKernelGGL((k_dot<double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r, c->s, c->Nh,
+                           c->d.measure, c->partial);
+        if ((rc = reduce(c, c->partial, c->nblk_vec, 1, RED_SUM)))
+            return rc;
+        if ((rc = read_scalars(c, sc, nullptr)))
+            return rc;
+        HIPCHK(c, hipMemcpyAsync(c->sc + SC_DELTAMID, &sc[SC_RES0], sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        EpiArgs<double> ep{};
+        ep.out     = c->s;
+        ep.r       = c->r;
+        ep.partial = c->partial;
+        if ((rc = convolve_dev<double>(c, c->fft64, c->r, c->spec64, c->G64, EPI_CG, ep, nullptr)))
+            return rc;
+        if ((rc = reduce(c, c->partial, (int) c->fft64.nblk_z, 1, RED_DELTA)))
+            return rc;
+        hipLaunchKernelGGL((k_direction<double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, d, c->s, c->sc,
+                           c->Nh);
+        // --- LineSearchSecant ---
+        const int    MaxIt = c->d.ls_max_iter;
+        const double ltol  = c->d.ls_tol;
+        double       lerr = 10.0, ap = 0.0, ac = c->alpha_warm;
+        int          lit = 0;
+        hipLaunchKernelGGL((k_dot<double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r, d, c->Nh,
+                           c->d.measure, c->partial);
+        if ((rc = reduce(c, c->partial, c->nblk_vec, 1, RED_SUM)) || (rc = read_scalars(c, sc, nullptr)))
+            return rc;
+        double rpd = sc[SC_RES0];
+        hipLaunchKernelGGL((k_axpby<double, double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->u, d, ac,
+                           1.0, c->Nh);
+        if ((rc = mixed_update(c)) || (rc = residual64(c, c->u, c->rnew, d)))
+            return rc;
+        if ((rc = reduce(c, c->partial + c->nblk_asm, c->nblk_asm, 1, RED_SUM)) ||
+            (rc = read_scalars(c, sc, nullptr)))
+            return rc;
+        double r1pd = sc[SC_RES0];
+        while (lit < MaxIt && lerr > ltol) {
+            double denom = r1pd - rpd;
+            if (std::fabs(denom) < 1e-14 * (std::fabs(r1pd) + std::fabs(rpd)))
+                break;
+            double an = ac - r1pd * (ac - ap) / denom;
+            if (an <= 0.0)
+                an = 0.5 * (ap + ac);
+            lerr = std::fabs(an - ac);
+            hipLaunchKernelGGL((k_axpby<double, double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->u, d,
+                               an - ac, 1.0, c->Nh);
+            ap  = ac;
+            rpd = r1pd;
+            ac  = an;
+            lit++;
+            if ((rc = mixed_update(c)) || (rc = residual64(c, c->u, c->rnew, d)))
+                return rc;
+            if ((rc = reduce(c, c->partial + c->nblk_asm, c->nblk_asm, 1, RED_SUM)) ||
+                (rc = read_scalars(c, sc, nullptr)))
+                return rc;
+            r1pd = sc[SC_RES0];
+        }
+        c->alpha_warm = (lit == MaxIt && lerr > ltol) ? 0.1 : ac;
+        std::swap(c->r, c->rnew);
+        // the last residual evaluation left norm(rnew) partials in slot 0
+        if ((rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_NEXT)) || (rc = read_scalars(c, nullptr, is)))
+            return rc;
+    }
+    c->iter = is[IS_ITER];
+    return fetch_history(c, c->iter);
+}
+
+int solve_fp_fp64(fans_gpu_ctx *c)
+{
+    int rc;
+    if ((rc = init_scalars(c, c->tol, c->tol_type, c->d.n_it)))
+        return rc;
+    if ((rc = residual64(c, c->u, c->r, nullptr)) || (rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_INIT)))
+        return rc;
+    int is[IS_COUNT];
+    if ((rc = read_scalars(c, nullptr, is)))
+        return rc;
+    const int *done = c->is + IS_DONE;
+    while (!is[IS_DONE]) {
+        const int chunk = c->mixed_active ? 1 : 8;
+        for (int k = 0; k < chunk; ++k) {
+            EpiArgs<double> ep{};
+            ep.u = c->u;
+            if ((rc = convolve_dev<double>(c, c->fft64, c->r, c->spec64, c->G64, EPI_FP, ep, done)))
+                return rc;
+            if ((rc = mixed_update(c)))
+                return rc;
+            AsmArgs<double> a = asm_args<double>(c);
+            a.phases          = c->dph64;
+            a.in              = c->u;
+            a.out             = c->r;
+            a.g0              = c->g0d;
+            a.done            = done;
+            if ((rc = launch_assemble<double, ASM_RESID>(c, a)) ||
+                (rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_NEXT)))
+                return rc;
+        }
+        if ((rc = read_scalars(c, nullptr, is)))
+            return rc;
+    }
+    c->iter = is[IS_ITER];
+    return fetch_history(c, c->iter);
+}
+
+// Mixed form: FP64 defect correction around an FP32 Green-preconditioned CG.
+// outer: r = R(u) in FP64 (the true residual), stop when it meets the
+// tolerance; inner: solve K x = r with FP32 operator/vectors and FP64 dots to a
+// relative reduction eta (or the FP32 noise floor), then u += x.
+int solve_cg_mixed(fans_gpu_ctx *c)
+{
+    int rc;
+    if (!(c->all_linear && !c->mixed_active))
+        return fail(c, FANS_GPU_EUNSUP, "mixed precision form supports linear materials without mixed BCs");
+    // outer residual
+    if ((rc = init_scalars(c, c->tol, c->tol_type, c->d.n_it)))
+        return rc;
+    if ((rc = residual64(c, c->u, c->r, nullptr)) || (rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_INIT)))
+        return rc;
+    double sc[SC_COUNT];
+    int    is[IS_COUNT];
+    if ((rc = read_scalars(c, sc, is)))
+        return rc;
+    std::vector<double> outer_hist{sc[SC_ERR]};
+    double              err0  = sc[SC_ERR];
+    int                 inner = 0, outer = 0;
+    bool                done  = is[IS_DONE];
+    const double        eta   = 2e-5;
+    while (!done && inner < c->d.n_it) {
+        // inner FP32 CG on K x = r, x = 0 (its own scalars/history)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r32, c->r,
+                           c->Nh);
+        if ((rc = fill<float>(c, c->x32, 0.f, c->Nh)) || (rc = fill<float>(c, c->dA32, 0.f, c->Nh)))
+            return rc;
+        const double err_out = sc[SC_ERR];
+        if ((rc = init_scalars(c, std::max(eta * err_out, 0.25 * c->tol), FANS_ERR_ABSOLUTE, c->d.n_it - inner)))
+            return rc;
+        hipLaunchKernelGGL((k_dot<float>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r32, (const float *) nullptr,
+                           c->Nh, c->d.measure, c->partial);
+        if ((rc = reduce(c, c->partial + c->nblk_vec, c->nblk_vec, 1, RED_ERR_INIT)))
+            return rc;
+        float *dcur = c->dA32, *dnext = c->dB32;
+        int    it   = 0;
+        if ((rc = cg_linear_loop<float, float>(c, c->fft32, c->r32, c->s32, dcur, dnext, c->rnew32, c->x32, c->spec32,
+                                               c->G32, c->dph32, &it)))
+            return rc;
+        inner += it;
+        outer += 1;
+        // u += x: the inner CG drove r(x) = R(u) + K x to zero (x -= alpha d)
+        hipLaunchKernelGGL((k_axpby<double, float>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->u, c->x32,
+                           1.0, 1.0, c->Nh);
+        // true residual
+        if ((rc = residual64(c, c->u, c->r, nullptr)))
+            return rc;
+        if ((rc = init_scalars(c, c->tol, c->tol_type, 1 << 30)))
+            return rc;
+        if ((rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_INIT)) || (rc = read_scalars(c, sc, is)))
+            return rc;
+        outer_hist.push_back(sc[SC_ERR]);
+        const double e = c->tol_type == FANS_ERR_ABSOLUTE ? sc[SC_ERR] : sc[SC_ERR] / err0;
+        done           = !(e > c->tol);
+        if (it == 0)
+            break;  // FP32 floor reached without progress
+    }
+    c->iter        = inner;
+    c->inner_iters = inner;
+    c->outer_iters = outer;
+    c->err_hist    = outer_hist;
+    return 0;
+}
+
+int alloc_fields(fans_gpu_ctx *c)
+{
+    const size_t vb = sizeof(double) * c->Nh;
+    HIPCHK(c, hipMalloc(&c->u, vb));
+    HIPCHK(c, hipMalloc(&c->u_prev, vb));
+    HIPCHK(c, hipMalloc(&c->r, vb));
+    HIPCHK(c, hipMalloc(&c->s, vb));
+    HIPCHK(c, hipMalloc(&c->dA, vb));
+    HIPCHK(c, hipMalloc(&c->dB, vb));
+    HIPCHK(c, hipMalloc(&c->rnew, vb));
+    HIPCHK(c, hipMemsetAsync(c->u, 0, vb, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->u_prev, 0, vb, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->r, 0, vb, c->stream));
+    HIPCHK(c, hipMalloc(&c->spec64, sizeof(cplx<double>) * c->Nf * c->h));
+    HIPCHK(c, hipMalloc(&c->g0d, sizeof(double) * 9));
+    HIPCHK(c, hipMalloc(&c->g0f, sizeof(float) * 9));
+    HIPCHK(c, hipMalloc(&c->sc, sizeof(double) * SC_COUNT));
+    HIPCHK(c, hipMalloc(&c->is, sizeof(int) * IS_COUNT));
+    HIPCHK(c, hipMalloc(&c->hist, sizeof(double) * (c->d.n_it + 2)));
+    HIPCHK(c, hipMalloc(&c->ms, c->N));
+    if (c->d.precision == FANS_PREC_MIXED) {
+        const size_t fb = sizeof(float) * c->Nh;
+        HIPCHK(c, hipMalloc(&c->r32, fb));
+        HIPCHK(c, hipMalloc(&c->s32, fb));
+        HIPCHK(c, hipMalloc(&c->dA32, fb));
+        HIPCHK(c, hipMalloc(&c->dB32, fb));
+        HIPCHK(c, hipMalloc(&c->rnew32, fb));
+        HIPCHK(c, hipMalloc(&c->x32, fb));
+        HIPCHK(c, hipMalloc(&c->spec32, sizeof(cplx<float>) * c->Nf * c->h));
+    }
+    // partial sums: the largest grid of any reducing kernel times its quantities
+    int64_t need = 2 * std::max<int64_t>({(int64_t) c->nblk_asm, (int64_t) c->nblk_vec, c->fft64.nblk_z});
+    need         = std::max<int64_t>(need, (int64_t) 6 * c->nblk_el);
+    c->partial_len = need;
+    HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * need));
+    return 0;
+}
+
+void free_all(fans_gpu_ctx *c)
+{
+    for (void *p : {(void *) c->ms, (void *) c->dph64, (void *) c->dph32, (void *) c->u, (void *) c->u_prev,
+                    (void *) c->r, (void *) c->s, (void *) c->dA, (void *) c->dB, (void *) c->rnew,
+                    (void *) c->spec64, (void *) c->G64, (void *) c->keref_d, (void *) c->G32, (void *) c->js,
+                    (void *) c->g0d, (void *) c->g0f, (void *) c->partial, (void *) c->sc, (void *) c->is,
+                    (void *) c->hist, (void *) c->r32, (void *) c->s32, (void *) c->dA32, (void *) c->dB32,
+                    (void *) c->rnew32, (void *) c->x32, (void *) c->spec32})
+        if (p)
+            (void) hipFree(p);
+    c->fft64.release();
+    c->fft32.release();
+    for (auto &p : c->ev_used) {
+        (void) hipEventDestroy(p.second.first);
+        (void) hipEventDestroy(p.second.second);
+    }
+    for (auto e : c->ev_pool)
+        (void) hipEventDestroy(e);
+}
+
+// ---- host material tables ---------------------------------------------------
+void linear_C(int nstr, int kind, const double *p, double *C)
+{
+    std::memset(C, 0, sizeof(double) * nstr * nstr);
+    if (kind == FANS_MODEL_THERMAL_ISO) {
+        for (int i = 0; i < 3; ++i)
+            C[i * 3 + i] = p[0];
+    } else if (kind == FANS_MODEL_THERMAL_TRI) {
+        const double K[9] = {p[0], p[1], p[2], p[1], p[3], p[4], p[2], p[4], p[5]};
+        std::memcpy(C, K, sizeof(K));
+    } else if (kind == FANS_MODEL_ELASTIC_TRI) {
+        int k = 0;
+        for (int row = 0; row < 6; ++row)
+            for (int col = row; col < 6; ++col) {
+                C[row * 6 + col] = p[k];
+                C[col * 6 + row] = p[k];
+                ++k;
+            }
+    } else {
+        const double lam = p[0] - (2.0 / 3.0) * p[1], mu = p[1];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                C[i * 6 + j] = lam;
+        for (int i = 0; i < 6; ++i)
+            C[i * 6 + i] += 2 * mu;
+    }
+}
+
+// B_int[p] (nstr x 8h) of matmodel.h:104-155 / 157-188 / 267-304 for the
+// reference element stiffness (matmodel.h:237-253)
+void host_B(const fans_gpu_ctx *c, double x, double y, double z, double *out)
+{
+    const double lx = c->d.L[0] / c->nx, ly = c->d.L[1] / c->ny, lz = c->d.L[2] / c->nz;
+    double       b[3][8];
+    b[0][0]        = -(1. - y) * (1. - z) / lx;
+    b[0][1]        = (1. - y) * (1. - z) / lx;
+    b[0][2]        = -y * (1. - z) / lx;
+    b[0][3]        = y * (1. - z) / lx;
+    b[0][4]        = -(1. - y) * z / lx;
+    b[0][5]        = (1. - y) * z / lx;
+    b[0][6]        = -y * z / lx;
+    b[0][7]        = y * z / lx;
+    b[1][0]        = -(1. - x) * (1. - z) / ly;
+    b[1][1]        = -x * (1. - z) / ly;
+    b[1][2]        = (1. - x) * (1. - z) / ly;
+    b[1][3]        = x * (1. - z) / ly;
+    b[1][4]        = -(1. - x) * z / ly;
+    b[1][5]        = -x * z / ly;
+    b[1][6]        = (1. - x) * z / ly;
+    b[1][7]        = x * z / ly;
+    b[2][0]        = -(1. - x) * (1. - y) / lz;
+    b[2][1]        = -x * (1. - y) / lz;
+    b[2][2]        = -(1. - x) * y / lz;
+    b[2][3]        = -x * y / lz;
+    b[2][4]        = (1. - x) * (1. - y) / lz;
+    b[2][5]        = x * (1. - y) / lz;
+    b[2][6]        = (1. - x) * y / lz;
+    b[2][7]        = x * y / lz;
+    const int ncol = 8 * c->h;
+    std::memset(out, 0, sizeof(double) * c->nstr * ncol);
+    if (c->h == 1) {
+        for (int i = 0; i < 3; ++i)
+            for (int q = 0; q < 8; ++q)
+                out[i * ncol + q] = b[i][q];
+        return;
+    }
+    const double sh = kSqrtHalf;
+    for (int q = 0; q < 8; ++q) {
+        out[0 * ncol + 3 * q + 0] = b[0][q];
+        out[1 * ncol + 3 * q + 1] = b[1][q];
+        out[2 * ncol + 3 * q + 2] = b[2][q];
+        out[3 * ncol + 3 * q + 0] = sh * b[1][q];
+        out[4 * ncol + 3 * q + 0] = sh * b[2][q];
+        out[5 * ncol + 3 * q + 1] = sh * b[2][q];
+        out[3 * ncol + 3 * q + 1] = sh * b[0][q];
+        out[4 * ncol + 3 * q + 2] = sh * b[0][q];
+        out[5 * ncol + 3 * q + 2] = sh * b[1][q];
+    }
+}
+
+void ref_element_stiffness(fans_gpu_ctx *c)
+{
+    const int    ne = 8 * c->h, ns = c->nstr;
+    const double xp = 0.5 + std::sqrt(3.) / 6., xm = 0.5 - std::sqrt(3.) / 6.;
+    const double xi[8][3] = {{xm, xm, xm}, {xp, xm, xm}, {xm, xp, xm}, {xp, xp, xm},
+                             {xm, xm, xp}, {xp, xm, xp}, {xm, xp, xp}, {xp, xp, xp}};
+    const double ve       = (c->d.L[0] / c->nx) * (c->d.L[1] / c->ny) * (c->d.L[2] / c->nz);
+    double       tmp[576] = {0};
+    for (int p = 0; p < c->ngp; ++p) {
+        double B[6 * 24];
+        if (c->ngp == 1)
+            host_B(c, 0.5, 0.5, 0.5, B);
+        else
+            host_B(c, xi[p][0], xi[p][1], xi[p][2], B);
+        double CB[6 * 24];
+        for (int i = 0; i < ns; ++i)
+            for (int j = 0; j < ne; ++j) {
+                double a = 0;
+                for (int k = 0; k < ns; ++k)
+                    a += c->kref[i * ns + k] * B[k * ne + j];
+                CB[i * ne + j] = a;
+            }
+        for (int i = 0; i < ne; ++i)
+            for (int j = 0; j < ne; ++j) {
+                double a = 0;
+                for (int k = 0; k < ns; ++k)
+                    a += B[k * ne + i] * CB[k * ne + j];
+                tmp[i * ne + j] += a * ve / c->ngp;
+            }
+    }
+    for (int i = 0; i < ne; ++i)
+        for (int j = 0; j < ne; ++j)
+            c->keref[((i % c->h) * 8 + i / c->h) * ne + (j % c->h) * 8 + j / c->h] = tmp[i * ne + j];
+}
+
+template <typename T>
+int build_fft(fans_gpu_ctx *c, Fft3D<T> &P)
+{
+    hipError_t e = P.init(c->nx, c->ny, c->nz, c->h);
+    if (e == hipErrorInvalidValue)
+        return fail(c, FANS_GPU_EUNSUP, "grid sizes must be products of 2, 3 and 5 (got %d x %d x %d)", c->nx, c->ny,
+                    c->nz);
+    HIPCHK(c, e);
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int fans_gpu_abi_version(void) { return FANS_GPU_ABI_VERSION; }
+
+const char *fans_gpu_last_error(const fans_gpu_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *d)
+{
+    if (!out || !d)
+        return FANS_GPU_EINVAL;
+    *out   = nullptr;
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0)
+        return FANS_GPU_ENODEV;
+    if (d->howmany != 1 && d->howmany != 3)
+        return FANS_GPU_EINVAL;
+    if (d->n_x < 2 || d->n_y < 2 || d->n_z < 2 || d->n_z % 2 != 0)
+        return FANS_GPU_EINVAL;
+    if (d->fe_type != FANS_FE_HEX8 && d->fe_type != FANS_FE_HEX8R && d->fe_type != FANS_FE_BBAR)
+        return FANS_GPU_EINVAL;
+    if (d->method != FANS_METHOD_CG && d->method != FANS_METHOD_FP)
+        return FANS_GPU_EINVAL;
+    if (d->n_it < 0 || d->device < 0 || d->device >= nd)
+        return FANS_GPU_EINVAL;
+    fans_gpu_ctx *c = new fans_gpu_ctx();
+    c->d            = *d;
+    if (hipSetDevice(d->device) != hipSuccess) {
+        delete c;
+        return FANS_GPU_EHIP;
+    }
+    c->nx       = d->n_x;
+    c->ny       = d->n_y;
+    c->nz       = d->n_z;
+    c->h        = d->howmany;
+    c->nstr     = d->howmany == 1 ? 3 : 6;
+    c->ngp      = d->fe_type == FANS_FE_HEX8R ? 1 : 8;
+    c->N        = (int64_t) c->nx * c->ny * c->nz;
+    c->Nh       = c->N * c->h;
+    c->Nf       = (int64_t) c->nx * c->ny * (c->nz / 2 + 1);
+    c->tol      = d->tol;
+    c->tol_type = d->err_type;
+    const double lx = d->L[0] / c->nx, ly = d->L[1] / c->ny, lz = d->L[2] / c->nz;
+    c->geom.ilx     = 1.0 / lx;
+    c->geom.ily     = 1.0 / ly;
+    c->geom.ilz     = 1.0 / lz;
+    c->geom.w       = lx * ly * lz / c->ngp;
+    c->geom.q0      = 0.5 - std::sqrt(3.) / 6.;
+    c->geom.q1      = 0.5 + std::sqrt(3.) / 6.;
+    *out            = c;
+    if (d->fe_type == FANS_FE_BBAR && c->h == 3)
+        return fail(c, FANS_GPU_EUNSUP, "FE_type BBAR is not implemented on the GPU");
+    if (hipStreamCreate(&c->stream) != hipSuccess)
+        return fail(c, FANS_GPU_EHIP, "hipStreamCreate failed");
+    // launch geometry
+    c->nty      = (c->ny + NOUT - 1) / NOUT;
+    c->ntz      = (c->nz + NOUT - 1) / NOUT;
+    c->cx       = c->nx >= 64 ? 16 : std::max(1, c->nx / 4);
+    const int nbx = (c->nx + c->cx - 1) / c->cx;
+    c->nblk_asm = nbx * c->nty * c->ntz;
+    c->nblk_vec = grid_vec(c->Nh);
+    c->nblk_el  = (int) ((c->N + kThreads - 1) / kThreads);
+    int rc;
+    if ((rc = build_fft<double>(c, c->fft64)))
+        return rc;
+    if (d->precision == FANS_PREC_MIXED && (rc = build_fft<float>(c, c->fft32)))
+        return rc;
+    if ((rc = alloc_fields(c)))
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fans_gpu_destroy(fans_gpu_ctx *c)
+{
+    if (!c)
+        return 0;
+    if (c->stream)
+        (void) hipStreamSynchronize(c->stream);
+    free_all(c);
+    if (c->stream)
+        (void) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int fans_gpu_set_microstructure(fans_gpu_ctx *c, const uint16_t *ms, int64_t count)
+{
+    if (!c || !ms || count != c->N)
+        return fail(c, FANS_GPU_EINVAL, "microstructure must hold n_x*n_y*n_z entries");
+    std::vector<uint8_t> m8(c->N);
+    for (int64_t i = 0; i < c->N; ++i) {
+        if (ms[i] > 255)
+            return fail(c, FANS_GPU_EUNSUP, "phase ids above 255 are not supported");
+        m8[i] = (uint8_t) ms[i];
+    }
+    HIPCHK(c, hipMemcpy(c->ms, m8.data(), c->N, hipMemcpyHostToDevice));
+    c->have_ms = true;
+    return 0;
+}
+
+int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, const int32_t *nphase,
+                           const int32_t *phase_ids, const double *params, int64_t n_params, const double *ref)
+{
+    if (!c)
+        return FANS_GPU_EINVAL;
+    if (ng <= 0 || !kind || !nphase || !phase_ids || !params)
+        return fail(c, FANS_GPU_EINVAL, "MaterialManager: 'materials' must be non-empty array");
+    int     maxp = -1, tot = 0;
+    int64_t np = 0;
+    for (int g = 0; g < ng; ++g) {
+        if (nparams(kind[g]) < 0)
+            return fail(c, FANS_GPU_EINVAL, "unknown material model kind %d", kind[g]);
+        const bool thermal = kind[g] <= 2;
+        if (thermal != (c->h == 1))
+            return fail(c, FANS_GPU_EINVAL, "material model kind %d does not match howmany=%d", kind[g], c->h);
+        for (int i = 0; i < nphase[g]; ++i)
+            maxp = std::max(maxp, phase_ids[tot + i]);
+        tot += nphase[g];
+        np += (int64_t) nphase[g] * nparams(kind[g]);
+    }
+    if (np != n_params)
+        return fail(c, FANS_GPU_EINVAL, "expected %lld material parameters, got %lld", (long long) np,
+                    (long long) n_params);
+    if (maxp < 0)
+        return fail(c, FANS_GPU_EINVAL, "MaterialManager: No phases defined");
+    if (maxp > 255)
+        return fail(c, FANS_GPU_EUNSUP, "phase ids above 255 are not supported");
+    c->nphases = maxp + 1;
+    c->phase_kind.assign(c->nphases, 0);
+    std::vector<int>          group_of(c->nphases, -1), local_of(c->nphases, 0);
+    std::vector<const double *> pp(c->nphases, nullptr);
+    c->all_linear = true;
+    c->has_j2     = false;
+    int     ofs   = 0;
+    int64_t pofs  = 0;
+    for (int g = 0; g < ng; ++g) {
+        if (!model_linear(kind[g]))
+            c->all_linear = false;
+        if (kind[g] == FANS_MODEL_J2_NEW)
+            c->has_j2 = true;
+        for (int i = 0; i < nphase[g]; ++i) {
+            const int p = phase_ids[ofs + i];
+            if (p < 0 || group_of[p] >= 0)
+                return fail(c, FANS_GPU_EINVAL, "MaterialManager: Invalid or duplicate phase %d", p);
+            group_of[p]      = g;
+            local_of[p]      = i;
+            c->phase_kind[p] = kind[g];
+            pp[p]            = params + pofs + (int64_t) i * nparams(kind[g]);
+        }
+        ofs += nphase[g];
+        pofs += (int64_t) nphase[g] * nparams(kind[g]);
+    }
+    for (int p = 0; p < c->nphases; ++p)
+        if (group_of[p] < 0)
+            return fail(c, FANS_GPU_EINVAL, "MaterialManager: Phase %d not assigned", p);
+    // reference stiffness (MaterialManager.h:177-205)
+    const int ns = c->nstr;
+    if (ref) {
+        std::memcpy(c->kref, ref, sizeof(double) * ns * ns);
+    } else {
+        std::memset(c->kref, 0, sizeof(c->kref));
+        ofs  = 0;
+        pofs = 0;
+        for (int g = 0; g < ng; ++g) {
+            double       C[36] = {0};
+            const int    npg   = nphase[g], npk = nparams(kind[g]);
+            const double *gp   = params + pofs;
+            if (kind[g] == FANS_MODEL_ELASTIC_ISO) {
+                double lmax = -INFINITY, lmin = INFINITY, mmax = -INFINITY, mmin = INFINITY;
+                for (int i = 0; i < npg; ++i) {
+                    const double lam = gp[i * npk] - (2.0 / 3.0) * gp[i * npk + 1];
+                    lmax = std::max(lmax, lam), lmin = std::min(lmin, lam);
+                    mmax = std::max(mmax, gp[i * npk + 1]), mmin = std::min(mmin, gp[i * npk + 1]);
+                }
+                const double lr = (lmax + lmin) / 2, mr = (mmax + mmin) / 2;
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        C[i * 6 + j] = lr;
+                for (int i = 0; i < 6; ++i)
+                    C[i * 6 + i] += 2 * mr;
+            } else if (kind[g] == FANS_MODEL_J2_NEW || kind[g] == FANS_MODEL_PSEUDO_PLASTIC) {
+                double Kb = 0, Gb = 0;
+                for (int i = 0; i < npg; ++i) {
+                    Kb += gp[i * npk];
+                    Gb += gp[i * npk + 1];
+                }
+                Kb /= npg;
+                Gb /= npg;
+                const double lb = Kb - 2.0 * Gb / 3.0;
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        C[i * 6 + j] = lb;
+                for (int i = 0; i < 6; ++i)
+                    C[i * 6 + i] += 2.0 * Gb;
+            } else {
+                double Ci[36];
+                for (int i = 0; i < npg; ++i) {
+                    linear_C(ns, kind[g], gp + i * npk, Ci);
+                    for (int k = 0; k < ns * ns; ++k)
+                        C[k] += Ci[k];
+                }
+                for (int k = 0; k < ns * ns; ++k)
+                    C[k] /= npg;
+            }
+            for (int k = 0; k < ns * ns; ++k)
+                c->kref[k] += C[k];
+            ofs += npg;
+            pofs += (int64_t) npg * npk;
+        }
+        for (int k = 0; k < ns * ns; ++k)
+            c->kref[k] /= (double) ng;
+    }
+    // per-phase device tables
+    c->ph64.assign(c->nphases, Phase<double>{});
+    c->ph32.assign(c->nphases, Phase<float>{});
+    for (int p = 0; p < c->nphases; ++p) {
+        Phase<double> &P = c->ph64[p];
+        P.kind           = c->phase_kind[p];
+        const double *q  = pp[p];
+        switch (P.kind) {
+        case FANS_MODEL_THERMAL_ISO: P.p[0] = q[0]; break;
+        case FANS_MODEL_THERMAL_TRI:
+        case FANS_MODEL_ELASTIC_TRI: linear_C(ns, P.kind, q, P.p); break;
+        case FANS_MODEL_ELASTIC_ISO:
+            P.p[0] = q[0] - (2.0 / 3.0) * q[1];
+            P.p[1] = q[1];
+            break;
+        case FANS_MODEL_J2_NEW:
+            for (int i = 0; i < 4; ++i)
+                P.p[i] = q[i];
+            break;
+        case FANS_MODEL_PSEUDO_PLASTIC:
+            for (int i = 0; i < 4; ++i)
+                P.p[i] = q[i];
+            P.p[4] = std::sqrt(2. / 3.) * q[2] / (2. * q[1]);
+            P.p[5] = (3. * q[1]) / (3. * q[1] + q[3]);
+            break;
+        }
+        c->ph32[p].kind = P.kind;
+        for (int i = 0; i < 36; ++i)
+            c->ph32[p].p[i] = (float) P.p[i];
+    }
+    if (!c->dph64)
+        HIPCHK(c, hipMalloc(&c->dph64, sizeof(Phase<double>) * 256));
+    if (!c->dph32)
+        HIPCHK(c, hipMalloc(&c->dph32, sizeof(Phase<float>) * 256));
+    HIPCHK(c, hipMemcpy(c->dph64, c->ph64.data(), sizeof(Phase<double>) * c->nphases, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->dph32, c->ph32.data(), sizeof(Phase<float>) * c->nphases, hipMemcpyHostToDevice));
+    // J2 state (zero, Solver ctor solver.h:139)
+    if (c->has_j2 && !c->js) {
+        const size_t bytes = sizeof(double) * 7 * c->ngp * c->N;
+        HIPCHK(c, hipMalloc(&c->js, bytes));
+        HIPCHK(c, hipMemset(c->js, 0, bytes));
+    }
+    // reference element stiffness + Green operator
+    ref_element_stiffness(c);
+    const int ne = 8 * c->h, npk = c->h * (c->h + 1) / 2;
+    if (!c->keref_d)
+        HIPCHK(c, hipMalloc(&c->keref_d, sizeof(double) * 576));
+    HIPCHK(c, hipMemcpy(c->keref_d, c->keref, sizeof(double) * ne * ne, hipMemcpyHostToDevice));
+    if (!c->G64)
+        HIPCHK(c, hipMalloc(&c->G64, sizeof(double) * c->Nf * npk));
+    const int gb = (int) ((c->Nf + kThreads - 1) / kThreads);
+    if (c->h == 1)
+        hipLaunchKernelGGL((k_build_green<1>), dim3(gb), dim3(kThreads), 0, c->stream, c->keref_d, c->G64, c->nx,
+                           c->ny, c->nz);
+    else
+        hipLaunchKernelGGL((k_build_green<3>), dim3(gb), dim3(kThreads), 0, c->stream, c->keref_d, c->G64, c->nx,
+                           c->ny, c->nz);
+    HIPCHK(c, hipGetLastError());
+    if (c->d.precision == FANS_PREC_MIXED) {
+        if (!c->G32)
+            HIPCHK(c, hipMalloc(&c->G32, sizeof(float) * c->Nf * npk));
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_vec(c->Nf * npk)), dim3(kThreads), 0, c->stream,
+                           c->G32, c->G64, c->Nf * npk);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_mat = true;
+    return 0;
+}
+
+int fans_gpu_set_gradient(fans_gpu_ctx *c, const double *g0, int32_t n)
+{
+    if (!c || !g0 || n != c->nstr)
+        return fail(c, FANS_GPU_EINVAL, "Invalid length of loading vector: expected %d", c ? c->nstr : 0);
+    for (int i = 0; i < n; ++i)
+        c->g0[i] = g0[i];
+    int rc = upload_g0(c);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fans_gpu_get_gradient(fans_gpu_ctx *c, double *g0, int32_t n)
+{
+    if (!c || !g0 || n != c->nstr)
+        return FANS_GPU_EINVAL;
+    for (int i = 0; i < n; ++i)
+        g0[i] = c->g0[i];
+    return 0;
+}
+
+int fans_gpu_enable_mixed_bc(fans_gpu_ctx *c, int32_t nE, const int32_t *iE, int32_t nF, const int32_t *iF,
+                             int32_t nsteps, const double *FE, const double *PF, int32_t step)
+{
+    if (!c)
+        return FANS_GPU_EINVAL;
+    if (!c->have_mat)
+        return fail(c, FANS_GPU_ESTATE, "set_materials must precede enable_mixed_bc");
+    const int n = c->nstr;
+    if (nE + nF != n || step < 0 || step >= nsteps)
+        return fail(c, FANS_GPU_EINVAL, "each component must be either strain- or stress-controlled");
+    std::vector<char> present(n, 0);
+    for (int i = 0; i < nE; ++i) {
+        if (iE[i] < 0 || iE[i] >= n)
+            return fail(c, FANS_GPU_EINVAL, "strain index out of range");
+        present[iE[i]] = 1;
+    }
+    for (int i = 0; i < nF; ++i) {
+        if (iF[i] < 0 || iF[i] >= n)
+            return fail(c, FANS_GPU_EINVAL, "stress index out of range");
+        if (present[iF[i]])
+            return fail(c, FANS_GPU_EINVAL, "index appears in both strain_indices and stress_indices");
+        present[iF[i]] = 1;
+    }
+    c->mixed_active = true;
+    c->idxE.assign(iE, iE + nE);
+    c->idxF.assign(iF, iF + nF);
+    if (FE)
+        c->FE.assign(FE, FE + (size_t) nsteps * nE);
+    else
+        c->FE.clear();
+    if (PF)
+        c->PF.assign(PF, PF + (size_t) nsteps * nF);
+    else
+        c->PF.clear();
+    c->nsteps = nsteps;
+    c->step   = step;
+    if (nF) {  // M = pinv(Q_F^T C0 Q_F) (mixedBCs.h:252-253)
+        std::vector<double> A(nF * nF);
+        for (int i = 0; i < nF; ++i)
+            for (int j = 0; j < nF; ++j)
+                A[i * nF + j] = c->kref[iF[i] * n + iF[j]];
+        c->M.assign(nF * nF, 0.0);
+        host_sym_pinv(nF, A.data(), c->M.data());
+    }
+    if (step == 0) {
+        std::memset(c->gvec, 0, sizeof(c->gvec));
+        std::memcpy(c->gvec_prev, c->gvec, sizeof(c->gvec));
+    } else {
+        double delta[9];
+        for (int i = 0; i < n; ++i)
+            delta[i] = c->gvec[i] - c->gvec_prev[i];
+        std::memcpy(c->gvec_prev, c->gvec, sizeof(c->gvec));
+        for (int i = 0; i < nF; ++i)
+            c->gvec[iF[i]] += delta[iF[i]];
+    }
+    for (int i = 0; i < nE; ++i)
+        c->gvec[iE[i]] = c->FE[(size_t) step * nE + i];
+    for (int i = 0; i < n; ++i)
+        c->g0[i] = c->gvec[i];
+    int rc = upload_g0(c);
+    if (rc)
+        return rc;
+    rc = mixed_update(c);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fans_gpu_disable_mixed_bc(fans_gpu_ctx *c)
+{
+    if (!c)
+        return FANS_GPU_EINVAL;
+    c->mixed_active = false;
+    return 0;
+}
+
+int fans_gpu_solve(fans_gpu_ctx *c, fans_gpu_stats *st)
+{
+    if (!c)
+        return FANS_GPU_EINVAL;
+    if (!c->have_ms || !c->have_mat)
+        return fail(c, FANS_GPU_ESTATE, "set_microstructure and set_materials must precede solve");
+    for (int k = 0; k < KF_COUNT; ++k) {
+        c->fam_ms[k] = 0;
+        c->fam_n[k]  = 0;
+    }
+    c->inner_iters = c->outer_iters = 0;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    int rc;
+    if (c->d.precision == FANS_PREC_MIXED && c->d.method == FANS_METHOD_CG)
+        rc = solve_cg_mixed(c);
+    else if (c->d.method == FANS_METHOD_FP)
+        rc = solve_fp_fp64(c);
+    else
+        rc = solve_cg_fp64(c);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    (void) hipEventDestroy(e0);
+    (void) hipEventDestroy(e1);
+    c->solve_ms = ms;
+    if (c->timing)
+        harvest_timing(c);
+    // MaterialManager::update_internal_variables (solver.h:299)
+    if ((rc = commit_state(c)))
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (st) {
+        st->iterations       = c->iter;
+        st->inner_iterations = c->inner_iters ? c->inner_iters : c->iter;
+        st->outer_iterations = c->outer_iters;
+        st->solve_ms         = c->solve_ms;
+        st->final_error      = c->err_hist.empty() ? 0.0 : c->err_hist.back();
+        const double e0v     = c->err_hist.empty() ? 1.0 : c->err_hist[0];
+        const double ev      = c->tol_type == FANS_ERR_ABSOLUTE ? st->final_error : st->final_error / e0v;
+        st->converged        = ev <= c->tol;
+    }
+    return 0;
+}
+
+int fans_gpu_error_history(fans_gpu_ctx *c, double *out, int32_t cap, int32_t *count)
+{
+    if (!c)
+        return FANS_GPU_EINVAL;
+    const int n = (int) c->err_hist.size();
+    if (count)
+        *count = n;
+    if (out)
+        std::memcpy(out, c->err_hist.data(), sizeof(double) * std::min(n, cap));
+    return 0;
+}
+
+int fans_gpu_homogenized_stress(fans_gpu_ctx *c, double *out, int32_t n)
+{
+    if (!c || !out || n != c->nstr)
+        return FANS_GPU_EINVAL;
+    if (!c->have_ms || !c->have_mat)
+        return fail(c, FANS_GPU_ESTATE, "materials not set");
+    return homog_stress(c, out);
+}
+
+int fans_gpu_homogenized_tangent(fans_gpu_ctx *c, double pert, double *out, int32_t n)
+{
+    if (!c || !out || n != c->nstr)
+        return FANS_GPU_EINVAL;
+    const int ns = c->nstr;
+    double    unpert[9], g0[9], pe[9], ps[9], C[81];
+    int       rc = homog_stress(c, unpert);
+    if (rc)
+        return rc;
+    std::memcpy(g0, c->g0, sizeof(g0));
+    // solver.h:749-750: later solves use the relative error with TOL >= 1e-6
+    c->tol_type = FANS_ERR_RELATIVE;
+    c->tol      = std::max(1e-6, c->tol);
+    for (int i = 0; i < ns; ++i) {
+        if (c->all_linear) {
+            std::memset(pe, 0, sizeof(pe));
+            pe[i] = 1.0;
+        } else {
+            std::memcpy(pe, g0, sizeof(pe));
+            pe[i] += pert;
+        }
+        if ((rc = fans_gpu_set_gradient(c, pe, ns)))
+            return rc;
+        c->mixed_active = false;
+        if ((rc = fans_gpu_solve(c, nullptr)) || (rc = homog_stress(c, ps)))
+            return rc;
+        for (int j = 0; j < ns; ++j)
+            C[j * ns + i] = c->all_linear ? ps[j] : (ps[j] - unpert[j]) / pert;
+    }
+    for (int i = 0; i < ns; ++i)
+        for (int j = 0; j < ns; ++j)
+            out[i * ns + j] = 0.5 * (C[i * ns + j] + C[j * ns + i]);
+    return 0;
+}
+
+int fans_gpu_extrapolate_displacement(fans_gpu_ctx *c)
+{
+    if (!c)
+        return FANS_GPU_EINVAL;
+    hipLaunchKernelGGL(k_extrapolate, dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->u, c->u_prev, c->Nh);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fans_gpu_postprocess(fans_gpu_ctx *c, double *strain_elem, double *stress_elem, double *strain_gp,
+                         double *stress_gp, double *stress_avg, double *strain_avg, double *phase_stress_avg,
+                         double *phase_strain_avg, int32_t *phase_counts, int32_t n_slots)
+{
+    if (!c)
+        return FANS_GPU_EINVAL;
+    if (!c->have_ms || !c->have_mat)
+        return fail(c, FANS_GPU_ESTATE, "materials not set");
+    const int ns = c->nstr, ng = c->ngp;
+    double   *de = nullptr, *ds = nullptr, *dge = nullptr, *dgs = nullptr;
+    HIPCHK(c, hipMalloc(&de, sizeof(double) * c->N * ns));
+    HIPCHK(c, hipMalloc(&ds, sizeof(double) * c->N * ns));
+    if (strain_gp)
+        HIPCHK(c, hipMalloc(&dge, sizeof(double) * c->N * ns * ng));
+    if (stress_gp)
+        HIPCHK(c, hipMalloc(&dgs, sizeof(double) * c->N * ns * ng));
+    ElemArgs<double> a{};
+    a.nx          = c->nx;
+    a.ny          = c->ny;
+    a.nz          = c->nz;
+    a.geom        = c->geom;
+    a.ms          = c->ms;
+    a.phases      = c->dph64;
+    a.js          = J2State{c->js, c->N};
+    a.u           = c->u;
+    a.g0          = c->g0d;
+    a.strain_elem = de;
+    a.stress_elem = ds;
+    a.strain_gp   = dge;
+    a.stress_gp   = dgs;
+#define EP(HH, GG)                                                                                               \
+    hipLaunchKernelGGL((k_element_pass<double, HH, GG, ELEM_FIELDS>), dim3(c->nblk_el), dim3(kThreads), 0, \
+                       c->stream, a)
+    if (c->h == 1) {
+        if (ng == 8)
+            EP(1, 8);
+        else
+            EP(1, 1);
+    } else {
+        if (ng == 8)
+            EP(3, 8);
+        else
+            EP(3, 1);
+    }
+#undef EP
+    HIPCHK(c, hipGetLastError());
+    std::vector<double>  he(c->N * ns), hs(c->N * ns);
+    std::vector<uint8_t> hm(c->N);
+    HIPCHK(c, hipMemcpyAsync(he.data(), de, sizeof(double) * c->N * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hs.data(), ds, sizeof(double) * c->N * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hm.data(), c->ms, c->N, hipMemcpyDeviceToHost, c->stream));
+    if (strain_gp)
+        HIPCHK(c, hipMemcpyAsync(strain_gp, dge, sizeof(double) * c->N * ns * ng, hipMemcpyDeviceToHost, c->stream));
+    if (stress_gp)
+        HIPCHK(c, hipMemcpyAsync(stress_gp, dgs, sizeof(double) * c->N * ns * ng, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (void *p : {(void *) de, (void *) ds, (void *) dge, (void *) dgs})
+        if (p)
+            (void) hipFree(p);
+    if (strain_elem)
+        std::memcpy(strain_elem, he.data(), sizeof(double) * c->N * ns);
+    if (stress_elem)
+        std::memcpy(stress_elem, hs.data(), sizeof(double) * c->N * ns);
+    // averages in element order like solver.h:546-574
+    double sa[9] = {0}, ea[9] = {0};
+    if (phase_stress_avg)
+        std::memset(phase_stress_avg, 0, sizeof(double) * n_slots * ns);
+    if (phase_strain_avg)
+        std::memset(phase_strain_avg, 0, sizeof(double) * n_slots * ns);
+    if (phase_counts)
+        std::memset(phase_counts, 0, sizeof(int32_t) * n_slots);
+    for (int64_t e = 0; e < c->N; ++e) {
+        const int p = hm[e];
+        for (int j = 0; j < ns; ++j) {
+            sa[j] += hs[e * ns + j];
+            ea[j] += he[e * ns + j];
+            if (p < n_slots) {
+                if (phase_stress_avg)
+                    phase_stress_avg[p * ns + j] += hs[e * ns + j];
+                if (phase_strain_avg)
+                    phase_strain_avg[p * ns + j] += he[e * ns + j];
+            }
+        }
+        if (phase_counts && p < n_slots)
+            phase_counts[p]++;
+    }
+    for (int j = 0; j < ns; ++j) {
+        if (stress_avg)
+            stress_avg[j] = sa[j] / (double) c->N;
+        if (strain_avg)
+            strain_avg[j] = ea[j] / (double) c->N;
+    }
+    if (phase_counts)
+        for (int p = 0; p < n_slots; ++p)
+            if (phase_counts[p] > 0)
+                for (int j = 0; j < ns; ++j) {
+                    if (phase_stress_avg)
+                        phase_stress_avg[p * ns + j] /= phase_counts[p];
+                    if (phase_strain_avg)
+                        phase_strain_avg[p * ns + j] /= phase_counts[p];
+                }
+    return 0;
+}
+
+int fans_gpu_get_field(fans_gpu_ctx *c, int32_t which, double *host, int64_t count)
+{
+    if (!c || !host)
+        return FANS_GPU_EINVAL;
+    const double *src = nullptr;
+    int64_t       n   = c->Nh;
+    if (which == FANS_FIELD_U)
+        src = c->u;
+    else if (which == FANS_FIELD_R)
+        src = c->r;
+    else if (which == FANS_FIELD_U_PREV)
+        src = c->u_prev;
+    else if (which == FANS_FIELD_PLASTIC_STRAIN || which == FANS_FIELD_HARDENING) {
+        const int ng = c->ngp;
+        n            = which == FANS_FIELD_PLASTIC_STRAIN ? c->N * ng * 6 : c->N * ng;
+        if (count != n)
+            return fail(c, FANS_GPU_EINVAL, "field size mismatch (expected %lld)", (long long) n);
+        if (!c->js) {
+            std::memset(host, 0, sizeof(double) * n);
+            return 0;
+        }
+        std::vector<double> st((size_t) 7 * ng * c->N);
+        HIPCHK(c, hipMemcpy(st.data(), c->js, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
+        for (int64_t e = 0; e < c->N; ++e)
+            for (int g = 0; g < ng; ++g) {
+                if (which == FANS_FIELD_PLASTIC_STRAIN)
+                    for (int k = 0; k < 6; ++k)
+                        host[(e * ng + g) * 6 + k] = st[((size_t) k * ng + g) * c->N + e];
+                else
+                    host[e * ng + g] = st[((size_t) 6 * ng + g) * c->N + e];
+            }
+        return 0;
+    } else
+        return fail(c, FANS_GPU_EINVAL, "unknown field %d", which);
+    if (count != n)
+        return fail(c, FANS_GPU_EINVAL, "field size mismatch (expected %lld)", (long long) n);
+    HIPCHK(c, hipMemcpy(host, src, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fans_gpu_set_field(fans_gpu_ctx *c, int32_t which, const double *host, int64_t count)
+{
+    if (!c || !host)
+        return FANS_GPU_EINVAL;
+    double *dst = which == FANS_FIELD_U ? c->u : (which == FANS_FIELD_U_PREV ? c->u_prev : nullptr);
+    if (!dst)
+        return fail(c, FANS_GPU_EINVAL, "field %d is not writable", which);
+    if (count != c->Nh)
+        return fail(c, FANS_GPU_EINVAL, "field size mismatch");
+    HIPCHK(c, hipMemcpy(dst, host, sizeof(double) * count, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int fans_gpu_residual(fans_gpu_ctx *c, const double *u, double *r, int32_t linear_kd)
+{
+    if (!c || !u || !r)
+        return FANS_GPU_EINVAL;
+    if (!c->have_ms || !c->have_mat)
+        return fail(c, FANS_GPU_ESTATE, "materials not set");
+    if (linear_kd && !c->all_linear)
+        return fail(c, FANS_GPU_EINVAL, "linear matvec needs linear materials");
+    HIPCHK(c, hipMemcpy(c->dB, u, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
+    int rc;
+    if (linear_kd) {
+        // rnew = K d with d = s + 0 * d: feed s = u and beta = 0
+        HIPCHK(c, hipMemcpy(c->s, u, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
+        if ((rc = init_scalars(c, 0.0, 0, 1)))
+            return rc;
+        AsmArgs<double> a = asm_args<double>(c);
+        a.phases          = c->dph64;
+        a.in              = c->s;
+        a.din             = c->dB;
+        a.dout            = c->dA;
+        a.out             = c->rnew;
+        a.scal            = c->sc;
+        rc                = launch_assemble<double, ASM_CGLIN>(c, a);
+    } else {
+        rc = residual64(c, c->dB, c->rnew, nullptr);
+    }
+    if (rc)
+        return rc;
+    HIPCHK(c, hipMemcpy(r, c->rnew, sizeof(double) * c->Nh, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fans_gpu_convolve(fans_gpu_ctx *c, const double *r, double *out)
+{
+    if (!c || !r || !out)
+        return FANS_GPU_EINVAL;
+    if (!c->have_mat)
+        return fail(c, FANS_GPU_ESTATE, "materials not set");
+    HIPCHK(c, hipMemcpy(c->dB, r, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
+    EpiArgs<double> ep{};
+    ep.out = c->rnew;
+    int rc = convolve_dev<double>(c, c->fft64, c->dB, c->spec64, c->G64, EPI_STORE, ep, nullptr);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipMemcpy(out, c->rnew, sizeof(double) * c->Nh, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fans_gpu_get_green(fans_gpu_ctx *c, double *out, int64_t count)
+{
+    if (!c || !out)
+        return FANS_GPU_EINVAL;
+    const int64_t n = c->Nf * c->h * (c->h + 1) / 2;
+    if (count != n || !c->G64)
+        return fail(c, FANS_GPU_EINVAL, "green size mismatch or not built");
+    HIPCHK(c, hipMemcpy(out, c->G64, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fans_gpu_get_reference_stiffness(fans_gpu_ctx *c, double *out, int32_t n)
+{
+    if (!c || !out || n != c->nstr)
+        return FANS_GPU_EINVAL;
+    std::memcpy(out, c->kref, sizeof(double) * n * n);
+    return 0;
+}
+
+static int fft_standalone(int dev, int nx, int ny, int nz, int h, const double *in, double *out, bool fwd)
+{
+    fans_gpu_desc d{};
+    d.n_x = nx, d.n_y = ny, d.n_z = nz, d.howmany = h, d.n_it = 1, d.device = dev;
+    d.L[0] = d.L[1] = d.L[2] = 1.0;
+    fans_gpu_ctx *c = nullptr;
+    int           rc = fans_gpu_create(&c, &d);
+    if (rc) {
+        fans_gpu_destroy(c);
+        return rc;
+    }
+    const int64_t nr = (int64_t) nx * ny * nz * h, nc = (int64_t) nx * ny * (nz / 2 + 1) * h;
+    auto         &P  = c->fft64;
+    const int64_t nlines = (int64_t) nx * ny;
+    const int     Kc     = P.Mc * h;
+    if (fwd) {
+        rc = hipMemcpy(c->dB, in, sizeof(double) * nr, hipMemcpyHostToDevice) == hipSuccess ? 0 : FANS_GPU_EHIP;
+        if (!rc) {
+            if (h == 1)
+                hipLaunchKernelGGL((k_fft_z_r2c<double, 1>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, c->dB,
+                                   c->spec64, P.Lz, P.twz, nz, nlines, P.lpb, nullptr);
+            else
+                hipLaunchKernelGGL((k_fft_z_r2c<double, 3>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, c->dB,
+                                   c->spec64, P.Lz, P.twz, nz, nlines, P.lpb, nullptr);
+            hipLaunchKernelGGL((k_fft_strided<double, false>), dim3((Kc + P.bc - 1) / P.bc, nx), dim3(kThreads), P.smy,
+                               c->stream, c->spec64, P.Ly, P.twy, (int64_t) Kc, Kc, (int64_t) ny * Kc, P.bc, nullptr);
+            // x pass: strided with columns = the whole (y, kz, h) block
+            hipLaunchKernelGGL((k_fft_strided<double, false>), dim3((ny * Kc + P.bc - 1) / P.bc, 1), dim3(kThreads),
+                               (size_t) 2 * sizeof(cplx<double>) * nx * P.bc, c->stream, c->spec64, P.Lx, P.twx,
+                               (int64_t) ny * Kc, ny * Kc, (int64_t) 0, P.bc, nullptr);
+            rc = hipMemcpy(out, c->spec64, sizeof(cplx<double>) * nc / 1, hipMemcpyDeviceToHost) == hipSuccess
+                     ? 0
+                     : FANS_GPU_EHIP;
+        }
+    } else {
+        rc = hipMemcpy(c->spec64, in, sizeof(cplx<double>) * nc, hipMemcpyHostToDevice) == hipSuccess ? 0
+                                                                                                      : FANS_GPU_EHIP;
+        if (!rc) {
+            hipLaunchKernelGGL((k_fft_strided<double, true>), dim3((ny * Kc + P.bc - 1) / P.bc, 1), dim3(kThreads),
+                               (size_t) 2 * sizeof(cplx<double>) * nx * P.bc, c->stream, c->spec64, P.Lx, P.twx,
+                               (int64_t) ny * Kc, ny * Kc, (int64_t) 0, P.bc, nullptr);
+            hipLaunchKernelGGL((k_fft_strided<double, true>), dim3((Kc + P.bc - 1) / P.bc, nx), dim3(kThreads), P.smy,
+                               c->stream, c->spec64, P.Ly, P.twy, (int64_t) Kc, Kc, (int64_t) ny * Kc, P.bc, nullptr);
+            EpiArgs<double> ep{};
+            ep.out = c->dB;
+            if (h == 1)
+                hipLaunchKernelGGL((k_fft_z_c2r<double, 1, EPI_STORE>), dim3(P.nblk_z), dim3(kThreads), P.smz,
+                                   c->stream, c->spec64, P.Lz, P.twz, nz, nlines, P.lpb, ep, nullptr);
+            else
+                hipLaunchKernelGGL((k_fft_z_c2r<double, 3, EPI_STORE>), dim3(P.nblk_z), dim3(kThreads), P.smz,
+                                   c->stream, c->spec64, P.Lz, P.twz, nz, nlines, P.lpb, ep, nullptr);
+            rc = hipMemcpy(out, c->dB, sizeof(double) * nr, hipMemcpyDeviceToHost) == hipSuccess ? 0 : FANS_GPU_EHIP;
+        }
+    }
+    if (!rc && hipGetLastError() != hipSuccess)
+        rc = FANS_GPU_EHIP;
+    fans_gpu_destroy(c);
+    return rc;
+}
+
+int fans_gpu_fft_r2c(int32_t dev, int32_t nx, int32_t ny, int32_t nz, int32_t h, const double *in, double *out)
+{
+    if (!in || !out || (h != 1 && h != 3))
+        return FANS_GPU_EINVAL;
+    return fft_standalone(dev, nx, ny, nz, h, in, out, true);
+}
+
+int fans_gpu_fft_c2r(int32_t dev, int32_t nx, int32_t ny, int32_t nz, int32_t h, const double *in, double *out)
+{
+    if (!in || !out || (h != 1 && h != 3))
+        return FANS_GPU_EINVAL;
+    return fft_standalone(dev, nx, ny, nz, h, in, out, false);
+}
+
+int fans_gpu_set_option(fans_gpu_ctx *c, int32_t opt, int64_t value)
+{
+    if (!c)
+        return FANS_GPU_EINVAL;
+    if (opt == FANS_OPT_TIMING) {
+        c->timing = value != 0;
+        return 0;
+    }
+    return fail(c, FANS_GPU_EINVAL, "unknown option %d", opt);
+}
+
+const char *fans_gpu_kernel_family_name(int32_t k) { return (k >= 0 && k < KF_COUNT) ? kFamilyNames[k] : ""; }
+
+int fans_gpu_kernel_times(fans_gpu_ctx *c, double *out, int32_t cap, int32_t *count)
+{
+    if (!c)
+        return FANS_GPU_EINVAL;
+    if (count)
+        *count = KF_COUNT;
+    for (int k = 0; k < KF_COUNT && 2 * k + 1 < cap; ++k) {
+        out[2 * k]     = c->fam_ms[k];
+        out[2 * k + 1] = c->fam_n[k];
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+// Streaming vector kernels, deterministic reductions and the device-side CG
+// scalar recurrences of include/solverCG.h:61-117.
+//
+// Every reduction is two-level: each block writes one partial per quantity to
+// a fixed slot, then a single block sums the partials in a fixed order.  The
+// launch geometry of every streaming kernel is fixed per problem size, so all
+// sums are bitwise reproducible from run to run.
+#pragma once
+#include "common.h"
+
+namespace fans {
+
+// device scalar slots (double) -- layout shared with the host
+enum {
+    SC_BETA     = 0,
+    SC_ALPHA    = 1,
+    SC_DELTA    = 2,
+    SC_DELTA0   = 3,
+    SC_DELTAMID = 4,
+    SC_ERR      = 5,
+    SC_ERR0     = 6,
+    SC_DN       = 7,
+    SC_TOL      = 8,
+    SC_RES0     = 16,  // generic reduction results [16, 32)
+    SC_COUNT    = 32
+};
+// device integer slots
+enum { IS_DONE = 0, IS_ITER = 1, IS_NIT = 2, IS_ERRTYPE = 3, IS_MEASURE = 4, IS_COUNT = 8 };
+
+enum {
+    RED_ERR_INIT = 0,  // q0 = norm partials      -> err0, iter = 0, delta = 1, deltamid = 0
+    RED_DELTA    = 1,  // q0 = <r,s> partials     -> delta, beta
+    RED_ALPHA    = 2,  // q0 = <d,rnew> partials  -> alpha
+    RED_UPDATE   = 3,  // q0 = norm, q1 = <r,s>   -> err, deltamid, iter++
+    RED_ERR_NEXT = 4,  // q0 = norm partials      -> err, iter++
+    RED_SUM      = 5   // q0..q(nq-1) sums        -> SC_RES0 + q
+};
+
+__device__ __forceinline__ double finish_norm(double v, int measure) { return measure == 1 ? sqrt(v) : v; }
+
+// sets done if the loop condition of solverCG.h:84 / solverFP.h:221 fails
+__device__ __forceinline__ void record_error(double *sc, int *is, double *hist, double err)
+{
+    const int it = is[IS_ITER];
+    hist[it]     = err;
+    if (it == 0)
+        sc[SC_ERR0] = err;
+    sc[SC_ERR]           = err;
+    const double err_rel = it == 0 ? 100.0 : err / sc[SC_ERR0];
+    const double e       = is[IS_ERRTYPE] == 0 ? err : err_rel;
+    is[IS_DONE]          = !(it < is[IS_NIT] && e > sc[SC_TOL]);
+}
+
+__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ partial, int nblk, int nq, int stage,
+                                                double *sc, int *is, double *hist)
+{
+    __shared__ double sm[4 * 8];
+    const int measure = is[IS_MEASURE];
+    if (stage != RED_ERR_INIT && stage != RED_SUM && is[IS_DONE])
+        return;
+    double v[2] = {0.0, 0.0};
+    bool   mx[2];
+    for (int q = 0; q < nq; q += 2) {
+        const int nqq = (nq - q) < 2 ? 1 : 2;
+        for (int k = 0; k < 2; ++k) {
+            const bool norm_slot = (k + q == 0) && (stage == RED_ERR_INIT || stage == RED_UPDATE || stage == RED_ERR_NEXT);
+            mx[k]                = norm_slot && measure == 2;
+            double a             = 0.0;
+            if (k < nqq)
+                for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+                    const double p = partial[(int64_t) (q + k) * nblk + b];
+                    a              = mx[k] ? fmax(a, p) : a + p;
+                }
+            v[k] = a;
+        }
+        block_reduce<2>(v, mx, sm);
+        if (threadIdx.x == 0) {
+            if (stage == RED_SUM) {
+                sc[SC_RES0 + q] = v[0];
+                if (nqq == 2)
+                    sc[SC_RES0 + q + 1] = v[1];
+            } else if (stage == RED_ERR_INIT) {
+                is[IS_ITER]     = 0;
+                sc[SC_DELTA]    = 1.0;
+                sc[SC_DELTAMID] = 0.0;
+                record_error(sc, is, hist, finish_norm(v[0], measure));
+            } else if (stage == RED_DELTA) {
+                sc[SC_DELTA0] = sc[SC_DELTA];
+                sc[SC_DELTA]  = v[0];
+                sc[SC_BETA]   = fmax(0.0, (sc[SC_DELTA] - sc[SC_DELTAMID]) / sc[SC_DELTA0]);
+            } else if (stage == RED_ALPHA) {
+                sc[SC_DN]    = v[0];
+                sc[SC_ALPHA] = sc[SC_DELTA] / v[0];
+            } else if (stage == RED_UPDATE) {
+                sc[SC_DELTAMID] = v[1];
+                is[IS_ITER] += 1;
+                record_error(sc, is, hist, finish_norm(v[0], measure));
+            } else if (stage == RED_ERR_NEXT) {
+                is[IS_ITER] += 1;
+                record_error(sc, is, hist, finish_norm(v[0], measure));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void acc_norm(double &acc, T v, int measure)
+{
+    const double a = fabs((double) v);
+    if (measure == 2)
+        acc = fmax(acc, a);
+    else if (measure == 1)
+        acc += a * a;
+    else
+        acc += a;
+}
+
+// r -= alpha rnew; u -= alpha d; partials: |r| (measure), <r, s>
+template <typename T, typename TU>
+__global__ __launch_bounds__(256) void k_cg_update(T *__restrict__ r, const T *__restrict__ rnew, TU *__restrict__ u,
+                                                   const T *__restrict__ d, const T *__restrict__ s, int64_t n,
+                                                   const double *__restrict__ sc, const int *__restrict__ is,
+                                                   double *__restrict__ partial)
+{
+    __shared__ double sm[8];
+    if (is[IS_DONE])
+        return;
+    const int    measure = is[IS_MEASURE];
+    const T      alpha   = (T) sc[SC_ALPHA];
+    const double alphad  = sc[SC_ALPHA];
+    double       a0 = 0.0, a1 = 0.0;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        const T rv = r[i] - alpha * rnew[i];
+        r[i]       = rv;
+        if constexpr (sizeof(TU) == sizeof(T))
+            u[i] -= alpha * d[i];
+        else
+            u[i] -= alphad * (double) d[i];
+        acc_norm(a0, rv, measure);
+        a1 += (double) rv * (double) s[i];
+    }
+    double     v[2]  = {a0, a1};
+    const bool mx[2] = {measure == 2, false};
+    block_reduce<2>(v, mx, sm);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x]             = v[0];
+        partial[gridDim.x + blockIdx.x] = v[1];
+    }
+}
+
+// partial <a, b> (slot 0) and optionally the norm of a (slot 1)
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot(const T *__restrict__ a, const T *__restrict__ b, int64_t n, int measure,
+                                             double *__restrict__ partial)
+{
+    __shared__ double sm[8];
+    double            a0 = 0.0, a1 = 0.0;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        if (b)
+            a0 += (double) a[i] * (double) b[i];
+        acc_norm(a1, a[i], measure);
+    }
+    double     v[2]  = {a0, a1};
+    const bool mx[2] = {false, measure == 2};
+    block_reduce<2>(v, mx, sm);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x]             = v[0];
+        partial[gridDim.x + blockIdx.x] = v[1];
+    }
+}
+
+// y = a * x + b * y  (a, b from host)
+template <typename T, typename TX>
+__global__ __launch_bounds__(256) void k_axpby(T *__restrict__ y, const TX *__restrict__ x, double a, double b, int64_t n)
+{
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
+        y[i] = (T) (a * (double) x[i] + b * (double) y[i]);
+}
+
+// d = s + beta d with beta from the device scalars (nonlinear CG, solverCG.h:94)
+template <typename T>
+__global__ __launch_bounds__(256) void k_direction(T *__restrict__ d, const T *__restrict__ s, const double *__restrict__ sc,
+                                                   int64_t n)
+{
+    const T beta = (T) sc[SC_BETA];
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
+        d[i] = s[i] + beta * d[i];
+}
+
+template <typename T, typename TS>
+__global__ __launch_bounds__(256) void k_convert(T *__restrict__ y, const TS *__restrict__ x, int64_t n)
+{
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
+        y[i] = (T) x[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T *__restrict__ y, T v, int64_t n)
+{
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
+        y[i] = v;
+}
+
+// Solver::extrapolateDisplacement (solver.h:302-311)
+__global__ __launch_bounds__(256) void k_extrapolate(double *__restrict__ u, double *__restrict__ up, int64_t n)
+{
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        const double delta = u[i] - up[i];
+        up[i]              = u[i];
+        u[i] += delta;
+    }
+}
+
+}  // namespace fans

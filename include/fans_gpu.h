@@ -1,0 +1,181 @@
+/*
+ * fans_gpu.h -- C ABI of the MI355X (gfx950) implementation of the FANS
+ * per-iteration solver path.  libfans_gpu.so exports exactly these symbols.
+ *
+ * One context = one load case of the reference: the pair (MaterialManager,
+ * Solver<howmany,n_str>) that src/main.cpp:15-16 creates per load case.  The
+ * context owns every device buffer; host arrays passed in or out are caller
+ * owned and copied.  All functions return 0 on success and a negative
+ * FANS_GPU_E* code otherwise; fans_gpu_last_error() gives the message.  The
+ * reference reports configuration errors as C++ exceptions
+ * (MaterialManager.h:69-108, mixedBCs.h:63-128); no exception crosses this ABI.
+ * A context is not thread safe (the reference is single threaded per rank);
+ * use one host thread per context.  There is no CPU fallback: without a HIP
+ * device fans_gpu_create() fails with FANS_GPU_ENODEV.
+ *
+ * Field layouts are the reference's in-memory layouts without ghost planes or
+ * FFT padding: nodal fields [x][y][z][howmany], microstructure [x][y][z]
+ * (Reader::ms after ReadMS, reader.cpp:380-399), element fields [x][y][z][n_str],
+ * Gauss point fields [x][y][z][n_gp][n_str].
+ */
+#ifndef FANS_GPU_H
+#define FANS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FANS_GPU_ABI_VERSION 1
+
+enum {
+    FANS_GPU_OK       = 0,
+    FANS_GPU_EINVAL   = -1, /* bad argument / configuration (reference: exception)   */
+    FANS_GPU_ENODEV   = -2, /* no HIP device                                         */
+    FANS_GPU_EHIP     = -3, /* HIP runtime error                                     */
+    FANS_GPU_ESTATE   = -4, /* call out of order (e.g. solve before set_materials)   */
+    FANS_GPU_EUNSUP   = -5  /* valid for the reference but not implemented here      */
+};
+
+/* FE_type (reader.cpp:120-128) */
+enum { FANS_FE_HEX8 = 0, FANS_FE_HEX8R = 1, FANS_FE_BBAR = 2 };
+/* method (setup.h:75-85) */
+enum { FANS_METHOD_CG = 0, FANS_METHOD_FP = 1 };
+/* error_parameters.measure / .type (solver.h:414-452) */
+enum { FANS_MEASURE_L1 = 0, FANS_MEASURE_L2 = 1, FANS_MEASURE_LINF = 2 };
+enum { FANS_ERR_ABSOLUTE = 0, FANS_ERR_RELATIVE = 1 };
+/* arithmetic of the operator apply */
+enum {
+    FANS_PREC_FP64  = 0, /* everything FP64: the drop-in form                         */
+    FANS_PREC_MIXED = 1  /* FP32 operator inside an FP64 defect-correction loop       */
+};
+/* material models (setup.h:21-61); per-phase parameters in brackets, in the order
+ * of the reference's material_properties keys */
+enum {
+    FANS_MODEL_THERMAL_ISO    = 1, /* LinearThermalIsotropic [conductivity]              */
+    FANS_MODEL_THERMAL_TRI    = 2, /* LinearThermalTriclinic [K_11 K_12 K_13 K_22 K_23 K_33] */
+    FANS_MODEL_ELASTIC_ISO    = 3, /* LinearElasticIsotropic [bulk_modulus shear_modulus] */
+    FANS_MODEL_ELASTIC_TRI    = 4, /* LinearElasticTriclinic [C_11 .. C_66, 21 values]   */
+    FANS_MODEL_J2_NEW         = 5, /* J2PlasticityNew_LinearIsotropicHardening
+                                      [bulk_modulus shear_modulus yield_stress
+                                       isotropic_hardening_parameter]                     */
+    FANS_MODEL_PSEUDO_PLASTIC = 6  /* PseudoPlasticLinearHardening
+                                      [bulk_modulus shear_modulus yield_stress hardening_parameter] */
+};
+/* fields for fans_gpu_get_field / fans_gpu_set_field */
+enum {
+    FANS_FIELD_U              = 0, /* displacement fluctuation v_u, double [N][h]      */
+    FANS_FIELD_R              = 1, /* residual v_r, double [N][h]                      */
+    FANS_FIELD_U_PREV         = 2, /* v_u_prev, double [N][h]                          */
+    FANS_FIELD_PLASTIC_STRAIN = 3, /* committed plastic strain, double [N][n_gp][6]    */
+    FANS_FIELD_HARDENING      = 4  /* committed isotropic hardening q, double [N][n_gp] */
+};
+
+typedef struct {
+    int32_t n_x, n_y, n_z;   /* grid (Reader::dims)                                  */
+    int32_t howmany;         /* 1 thermal, 3 small-strain mechanics                  */
+    int32_t fe_type;         /* FANS_FE_*                                            */
+    int32_t method;          /* FANS_METHOD_*                                        */
+    int32_t measure;         /* FANS_MEASURE_*                                       */
+    int32_t err_type;        /* FANS_ERR_*                                           */
+    int32_t n_it;            /* max iterations (Reader::n_it)                        */
+    int32_t ls_max_iter;     /* linesearch_parameters.max_iter (default 5)           */
+    double  L[3];            /* microstructure.L                                     */
+    double  tol;             /* error_parameters.tolerance                           */
+    double  ls_tol;          /* linesearch_parameters.tol (default 1e-2)             */
+    int32_t precision;       /* FANS_PREC_*                                          */
+    int32_t device;          /* HIP device ordinal                                   */
+    int32_t reserved[6];     /* must be zero                                         */
+} fans_gpu_desc;
+
+typedef struct {
+    int32_t iterations;       /* CG/FP iterations of the last solve (Solver::iter)   */
+    int32_t inner_iterations; /* FP32 inner iterations (mixed form), else = iterations */
+    int32_t outer_iterations; /* FP64 residual refreshes (mixed form), else 0        */
+    int32_t converged;        /* final error <= tolerance                            */
+    double  solve_ms;         /* device time of internalSolve (HIP events)           */
+    double  final_error;      /* last entry of the error history                     */
+} fans_gpu_stats;
+
+typedef struct fans_gpu_ctx fans_gpu_ctx;
+
+int         fans_gpu_abi_version(void);
+const char *fans_gpu_last_error(const fans_gpu_ctx *ctx);
+
+/* Solver + Reader setup (solver.h:105-142, reader.cpp:227-411).  Fails with
+ * FANS_GPU_ENODEV when no HIP device is present. */
+int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *desc);
+int fans_gpu_destroy(fans_gpu_ctx *ctx);
+
+/* Reader::ms (reader.cpp:380-399): n_x*n_y*n_z phase ids in [x][y][z] order */
+int fans_gpu_set_microstructure(fans_gpu_ctx *ctx, const uint16_t *ms_xyz, int64_t count);
+
+/* MaterialManager ctor (MaterialManager.h:38-159): n_groups material groups, group
+ * g of model kind[g] owning nphase[g] phases (listed consecutively in phase_ids)
+ * with nphase[g] * nparams(kind[g]) parameters.  ref_material (n_str x n_str row
+ * major, MaterialManager.h:179-197) may be NULL.  Builds the per-phase tables, the
+ * reference stiffness and the Green operator (solver.h:144-204) on the device. */
+int fans_gpu_set_materials(fans_gpu_ctx *ctx, int32_t n_groups, const int32_t *kind, const int32_t *nphase,
+                           const int32_t *phase_ids, const double *params, int64_t n_params,
+                           const double *ref_material);
+
+/* MaterialManager::set_gradient (MaterialManager.h:216-221) */
+int fans_gpu_set_gradient(fans_gpu_ctx *ctx, const double *g0, int32_t n_str);
+int fans_gpu_get_gradient(fans_gpu_ctx *ctx, double *g0, int32_t n_str);
+
+/* Solver::enableMixedBC / disableMixedBC (solver.h:71-78, mixedBCs.h:181-226):
+ * strain path [n_steps][n_E], stress path [n_steps][n_F] */
+int fans_gpu_enable_mixed_bc(fans_gpu_ctx *ctx, int32_t n_E, const int32_t *idx_E, int32_t n_F, const int32_t *idx_F,
+                             int32_t n_steps, const double *strain_path, const double *stress_path, int32_t step);
+int fans_gpu_disable_mixed_bc(fans_gpu_ctx *ctx);
+
+/* Solver::solve (solver.h:282-300): internalSolve + update_internal_variables */
+int fans_gpu_solve(fans_gpu_ctx *ctx, fans_gpu_stats *stats);
+/* Solver::err_all[0..iter] */
+int fans_gpu_error_history(fans_gpu_ctx *ctx, double *out, int32_t cap, int32_t *count);
+
+/* Solver::get_homogenized_stress (solver.h:707-737) */
+int fans_gpu_homogenized_stress(fans_gpu_ctx *ctx, double *out, int32_t n_str);
+/* Solver::get_homogenized_tangent (solver.h:739-778): out n_str x n_str row major */
+int fans_gpu_homogenized_tangent(fans_gpu_ctx *ctx, double pert, double *out, int32_t n_str);
+/* Solver::extrapolateDisplacement (solver.h:302-311) */
+int fans_gpu_extrapolate_displacement(fans_gpu_ctx *ctx);
+
+/* element pass of Solver::postprocess (solver.h:503-574); any output may be NULL.
+ * phase_* arrays have n_slots rows (phase ids 0..n_slots-1). */
+int fans_gpu_postprocess(fans_gpu_ctx *ctx, double *strain_elem, double *stress_elem, double *strain_gp,
+                         double *stress_gp, double *stress_avg, double *strain_avg, double *phase_stress_avg,
+                         double *phase_strain_avg, int32_t *phase_counts, int32_t n_slots);
+
+int fans_gpu_get_field(fans_gpu_ctx *ctx, int32_t which, double *host, int64_t count);
+int fans_gpu_set_field(fans_gpu_ctx *ctx, int32_t which, const double *host, int64_t count);
+
+/* single operators (Solver::compute_residual, solver.h:228-280, and
+ * Solver::convolution, solver.h:387-412) on host arrays [N][h]; linear_kd=1 is
+ * the linear CG matvec of solverCG.h:97-103 (no macroscopic gradient) */
+int fans_gpu_residual(fans_gpu_ctx *ctx, const double *u, double *r, int32_t linear_kd);
+int fans_gpu_convolve(fans_gpu_ctx *ctx, const double *r, double *out);
+/* Green operator, [n_x][n_y][n_z/2+1][h(h+1)/2] packed (00 01 02 11 12 22) */
+int fans_gpu_get_green(fans_gpu_ctx *ctx, double *out, int64_t count);
+/* reference stiffness kapparef_mat (MaterialManager.h:177-205), n_str x n_str */
+int fans_gpu_get_reference_stiffness(fans_gpu_ctx *ctx, double *out, int32_t n_str);
+
+/* standalone 3-D r2c / c2r of a [nx][ny][nz][h] real field (unnormalised, like
+ * the FFTW plans of solver.h:221-222); spectrum [nx][ny][nz/2+1][h] interleaved */
+int fans_gpu_fft_r2c(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t h, const double *in, double *out_cplx);
+int fans_gpu_fft_c2r(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t h, const double *in_cplx, double *out);
+
+/* options: FANS_OPT_TIMING=1 brackets every launch with HIP events on the
+ * context's stream (no reference counterpart; replaces the clock() counters of
+ * solver.h:286-297) */
+enum { FANS_OPT_TIMING = 1 };
+int fans_gpu_set_option(fans_gpu_ctx *ctx, int32_t opt, int64_t value);
+/* per kernel family of the last solve: out[2k] = total ms, out[2k+1] = launches */
+int         fans_gpu_kernel_times(fans_gpu_ctx *ctx, double *out, int32_t cap, int32_t *count);
+const char *fans_gpu_kernel_family_name(int32_t family);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

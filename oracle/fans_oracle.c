@@ -1,0 +1,1399 @@
+/*
+ * fans_oracle.c -- CPU FP64 restatement of the FANS per-iteration solver path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path in fans_amd/.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * this library; the product never routes through it.
+ *
+ * Each function cites the reference file:line (FANS 0.6.2) it restates.  The
+ * restatement is single-rank: the ghost x-plane of u holds a copy of plane 0
+ * and the ghost plane of r is folded back into plane 0 after the element loop,
+ * exactly as the MPI_Sendrecv pair of solver.h:244-245,264-269 does at P=1.
+ *
+ * Parity status of each part (pins live in tests/test_oracle_*.py):
+ *   FFT r2c/c2r            pinned: numpy.fft.rfftn/irfftn
+ *   B, Ke, Ke_ref          pinned: partition of unity, exact linear gradients,
+ *                                  symmetry/PSD/translation null space
+ *   residual assembly      pinned: brute-force sparse assembly in numpy
+ *   Green operator         pinned: G(xi) K(xi) = I from plane-wave probing
+ *   CG / FP                pinned: scipy direct solve on tiny grids, homogeneous
+ *                                  RVE, laminate closed form
+ *   J2PlasticityNew        pinned: textbook radial return (Simo & Hughes Box 3.1)
+ *   mixed BC + secant LS   pinned: golden macro strains of test_MixedBCs.json
+ *   homogenized tangent    pinned: symmetry/SPD, Voigt/Reuss bounds, laminate
+ */
+#include "fans_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+#define MAXH 3
+#define MAXSTR 6
+#define MAXE 24 /* 8 * howmany */
+
+/* ------------------------------------------------------------------------ */
+/* FFT: recursive mixed-radix decimation in time, any length                 */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int   n;
+    int   nf;
+    int   f[40];
+    cplx *w; /* w[k] = exp(-2 pi i k / n) */
+} fft_plan;
+
+static void plan_init(fft_plan *P, int n)
+{
+    P->n  = n;
+    P->nf = 0;
+    int m = n;
+    while (m % 4 == 0) { P->f[P->nf++] = 4; m /= 4; }
+    while (m % 2 == 0) { P->f[P->nf++] = 2; m /= 2; }
+    for (int p = 3; m > 1; p += 2) {
+        while (m % p == 0) { P->f[P->nf++] = p; m /= p; }
+    }
+    P->w = (cplx *) malloc(sizeof(cplx) * (size_t) n);
+    for (int k = 0; k < n; k++) {
+        long double th = -2.0L * 3.14159265358979323846264338327950288L * (long double) k / (long double) n;
+        P->w[k]        = (double) cosl(th) + I * (double) sinl(th);
+    }
+}
+
+static void plan_free(fft_plan *P) { free(P->w); }
+
+static inline cplx tw(const fft_plan *P, long e, int sign)
+{
+    e %= P->n;
+    return sign < 0 ? P->w[e] : conj(P->w[e]);
+}
+
+static void fft_rec(const cplx *in, ptrdiff_t is, cplx *out, int n, const fft_plan *P, long wstep, const int *f, int sign)
+{
+    if (n == 1) {
+        out[0] = in[0];
+        return;
+    }
+    int p = f[0], m = n / p;
+    for (int q = 0; q < p; q++)
+        fft_rec(in + q * is, is * p, out + q * m, m, P, wstep * p, f + 1, sign);
+    cplx  stackbuf[16];
+    cplx *tmp = p <= 16 ? stackbuf : (cplx *) malloc(sizeof(cplx) * (size_t) p);
+    for (int k = 0; k < m; k++) {
+        for (int q = 0; q < p; q++)
+            tmp[q] = out[q * m + k] * tw(P, (long) q * k * wstep, sign);
+        for (int t = 0; t < p; t++) {
+            cplx acc = 0;
+            for (int q = 0; q < p; q++)
+                acc += tmp[q] * tw(P, (long) q * t * m * wstep, sign);
+            out[k + t * m] = acc;
+        }
+    }
+    if (tmp != stackbuf)
+        free(tmp);
+}
+
+/* transform every line of one axis of a [nx][ny][nzc][h] complex array */
+static void fft_axis(cplx *F, int nx, int ny, int nzc, int h, int axis, int sign)
+{
+    int       n = axis == 0 ? nx : (axis == 1 ? ny : nzc);
+    ptrdiff_t stride, nlines;
+    if (axis == 2) {
+        stride = h;
+        nlines = (ptrdiff_t) nx * ny * h;
+    } else if (axis == 1) {
+        stride = (ptrdiff_t) nzc * h;
+        nlines = (ptrdiff_t) nx * nzc * h;
+    } else {
+        stride = (ptrdiff_t) ny * nzc * h;
+        nlines = (ptrdiff_t) ny * nzc * h;
+    }
+    fft_plan P;
+    plan_init(&P, n);
+#pragma omp parallel
+    {
+        cplx *a = (cplx *) malloc(sizeof(cplx) * (size_t) n);
+        cplx *b = (cplx *) malloc(sizeof(cplx) * (size_t) n);
+#pragma omp for schedule(static)
+        for (ptrdiff_t L = 0; L < nlines; L++) {
+            ptrdiff_t off;
+            if (axis == 2) {
+                ptrdiff_t xy = L / h, c = L % h;
+                off          = xy * nzc * h + c;
+            } else if (axis == 1) {
+                ptrdiff_t x = L / ((ptrdiff_t) nzc * h), zc = L % ((ptrdiff_t) nzc * h);
+                off         = x * (ptrdiff_t) ny * nzc * h + zc;
+            } else {
+                off = L;
+            }
+            for (int i = 0; i < n; i++)
+                a[i] = F[off + i * stride];
+            fft_rec(a, 1, b, n, &P, 1, P.f, sign);
+            for (int i = 0; i < n; i++)
+                F[off + i * stride] = b[i];
+        }
+        free(a);
+        free(b);
+    }
+    plan_free(&P);
+}
+
+/* unnormalised 3-D r2c like fftw_mpi_plan_many_dft_r2c (solver.h:221) */
+void or_fft_r2c(int nx, int ny, int nz, int h, const double *in, double *out_c)
+{
+    ptrdiff_t N  = (ptrdiff_t) nx * ny * nz;
+    cplx     *F  = (cplx *) malloc(sizeof(cplx) * (size_t) (N * h));
+    cplx     *o  = (cplx *) out_c;
+    int       nc = nz / 2 + 1;
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t i = 0; i < N * h; i++)
+        F[i] = in[i];
+    fft_axis(F, nx, ny, nz, h, 2, -1);
+    fft_axis(F, nx, ny, nz, h, 1, -1);
+    fft_axis(F, nx, ny, nz, h, 0, -1);
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t xy = 0; xy < (ptrdiff_t) nx * ny; xy++)
+        for (int kz = 0; kz < nc; kz++)
+            for (int c = 0; c < h; c++)
+                o[(xy * nc + kz) * h + c] = F[(xy * nz + kz) * h + c];
+    free(F);
+}
+
+/* unnormalised 3-D c2r like fftw_mpi_plan_many_dft_c2r (solver.h:222) */
+void or_fft_c2r(int nx, int ny, int nz, int h, const double *in_c, double *out)
+{
+    ptrdiff_t   N  = (ptrdiff_t) nx * ny * nz;
+    cplx       *F  = (cplx *) malloc(sizeof(cplx) * (size_t) (N * h));
+    const cplx *X  = (const cplx *) in_c;
+    int         nc = nz / 2 + 1;
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t xy = 0; xy < (ptrdiff_t) nx * ny; xy++) {
+        int x = (int) (xy / ny), y = (int) (xy % ny);
+        int xm = (nx - x) % nx, ym = (ny - y) % ny;
+        for (int kz = 0; kz < nz; kz++)
+            for (int c = 0; c < h; c++) {
+                if (kz < nc)
+                    F[(xy * nz + kz) * h + c] = X[(xy * nc + kz) * h + c];
+                else
+                    F[(xy * nz + kz) * h + c] = conj(X[(((ptrdiff_t) xm * ny + ym) * nc + (nz - kz)) * h + c]);
+            }
+    }
+    fft_axis(F, nx, ny, nz, h, 0, +1);
+    fft_axis(F, nx, ny, nz, h, 1, +1);
+    fft_axis(F, nx, ny, nz, h, 2, +1);
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t i = 0; i < N * h; i++)
+        out[i] = creal(F[i]);
+    free(F);
+}
+
+/* ------------------------------------------------------------------------ */
+/* small dense helpers                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* eigen-decomposition of a symmetric n x n (n <= 6) matrix by cyclic Jacobi
+ * rotations; used for the pseudo-inverse of solver.h:89-96 (JacobiSVD with an
+ * absolute cutoff on singular values = |eigenvalues| for symmetric input) */
+static void sym_eig(int n, const double *A, double *lam, double *V)
+{
+    double a[36];
+    memcpy(a, A, sizeof(double) * n * n);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            V[i * n + j] = (i == j);
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                tot += a[i * n + j] * a[i * n + j];
+                if (i != j)
+                    off += a[i * n + j] * a[i * n + j];
+            }
+        if (off <= 1e-32 * tot || off == 0.0)
+            break;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                double apq = a[p * n + q];
+                if (apq == 0.0)
+                    continue;
+                double app = a[p * n + p], aqq = a[q * n + q];
+                double theta = (aqq - app) / (2 * apq);
+                double t     = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < n; k++) {
+                    double akp = a[k * n + p], akq = a[k * n + q];
+                    a[k * n + p] = c * akp - s * akq;
+                    a[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    double apk = a[p * n + k], aqk = a[q * n + k];
+                    a[p * n + k] = c * apk - s * aqk;
+                    a[q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = c * vkp - s * vkq;
+                    V[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < n; i++)
+        lam[i] = a[i * n + i];
+}
+
+static void sym_pinv(int n, const double *A, double cutoff, double *P)
+{
+    double lam[6], V[36];
+    sym_eig(n, A, lam, V);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            double acc = 0;
+            for (int k = 0; k < n; k++)
+                if (fabs(lam[k]) > cutoff)
+                    acc += V[i * n + k] * V[j * n + k] / lam[k];
+            P[i * n + j] = acc;
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* solver state                                                              */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int     kind;
+    int     nphase;
+    int     np;
+    double *params; /* [nphase][np] */
+} or_group;
+
+struct or_solver {
+    or_desc   cfg;
+    int       nx, ny, nz, h, nstr, ngp;
+    ptrdiff_t N, Nf;
+    double    le[3], ve;
+    double    Bint[8][MAXSTR * MAXE]; /* B_int[p]: nstr x 8h, row major */
+    uint16_t *ms;
+    int       ngroups;
+    or_group *groups;
+    int       nphases;
+    int      *phase_group, *phase_local;
+    int       all_linear;
+    double    kref[MAXSTR * MAXSTR];
+    double    keref[MAXE * MAXE];
+    double   *Ke; /* [nphases][8h][8h], linear phases only */
+    double   *G;  /* [Nf][h(h+1)/2]: (00 01 02 11 12 22) */
+    double   *u, *u_prev, *r, *s, *d, *rnew, *rghost;
+    double    g0[MAXSTR];
+    double   *ep, *ep_t, *q, *q_t; /* per element per Gauss point state */
+    int       has_state;
+    double    alpha_warm;
+    double   *err_all;
+    int       iter;
+    double    tol;
+    int       err_type;
+    /* mixed BC (mixedBCs.h:150-227) */
+    int       mixed_active;
+    int       nE, nF, nsteps, step;
+    int       idxE[9], idxF[9];
+    double   *FE, *PF;
+    double    M[81];
+    double    gvec[9], gvec_prev[9];
+};
+
+int or_model_nparams(int kind)
+{
+    switch (kind) {
+    case OR_MODEL_THERMAL_ISO: return 1;
+    case OR_MODEL_THERMAL_TRI: return 6;
+    case OR_MODEL_ELASTIC_ISO: return 2;
+    case OR_MODEL_ELASTIC_TRI: return 21;
+    case OR_MODEL_J2_NEW: return 4;
+    case OR_MODEL_PSEUDO_PLASTIC: return 4;
+    default: return -1;
+    }
+}
+
+static int model_is_linear(int kind)
+{
+    return kind == OR_MODEL_THERMAL_ISO || kind == OR_MODEL_THERMAL_TRI || kind == OR_MODEL_ELASTIC_ISO ||
+           kind == OR_MODEL_ELASTIC_TRI;
+}
+
+void or_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void) n;
+#endif
+}
+
+/* matmodel.h:157-188 */
+static void basic_B(const or_solver *S, double x, double y, double z, double o[3][8])
+{
+    const double lx = S->le[0], ly = S->le[1], lz = S->le[2];
+    o[0][0] = -(1. - y) * (1. - z) / lx;
+    o[0][1] = (1. - y) * (1. - z) / lx;
+    o[0][2] = -y * (1. - z) / lx;
+    o[0][3] = y * (1. - z) / lx;
+    o[0][4] = -(1. - y) * z / lx;
+    o[0][5] = (1. - y) * z / lx;
+    o[0][6] = -y * z / lx;
+    o[0][7] = y * z / lx;
+
+    o[1][0] = -(1. - x) * (1. - z) / ly;
+    o[1][1] = -x * (1. - z) / ly;
+    o[1][2] = (1. - x) * (1. - z) / ly;
+    o[1][3] = x * (1. - z) / ly;
+    o[1][4] = -(1. - x) * z / ly;
+    o[1][5] = -x * z / ly;
+    o[1][6] = (1. - x) * z / ly;
+    o[1][7] = x * z / ly;
+
+    o[2][0] = -(1. - x) * (1. - y) / lz;
+    o[2][1] = -x * (1. - y) / lz;
+    o[2][2] = -(1. - x) * y / lz;
+    o[2][3] = -x * y / lz;
+    o[2][4] = (1. - x) * (1. - y) / lz;
+    o[2][5] = x * (1. - y) / lz;
+    o[2][6] = (1. - x) * y / lz;
+    o[2][7] = x * y / lz;
+}
+
+/* ThermalModel::Compute_B (matmodel.h:267-270) and SmallStrainMechModel::
+ * Compute_B (matmodel.h:284-304): nstr x 8h row major */
+static void compute_B(const or_solver *S, double x, double y, double z, double *out)
+{
+    double b[3][8];
+    basic_B(S, x, y, z, b);
+    int ncol = 8 * S->h;
+    memset(out, 0, sizeof(double) * S->nstr * ncol);
+    if (S->h == 1) {
+        for (int i = 0; i < 3; i++)
+            for (int q = 0; q < 8; q++)
+                out[i * ncol + q] = b[i][q];
+        return;
+    }
+    const double sh = 7.071067811865476e-01;
+    for (int q = 0; q < 8; q++) {
+        out[0 * ncol + 3 * q + 0] = b[0][q];
+        out[1 * ncol + 3 * q + 1] = b[1][q];
+        out[2 * ncol + 3 * q + 2] = b[2][q];
+        out[3 * ncol + 3 * q + 0] = sh * b[1][q];
+        out[4 * ncol + 3 * q + 0] = sh * b[2][q];
+        out[5 * ncol + 3 * q + 1] = sh * b[2][q];
+        out[3 * ncol + 3 * q + 1] = sh * b[0][q];
+        out[4 * ncol + 3 * q + 2] = sh * b[0][q];
+        out[5 * ncol + 3 * q + 2] = sh * b[1][q];
+    }
+}
+
+/* Matmodel::Construct_B (matmodel.h:104-155) */
+static void construct_B(or_solver *S)
+{
+    int          ncol = 8 * S->h, nstr = S->nstr;
+    const double xp = 0.5 + sqrt(3.) / 6., xm = 0.5 - sqrt(3.) / 6.;
+    const double xi[8][3] = {{xm, xm, xm}, {xp, xm, xm}, {xm, xp, xm}, {xp, xp, xm},
+                             {xm, xm, xp}, {xp, xm, xp}, {xm, xp, xp}, {xp, xp, xp}};
+    if (S->cfg.fe_type == OR_FE_HEX8R) {
+        compute_B(S, 0.5, 0.5, 0.5, S->Bint[0]);
+        return;
+    }
+    if (S->cfg.fe_type == OR_FE_BBAR && nstr > 3) {
+        double Bvol[MAXSTR * MAXE], Bfull[MAXSTR * MAXE];
+        compute_B(S, 0.5, 0.5, 0.5, Bvol);
+        for (int p = 0; p < 8; p++) {
+            compute_B(S, xi[p][0], xi[p][1], xi[p][2], Bfull);
+            for (int col = 0; col < ncol; col++) {
+                double vf = (Bfull[0 * ncol + col] + Bfull[1 * ncol + col] + Bfull[2 * ncol + col]) / 3.0;
+                double vb = (Bvol[0 * ncol + col] + Bvol[1 * ncol + col] + Bvol[2 * ncol + col]) / 3.0;
+                for (int row = 0; row < 3; row++)
+                    S->Bint[p][row * ncol + col] = Bfull[row * ncol + col] - vf + vb;
+                for (int row = 3; row < nstr; row++)
+                    S->Bint[p][row * ncol + col] = Bfull[row * ncol + col];
+            }
+        }
+        return;
+    }
+    for (int p = 0; p < 8; p++)
+        compute_B(S, xi[p][0], xi[p][1], xi[p][2], S->Bint[p]);
+}
+
+or_solver *or_create(const or_desc *d)
+{
+    if (d->howmany != 1 && d->howmany != 3)
+        return NULL;
+    if (d->n_x < 1 || d->n_y < 1 || d->n_z < 2 || (d->n_z % 2) != 0)
+        return NULL;
+    or_solver *S = (or_solver *) calloc(1, sizeof(or_solver));
+    S->cfg       = *d;
+    S->nx        = d->n_x;
+    S->ny        = d->n_y;
+    S->nz        = d->n_z;
+    S->h         = d->howmany;
+    S->nstr      = d->howmany == 1 ? 3 : 6;
+    S->ngp       = d->fe_type == OR_FE_HEX8R ? 1 : 8;
+    S->N         = (ptrdiff_t) S->nx * S->ny * S->nz;
+    S->Nf        = (ptrdiff_t) S->nx * S->ny * (S->nz / 2 + 1);
+    /* reader.cpp:293-296 */
+    S->le[0]     = d->L[0] / (double) S->nx;
+    S->le[1]     = d->L[1] / (double) S->ny;
+    S->le[2]     = d->L[2] / (double) S->nz;
+    S->ve        = S->le[0] * S->le[1] * S->le[2];
+    S->tol       = d->tol;
+    S->err_type  = d->err_type;
+    size_t nh    = (size_t) S->N * S->h;
+    S->u         = (double *) calloc(nh, sizeof(double));
+    S->u_prev    = (double *) calloc(nh, sizeof(double));
+    S->r         = (double *) calloc(nh, sizeof(double));
+    S->s         = (double *) calloc(nh, sizeof(double));
+    S->d         = (double *) calloc(nh, sizeof(double));
+    S->rnew      = (double *) calloc(nh, sizeof(double));
+    S->rghost    = (double *) calloc((size_t) S->ny * S->nz * S->h, sizeof(double));
+    S->ms        = (uint16_t *) calloc((size_t) S->N, sizeof(uint16_t));
+    S->err_all   = (double *) calloc((size_t) d->n_it + 1, sizeof(double));
+    construct_B(S);
+    return S;
+}
+
+void or_destroy(or_solver *S)
+{
+    if (!S)
+        return;
+    free(S->u);
+    free(S->u_prev);
+    free(S->r);
+    free(S->s);
+    free(S->d);
+    free(S->rnew);
+    free(S->rghost);
+    free(S->ms);
+    free(S->err_all);
+    free(S->Ke);
+    free(S->G);
+    free(S->ep);
+    free(S->ep_t);
+    free(S->q);
+    free(S->q_t);
+    free(S->FE);
+    free(S->PF);
+    for (int g = 0; g < S->ngroups; g++)
+        free(S->groups[g].params);
+    free(S->groups);
+    free(S->phase_group);
+    free(S->phase_local);
+    free(S);
+}
+
+int or_set_microstructure(or_solver *S, const uint16_t *ms)
+{
+    memcpy(S->ms, ms, sizeof(uint16_t) * (size_t) S->N);
+    return 0;
+}
+
+int or_n_gp(or_solver *S) { return S->ngp; }
+
+static const double *phase_params(const or_solver *S, int phase, int *kind)
+{
+    const or_group *g = &S->groups[S->phase_group[phase]];
+    *kind             = g->kind;
+    return g->params + (size_t) S->phase_local[phase] * g->np;
+}
+
+/* C matrix of a linear phase (LinearThermal.h:19-31, :88-99; LinearElastic.h:150-160, :234-249) */
+static void linear_C(int n, int kind, const double *p, double *C)
+{
+    memset(C, 0, sizeof(double) * n * n);
+    if (kind == OR_MODEL_THERMAL_ISO) {
+        for (int i = 0; i < 3; i++)
+            C[i * 3 + i] = p[0];
+    } else if (kind == OR_MODEL_THERMAL_TRI) {
+        double K[9] = {p[0], p[1], p[2], p[1], p[3], p[4], p[2], p[4], p[5]};
+        memcpy(C, K, sizeof(K));
+    } else if (kind == OR_MODEL_ELASTIC_ISO || kind == OR_MODEL_J2_NEW || kind == OR_MODEL_PSEUDO_PLASTIC) {
+        double lam = p[0] - (2.0 / 3.0) * p[1], mu = p[1];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                C[i * 6 + j] = lam;
+        for (int i = 0; i < 6; i++)
+            C[i * 6 + i] += 2 * mu;
+    } else if (kind == OR_MODEL_ELASTIC_TRI) {
+        int k = 0;
+        for (int row = 0; row < 6; row++)
+            for (int col = row; col < 6; col++) {
+                C[row * 6 + col] = p[k];
+                C[col * 6 + row] = p[k];
+                k++;
+            }
+    }
+}
+
+/* sum_p B_p^T C B_p v_e / n_gp (LinearElastic.h:157-159, matmodel.h:243-245) */
+static void element_stiffness(const or_solver *S, const double *C, double *Ke)
+{
+    int ne = 8 * S->h, ns = S->nstr;
+    memset(Ke, 0, sizeof(double) * ne * ne);
+    for (int p = 0; p < S->ngp; p++) {
+        const double *B = S->Bint[p];
+        double        CB[MAXSTR * MAXE];
+        for (int i = 0; i < ns; i++)
+            for (int j = 0; j < ne; j++) {
+                double a = 0;
+                for (int k = 0; k < ns; k++)
+                    a += C[i * ns + k] * B[k * ne + j];
+                CB[i * ne + j] = a;
+            }
+        for (int i = 0; i < ne; i++)
+            for (int j = 0; j < ne; j++) {
+                double a = 0;
+                for (int k = 0; k < ns; k++)
+                    a += B[k * ne + i] * CB[k * ne + j];
+                Ke[i * ne + j] += a * S->ve / S->ngp;
+            }
+    }
+}
+
+/* model->get_reference_stiffness() averaged over groups (MaterialManager.h:198-204) */
+static void group_reference_stiffness(const or_solver *S, const or_group *g, double *C)
+{
+    int n = S->nstr;
+    memset(C, 0, sizeof(double) * n * n);
+    if (g->kind == OR_MODEL_THERMAL_ISO || g->kind == OR_MODEL_THERMAL_TRI || g->kind == OR_MODEL_ELASTIC_TRI) {
+        /* LinearThermal.h:42-45 (kappa_average), LinearElastic.h:266-274 */
+        double Ci[36];
+        for (int i = 0; i < g->nphase; i++) {
+            linear_C(S->nstr, g->kind, g->params + (size_t) i * g->np, Ci);
+            for (int k = 0; k < n * n; k++)
+                C[k] += Ci[k];
+        }
+        for (int k = 0; k < n * n; k++)
+            C[k] /= g->nphase;
+    } else if (g->kind == OR_MODEL_ELASTIC_ISO) {
+        /* LinearElastic.h:175-189: (max+min)/2 of lambda and mu */
+        double lmax = -INFINITY, lmin = INFINITY, mmax = -INFINITY, mmin = INFINITY;
+        for (int i = 0; i < g->nphase; i++) {
+            const double *p   = g->params + (size_t) i * g->np;
+            double        lam = p[0] - (2.0 / 3.0) * p[1];
+            lmax              = fmax(lmax, lam);
+            lmin              = fmin(lmin, lam);
+            mmax              = fmax(mmax, p[1]);
+            mmin              = fmin(mmin, p[1]);
+        }
+        double lr = (lmax + lmin) / 2, mr = (mmax + mmin) / 2;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                C[i * 6 + j] = lr;
+        for (int i = 0; i < 6; i++)
+            C[i * 6 + i] += 2 * mr;
+    } else {
+        /* J2PlasticityNew.h:111-121, PseudoPlastic.h:43-54: mean K, mean G */
+        double Kb = 0, Gb = 0;
+        for (int i = 0; i < g->nphase; i++) {
+            Kb += g->params[(size_t) i * g->np + 0];
+            Gb += g->params[(size_t) i * g->np + 1];
+        }
+        Kb /= g->nphase;
+        Gb /= g->nphase;
+        double lb = Kb - 2.0 * Gb / 3.0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                C[i * 6 + j] = lb;
+        for (int i = 0; i < 6; i++)
+            C[i * 6 + i] += 2.0 * Gb;
+    }
+}
+
+/* Solver::computeFundamentalSolution (solver.h:144-204) */
+static void build_green(or_solver *S)
+{
+    int          h = S->h, nx = S->nx, ny = S->ny, nz = S->nz, nc = nz / 2 + 1, np = h * (h + 1) / 2;
+    const double tpi = 2 * acos(-1.0);
+    const double *K0 = S->keref;
+    int           ne = 8 * h;
+    free(S->G);
+    S->G = (double *) calloc((size_t) S->Nf * np, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S->Nf; f++) {
+        int ix = (int) (f / ((ptrdiff_t) ny * nc));
+        int iy = (int) ((f / nc) % ny);
+        int iz = (int) (f % nc);
+        if (ix == 0 && iy == 0 && iz == 0)
+            continue;
+        double thx = tpi * ix / (double) nx, thy = tpi * iy / (double) ny, thz = tpi * iz / (double) nz;
+        cplx   ex = cos(thx) + I * sin(thx), ey = cos(thy) + I * sin(thy), ez = cos(thz) + I * sin(thz);
+        cplx   A[8];
+        A[0] = 1.0;
+        A[1] = ex;
+        A[2] = ey;
+        A[3] = ex * ey;
+        A[4] = ez;
+        A[5] = ex * ez;
+        A[6] = ez * ey;
+        A[7] = ex * ey * ez;
+        double AA[8][8];
+        for (int a = 0; a < 8; a++)
+            for (int b = 0; b < 8; b++)
+                AA[a][b] = creal(A[a]) * creal(A[b]) + cimag(A[a]) * cimag(A[b]);
+        double blk[9], P[9];
+        for (int i = 0; i < h; i++)
+            for (int j = i; j < h; j++) {
+                double acc = 0;
+                for (int a = 0; a < 8; a++)
+                    for (int b = 0; b < 8; b++)
+                        acc += K0[(8 * i + a) * ne + 8 * j + b] * AA[a][b];
+                blk[i * h + j] = acc;
+                blk[j * h + i] = acc;
+            }
+        sym_pinv(h, blk, 1e-14, P);
+        double *g = S->G + f * np;
+        int     k = 0;
+        for (int i = 0; i < h; i++)
+            for (int j = i; j < h; j++)
+                g[k++] = P[i * h + j] / (double) ((ptrdiff_t) nx * ny * nz);
+    }
+}
+
+int or_set_materials(or_solver *S, int n_groups, const int32_t *kind, const int32_t *nphase, const int32_t *phase_ids,
+                     const double *params, const double *ref_material)
+{
+    int maxp = -1, tot = 0;
+    for (int g = 0; g < n_groups; g++) {
+        if (or_model_nparams(kind[g]) < 0)
+            return -1;
+        int thermal = kind[g] == OR_MODEL_THERMAL_ISO || kind[g] == OR_MODEL_THERMAL_TRI;
+        if (thermal != (S->h == 1))
+            return -2;
+        for (int i = 0; i < nphase[g]; i++)
+            maxp = phase_ids[tot + i] > maxp ? phase_ids[tot + i] : maxp;
+        tot += nphase[g];
+    }
+    if (maxp < 0)
+        return -3;
+    S->ngroups     = n_groups;
+    S->groups      = (or_group *) calloc((size_t) n_groups, sizeof(or_group));
+    S->nphases     = maxp + 1;
+    S->phase_group = (int *) malloc(sizeof(int) * (size_t) S->nphases);
+    S->phase_local = (int *) malloc(sizeof(int) * (size_t) S->nphases);
+    for (int p = 0; p < S->nphases; p++)
+        S->phase_group[p] = -1;
+    S->all_linear = 1;
+    int ofs = 0, pofs = 0;
+    for (int g = 0; g < n_groups; g++) {
+        or_group *G = &S->groups[g];
+        G->kind     = kind[g];
+        G->nphase   = nphase[g];
+        G->np       = or_model_nparams(kind[g]);
+        G->params   = (double *) malloc(sizeof(double) * (size_t) G->nphase * G->np);
+        memcpy(G->params, params + pofs, sizeof(double) * (size_t) G->nphase * G->np);
+        pofs += G->nphase * G->np;
+        if (!model_is_linear(G->kind))
+            S->all_linear = 0;
+        for (int i = 0; i < nphase[g]; i++) {
+            int p = phase_ids[ofs + i];
+            if (p < 0 || S->phase_group[p] >= 0)
+                return -4; /* MaterialManager.h:99-100 */
+            S->phase_group[p] = g;
+            S->phase_local[p] = i;
+        }
+        ofs += nphase[g];
+    }
+    for (int p = 0; p < S->nphases; p++)
+        if (S->phase_group[p] < 0)
+            return -5; /* MaterialManager.h:106-108 */
+    /* reference stiffness (MaterialManager.h:177-205) */
+    int n = S->nstr;
+    if (ref_material) {
+        memcpy(S->kref, ref_material, sizeof(double) * n * n);
+    } else {
+        memset(S->kref, 0, sizeof(S->kref));
+        double C[36];
+        for (int g = 0; g < n_groups; g++) {
+            group_reference_stiffness(S, &S->groups[g], C);
+            for (int k = 0; k < n * n; k++)
+                S->kref[k] += C[k];
+        }
+        for (int k = 0; k < n * n; k++)
+            S->kref[k] /= (double) n_groups;
+    }
+    /* per-phase element stiffness for linear phases (LinearModel::phase_stiffness) */
+    int ne = 8 * S->h;
+    S->Ke  = (double *) calloc((size_t) S->nphases * ne * ne, sizeof(double));
+    for (int p = 0; p < S->nphases; p++) {
+        int           k;
+        const double *pp = phase_params(S, p, &k);
+        if (model_is_linear(k)) {
+            double C[36];
+            linear_C(S->nstr, k, pp, C);
+            element_stiffness(S, C, S->Ke + (size_t) p * ne * ne);
+        }
+    }
+    /* Compute_Reference_ElementStiffness (matmodel.h:237-253): component-major */
+    double tmp[MAXE * MAXE];
+    element_stiffness(S, S->kref, tmp);
+    for (int i = 0; i < ne; i++)
+        for (int j = 0; j < ne; j++)
+            S->keref[((i % S->h) * 8 + i / S->h) * ne + (j % S->h) * 8 + j / S->h] = tmp[i * ne + j];
+    /* internal variables (Solver ctor solver.h:139) */
+    S->has_state = 0;
+    for (int g = 0; g < n_groups; g++)
+        if (S->groups[g].kind == OR_MODEL_J2_NEW)
+            S->has_state = 1;
+    if (S->has_state) {
+        size_t ng = (size_t) S->N * S->ngp;
+        S->ep     = (double *) calloc(ng * 6, sizeof(double));
+        S->ep_t   = (double *) calloc(ng * 6, sizeof(double));
+        S->q      = (double *) calloc(ng, sizeof(double));
+        S->q_t    = (double *) calloc(ng, sizeof(double));
+    }
+    build_green(S);
+    return 0;
+}
+
+void or_get_green(or_solver *S, double *packed)
+{
+    memcpy(packed, S->G, sizeof(double) * (size_t) S->Nf * S->h * (S->h + 1) / 2);
+}
+void or_get_reference_stiffness(or_solver *S, double *k) { memcpy(k, S->kref, sizeof(double) * S->nstr * S->nstr); }
+void or_get_ref_element_stiffness(or_solver *S, double *ke)
+{
+    memcpy(ke, S->keref, sizeof(double) * 64 * S->h * S->h);
+}
+void or_get_B(or_solver *S, double *B)
+{
+    int sz = S->nstr * 8 * S->h;
+    for (int p = 0; p < S->ngp; p++)
+        memcpy(B + (size_t) p * sz, S->Bint[p], sizeof(double) * sz);
+}
+int or_get_phase_stiffness(or_solver *S, int phase, double *ke)
+{
+    int ne = 8 * S->h;
+    if (phase < 0 || phase >= S->nphases)
+        return -1;
+    memcpy(ke, S->Ke + (size_t) phase * ne * ne, sizeof(double) * ne * ne);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* constitutive updates (get_sigma)                                          */
+/* ------------------------------------------------------------------------ */
+
+/* J2PlasticityNew::get_sigma (J2PlasticityNew.h:43-109) */
+static void j2new_point(const double *p, const double *e, const double *ep_in, double q_in, double *s, double *ep,
+                        double *qo)
+{
+    const double c23 = sqrt(2.0 / 3.0);
+    double       G   = p[1];
+    for (int i = 0; i < 6; i++)
+        s[i] = 2.0 * G * (e[i] - ep_in[i]);
+    double lam = p[0] - 2.0 / 3.0 * G;
+    double tr  = e[0] + e[1] + e[2];
+    for (int i = 0; i < 3; i++)
+        s[i] += lam * tr;
+    double st[6];
+    double mean = (s[0] + s[1] + s[2]) / 3.0;
+    for (int i = 0; i < 6; i++)
+        st[i] = s[i] - (i < 3 ? mean : 0.0);
+    double s_t = 0;
+    for (int i = 0; i < 6; i++)
+        s_t += st[i] * st[i];
+    s_t           = sqrt(s_t);
+    double sy     = p[2] + q_in * p[3];
+    double dsy    = p[3];
+    double phi    = s_t - c23 * sy;
+    double dgamma = fmax(0.0, phi / (2.0 * G + 2.0 / 3.0 * dsy));
+    for (int i = 0; i < 6; i++)
+        ep[i] = s_t > 1e-12 ? ep_in[i] + dgamma * st[i] / s_t : ep_in[i];
+    *qo = q_in + c23 * dgamma;
+    if (s_t > 1e-12)
+        for (int i = 0; i < 6; i++)
+            s[i] = s[i] - dgamma * 2.0 * G * st[i] / s_t;
+}
+
+/* PseudoPlasticLinearHardening::get_sigma (PseudoPlastic.h:92-116) */
+static void pseudo_point(const double *p, const double *e, double *s)
+{
+    const double a = 2. / 3, b = sqrt(a);
+    double       K = p[0], G = p[1], sy = p[2], hp = p[3];
+    double       eps_crit = sqrt(2. / 3.) * sy / (2. * G);
+    double       E_s      = (3. * G) / (3. * G + hp);
+    double       tr       = e[0] + e[1] + e[2];
+    double       dev[6];
+    for (int i = 0; i < 3; i++)
+        dev[i] = e[i] - (1.0 / 3.0) * tr;
+    for (int i = 3; i < 6; i++)
+        dev[i] = e[i];
+    double nd = 0;
+    for (int i = 0; i < 6; i++)
+        nd += dev[i] * dev[i];
+    nd          = sqrt(nd);
+    double buf1 = K * tr, buf2;
+    if (nd <= eps_crit)
+        buf2 = 2.0 * G;
+    else
+        buf2 = (b * sy + a * E_s * hp * (nd - eps_crit)) / nd;
+    for (int i = 0; i < 3; i++)
+        s[i] = buf1 + buf2 * dev[i];
+    for (int i = 3; i < 6; i++)
+        s[i] = buf2 * dev[i];
+}
+
+void or_material_point(int kind, const double *p, const double *e, const double *ep_t, double q_t, double *sigma,
+                       double *ep, double *q)
+{
+    if (kind == OR_MODEL_J2_NEW) {
+        j2new_point(p, e, ep_t, q_t, sigma, ep, q);
+    } else if (kind == OR_MODEL_PSEUDO_PLASTIC) {
+        pseudo_point(p, e, sigma);
+    } else if (kind == OR_MODEL_ELASTIC_ISO) {
+        /* LinearElastic.h:163-173 */
+        double lam = p[0] - (2.0 / 3.0) * p[1];
+        double b1 = lam * (e[0] + e[1] + e[2]), b2 = 2 * p[1];
+        for (int i = 0; i < 3; i++)
+            sigma[i] = b1 + b2 * e[i];
+        for (int i = 3; i < 6; i++)
+            sigma[i] = b2 * e[i];
+    } else if (kind == OR_MODEL_THERMAL_ISO) {
+        for (int i = 0; i < 3; i++)
+            sigma[i] = p[0] * e[i];
+    } else {
+        int    n = kind == OR_MODEL_THERMAL_TRI ? 3 : 6;
+        double C[36];
+        linear_C(n, kind, p, C);
+        for (int i = 0; i < n; i++) {
+            double a = 0;
+            for (int j = 0; j < n; j++)
+                a += C[i * n + j] * e[j];
+            sigma[i] = a;
+        }
+    }
+}
+
+/* eps = B ue + g0; sigma = get_sigma per GP (matmodel.h:190-201, 202-209) */
+static void element_strain_stress(or_solver *S, const double *ue, int phase, ptrdiff_t el, double *eps, double *sig)
+{
+    int ne = 8 * S->h, ns = S->nstr;
+    for (int p = 0; p < S->ngp; p++)
+        for (int i = 0; i < ns; i++) {
+            double a = 0;
+            for (int j = 0; j < ne; j++)
+                a += S->Bint[p][i * ne + j] * ue[j];
+            eps[p * ns + i] = a + S->g0[i];
+        }
+    int           kind;
+    const double *pp = phase_params(S, phase, &kind);
+    for (int p = 0; p < S->ngp; p++) {
+        double *e = eps + p * ns, *s = sig + p * ns;
+        if (kind == OR_MODEL_J2_NEW) {
+            size_t gi = (size_t) el * S->ngp + p;
+            j2new_point(pp, e, S->ep_t + gi * 6, S->q_t[gi], s, S->ep + gi * 6, S->q + gi);
+        } else {
+            or_material_point(kind, pp, e, NULL, 0.0, s, NULL, NULL);
+        }
+    }
+}
+
+static void element_residual(or_solver *S, const double *ue, int phase, ptrdiff_t el, double *res)
+{
+    double eps[8 * MAXSTR], sig[8 * MAXSTR];
+    int    ne = 8 * S->h, ns = S->nstr;
+    element_strain_stress(S, ue, phase, el, eps, sig);
+    for (int j = 0; j < ne; j++) {
+        double a = 0;
+        for (int p = 0; p < S->ngp; p++)
+            for (int i = 0; i < ns; i++)
+                a += S->Bint[p][i * ne + j] * sig[p * ns + i];
+        res[j] = a * S->ve / S->ngp;
+    }
+}
+
+/* node offsets of element (ix,iy,iz) in the order of solver.h:318-382; x+1 of the
+ * last plane lands in the ghost plane (index N..N+ny*nz-1) */
+static inline void element_nodes(const or_solver *S, int ix, int iy, int iz, ptrdiff_t *idx)
+{
+    int       y1 = iy + 1 == S->ny ? 0 : iy + 1;
+    int       z1 = iz + 1 == S->nz ? 0 : iz + 1;
+    ptrdiff_t nz = S->nz, nyz = (ptrdiff_t) S->ny * S->nz;
+    ptrdiff_t x0 = ix * nyz, x1 = (ptrdiff_t) (ix + 1) * nyz;
+    idx[0]       = x0 + iy * nz + iz;
+    idx[1]       = x1 + iy * nz + iz;
+    idx[2]       = x0 + y1 * nz + iz;
+    idx[3]       = x1 + y1 * nz + iz;
+    idx[4]       = x0 + iy * nz + z1;
+    idx[5]       = x1 + iy * nz + z1;
+    idx[6]       = x0 + y1 * nz + z1;
+    idx[7]       = x1 + y1 * nz + z1;
+}
+
+/* Solver::compute_residual_basic (solver.h:228-270) with f = element_residual
+ * (compute_residual, :272-280) or f = Ke[phase] ue (solverCG.h:97-103).  x-planes
+ * are processed in two colours so a threaded run stays race free and gives the
+ * same bits for any thread count. */
+void or_residual(or_solver *S, const double *u, double *r, int linear_kd)
+{
+    int       h = S->h, ne = 8 * h;
+    ptrdiff_t N = S->N, nyz = (ptrdiff_t) S->ny * S->nz;
+    memset(r, 0, sizeof(double) * (size_t) N * h);
+    memset(S->rghost, 0, sizeof(double) * (size_t) nyz * h);
+    for (int color = 0; color < 2; color++) {
+#pragma omp parallel for schedule(static)
+        for (int ix = color; ix < S->nx; ix += 2) {
+            ptrdiff_t idx[8];
+            double    ue[MAXE], res[MAXE];
+            for (int iy = 0; iy < S->ny; iy++)
+                for (int iz = 0; iz < S->nz; iz++) {
+                    element_nodes(S, ix, iy, iz, idx);
+                    for (int a = 0; a < 8; a++) {
+                        ptrdiff_t n = idx[a] >= N ? idx[a] - N : idx[a]; /* ghost plane of u = plane 0 */
+                        for (int c = 0; c < h; c++)
+                            ue[h * a + c] = u[h * n + c] - u[h * idx[0] + c];
+                    }
+                    int phase = S->ms[idx[0]];
+                    if (linear_kd) {
+                        const double *K = S->Ke + (size_t) phase * ne * ne;
+                        for (int i = 0; i < ne; i++) {
+                            double a = 0;
+                            for (int j = 0; j < ne; j++)
+                                a += K[i * ne + j] * ue[j];
+                            res[i] = a;
+                        }
+                    } else {
+                        element_residual(S, ue, phase, idx[0], res);
+                    }
+                    for (int a = 0; a < 8; a++) {
+                        double *dst = idx[a] >= N ? S->rghost + (idx[a] - N) * h : r + idx[a] * h;
+                        for (int c = 0; c < h; c++)
+                            dst[c] += res[h * a + c];
+                    }
+                }
+        }
+    }
+    for (ptrdiff_t i = 0; i < nyz * h; i++)
+        r[i] += S->rghost[i];
+}
+
+/* Solver::convolution (solver.h:387-412): r2c, per-frequency G r_hat, c2r */
+void or_convolve(or_solver *S, const double *r, double *out)
+{
+    int   h = S->h, np = h * (h + 1) / 2;
+    cplx *X = (cplx *) malloc(sizeof(cplx) * (size_t) S->Nf * h);
+    or_fft_r2c(S->nx, S->ny, S->nz, h, r, (double *) X);
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S->Nf; f++) {
+        const double *g = S->G + f * np;
+        cplx         *x = X + f * h;
+        if (h == 1) {
+            x[0] = g[0] * x[0];
+        } else {
+            cplx a = x[0], b = x[1], c = x[2];
+            x[0]   = g[0] * a + g[1] * b + g[2] * c;
+            x[1]   = g[1] * a + g[3] * b + g[4] * c;
+            x[2]   = g[2] * a + g[4] * b + g[5] * c;
+        }
+    }
+    or_fft_c2r(S->nx, S->ny, S->nz, h, (double *) X, out);
+    free(X);
+}
+
+/* deterministic dot: per x-plane sequential partials, summed in plane order */
+static double dotp(const or_solver *S, const double *a, const double *b)
+{
+    ptrdiff_t pl = (ptrdiff_t) S->ny * S->nz * S->h;
+    double   *part = (double *) malloc(sizeof(double) * (size_t) S->nx);
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < S->nx; x++) {
+        double acc = 0;
+        for (ptrdiff_t i = x * pl; i < (x + 1) * pl; i++)
+            acc += a[i] * b[i];
+        part[x] = acc;
+    }
+    double t = 0;
+    for (int x = 0; x < S->nx; x++)
+        t += part[x];
+    free(part);
+    return t;
+}
+
+/* Solver::compute_error (solver.h:414-452) */
+static double compute_error(or_solver *S, const double *r)
+{
+    ptrdiff_t n = S->N * S->h;
+    double    err;
+    if (S->cfg.measure == OR_MEASURE_L1) {
+        err = 0;
+        for (ptrdiff_t i = 0; i < n; i++)
+            err += fabs(r[i]);
+    } else if (S->cfg.measure == OR_MEASURE_L2) {
+        err = sqrt(dotp(S, r, r));
+    } else {
+        err = 0;
+        for (ptrdiff_t i = 0; i < n; i++)
+            err = fmax(err, fabs(r[i]));
+    }
+    S->err_all[S->iter] = err;
+    double err0         = S->err_all[0];
+    double err_rel      = S->iter == 0 ? 100 : err / err0;
+    return S->err_type == OR_ERR_ABSOLUTE ? err : err_rel;
+}
+
+void or_set_gradient(or_solver *S, const double *g0)
+{
+    memcpy(S->g0, g0, sizeof(double) * S->nstr);
+}
+
+void or_get_macro_gradient(or_solver *S, double *g0) { memcpy(g0, S->g0, sizeof(double) * S->nstr); }
+
+/* Solver::get_homogenized_stress (solver.h:707-737) */
+void or_homogenized_stress(or_solver *S, double *out)
+{
+    int       h = S->h, ns = S->nstr;
+    ptrdiff_t N = S->N;
+    double    acc[MAXSTR] = {0};
+    for (int ix = 0; ix < S->nx; ix++)
+        for (int iy = 0; iy < S->ny; iy++)
+            for (int iz = 0; iz < S->nz; iz++) {
+                ptrdiff_t idx[8];
+                double    ue[MAXE], eps[8 * MAXSTR], sig[8 * MAXSTR];
+                element_nodes(S, ix, iy, iz, idx);
+                for (int a = 0; a < 8; a++) {
+                    ptrdiff_t n = idx[a] >= N ? idx[a] - N : idx[a];
+                    for (int c = 0; c < h; c++)
+                        ue[h * a + c] = S->u[h * n + c];
+                }
+                element_strain_stress(S, ue, S->ms[idx[0]], idx[0], eps, sig);
+                for (int j = 0; j < ns; j++) {
+                    double av = 0;
+                    for (int p = 0; p < S->ngp; p++)
+                        av += sig[p * ns + j] / S->ngp;
+                    acc[j] += av;
+                }
+            }
+    for (int j = 0; j < ns; j++)
+        out[j] = acc[j] / (double) N;
+}
+
+/* MixedBCController::update (mixedBCs.h:161-178) */
+static void mixed_update(or_solver *S)
+{
+    if (!S->mixed_active)
+        return;
+    double Pbar[MAXSTR];
+    or_homogenized_stress(S, Pbar);
+    if (S->nF) {
+        double rhs[9], dE[9];
+        for (int i = 0; i < S->nF; i++)
+            rhs[i] = S->PF[(size_t) S->step * S->nF + i] - Pbar[S->idxF[i]];
+        for (int i = 0; i < S->nF; i++) {
+            double a = 0;
+            for (int j = 0; j < S->nF; j++)
+                a += S->M[i * S->nF + j] * rhs[j];
+            dE[i] = a;
+        }
+        for (int i = 0; i < S->nF; i++)
+            S->gvec[S->idxF[i]] += dE[i];
+    }
+    or_set_gradient(S, S->gvec);
+}
+
+/* MixedBCController::activate (mixedBCs.h:181-226) + MixedBC::finalize (:30-46) */
+int or_enable_mixed_bc(or_solver *S, int nE, const int32_t *idxE, int nF, const int32_t *idxF, int nsteps,
+                       const double *FE, const double *PF, int step)
+{
+    int n = S->nstr;
+    if (nE + nF != n || step < 0 || step >= nsteps)
+        return -1;
+    S->mixed_active = 1;
+    S->nE           = nE;
+    S->nF           = nF;
+    S->nsteps       = nsteps;
+    S->step         = step;
+    for (int i = 0; i < nE; i++)
+        S->idxE[i] = idxE[i];
+    for (int i = 0; i < nF; i++)
+        S->idxF[i] = idxF[i];
+    free(S->FE);
+    free(S->PF);
+    S->FE = (double *) malloc(sizeof(double) * (size_t) (nsteps * nE + 1));
+    S->PF = (double *) malloc(sizeof(double) * (size_t) (nsteps * nF + 1));
+    memcpy(S->FE, FE, sizeof(double) * (size_t) nsteps * nE);
+    memcpy(S->PF, PF, sizeof(double) * (size_t) nsteps * nF);
+    if (nF) {
+        /* M = pinv(Q_F^T C0 Q_F): a principal submatrix of the SPD reference stiffness */
+        double A[81];
+        for (int i = 0; i < nF; i++)
+            for (int j = 0; j < nF; j++)
+                A[i * nF + j] = S->kref[S->idxF[i] * n + S->idxF[j]];
+        sym_pinv(nF, A, 1e-300, S->M);
+    }
+    if (step == 0) {
+        memset(S->gvec, 0, sizeof(S->gvec));
+        memcpy(S->gvec_prev, S->gvec, sizeof(S->gvec));
+    } else {
+        double delta[9];
+        for (int i = 0; i < n; i++)
+            delta[i] = S->gvec[i] - S->gvec_prev[i];
+        memcpy(S->gvec_prev, S->gvec, sizeof(S->gvec));
+        for (int i = 0; i < nF; i++)
+            S->gvec[S->idxF[i]] += delta[S->idxF[i]];
+    }
+    for (int i = 0; i < nE; i++)
+        S->gvec[S->idxE[i]] = S->FE[(size_t) step * nE + i];
+    or_set_gradient(S, S->gvec);
+    mixed_update(S);
+    return 0;
+}
+
+void or_disable_mixed_bc(or_solver *S) { S->mixed_active = 0; }
+int  or_mixed_active(or_solver *S) { return S->mixed_active; }
+
+/* SolverCG::LineSearchSecant (solverCG.h:119-160) */
+static void line_search(or_solver *S)
+{
+    ptrdiff_t    n     = S->N * S->h;
+    double       err   = 10.0;
+    const int    MaxIt = S->cfg.ls_max_iter;
+    const double tol   = S->cfg.ls_tol;
+    int          it    = 0;
+    double       ap = 0.0, ac = S->alpha_warm;
+    double       rpd = dotp(S, S->r, S->d);
+    for (ptrdiff_t i = 0; i < n; i++)
+        S->u[i] += S->d[i] * ac;
+    mixed_update(S);
+    or_residual(S, S->u, S->rnew, 0);
+    double r1pd = dotp(S, S->rnew, S->d);
+    while (it < MaxIt && err > tol) {
+        double denom = r1pd - rpd;
+        if (fabs(denom) < 1e-14 * (fabs(r1pd) + fabs(rpd)))
+            break;
+        double an = ac - r1pd * (ac - ap) / denom;
+        if (an <= 0.0)
+            an = 0.5 * (ap + ac);
+        err = fabs(an - ac);
+        for (ptrdiff_t i = 0; i < n; i++)
+            S->u[i] += S->d[i] * (an - ac);
+        ap  = ac;
+        rpd = r1pd;
+        ac  = an;
+        it++;
+        mixed_update(S);
+        or_residual(S, S->u, S->rnew, 0);
+        r1pd = dotp(S, S->rnew, S->d);
+    }
+    S->alpha_warm = (it == MaxIt && err > tol) ? 0.1 : ac;
+    memcpy(S->r, S->rnew, sizeof(double) * (size_t) n);
+}
+
+/* SolverCG::internalSolve (solverCG.h:61-117) */
+static void solve_cg(or_solver *S)
+{
+    ptrdiff_t n = S->N * S->h;
+    S->alpha_warm = 0.1;
+    memset(S->s, 0, sizeof(double) * (size_t) n);
+    memset(S->d, 0, sizeof(double) * (size_t) n);
+    or_residual(S, S->u, S->r, 0);
+    S->iter        = 0;
+    double err_rel = compute_error(S, S->r);
+    double delta = 1.0, delta0, deltamid;
+    while (S->iter < S->cfg.n_it && err_rel > S->tol) {
+        deltamid = dotp(S, S->r, S->s);
+        or_convolve(S, S->r, S->s);
+        for (ptrdiff_t i = 0; i < n; i++)
+            S->s[i] = -S->s[i];
+        delta0      = delta;
+        delta       = dotp(S, S->r, S->s);
+        double beta = fmax(0, (delta - deltamid) / delta0);
+        for (ptrdiff_t i = 0; i < n; i++)
+            S->d[i] = S->s[i] + beta * S->d[i];
+        if (S->all_linear && !S->mixed_active) {
+            or_residual(S, S->d, S->rnew, 1);
+            double alpha = delta / dotp(S, S->d, S->rnew);
+            for (ptrdiff_t i = 0; i < n; i++)
+                S->r[i] -= alpha * S->rnew[i];
+            for (ptrdiff_t i = 0; i < n; i++)
+                S->u[i] -= alpha * S->d[i];
+        } else {
+            line_search(S);
+        }
+        S->iter++;
+        err_rel = compute_error(S, S->r);
+    }
+}
+
+/* SolverFP::internalSolve (solverFP.h:33-55) */
+static void solve_fp(or_solver *S)
+{
+    ptrdiff_t n = S->N * S->h;
+    or_residual(S, S->u, S->r, 0);
+    S->iter        = 0;
+    double err_rel = compute_error(S, S->r);
+    while (S->iter < S->cfg.n_it && err_rel > S->tol) {
+        or_convolve(S, S->r, S->r);
+        for (ptrdiff_t i = 0; i < n; i++)
+            S->u[i] -= S->r[i];
+        mixed_update(S);
+        or_residual(S, S->u, S->r, 0);
+        S->iter++;
+        err_rel = compute_error(S, S->r);
+    }
+}
+
+/* Solver::solve (solver.h:282-300) */
+int or_solve(or_solver *S)
+{
+    memset(S->err_all, 0, sizeof(double) * (size_t) (S->cfg.n_it + 1));
+    if (S->cfg.method == OR_METHOD_FP)
+        solve_fp(S);
+    else
+        solve_cg(S);
+    if (S->has_state) { /* J2PlasticityNew::updateInternalVariables (:37-41) */
+        size_t ng = (size_t) S->N * S->ngp;
+        memcpy(S->ep_t, S->ep, sizeof(double) * ng * 6);
+        memcpy(S->q_t, S->q, sizeof(double) * ng);
+    }
+    return S->iter;
+}
+
+int or_error_history(or_solver *S, double *out, int cap)
+{
+    int n = S->iter + 1 < cap ? S->iter + 1 : cap;
+    memcpy(out, S->err_all, sizeof(double) * (size_t) n);
+    return S->iter + 1;
+}
+
+/* Solver::get_homogenized_tangent (solver.h:739-778); like the reference it
+ * permanently switches the error type to relative with TOL >= 1e-6 */
+int or_homogenized_tangent(or_solver *S, double pert, double *out)
+{
+    int    n = S->nstr;
+    double unpert[MAXSTR], pstress[MAXSTR], g0[MAXSTR], pe[MAXSTR];
+    or_homogenized_stress(S, unpert);
+    memcpy(g0, S->g0, sizeof(g0));
+    S->err_type = OR_ERR_RELATIVE;
+    S->tol      = fmax(1e-6, S->tol);
+    double C[36];
+    for (int i = 0; i < n; i++) {
+        if (S->all_linear) {
+            memset(pe, 0, sizeof(pe));
+            pe[i] = 1.0;
+        } else {
+            memcpy(pe, g0, sizeof(pe));
+            pe[i] += pert;
+        }
+        or_set_gradient(S, pe);
+        S->mixed_active = 0;
+        or_solve(S);
+        or_homogenized_stress(S, pstress);
+        for (int j = 0; j < n; j++)
+            C[j * n + i] = S->all_linear ? pstress[j] : (pstress[j] - unpert[j]) / pert;
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            out[i * n + j] = 0.5 * (C[i * n + j] + C[j * n + i]);
+    return 0;
+}
+
+/* Solver::extrapolateDisplacement (solver.h:302-311) */
+void or_extrapolate_displacement(or_solver *S)
+{
+    ptrdiff_t n = S->N * S->h;
+    for (ptrdiff_t i = 0; i < n; i++) {
+        double delta = S->u[i] - S->u_prev[i];
+        S->u_prev[i] = S->u[i];
+        S->u[i] += delta;
+    }
+}
+
+void or_get_u(or_solver *S, double *out) { memcpy(out, S->u, sizeof(double) * (size_t) S->N * S->h); }
+void or_set_u(or_solver *S, const double *in) { memcpy(S->u, in, sizeof(double) * (size_t) S->N * S->h); }
+void or_get_r(or_solver *S, double *out) { memcpy(out, S->r, sizeof(double) * (size_t) S->N * S->h); }
+
+void or_get_state(or_solver *S, double *pe, double *q, int committed)
+{
+    size_t ng = (size_t) S->N * S->ngp;
+    if (!S->has_state) {
+        if (pe)
+            memset(pe, 0, sizeof(double) * ng * 6);
+        if (q)
+            memset(q, 0, sizeof(double) * ng);
+        return;
+    }
+    if (pe)
+        memcpy(pe, committed ? S->ep_t : S->ep, sizeof(double) * ng * 6);
+    if (q)
+        memcpy(q, committed ? S->q_t : S->q, sizeof(double) * ng);
+}
+
+/* element pass of Solver::postprocess (solver.h:503-574) */
+void or_element_fields(or_solver *S, double *strain_elem, double *stress_elem, double *strain_gp, double *stress_gp,
+                       double *stress_avg, double *strain_avg, double *phase_stress_avg, double *phase_strain_avg,
+                       int32_t *phase_counts, int n_slots)
+{
+    int       h = S->h, ns = S->nstr, ng = S->ngp;
+    ptrdiff_t N = S->N;
+    double    sa[MAXSTR] = {0}, ea[MAXSTR] = {0};
+    if (phase_stress_avg)
+        memset(phase_stress_avg, 0, sizeof(double) * n_slots * ns);
+    if (phase_strain_avg)
+        memset(phase_strain_avg, 0, sizeof(double) * n_slots * ns);
+    if (phase_counts)
+        memset(phase_counts, 0, sizeof(int32_t) * n_slots);
+    for (int ix = 0; ix < S->nx; ix++)
+        for (int iy = 0; iy < S->ny; iy++)
+            for (int iz = 0; iz < S->nz; iz++) {
+                ptrdiff_t idx[8];
+                double    ue[MAXE], eps[8 * MAXSTR], sig[8 * MAXSTR];
+                element_nodes(S, ix, iy, iz, idx);
+                for (int a = 0; a < 8; a++) {
+                    ptrdiff_t n = idx[a] >= N ? idx[a] - N : idx[a];
+                    for (int c = 0; c < h; c++)
+                        ue[h * a + c] = S->u[h * n + c];
+                }
+                int phase = S->ms[idx[0]];
+                element_strain_stress(S, ue, phase, idx[0], eps, sig);
+                double es[MAXSTR] = {0}, ss[MAXSTR] = {0};
+                for (int p = 0; p < ng; p++)
+                    for (int j = 0; j < ns; j++) {
+                        es[j] += eps[p * ns + j] / ng;
+                        ss[j] += sig[p * ns + j] / ng;
+                    }
+                ptrdiff_t e = idx[0];
+                for (int j = 0; j < ns; j++) {
+                    if (strain_elem)
+                        strain_elem[e * ns + j] = es[j];
+                    if (stress_elem)
+                        stress_elem[e * ns + j] = ss[j];
+                    sa[j] += ss[j];
+                    ea[j] += es[j];
+                    if (phase < n_slots) {
+                        if (phase_stress_avg)
+                            phase_stress_avg[phase * ns + j] += ss[j];
+                        if (phase_strain_avg)
+                            phase_strain_avg[phase * ns + j] += es[j];
+                    }
+                }
+                if (phase_counts && phase < n_slots)
+                    phase_counts[phase]++;
+                if (strain_gp)
+                    memcpy(strain_gp + e * ng * ns, eps, sizeof(double) * ng * ns);
+                if (stress_gp)
+                    memcpy(stress_gp + e * ng * ns, sig, sizeof(double) * ng * ns);
+            }
+    for (int j = 0; j < ns; j++) {
+        if (stress_avg)
+            stress_avg[j] = sa[j] / (double) N;
+        if (strain_avg)
+            strain_avg[j] = ea[j] / (double) N;
+    }
+    if (phase_counts)
+        for (int p = 0; p < n_slots; p++)
+            if (phase_counts[p] > 0)
+                for (int j = 0; j < ns; j++) {
+                    if (phase_stress_avg)
+                        phase_stress_avg[p * ns + j] /= phase_counts[p];
+                    if (phase_strain_avg)
+                        phase_strain_avg[p * ns + j] /= phase_counts[p];
+                }
+}

@@ -1,0 +1,150 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Tolerances: FP64 operators 1e-12 relative to the field's max (rounding-order
+differences only); FP64 solves reach the oracle's homogenized quantities to
+1e-10 relative with CG iteration counts within +-2; the mixed FP32/FP64 form
+reaches them to 1e-8 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from fans_amd import gpu
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8, 1), (16, 12, 10, 3), (32, 32, 32, 3), (24, 20, 30, 1), (40, 48, 16, 3)])
+def test_fft_matches_numpy(shape):
+    say(f"fft shape={shape}")
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(shape)
+    X = gpu.fft_r2c(x)
+    ref = np.fft.rfftn(x, axes=(0, 1, 2))
+    assert relerr(X, ref) < 1e-13
+    y = gpu.fft_c2r(ref, shape[2])
+    refi = np.fft.irfftn(ref, s=shape[:3], axes=(0, 1, 2)) * np.prod(shape[:3])
+    assert relerr(y, refi) < 1e-13
+
+
+@pytest.mark.parametrize("maker,fe", [(cases.thermal, "HEX8"), (cases.thermal, "HEX8R"), (cases.elastic, "HEX8"),
+                                      (cases.elastic, "HEX8R")])
+def test_green_and_operators(maker, fe):
+    p = maker(n=12, fe=fe, dims=(12, 10, 8))
+    say(f"operators {maker.__name__} {fe} dims={p.dims}")
+    o = oracle.Oracle(p)
+    g = gpu.GpuSolver(p)
+    assert relerr(g.reference_stiffness(), o.reference_stiffness()) < 1e-15
+    assert relerr(g.green(), o.green()) < 1e-12
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(p.dims + (p.howmany,))
+    g0 = rng.standard_normal(p.n_str) * 0.1
+    o.set_gradient(g0)
+    g.set_gradient(g0)
+    assert relerr(g.residual(u), o.residual(u)) < 1e-12
+    assert relerr(g.residual(u, linear_kd=True), o.residual(u, linear_kd=True)) < 1e-12
+    r = rng.standard_normal(p.dims + (p.howmany,))
+    assert relerr(g.convolve(r), o.convolve(r)) < 1e-12
+    g.close()
+
+
+def _solve_pair(p, g0, precision="fp64"):
+    o = oracle.Oracle(p)
+    g = gpu.GpuSolver(p, precision=precision)
+    o.set_gradient(g0)
+    g.set_gradient(g0)
+    it_o = o.solve()
+    it_g = g.solve()
+    return o, g, it_o, it_g
+
+
+@pytest.mark.parametrize("fe", ["HEX8R", "HEX8"])
+def test_reference_thermal_input(sphere32_ms, fe):
+    """test/input_files/test_LinearThermal.json on sphere32 (CG, Linf 1e-10)."""
+    p = cases.reference_input("test_LinearThermal", sphere32_ms)
+    p.fe_type = fe
+    say(f"reference thermal sphere32 {fe}")
+    o, g, it_o, it_g = _solve_pair(p, p.load_cases[0].g0_path[0])
+    assert abs(it_o - it_g) <= 2
+    assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-10
+    assert g.error_history()[-1] <= p.tol
+    assert relerr(g.get_u(), o.get_u()) < 1e-7
+
+
+def test_reference_elastic_input(sphere32_ms):
+    """test/input_files/test_LinearElastic.json on sphere32 (HEX8R, CG, Linf 1e-10)."""
+    p = cases.reference_input("test_LinearElastic", sphere32_ms)
+    say("reference elastic sphere32")
+    o, g, it_o, it_g = _solve_pair(p, p.load_cases[0].g0_path[0])
+    assert abs(it_o - it_g) <= 2
+    assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-10
+    fo, fg = o.element_fields(), g.element_fields()
+    for k in ("strain", "stress", "stress_average", "strain_average", "phase_stress_average"):
+        assert relerr(fg[k], fo[k]) < 1e-8, k
+    assert np.array_equal(fg["phase_counts"], fo["phase_counts"])
+
+
+@pytest.mark.parametrize("method", ["cg", "fp"])
+def test_elastic_hex8_methods(method):
+    p = cases.elastic(n=16, seed=2, method=method, n_it=2000 if method == "fp" else 300)
+    say(f"elastic 16^3 HEX8 {method}")
+    g0 = np.array([0.001, -0.002, 0.003, 0.0015, -0.0025, 0.001])
+    o, g, it_o, it_g = _solve_pair(p, g0)
+    assert abs(it_o - it_g) <= 2
+    assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-10
+
+
+def test_homogenized_tangent_thermal():
+    p = cases.thermal(n=12, seed=4)
+    say("tangent thermal 12^3")
+    o, g, _, _ = _solve_pair(p, [0.01, 0.0, 0.0])
+    Co, Cg = o.homogenized_tangent(), g.homogenized_tangent()
+    assert relerr(Cg, Co) < 1e-6
+    assert np.allclose(Cg, Cg.T)
+
+
+def test_mixed_precision_elastic():
+    p = cases.elastic(n=32, seed=5, count=10, rmin=3, rmax=6)
+    say("mixed precision elastic 32^3")
+    g0 = np.array([1.0, 0, 0, 0, 0, 0])
+    o = oracle.Oracle(p)
+    o.set_gradient(g0)
+    o.solve()
+    g = gpu.GpuSolver(p, precision="mixed")
+    g.set_gradient(g0)
+    g.solve()
+    assert g.error_history()[-1] <= p.tol
+    assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-8
+
+
+def test_j2_mixed_bc_steps():
+    """J2 matrix + elastic spheres, uniaxial strain with zero lateral stress (config-3 control law)."""
+    p = cases.j2(n=12, seed=1)
+    mbc = cases.uniaxial_mixed(4, 0.02)
+    say("j2 mixed bc 12^3 4 steps")
+    o = oracle.Oracle(p)
+    g = gpu.GpuSolver(p)
+    for t in range(mbc["n_steps"]):
+        o.enable_mixed_bc(mbc, t)
+        g.enable_mixed_bc(mbc, t)
+        it_o, it_g = o.solve(), g.solve()
+        assert abs(it_o - it_g) <= 2, (t, it_o, it_g)
+        so, sg = o.homogenized_stress(), g.homogenized_stress()
+        assert relerr(sg, so) < 1e-8, t
+        assert relerr(g.macro_gradient(), o.macro_gradient()) < 1e-8
+        o.extrapolate_displacement()
+        g.extrapolate_displacement()
+    epo, qo = o.get_state()
+    epg, qg = g.get_state()
+    assert relerr(qg, qo) < 1e-6
+    assert qo.max() > 0  # the matrix yielded
