@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Headline benchmark: voxel-iterations/s of the FANS CG solve, 256^3 linear
+elastic, on one MI355X; FP64 (drop-in) form as `value`, the mixed FP32/FP64
+form reported beside it.
+
+A step is one full solve (Solver::solve, include/solver.h:282-300) of the
+256^3 problem from a fresh solver state.  Inputs (microstructure, material
+tables, Green operator) are resident in HBM before the timed region starts.
+
+    python bench.py [--steps K] [--warmup W] [--size 256] [--no-cpu-baseline] [--no-mixed]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "voxel-iters/s at 256^3 elastic, mixed vs FP64 GPU @1e-8 err; % of HBM peak"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+
+# Algorithmic (compulsory) bytes per voxel of each kernel family, FP64, h = 3:
+# the kernel's own inputs and outputs read/written once.  The spectrum holds
+# (nz/2+1)/nz complex values per real value.
+def family_bytes_per_voxel(nz, h=3, wb=8):
+    V = wb * h
+    S = 2 * wb * h * (nz // 2 + 1) / nz  # spectrum bytes per voxel
+    Gp = wb * h * (h + 1) // 2 * (nz // 2 + 1) / nz  # packed Green operator
+    return {
+        "fft_z_r2c": V + S,
+        "fft_y_fwd": 2 * S,
+        "fft_x_green": 2 * S + Gp,
+        "fft_y_inv": 2 * S,
+        "fft_z_c2r": S + 2 * V,      # read spectrum + r, write s
+        "assemble_lin": 4 * V + 1,   # read s, d, phase; write d, rnew
+        "cg_update": 7 * V,          # read r, rnew, u, d, s; write r, u
+    }
+
+
+B_ALG_FP64 = 409.0  # compulsory bytes per voxel-iteration, FP64 h=3 (CG, single-pass FFT ideal)
+B_ALG_MIXED = 205.0
+
+
+def headline_problem(n=256, seed=0, tol=1e-10):
+    from fans_amd.microstructure import random_spheres
+    from fans_amd.problem import MaterialGroup, Problem, isotropic_KG
+    s = n / 256.0
+    ms = random_spheres(n, max(1, int(round(150 * s ** 3))), 8 * s, 20 * s, seed=seed)
+    Km, Gm = isotropic_KG(3.0, 0.35)
+    Ki, Gi = isotropic_KG(70.0, 0.22)
+    groups = [MaterialGroup("LinearElasticIsotropic", [0, 1], {"bulk_modulus": [Km, Ki], "shear_modulus": [Gm, Gi]})]
+    return Problem(dims=(n, n, n), howmany=3, ms=ms, groups=groups, fe_type="HEX8", method="cg",
+                   measure="Linfinity", err_type="absolute", tol=tol, n_it=2000)
+
+
+def run_gpu(p, precision, steps, warmup, g0):
+    from fans_amd.gpu import GpuSolver
+    g = GpuSolver(p, precision=precision)
+    g.set_gradient(g0)
+    for _ in range(warmup):
+        g.reset_state()
+        g.set_gradient(g0)
+        g.solve()
+    g.set_timing(True)
+    iters, solve_ms = [], []
+    fam = {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.reset_state()
+        g.set_gradient(g0)
+        g.solve()
+        iters.append(int(g.stats.inner_iterations))
+        solve_ms.append(float(g.stats.solve_ms))
+        for k, (ms, n) in g.kernel_times().items():
+            a, b = fam.get(k, (0.0, 0))
+            fam[k] = (a + ms, b + n)
+    t1 = time.perf_counter()
+    res = dict(iters=iters, solve_ms=solve_ms, wall_s=t1 - t0, fam=fam, stress=g.homogenized_stress(),
+               final_error=float(g.stats.final_error), outer=int(g.stats.outer_iterations),
+               converged=bool(g.stats.converged))
+    g.close()
+    return res
+
+
+def cpu_baseline(p, g0, iters=2):
+    """The CPU restatement (oracle, OpenMP over the host cores) on the same
+    256^3 workload, bounded to `iters` CG iterations."""
+    import oracle
+    cores = min(16, os.cpu_count() or 1)
+    oracle.set_threads(cores)
+    from dataclasses import replace
+    q = replace(p, n_it=iters, tol=0.0)
+    o = oracle.Oracle(q)
+    o.set_gradient(g0)
+    t0 = time.perf_counter()
+    it = o.solve()
+    dt = time.perf_counter() - t0
+    n = int(np.prod(p.dims))
+    return {"value": n * it / dt, "unit": "voxel-iters/s", "cores": cores, "kind": "port",
+            "sample": f"oracle CG at {p.dims[0]}^3 linear elastic, {it} iterations ({dt:.1f} s), same microstructure"}
+
+
+def traffic_from_profiles(family):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get(family)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is a design "
+              "described in DESIGN.md and is not executed", file=sys.stderr)
+        sys.exit(2)
+    n = args.size
+    p = headline_problem(n)
+    N = n ** 3
+    g0 = np.array([1.0, 0, 0, 0, 0, 0])
+    print(f"[bench] {n}^3 elastic: FP64 form, {args.warmup} warmup + {args.steps} timed solves", file=sys.stderr,
+          flush=True)
+    r64 = run_gpu(p, "fp64", args.steps, args.warmup, g0)
+    tot_it = sum(r64["iters"])
+    value = N * tot_it / r64["wall_s"]
+    ms_per_step = 1000.0 * r64["wall_s"] / args.steps
+    # dominant kernel family by device time
+    fam = r64["fam"]
+    bpv = family_bytes_per_voxel(n)
+    dom = max((k for k in fam if k in bpv), key=lambda k: fam[k][0])
+    avg_ms = fam[dom][0] / max(fam[dom][1], 1)
+    alg_bytes = bpv[dom] * N
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(achieved / 8000.0, 4), "traffic": traffic_from_profiles(dom),
+                "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_ms, 4)}
+    t_solve = sum(r64["solve_ms"]) * 1e-3
+    iter_frac = B_ALG_FP64 * N * tot_it / t_solve / HBM_PEAK
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "voxel-iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n}^3 linear elastic CG (HEX8, Linf abs 1e-10), 150 spheres r~U[8,20] seed 0, "
+                               "matrix E=3 nu=0.35 / spheres E=70 nu=0.22, Mandel e11 load",
+                   "grid": n, "cg_iterations": r64["iters"][0], "form": "fp64"},
+        "roofline": roofline,
+        "iteration_hbm_frac": round(iter_frac, 4),
+        "kernel_ms_per_iteration": {k: round(v[0] / max(tot_it, 1), 4) for k, v in fam.items()},
+    }
+    if not args.no_mixed:
+        print(f"[bench] {n}^3 elastic: mixed form", file=sys.stderr, flush=True)
+        rmx = run_gpu(p, "mixed", args.steps, args.warmup, g0)
+        rel = float(np.max(np.abs(rmx["stress"] - r64["stress"])) / np.max(np.abs(r64["stress"])))
+        t_mx = sum(rmx["solve_ms"]) / len(rmx["solve_ms"])
+        t_64 = sum(r64["solve_ms"]) / len(r64["solve_ms"])
+        out["mixed"] = {"voxel_iters_per_s": round(N * sum(rmx["iters"]) / rmx["wall_s"], 1),
+                        "inner_iterations": rmx["iters"][0], "outer_iterations": rmx["outer"],
+                        "time_to_solution_speedup": round(t_64 / t_mx, 3), "homog_stress_rel_err": rel,
+                        "converged": rmx["converged"], "solve_ms": round(t_mx, 2), "fp64_solve_ms": round(t_64, 2),
+                        "iteration_hbm_frac": round(B_ALG_MIXED * N * sum(rmx["iters"]) /
+                                                    (sum(rmx["solve_ms"]) * 1e-3) / HBM_PEAK, 4)}
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(p, g0)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

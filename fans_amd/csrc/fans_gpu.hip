@@ -1576,6 +1576,25 @@ int fans_gpu_extrapolate_displacement(fans_gpu_ctx *c)
     return 0;
 }
 
+int fans_gpu_reset_state(fans_gpu_ctx *c)
+{
+    if (!c)
+        return FANS_GPU_EINVAL;
+    int rc;
+    if ((rc = fill<double>(c, c->u, 0.0, c->Nh)) || (rc = fill<double>(c, c->u_prev, 0.0, c->Nh)) ||
+        (rc = fill<double>(c, c->r, 0.0, c->Nh)))
+        return rc;
+    if (c->js && (rc = fill<double>(c, c->js, 0.0, (int64_t) 7 * c->ngp * c->N)))
+        return rc;
+    c->mixed_active = false;
+    std::memset(c->gvec, 0, sizeof(c->gvec));
+    std::memset(c->gvec_prev, 0, sizeof(c->gvec_prev));
+    c->tol      = c->d.tol;
+    c->tol_type = c->d.err_type;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int fans_gpu_postprocess(fans_gpu_ctx *c, double *strain_elem, double *stress_elem, double *strain_gp,
                          double *stress_gp, double *stress_avg, double *strain_avg, double *phase_stress_avg,
                          double *phase_strain_avg, int32_t *phase_counts, int32_t n_slots)

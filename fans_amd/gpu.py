@@ -55,7 +55,7 @@ EXPORTS = [
     "fans_gpu_homogenized_stress", "fans_gpu_homogenized_tangent", "fans_gpu_extrapolate_displacement",
     "fans_gpu_postprocess", "fans_gpu_get_field", "fans_gpu_set_field", "fans_gpu_residual", "fans_gpu_convolve",
     "fans_gpu_get_green", "fans_gpu_get_reference_stiffness", "fans_gpu_fft_r2c", "fans_gpu_fft_c2r",
-    "fans_gpu_set_option", "fans_gpu_kernel_times", "fans_gpu_kernel_family_name",
+    "fans_gpu_set_option", "fans_gpu_kernel_times", "fans_gpu_kernel_family_name", "fans_gpu_reset_state",
 ]
 
 
@@ -81,6 +81,7 @@ def lib():
         L.fans_gpu_homogenized_stress.argtypes = [vp, dp, C.c_int32]
         L.fans_gpu_homogenized_tangent.argtypes = [vp, C.c_double, dp, C.c_int32]
         L.fans_gpu_extrapolate_displacement.argtypes = [vp]
+        L.fans_gpu_reset_state.argtypes = [vp]
         L.fans_gpu_postprocess.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, ip, C.c_int32]
         L.fans_gpu_get_field.argtypes = [vp, C.c_int32, dp, i64]
         L.fans_gpu_set_field.argtypes = [vp, C.c_int32, dp, i64]
@@ -234,6 +235,12 @@ class GpuSolver:
 
     def extrapolate_displacement(self):
         self._check(self.L.fans_gpu_extrapolate_displacement(self.ptr))
+
+    def reset_state(self):
+        self._check(self.L.fans_gpu_reset_state(self.ptr))
+
+    def set_timing(self, on=True):
+        self._check(self.L.fans_gpu_set_option(self.ptr, OPT_TIMING, int(bool(on))))
 
     def get_field(self, name):
         if name in ("plastic_strain",):

@@ -141,6 +141,10 @@ int fans_gpu_homogenized_stress(fans_gpu_ctx *ctx, double *out, int32_t n_str);
 int fans_gpu_homogenized_tangent(fans_gpu_ctx *ctx, double pert, double *out, int32_t n_str);
 /* Solver::extrapolateDisplacement (solver.h:302-311) */
 int fans_gpu_extrapolate_displacement(fans_gpu_ctx *ctx);
+/* State of a freshly constructed Solver (solver.h:133-139): u = u_prev = 0,
+ * internal variables zero, mixed BC off.  Keeps the Green operator and tables
+ * (the reference rebuilds them per load case, main.cpp:15-16, with equal result). */
+int fans_gpu_reset_state(fans_gpu_ctx *ctx);
 
 /* element pass of Solver::postprocess (solver.h:503-574); any output may be NULL.
  * phase_* arrays have n_slots rows (phase ids 0..n_slots-1). */
