@@ -112,6 +112,7 @@ class Problem:
             thermal = g.matmodel in THERMAL_MODELS
             if thermal != (self.howmany == 1):
                 raise ValueError(f"{g.matmodel} is not a valid matmodel for this problem type")
+            g.param_matrix()  # missing / inconsistent material properties raise here
         seen = set()
         for g in self.groups:
             for p in g.phases:
