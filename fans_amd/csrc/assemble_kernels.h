@@ -1,0 +1,256 @@
+// Residual assembly kernel (Solver::compute_residual_basic, solver.h:228-270,
+// and the linear CG matvec of solverCG.h:94-105) for the periodic voxel grid.
+//
+// The kernel marches along x.  A workgroup of 256 threads owns a 16x16 (y,z)
+// tile of elements (one element per thread, lanes ordered z-fastest so one
+// wave holds 4 rows of 16) and emits the 15x15 nodes whose four surrounding
+// elements of the current plane lie in the tile.  Per x plane:
+//   1. the 17x17 node plane x+1 is staged in LDS (in CG mode the direction
+//      update d = s + beta d happens here and the owned nodes are written);
+//   2. each thread gathers its 8 corners from LDS and computes its element;
+//   3. corner forces are combined along z with DPP row_shr (in registers),
+//      along y with a 16-lane shuffle (LDS only for the wave-boundary row),
+//      and along x by a register carry into the next plane.
+// Two barriers per plane; every node is produced by exactly one thread in a
+// fixed summation order, so results are deterministic and atomic-free.
+#pragma once
+#include "element_kernels.h"
+
+namespace fans {
+
+enum { ASM_RESID = 0, ASM_CGLIN = 1 };
+constexpr int TILE = 16;  // element tile edge (y and z)
+constexpr int NOUT = TILE - 1;
+
+template <typename T>
+struct AsmArgs {
+    int             nx, ny, nz;
+    int             cx;        // x planes per block
+    int             nty, ntz;  // number of tiles in y, z
+    Geom            geom;
+    const uint8_t  *ms;        // phase ids [x][y][z]
+    const Phase<T> *phases;
+    J2State         js;
+    // ASM_RESID: in = u, out = r; norm/dot partials
+    // ASM_CGLIN: in = s, din = d_cur, dout = d_next (d = s + beta d), out = rnew
+    const T        *in;
+    const T        *din;
+    T              *dout;
+    T              *out;
+    const T        *dotv;      // ASM_RESID optional: partial <out, dotv>
+    const double   *scal;      // device scalars (beta at scal[0] for CGLIN)
+    const T        *g0;        // macroscopic gradient (n_str), device
+    double         *partial;   // [2][gridDim] : slot 0 = norm/dot, slot 1 = dot
+    int             measure;   // 0 L1, 1 L2, 2 Linf  (ASM_RESID)
+    const int      *done;
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so halo re-reads of neighbouring tiles hit the
+// same L2.
+__device__ __forceinline__ int xcd_tile_index(int bid, int nb)
+{
+    const int q = nb / 8, r = nb % 8, k = bid % 8, p = bid / 8;
+    return k * q + min(k, r) + p;
+}
+
+template <typename T, int H>
+__device__ __forceinline__ T shfl_up16(T v)
+{
+    return __shfl_up(v, 16, 64);
+}
+
+// ISO: every phase is LinearElasticIsotropic (H=3) or LinearThermalIsotropic
+// (H=1) and the collapsed element arithmetic is used; otherwise the general
+// per-Gauss-point constitutive path.
+template <typename T, int H, int NGP, int MODE, bool ISO>
+__global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
+{
+    if (a.done && *a.done)
+        return;
+    constexpr int NL = TILE + 1;  // node tile edge
+    constexpr int NS = H == 1 ? 3 : 6;
+    __shared__ T      Un[2][NL * NL * H];
+    __shared__ T      Yb[4][TILE * 2 * H];  // last row of each wave for the y exchange
+    __shared__ double red[8];
+
+    const int t = threadIdx.x, ey = t / TILE, ez = t - ey * TILE;
+    const int wave = t >> 6, lane = t & 63;
+    int       bid  = xcd_tile_index(blockIdx.x, gridDim.x);
+    const int tz   = bid % a.ntz;
+    bid /= a.ntz;
+    const int     ty = bid % a.nty;
+    const int     bx = bid / a.nty;
+    const int     Y0 = ty * NOUT, Z0 = tz * NOUT, X0 = bx * a.cx;
+    const int     X1 = min(X0 + a.cx, a.nx);
+    const int     ny = a.ny, nz = a.nz, nx = a.nx;
+    const int64_t plane = (int64_t) ny * nz;
+
+    auto wrapy = [&](int v) { int y = Y0 - 1 + v; y += (y < 0) ? ny : 0; return y >= ny ? y - ny : y; };
+    auto wrapz = [&](int v) { int z = Z0 - 1 + v; z += (z < 0) ? nz : 0; return z >= nz ? z - nz : z; };
+    const int  my_y   = Y0 - 1 + ey, my_z = Z0 - 1 + ez;
+    const bool own_yz = ey >= 1 && ez >= 1 && my_y < ny && my_z < nz;
+    // node-plane slots of this thread: its own node (ey, ez) and, for 33 threads
+    // spread over the waves, one of the extra row/column 16 nodes
+    const int  ext    = (t % 7 == 0 && t / 7 < 2 * NL - 1) ? t / 7 : -1;
+    const int  e_a    = ext < 0 ? 0 : (ext < NL ? TILE : ext - NL);
+    const int  e_b    = ext < 0 ? 0 : (ext < NL ? ext : TILE);
+    const int64_t off_own = (int64_t) wrapy(ey) * nz + wrapz(ez);
+    const int64_t off_ext = (int64_t) wrapy(e_a) * nz + wrapz(e_b);
+
+    T beta = (T) 0;
+    if constexpr (MODE == ASM_CGLIN)
+        beta = (T) a.scal[0];
+
+    auto load_node = [&](int64_t n, int slot, int buf, bool wr) {
+#pragma unroll
+        for (int c = 0; c < H; ++c) {
+            T val;
+            if constexpr (MODE == ASM_CGLIN) {
+                val = a.in[n * H + c] + beta * a.din[n * H + c];
+                if (wr)
+                    a.dout[n * H + c] = val;
+            } else {
+                val = a.in[n * H + c];
+            }
+            Un[buf][slot * H + c] = val;
+        }
+    };
+    auto load_plane = [&](int xp, int buf) {
+        const int     xg  = xp < 0 ? xp + nx : (xp >= nx ? xp - nx : xp);
+        const bool    wr  = (MODE == ASM_CGLIN) && xp >= X0 && xp < X1 && own_yz;
+        const int64_t pb  = xg * plane;
+        load_node(pb + off_own, ey * NL + ez, buf, wr);
+        if (ext >= 0)
+            load_node(pb + off_ext, e_a * NL + e_b, buf, false);
+    };
+
+    T      carry[H];
+    double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+    for (int c = 0; c < H; ++c)
+        carry[c] = (T) 0;
+    T g0[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+        g0[i] = (MODE == ASM_RESID) ? a.g0[i] : (T) 0;
+    const T w = (T) a.geom.w;
+
+    load_plane(X0 - 1, 0);
+    for (int i = 0; X0 - 1 + i < X1; ++i) {
+        const int xe = X0 - 1 + i;
+        load_plane(xe + 1, (i + 1) & 1);
+        __syncthreads();
+        T u[8][H];
+#pragma unroll
+        for (int cr = 0; cr < 8; ++cr) {
+            const int ax = cr & 1, ay = (cr >> 1) & 1, az = cr >> 2;
+            const T  *src = &Un[(i + ax) & 1][((ey + ay) * NL + ez + az) * H];
+#pragma unroll
+            for (int c = 0; c < H; ++c)
+                u[cr][c] = src[c];
+        }
+        T dv[H];  // d_next at node plane xe (CG mode)
+#pragma unroll
+        for (int c = 0; c < H; ++c)
+            dv[c] = u[0][c];
+        const int       xg = xe < 0 ? xe + nx : xe;
+        const int64_t   el = xg * plane + off_own;
+        const Phase<T> &P  = a.phases[a.ms[el]];
+        T f[8][H];
+        if constexpr (ISO) {
+            if constexpr (H == 3) {
+                const T lw = P.p[0] * w, mw = P.p[1] * w;
+                T       T0[3][3];
+                if constexpr (MODE == ASM_RESID) {
+                    const T r = (T) kSqrtHalf, tr = g0[0] + g0[1] + g0[2];
+                    const T E[3][3] = {{g0[0], r * g0[3], r * g0[4]}, {r * g0[3], g0[1], r * g0[5]},
+                                       {r * g0[4], r * g0[5], g0[2]}};
+#pragma unroll
+                    for (int p = 0; p < 3; ++p)
+#pragma unroll
+                        for (int q = 0; q < 3; ++q)
+                            T0[p][q] = (T) 2 * mw * E[p][q] + (p == q ? lw * tr : (T) 0);
+                } else {
+#pragma unroll
+                    for (int p = 0; p < 3; ++p)
+#pragma unroll
+                        for (int q = 0; q < 3; ++q)
+                            T0[p][q] = (T) 0;
+                }
+                element_forces_iso3<T, NGP>(reinterpret_cast<const T(&)[8][3]>(u), a.geom, lw, mw, T0,
+                                            reinterpret_cast<T(&)[8][3]>(f));
+            } else {
+                const T kw    = P.p[0] * w;
+                const T T0[3] = {kw * g0[0], kw * g0[1], kw * g0[2]};
+                element_forces_iso1<T, NGP>(reinterpret_cast<const T(&)[8][1]>(u), a.geom, kw, T0,
+                                            reinterpret_cast<T(&)[8][1]>(f));
+            }
+        } else {
+            auto mat = [&](int gp, const T(&e)[NS], T(&s)[NS]) { material_point<T, H>(P, e, s, a.js, el, gp, NGP); };
+            element_forces<T, H, NGP>(u, a.geom, g0, mat, f);
+        }
+        // z: node (ey,ez) <- az=0 corners of element ez + az=1 corners of element ez-1
+        T P2[2][2][H];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < H; ++c)
+                P2[q & 1][q >> 1][c] = f[q][c] + dpp_row_shr1(f[q + 4][c]);
+        // y: node (ey,ez) <- ay=0 of own row + ay=1 of row ey-1 (lane-16, or previous wave)
+        if (lane >= 48)
+#pragma unroll
+            for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+                for (int c = 0; c < H; ++c)
+                    Yb[wave][(ez * 2 + ax) * H + c] = P2[ax][1][c];
+        T up[2][H];
+#pragma unroll
+        for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+            for (int c = 0; c < H; ++c)
+                up[ax][c] = shfl_up16<T, H>(P2[ax][1][c]);
+        __syncthreads();
+        if (lane < 16 && wave > 0)
+#pragma unroll
+            for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+                for (int c = 0; c < H; ++c)
+                    up[ax][c] = Yb[wave - 1][(ez * 2 + ax) * H + c];
+        if (i >= 1 && own_yz) {
+            const int64_t n = xe * plane + off_own;
+#pragma unroll
+            for (int c = 0; c < H; ++c) {
+                const T res      = carry[c] + P2[0][0][c] + up[0][c];
+                a.out[n * H + c] = res;
+                if constexpr (MODE == ASM_CGLIN) {
+                    acc0 += (double) dv[c] * (double) res;
+                } else {
+                    const double ar = fabs((double) res);
+                    if (a.measure == 2)
+                        acc0 = fmax(acc0, ar);
+                    else if (a.measure == 1)
+                        acc0 += ar * ar;
+                    else
+                        acc0 += ar;
+                    if (a.dotv)
+                        acc1 += (double) res * (double) a.dotv[n * H + c];
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < H; ++c)
+            carry[c] = P2[1][0][c] + up[1][c];
+    }
+    if (a.partial) {
+        double     v[2]  = {acc0, acc1};
+        const bool mx[2] = {MODE == ASM_RESID && a.measure == 2, false};
+        block_reduce<2>(v, mx, red);
+        if (t == 0) {
+            a.partial[blockIdx.x]             = v[0];
+            a.partial[gridDim.x + blockIdx.x] = v[1];
+        }
+    }
+}
+
+}  // namespace fans

@@ -9,11 +9,8 @@
 // B^T sigma is its exact transpose.  Mandel shear scaling follows
 // include/matmodel.h:284-304.
 //
-// The assembly kernel marches along x.  A workgroup owns a 16x16 (y,z) tile of
-// elements and emits the 15x15 nodes whose 4 surrounding elements in that
-// plane are all in the tile; the x-neighbour contribution is carried in
-// registers from one plane to the next.  Corner forces are combined through LDS
-// in a fixed order, so results are deterministic and atomic-free.
+// This header holds the per-element arithmetic and the element-local passes;
+// the assembly (gather/scatter) kernel built on it is in assemble_kernels.h.
 #pragma once
 #include "common.h"
 
@@ -381,184 +378,100 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][H], const Geom &g
     grad_factors_T<T, H, NGP>(Sx, Sy, Sz, g, f);
 }
 
-// ---------------------------------------------------------------------------
-// assembly kernel (x-marching tile)
-// ---------------------------------------------------------------------------
-enum { ASM_RESID = 0, ASM_CGLIN = 1 };
-constexpr int TILE = 16;  // element tile edge (y and z)
-constexpr int NOUT = TILE - 1;
-
-template <typename T>
-struct AsmArgs {
-    int            nx, ny, nz;
-    int            cx;          // x planes per block
-    int            nty, ntz;    // number of tiles in y, z
-    Geom           geom;
-    const uint8_t *ms;          // phase ids [x][y][z]
-    const Phase<T> *phases;
-    J2State        js;
-    // ASM_RESID: in = u, out = r; norm/dot partials
-    // ASM_CGLIN: in = s, din = d_cur, dout = d_next (d = s + beta d), out = rnew
-    const T       *in;
-    const T       *din;
-    T             *dout;
-    T             *out;
-    const T       *dotv;        // ASM_RESID optional: partial <out, dotv>
-    const double  *scal;        // device scalars (beta at scal[0] for CGLIN)
-    const T       *g0;          // macroscopic gradient (n_str), device
-    double        *partial;     // [2][gridDim] : slot 0 = norm/dot, slot 1 = dot
-    int            measure;     // 0 L1, 1 L2, 2 Linf  (ASM_RESID)
-    const int     *done;
-};
-
-template <typename T, int H, int NGP, int MODE>
-__global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
+// Isotropic fast path: with sigma = lambda tr(eps) I + 2 mu eps (Mandel), the
+// Gauss-point loop collapses.  T[c][d] = lambda tr(grad u) delta_cd +
+// mu (du_c/dx_d + du_d/dx_c) is linear in the gradient factors, and the sums
+// over the Gauss index that a factor does not depend on reduce to pairwise
+// sums.  lw = lambda * w, mw = mu * w (w = v_e / n_gp); T0 is the traction
+// tensor of the macroscopic gradient (already multiplied by w), added once
+// per Gauss point.
+template <typename T, int NGP>
+__device__ __forceinline__ void element_forces_iso3(const T (&u)[8][3], const Geom &g, T lw, T mw, const T (&T0)[3][3],
+                                                    T (&f)[8][3])
 {
-    if (a.done && *a.done)
-        return;
-    constexpr int NL = TILE + 1;  // node tile edge
-    __shared__ T      Un[2][NL * NL * H];
-    __shared__ T      Xz[TILE * TILE * 4 * H];
-    __shared__ T      Xy[TILE * TILE * 2 * H];
-    __shared__ double red[8];
-
-    const int t = threadIdx.x, ey = t / TILE, ez = t - ey * TILE;
-    int       bid  = blockIdx.x;
-    const int tz   = bid % a.ntz;
-    bid /= a.ntz;
-    const int ty   = bid % a.nty;
-    const int bx   = bid / a.nty;
-    const int Y0   = ty * NOUT, Z0 = tz * NOUT, X0 = bx * a.cx;
-    const int X1   = min(X0 + a.cx, a.nx);
-    const int ny = a.ny, nz = a.nz, nx = a.nx;
-    const int64_t plane = (int64_t) ny * nz;
-
-    // node tile coords (ty_, tz_) in [0,17) map to global (Y0-1+ty_, Z0-1+tz_) mod n
-    auto gy = [&](int v) { int y = Y0 - 1 + v; y += (y < 0) ? ny : 0; return y >= ny ? y - ny : y; };
-    auto gz = [&](int v) { int z = Z0 - 1 + v; z += (z < 0) ? nz : 0; return z >= nz ? z - nz : z; };
-    const int  my_y   = Y0 - 1 + ey, my_z = Z0 - 1 + ez;
-    const bool own_yz = ey >= 1 && ez >= 1 && my_y < ny && my_z < nz;
-    T          beta   = (T) 0;
-    if constexpr (MODE == ASM_CGLIN)
-        beta = (T) a.scal[0];
-
-    auto load_plane = [&](int xp, int buf) {
-        const int     xg = xp < 0 ? xp + nx : (xp >= nx ? xp - nx : xp);
-        const bool    wr = (MODE == ASM_CGLIN) && xp >= X0 && xp < X1;
-        for (int v = t; v < NL * NL; v += 256) {
-            const int     a1 = v < 256 ? v / TILE : (v - 256 < NL ? TILE : (v - 256 - NL));
-            const int     b1 = v < 256 ? v - (v / TILE) * TILE : (v - 256 < NL ? v - 256 : TILE);
-            const int64_t n  = xg * plane + (int64_t) gy(a1) * nz + gz(b1);
+    T Gx[2][2][3], Gy[2][2][3], Gz[2][2][3];
+    grad_factors<T, 3, NGP>(u, g, Gx, Gy, Gz);
+    T Sx[2][2][3], Sy[2][2][3], Sz[2][2][3];
+    if constexpr (NGP == 8) {
+        const T l2m2 = (T) 2 * (lw + (T) 2 * mw), m2 = (T) 2 * mw, g2 = (T) 2;
 #pragma unroll
-            for (int c = 0; c < H; ++c) {
-                T val;
-                if constexpr (MODE == ASM_CGLIN) {
-                    val = a.in[n * H + c] + beta * a.din[n * H + c];
-                    if (wr && v < 256 && own_yz)
-                        a.dout[n * H + c] = val;
-                } else {
-                    val = a.in[n * H + c];
-                }
-                Un[buf][(a1 * NL + b1) * H + c] = val;
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                // sums over the Gauss index the factor does not depend on
+                const T sGy_i0 = Gy[0][b][0] + Gy[1][b][0], sGy_i1 = Gy[0][b][1] + Gy[1][b][1];  // [k=b]
+                const T sGz_i0 = Gz[0][a][0] + Gz[1][a][0], sGz_i2 = Gz[0][a][2] + Gz[1][a][2];  // [j=a]
+                Sx[a][b][0] = l2m2 * Gx[a][b][0] + lw * (sGy_i1 + sGz_i2) + g2 * T0[0][0];
+                Sx[a][b][1] = m2 * Gx[a][b][1] + mw * sGy_i0 + g2 * T0[1][0];
+                Sx[a][b][2] = m2 * Gx[a][b][2] + mw * sGz_i0 + g2 * T0[2][0];
+                const T sGx_j0 = Gx[0][b][0] + Gx[1][b][0], sGx_j1 = Gx[0][b][1] + Gx[1][b][1];  // [k=b]
+                const T sGz_j1 = Gz[a][0][1] + Gz[a][1][1], sGz_j2 = Gz[a][0][2] + Gz[a][1][2];  // [i=a]
+                Sy[a][b][0] = m2 * Gy[a][b][0] + mw * sGx_j1 + g2 * T0[0][1];
+                Sy[a][b][1] = l2m2 * Gy[a][b][1] + lw * (sGx_j0 + sGz_j2) + g2 * T0[1][1];
+                Sy[a][b][2] = m2 * Gy[a][b][2] + mw * sGz_j1 + g2 * T0[2][1];
+                const T sGx_k0 = Gx[b][0][0] + Gx[b][1][0], sGx_k2 = Gx[b][0][2] + Gx[b][1][2];  // [j=b]
+                const T sGy_k1 = Gy[a][0][1] + Gy[a][1][1], sGy_k2 = Gy[a][0][2] + Gy[a][1][2];  // [i=a]
+                Sz[a][b][0] = m2 * Gz[a][b][0] + mw * sGx_k2 + g2 * T0[0][2];
+                Sz[a][b][1] = m2 * Gz[a][b][1] + mw * sGy_k2 + g2 * T0[1][2];
+                Sz[a][b][2] = l2m2 * Gz[a][b][2] + lw * (sGx_k0 + sGy_k1) + g2 * T0[2][2];
             }
-        }
-    };
-
-    T      carry[H];
-    double acc0 = 0.0, acc1 = 0.0;
+    } else {
+        const T gr[3][3] = {{Gx[0][0][0], Gy[0][0][0], Gz[0][0][0]},
+                            {Gx[0][0][1], Gy[0][0][1], Gz[0][0][1]},
+                            {Gx[0][0][2], Gy[0][0][2], Gz[0][0][2]}};
+        const T tr       = gr[0][0] + gr[1][1] + gr[2][2];
+        T       Tt[3][3];
 #pragma unroll
-    for (int c = 0; c < H; ++c)
-        carry[c] = (T) 0;
-    T g0[H == 1 ? 3 : 6];
+        for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int i = 0; i < (H == 1 ? 3 : 6); ++i)
-        g0[i] = (MODE == ASM_RESID) ? a.g0[i] : (T) 0;
-
-    load_plane(X0 - 1, 0);
-    for (int i = 0; X0 - 1 + i < X1; ++i) {
-        const int xe = X0 - 1 + i;
-        load_plane(xe + 1, (i + 1) & 1);
-        __syncthreads();
-        T u[8][H];
+            for (int d = 0; d < 3; ++d)
+                Tt[c][d] = mw * (gr[c][d] + gr[d][c]) + (c == d ? lw * tr : (T) 0) + T0[c][d];
 #pragma unroll
-        for (int cr = 0; cr < 8; ++cr) {
-            const int ax = cr & 1, ay = (cr >> 1) & 1, az = cr >> 2;
-            const T  *src = &Un[(i + ax) & 1][((ey + ay) * NL + ez + az) * H];
-#pragma unroll
-            for (int c = 0; c < H; ++c)
-                u[cr][c] = src[c];
-        }
-        T dv[H];  // d_next at node plane xe (CGLIN); read before the buffer is recycled
-#pragma unroll
-        for (int c = 0; c < H; ++c)
-            dv[c] = Un[i & 1][(ey * NL + ez) * H + c];
-        const int     xg  = xe < 0 ? xe + nx : xe;
-        const int64_t el  = xg * plane + (int64_t) gy(ey) * nz + gz(ez);
-        const Phase<T> &P = a.phases[a.ms[el]];
-        T f[8][H];
-        auto mat = [&](int gp, const T(&e)[H == 1 ? 3 : 6], T(&s)[H == 1 ? 3 : 6]) {
-            material_point<T, H>(P, e, s, a.js, el, gp, NGP);
-        };
-        element_forces<T, H, NGP>(u, a.geom, g0, mat, f);
-        // combine along z: node (ey, ez) gets az=0 corners of element ez and az=1 of element ez-1
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int c = 0; c < H; ++c)
-                Xz[(t * 4 + q) * H + c] = f[q + 4][c];
-        __syncthreads();
-        T P2[2][2][H];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int c = 0; c < H; ++c)
-                P2[q & 1][q >> 1][c] = f[q][c] + (ez >= 1 ? Xz[((t - 1) * 4 + q) * H + c] : (T) 0);
-#pragma unroll
-        for (int ax = 0; ax < 2; ++ax)
-#pragma unroll
-            for (int c = 0; c < H; ++c)
-                Xy[(t * 2 + ax) * H + c] = P2[ax][1][c];
-        __syncthreads();
-        T Q[2][H];
-#pragma unroll
-        for (int ax = 0; ax < 2; ++ax)
-#pragma unroll
-            for (int c = 0; c < H; ++c)
-                Q[ax][c] = P2[ax][0][c] + (ey >= 1 ? Xy[((t - TILE) * 2 + ax) * H + c] : (T) 0);
-        if (i >= 1 && own_yz) {
-            const int64_t n = xe * plane + (int64_t) my_y * nz + my_z;
-#pragma unroll
-            for (int c = 0; c < H; ++c) {
-                const T res    = carry[c] + Q[0][c];
-                a.out[n * H + c] = res;
-                if constexpr (MODE == ASM_CGLIN) {
-                    acc0 += (double) dv[c] * (double) res;
-                } else {
-                    const double ar = fabs((double) res);
-                    if (a.measure == 2)
-                        acc0 = fmax(acc0, ar);
-                    else if (a.measure == 1)
-                        acc0 += ar * ar;
-                    else
-                        acc0 += ar;
-                    if (a.dotv)
-                        acc1 += (double) res * (double) a.dotv[n * H + c];
-                }
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < H; ++c)
-            carry[c] = Q[1][c];
-    }
-    if (a.partial) {
-        double     v[2]  = {acc0, acc1};
-        const bool mx[2] = {MODE == ASM_RESID && a.measure == 2, false};
-        block_reduce<2>(v, mx, red);
-        if (t == 0) {
-            a.partial[blockIdx.x]             = v[0];
-            a.partial[gridDim.x + blockIdx.x] = v[1];
+        for (int c = 0; c < 3; ++c) {
+            Sx[0][0][c] = Tt[c][0];
+            Sy[0][0][c] = Tt[c][1];
+            Sz[0][0][c] = Tt[c][2];
         }
     }
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            f[a][c] = (T) 0;
+    grad_factors_T<T, 3, NGP>(Sx, Sy, Sz, g, f);
+}
+
+// isotropic conduction: sigma = k grad + k g0, the Gauss sums are factors of 2
+template <typename T, int NGP>
+__device__ __forceinline__ void element_forces_iso1(const T (&u)[8][1], const Geom &g, T kw, const T (&T0)[3],
+                                                    T (&f)[8][1])
+{
+    T Gx[2][2][1], Gy[2][2][1], Gz[2][2][1];
+    grad_factors<T, 1, NGP>(u, g, Gx, Gy, Gz);
+    const T s = NGP == 8 ? (T) 2 * kw : kw, s0 = NGP == 8 ? (T) 2 : (T) 1;
+    T       Sx[2][2][1], Sy[2][2][1], Sz[2][2][1];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            Sx[a][b][0] = s * Gx[a][b][0] + s0 * T0[0];
+            Sy[a][b][0] = s * Gy[a][b][0] + s0 * T0[1];
+            Sz[a][b][0] = s * Gz[a][b][0] + s0 * T0[2];
+        }
+    f[0][0] = f[1][0] = f[2][0] = f[3][0] = f[4][0] = f[5][0] = f[6][0] = f[7][0] = (T) 0;
+    grad_factors_T<T, 1, NGP>(Sx, Sy, Sz, g, f);
+}
+
+// lane l <- lane l-1 inside each row of 16 lanes (DPP row_shr:1); row starts get 0
+__device__ __forceinline__ double dpp_row_shr1(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x111, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x111, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float dpp_row_shr1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
 }
 
 // ---------------------------------------------------------------------------

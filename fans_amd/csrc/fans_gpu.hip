@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "assemble_kernels.h"
 #include "common.h"
 #include "element_kernels.h"
 #include "fft_kernels.h"
@@ -109,8 +110,17 @@ struct Fft3D {
     int           nx = 0, ny = 0, nz = 0, h = 0, M = 0, Mc = 0;
     FftLen        Lz{}, Ly{}, Lx{};
     cplx<T>      *twz = nullptr, *twy = nullptr, *twx = nullptr;
-    int           lpb = 1, bc = 8, bk = 2;
-    size_t        smz = 0, smy = 0, smx = 0;
+    int           lpb = 1, bc = 8, bk = 2, bcx = 8;
+    size_t        smz = 0, smy = 0, smx = 0, smx_plain = 0;
+
+    // largest power of two <= min(limit, cap), at least 1
+    static int pow2_cols(int limit, int cap)
+    {
+        int v = 1;
+        while (v * 2 <= limit && v * 2 <= cap)
+            v *= 2;
+        return v;
+    }
     int64_t       nblk_z = 0;
     bool          ready  = false;
 
@@ -134,22 +144,22 @@ struct Fft3D {
             return e;
         if ((e = up(twx, twiddles<T>(nx))) != hipSuccess)
             return e;
-        // LDS budget per workgroup: 60 KiB for the two ping-pong buffers
-        const size_t cb     = sizeof(cplx<T>);
-        const size_t budget = 60 * 1024;
-        lpb                 = std::max(1, (int) (budget / (2 * cb * M * h)));
-        lpb                 = std::min(lpb, 16);
-        smz                 = 2 * cb * M * h * lpb;
-        const int Kc        = Mc * h;
-        bc                  = std::max(1, (int) (budget / (2 * cb * ny)));
-        bc                  = std::min(bc, Kc);
-        bc                  = std::min(bc, 32);
-        smy                 = 2 * cb * ny * bc;
-        bk                  = std::max(1, (int) (budget / (2 * cb * nx * h)));
-        bk                  = std::min(bk, Mc);
-        smx                 = 2 * cb * nx * h * bk;
-        nblk_z              = ((int64_t) nx * ny + lpb - 1) / lpb;
-        ready               = true;
+        // One padded LDS tile per workgroup, n * columns <= FFT_MAX_ELEMS so the
+        // in-place stages keep every butterfly input in registers.
+        const size_t cb = sizeof(cplx<T>);
+        if ((int64_t) M * h > FFT_MAX_ELEMS || ny > FFT_MAX_ELEMS || (int64_t) nx * h > FFT_MAX_ELEMS)
+            return hipErrorInvalidValue;
+        lpb          = std::max(1, std::min(16, FFT_MAX_ELEMS / (M * h)));
+        smz          = cb * M * (lpb * h + 1);
+        const int Kc = Mc * h;
+        bc           = pow2_cols(FFT_MAX_ELEMS / ny, std::min(Kc, 16));
+        smy          = cb * ny * (bc + 1);
+        bk           = std::max(1, std::min(Mc, FFT_MAX_ELEMS / (nx * h)));
+        smx          = cb * nx * (bk * h + 1);
+        bcx          = pow2_cols(FFT_MAX_ELEMS / nx, 16);
+        smx_plain    = cb * nx * (bcx + 1);
+        nblk_z       = ((int64_t) nx * ny + lpb - 1) / lpb;
+        ready        = true;
         return hipSuccess;
     }
     void release()
@@ -177,7 +187,7 @@ struct fans_gpu_ctx {
     bool                        have_ms = false, have_mat = false;
     int                         nphases = 0;
     std::vector<int>            phase_kind;
-    bool                        all_linear = true, has_j2 = false;
+    bool                        all_linear = true, has_j2 = false, all_iso = false;
     double                      kref[36]{};
     double                      keref[576]{};
     std::vector<Phase<double>>  ph64;
@@ -394,8 +404,15 @@ template <typename T, int MODE>
 int launch_assemble(fans_gpu_ctx *c, AsmArgs<T> a)
 {
     Timed t(c, MODE == ASM_CGLIN ? KF_ASSEMBLE_LIN : KF_ASSEMBLE_RES);
-#define ASM(HH, GG) \
-    hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE>), dim3(c->nblk_asm), dim3(kThreads), 0, c->stream, a)
+#define ASM(HH, GG)                                                                                          \
+    do {                                                                                                     \
+        if (c->all_iso)                                                                                      \
+            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, true>), dim3(c->nblk_asm), dim3(kThreads), 0,  \
+                               c->stream, a);                                                                \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, false>), dim3(c->nblk_asm), dim3(kThreads), 0, \
+                               c->stream, a);                                                                \
+    } while (0)
     if (c->h == 1) {
         if (c->ngp == 8)
             ASM(1, 8);
@@ -1357,6 +1374,10 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->all_iso = true;
+    for (int p = 0; p < c->nphases; ++p)
+        if (c->phase_kind[p] != (c->h == 1 ? FANS_MODEL_THERMAL_ISO : FANS_MODEL_ELASTIC_ISO))
+            c->all_iso = false;
     c->have_mat = true;
     return 0;
 }
@@ -1846,9 +1867,9 @@ static int fft_standalone(int dev, int nx, int ny, int nz, int h, const double *
             hipLaunchKernelGGL((k_fft_strided<double, false>), dim3((Kc + P.bc - 1) / P.bc, nx), dim3(kThreads), P.smy,
                                c->stream, c->spec64, P.Ly, P.twy, (int64_t) Kc, Kc, (int64_t) ny * Kc, P.bc, nullptr);
             // x pass: strided with columns = the whole (y, kz, h) block
-            hipLaunchKernelGGL((k_fft_strided<double, false>), dim3((ny * Kc + P.bc - 1) / P.bc, 1), dim3(kThreads),
-                               (size_t) 2 * sizeof(cplx<double>) * nx * P.bc, c->stream, c->spec64, P.Lx, P.twx,
-                               (int64_t) ny * Kc, ny * Kc, (int64_t) 0, P.bc, nullptr);
+            hipLaunchKernelGGL((k_fft_strided<double, false>), dim3((ny * Kc + P.bcx - 1) / P.bcx, 1), dim3(kThreads),
+                               P.smx_plain, c->stream, c->spec64, P.Lx, P.twx, (int64_t) ny * Kc, ny * Kc,
+                               (int64_t) 0, P.bcx, nullptr);
             rc = hipMemcpy(out, c->spec64, sizeof(cplx<double>) * nc / 1, hipMemcpyDeviceToHost) == hipSuccess
                      ? 0
                      : FANS_GPU_EHIP;
@@ -1857,9 +1878,9 @@ static int fft_standalone(int dev, int nx, int ny, int nz, int h, const double *
         rc = hipMemcpy(c->spec64, in, sizeof(cplx<double>) * nc, hipMemcpyHostToDevice) == hipSuccess ? 0
                                                                                                       : FANS_GPU_EHIP;
         if (!rc) {
-            hipLaunchKernelGGL((k_fft_strided<double, true>), dim3((ny * Kc + P.bc - 1) / P.bc, 1), dim3(kThreads),
-                               (size_t) 2 * sizeof(cplx<double>) * nx * P.bc, c->stream, c->spec64, P.Lx, P.twx,
-                               (int64_t) ny * Kc, ny * Kc, (int64_t) 0, P.bc, nullptr);
+            hipLaunchKernelGGL((k_fft_strided<double, true>), dim3((ny * Kc + P.bcx - 1) / P.bcx, 1), dim3(kThreads),
+                               P.smx_plain, c->stream, c->spec64, P.Lx, P.twx, (int64_t) ny * Kc, ny * Kc,
+                               (int64_t) 0, P.bcx, nullptr);
             hipLaunchKernelGGL((k_fft_strided<double, true>), dim3((Kc + P.bc - 1) / P.bc, nx), dim3(kThreads), P.smy,
                                c->stream, c->spec64, P.Ly, P.twy, (int64_t) Kc, Kc, (int64_t) ny * Kc, P.bc, nullptr);
             EpiArgs<double> ep{};
