@@ -30,6 +30,7 @@ struct AsmArgs {
     Geom            geom;
     const uint8_t  *ms;        // phase ids [x][y][z]
     const Phase<T> *phases;
+    int             nphases;
     J2State         js;
     // ASM_RESID: in = u, out = r; norm/dot partials
     // ASM_CGLIN: in = s, din = d_cur, dout = d_next (d = s + beta d), out = rnew
@@ -45,27 +46,21 @@ struct AsmArgs {
     const int      *done;
 };
 
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous range of tiles so halo re-reads of neighbouring tiles hit the
-// same L2.
-__device__ __forceinline__ int xcd_tile_index(int bid, int nb)
-{
-    const int q = nb / 8, r = nb % 8, k = bid % 8, p = bid / 8;
-    return k * q + min(k, r) + p;
-}
-
 template <typename T, int H>
 __device__ __forceinline__ T shfl_up16(T v)
 {
     return __shfl_up(v, 16, 64);
 }
 
-// ISO: every phase is LinearElasticIsotropic (H=3) or LinearThermalIsotropic
-// (H=1) and the collapsed element arithmetic is used; otherwise the general
-// per-Gauss-point constitutive path.
-template <typename T, int H, int NGP, int MODE, bool ISO>
+// VAR: ASM_GENERAL = per-Gauss-point constitutive path (any model);
+// ASM_ISO = every phase LinearElasticIsotropic (H=3) or LinearThermalIsotropic
+// (H=1), modal element arithmetic; ASM_ISO_CUBIC = the same on cubic voxels.
+enum { ASM_GENERAL = 0, ASM_ISO = 1, ASM_ISO_CUBIC = 2 };
+
+template <typename T, int H, int NGP, int MODE, int VAR>
 __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
 {
+    constexpr bool ISO = VAR != ASM_GENERAL, CUBIC = VAR == ASM_ISO_CUBIC;
     if (a.done && *a.done)
         return;
     constexpr int NL = TILE + 1;  // node tile edge
@@ -102,29 +97,6 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
     if constexpr (MODE == ASM_CGLIN)
         beta = (T) a.scal[0];
 
-    auto load_node = [&](int64_t n, int slot, int buf, bool wr) {
-#pragma unroll
-        for (int c = 0; c < H; ++c) {
-            T val;
-            if constexpr (MODE == ASM_CGLIN) {
-                val = a.in[n * H + c] + beta * a.din[n * H + c];
-                if (wr)
-                    a.dout[n * H + c] = val;
-            } else {
-                val = a.in[n * H + c];
-            }
-            Un[buf][slot * H + c] = val;
-        }
-    };
-    auto load_plane = [&](int xp, int buf) {
-        const int     xg  = xp < 0 ? xp + nx : (xp >= nx ? xp - nx : xp);
-        const bool    wr  = (MODE == ASM_CGLIN) && xp >= X0 && xp < X1 && own_yz;
-        const int64_t pb  = xg * plane;
-        load_node(pb + off_own, ey * NL + ez, buf, wr);
-        if (ext >= 0)
-            load_node(pb + off_ext, e_a * NL + e_b, buf, false);
-    };
-
     T      carry[H];
     double acc0 = 0.0, acc1 = 0.0;
 #pragma unroll
@@ -135,11 +107,73 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
     for (int i = 0; i < NS; ++i)
         g0[i] = (MODE == ASM_RESID) ? a.g0[i] : (T) 0;
     const T w = (T) a.geom.w;
+    // isotropic moduli of every phase, staged once per block
+    __shared__ T PL[256][2];
+    if constexpr (ISO) {
+        for (int p = t; p < a.nphases; p += blockDim.x) {
+            PL[p][0] = a.phases[p].p[0];
+            PL[p][1] = H == 3 ? a.phases[p].p[1] : (T) 0;
+        }
+    }
 
-    load_plane(X0 - 1, 0);
+    // node plane x+1: fetch (global) then commit (d update, LDS); the phase id
+    // of the next element plane is loaded one plane ahead
+    T pf_own[MODE == ASM_CGLIN ? 2 : 1][H], pf_ext[MODE == ASM_CGLIN ? 2 : 1][H];
+    auto fetch_plane = [&](int xp) {
+        const int     xg = xp < 0 ? xp + nx : (xp >= nx ? xp - nx : xp);
+        const int64_t pb = xg * plane;
+#pragma unroll
+        for (int c = 0; c < H; ++c) {
+            pf_own[0][c] = a.in[(pb + off_own) * H + c];
+            if constexpr (MODE == ASM_CGLIN)
+                pf_own[1][c] = a.din[(pb + off_own) * H + c];
+        }
+        if (ext >= 0)
+#pragma unroll
+            for (int c = 0; c < H; ++c) {
+                pf_ext[0][c] = a.in[(pb + off_ext) * H + c];
+                if constexpr (MODE == ASM_CGLIN)
+                    pf_ext[1][c] = a.din[(pb + off_ext) * H + c];
+            }
+    };
+    auto commit_plane = [&](int xp, int buf) {
+        const int     xg = xp < 0 ? xp + nx : (xp >= nx ? xp - nx : xp);
+        const bool    wr = (MODE == ASM_CGLIN) && xp >= X0 && xp < X1 && own_yz;
+        const int64_t n  = xg * plane + off_own;
+#pragma unroll
+        for (int c = 0; c < H; ++c) {
+            T val = pf_own[0][c];
+            if constexpr (MODE == ASM_CGLIN) {
+                val = val + beta * pf_own[1][c];
+                if (wr)
+                    a.dout[n * H + c] = val;
+            }
+            Un[buf][(ey * NL + ez) * H + c] = val;
+        }
+        if (ext >= 0)
+#pragma unroll
+            for (int c = 0; c < H; ++c) {
+                T val = pf_ext[0][c];
+                if constexpr (MODE == ASM_CGLIN)
+                    val = val + beta * pf_ext[1][c];
+                Un[buf][(e_a * NL + e_b) * H + c] = val;
+            }
+    };
+    auto phase_of = [&](int xe) {
+        const int xg = xe < 0 ? xe + nx : xe;
+        return (int) a.ms[xg * plane + off_own];
+    };
+
+    fetch_plane(X0 - 1);
+    commit_plane(X0 - 1, 0);
+    int ph_next = phase_of(X0 - 1);
     for (int i = 0; X0 - 1 + i < X1; ++i) {
         const int xe = X0 - 1 + i;
-        load_plane(xe + 1, (i + 1) & 1);
+        fetch_plane(xe + 1);
+        commit_plane(xe + 1, (i + 1) & 1);
+        const int ph = ph_next;
+        if (xe + 1 < X1)
+            ph_next = phase_of(xe + 1);
         __syncthreads();
         T u[8][H];
 #pragma unroll
@@ -154,13 +188,12 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
 #pragma unroll
         for (int c = 0; c < H; ++c)
             dv[c] = u[0][c];
-        const int       xg = xe < 0 ? xe + nx : xe;
-        const int64_t   el = xg * plane + off_own;
-        const Phase<T> &P  = a.phases[a.ms[el]];
-        T f[8][H];
+        const int     xg = xe < 0 ? xe + nx : xe;
+        const int64_t el = xg * plane + off_own;
+        T             f[8][H];
         if constexpr (ISO) {
             if constexpr (H == 3) {
-                const T lw = P.p[0] * w, mw = P.p[1] * w;
+                const T lw = PL[ph][0] * w, mw = PL[ph][1] * w;
                 T       T0[3][3];
                 if constexpr (MODE == ASM_RESID) {
                     const T r = (T) kSqrtHalf, tr = g0[0] + g0[1] + g0[2];
@@ -178,16 +211,19 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
                         for (int q = 0; q < 3; ++q)
                             T0[p][q] = (T) 0;
                 }
-                element_forces_iso3<T, NGP>(reinterpret_cast<const T(&)[8][3]>(u), a.geom, lw, mw, T0,
+                element_forces_iso3_modal<T, NGP, CUBIC>(reinterpret_cast<const T(&)[8][3]>(u), a.geom, lw, mw, T0,
                                             reinterpret_cast<T(&)[8][3]>(f));
             } else {
-                const T kw    = P.p[0] * w;
+                const T kw    = PL[ph][0] * w;
                 const T T0[3] = {kw * g0[0], kw * g0[1], kw * g0[2]};
-                element_forces_iso1<T, NGP>(reinterpret_cast<const T(&)[8][1]>(u), a.geom, kw, T0,
+                element_forces_iso1_modal<T, NGP, CUBIC>(reinterpret_cast<const T(&)[8][1]>(u), a.geom, kw, T0,
                                             reinterpret_cast<T(&)[8][1]>(f));
             }
         } else {
-            auto mat = [&](int gp, const T(&e)[NS], T(&s)[NS]) { material_point<T, H>(P, e, s, a.js, el, gp, NGP); };
+            const Phase<T> &P   = a.phases[ph];
+            auto            mat = [&](int gp, const T(&e)[NS], T(&s)[NS]) {
+                material_point<T, H>(P, e, s, a.js, el, gp, NGP);
+            };
             element_forces<T, H, NGP>(u, a.geom, g0, mat, f);
         }
         // z: node (ey,ez) <- az=0 corners of element ez + az=1 corners of element ez-1

@@ -97,4 +97,27 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], const bool (&is_ma
     }
 }
 
+// accumulate |v| into a norm: measure 0 = L1, 1 = L2 (squared), 2 = Linf
+template <typename T>
+__device__ __forceinline__ void acc_norm(double &acc, T v, int measure)
+{
+    const double a = fabs((double) v);
+    if (measure == 2)
+        acc = fmax(acc, a);
+    else if (measure == 1)
+        acc += a * a;
+    else
+        acc += a;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Map the hardware block id so that each XCD gets a contiguous range
+// of logical blocks: neighbouring blocks that share cache lines (halo nodes,
+// partial lines of narrow column groups) then meet in the same L2.
+__device__ __forceinline__ int xcd_tile_index(int bid, int nb)
+{
+    const int q = nb / 8, r = nb % 8, k = bid % 8, p = bid / 8;
+    return k * q + min(k, r) + p;
+}
+
 }  // namespace fans

@@ -462,6 +462,227 @@ __device__ __forceinline__ void element_forces_iso1(const T (&u)[8][1], const Ge
     grad_factors_T<T, 1, NGP>(Sx, Sy, Sz, g, f);
 }
 
+// ---------------------------------------------------------------------------
+// Modal form of the isotropic element operators.
+//
+// With reference coordinates (xi, eta, zeta) in [-1,1]^3, the derivative of the
+// trilinear field along x is bilinear in (eta, zeta):
+//   du/dx = kx/4 (A0 + A1 eta + A2 zeta + A3 eta zeta),  kx = 1/hx,
+// where A = H Dx is the 4-point Hadamard transform of the four x-edge
+// differences (likewise for y over (xi, zeta) and z over (xi, eta)).  The
+// products of two such fields have degree <= 3 in every coordinate, so the
+// 2x2x2 Gauss rule integrates the energy density exactly, and the modes
+// {1, xi, eta, zeta, xi eta, ...} are orthogonal under it (norms 8 / 3^deg).
+// The element energy lambda/2 tr(eps)^2 + mu eps:eps therefore splits into
+// independent modal blocks:
+//   mode 1        : all nine gradient entries
+//   mode xi       : d/dy, d/dz   (eta: d/dx, d/dz; zeta: d/dx, d/dy)
+//   mode eta zeta : d/dx only    (xi zeta: d/dy; xi eta: d/dz)
+// and the forces are the transposed Hadamard / edge-difference maps applied to
+// the modal stresses.  This is the same operator as the Gauss-point loop (the
+// quadrature is exact for it) at about 360 instead of ~600 FP64 operations per
+// element, with a third of the live registers.  For one Gauss point (HEX8R)
+// only mode 1 remains.
+// ---------------------------------------------------------------------------
+
+// forward: D[p][q] (p, q in {0,1}) -> A0..A3
+template <typename T>
+__device__ __forceinline__ void hadamard4(T &d00, T &d10, T &d01, T &d11)
+{
+    const T s0 = d00 + d01, s1 = d10 + d11, e0 = d01 - d00, e1 = d11 - d10;
+    d00 = s0 + s1;  // A0
+    d10 = s1 - s0;  // A1 (slope in p)
+    d01 = e0 + e1;  // A2 (slope in q)
+    d11 = e1 - e0;  // A3 (p q)
+}
+// transpose: A0..A3 -> D[p][q]
+template <typename T>
+__device__ __forceinline__ void hadamard4_T(T &a0, T &a1, T &a2, T &a3)
+{
+    const T p0 = a0 - a2, p1 = a1 - a3, q0 = a0 + a2, q1 = a1 + a3;
+    a0 = p0 - p1;  // D00
+    a1 = p0 + p1;  // D10
+    a2 = q0 - q1;  // D01
+    a3 = q0 + q1;  // D11
+}
+
+// modal block with two derivative directions dA, dB present (third missing):
+// a[c] = coefficient of d u_c / d x_dA, b[c] of d u_c / d x_dB
+template <typename T, int dA, int dB>
+__device__ __forceinline__ void modal_pair(T (&a)[3], T (&b)[3], T L, T M)
+{
+    constexpr int c0 = 3 - dA - dB;
+    const T       t  = L * (a[dA] + b[dB]);
+    const T       m2 = M + M;
+    const T       off = M * (a[dB] + b[dA]);
+    a[dA]            = t + m2 * a[dA];
+    b[dB]            = t + m2 * b[dB];
+    a[dB]            = off;
+    b[dA]            = off;
+    a[c0] *= M;
+    b[c0] *= M;
+}
+
+template <typename T, int d>
+__device__ __forceinline__ void modal_single(T (&a)[3], T L, T M)
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        a[c] *= (c == d) ? (L + M + M) : M;
+}
+
+// f = Ke(lambda, mu) u (+ the forces of the traction T0 = w sigma(g0)); same
+// arguments as element_forces_iso3.  CUBIC (hx = hy = hz): the 1/h of the edge
+// differences and of their transpose fold into the modal constants (k^2) and
+// the traction term (k), which saves 72 multiplications per element.
+template <typename T, int NGP, bool CUBIC = false>
+__device__ __forceinline__ void element_forces_iso3_modal(const T (&u)[8][3], const Geom &g, T lw, T mw,
+                                                          const T (&T0)[3][3], T (&f)[8][3])
+{
+    const T kc   = (T) g.ilx;
+    const T k[3] = {CUBIC ? (T) 1 : (T) g.ilx, CUBIC ? (T) 1 : (T) g.ily, CUBIC ? (T) 1 : (T) g.ilz};
+    // A[d][mode][c]; edge differences along d (scaled by 1/h_d), then Hadamard
+    T A[3][4][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                A[0][p + 2 * q][c] = (u[1 + 2 * p + 4 * q][c] - u[2 * p + 4 * q][c]) * k[0];
+                A[1][p + 2 * q][c] = (u[p + 2 + 4 * q][c] - u[p + 4 * q][c]) * k[1];
+                A[2][p + 2 * q][c] = (u[p + 2 * q + 4][c] - u[p + 2 * q][c]) * k[2];
+            }
+    // Gauss weight w = V / NGP; mode weights NGP / 3^deg; 1/16 from the two
+    // factors 1/4 of the bilinear expansion
+    const T s  = CUBIC ? (T) NGP / (T) 16 * kc * kc : (T) NGP / (T) 16;
+    const T L0 = s * lw, M0 = s * mw;
+    if constexpr (NGP == 8) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                hadamard4(A[d][0][c], A[d][1][c], A[d][2][c], A[d][3][c]);
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                A[d][0][c] = (A[d][0][c] + A[d][2][c]) + (A[d][1][c] + A[d][3][c]);
+    }
+    // mode 1: all nine entries
+    {
+        const T t  = L0 * (A[0][0][0] + A[1][0][1] + A[2][0][2]);
+        const T m2 = M0 + M0;
+        const T t0 = CUBIC ? (T) NGP * (T) 0.25 * kc : (T) NGP * (T) 0.25;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            A[d][0][d] = t + m2 * A[d][0][d] + t0 * T0[d][d];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = d + 1; c < 3; ++c) {
+                // entry (c, d) holds du_c/dx_d, entry (d, c) du_d/dx_c
+                const T v  = M0 * (A[d][0][c] + A[c][0][d]);
+                A[d][0][c] = v + t0 * T0[c][d];
+                A[c][0][d] = v + t0 * T0[d][c];
+            }
+    }
+    if constexpr (NGP == 8) {
+        const T third = (T) (1.0 / 3.0);
+        const T L1 = L0 * third, M1 = M0 * third, L2 = L1 * third, M2 = M1 * third;
+        // xi: d/dy mode 1, d/dz mode 1; eta: d/dx mode 1, d/dz mode 2; zeta: d/dx mode 2, d/dy mode 2
+        modal_pair<T, 1, 2>(A[1][1], A[2][1], L1, M1);
+        modal_pair<T, 0, 2>(A[0][1], A[2][2], L1, M1);
+        modal_pair<T, 0, 1>(A[0][2], A[1][2], L1, M1);
+        modal_single<T, 0>(A[0][3], L2, M2);  // eta zeta
+        modal_single<T, 1>(A[1][3], L2, M2);  // xi zeta
+        modal_single<T, 2>(A[2][3], L2, M2);  // xi eta
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                hadamard4_T(A[d][0][c], A[d][1][c], A[d][2][c], A[d][3][c]);
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                A[d][1][c] = A[d][2][c] = A[d][3][c] = A[d][0][c];
+    }
+    // transposed edge differences
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+            f[n][c] = (T) 0;
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const T dx = A[0][p + 2 * q][c] * k[0], dy = A[1][p + 2 * q][c] * k[1],
+                        dz = A[2][p + 2 * q][c] * k[2];
+                f[1 + 2 * p + 4 * q][c] += dx;
+                f[2 * p + 4 * q][c] -= dx;
+                f[p + 2 + 4 * q][c] += dy;
+                f[p + 4 * q][c] -= dy;
+                f[p + 2 * q + 4][c] += dz;
+                f[p + 2 * q][c] -= dz;
+            }
+    }
+}
+
+// isotropic conduction in modal form: every mode of every derivative is its
+// own block, weighted by NGP / 3^deg
+template <typename T, int NGP, bool CUBIC = false>
+__device__ __forceinline__ void element_forces_iso1_modal(const T (&u)[8][1], const Geom &g, T kw, const T (&T0)[3],
+                                                          T (&f)[8][1])
+{
+    const T kc   = (T) g.ilx;
+    const T k[3] = {CUBIC ? (T) 1 : (T) g.ilx, CUBIC ? (T) 1 : (T) g.ily, CUBIC ? (T) 1 : (T) g.ilz};
+    T       A[3][4];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            A[0][p + 2 * q] = (u[1 + 2 * p + 4 * q][0] - u[2 * p + 4 * q][0]) * k[0];
+            A[1][p + 2 * q] = (u[p + 2 + 4 * q][0] - u[p + 4 * q][0]) * k[1];
+            A[2][p + 2 * q] = (u[p + 2 * q + 4][0] - u[p + 2 * q][0]) * k[2];
+        }
+    const T c0 = CUBIC ? (T) NGP / (T) 16 * kw * kc * kc : (T) NGP / (T) 16 * kw;
+    const T t0 = CUBIC ? (T) NGP * (T) 0.25 * kc : (T) NGP * (T) 0.25;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if constexpr (NGP == 8) {
+            const T c1 = c0 * (T) (1.0 / 3.0), c2 = c1 * (T) (1.0 / 3.0);
+            hadamard4(A[d][0], A[d][1], A[d][2], A[d][3]);
+            A[d][0] = c0 * A[d][0] + t0 * T0[d];
+            A[d][1] *= c1;
+            A[d][2] *= c1;
+            A[d][3] *= c2;
+            hadamard4_T(A[d][0], A[d][1], A[d][2], A[d][3]);
+        } else {
+            const T a = c0 * ((A[d][0] + A[d][2]) + (A[d][1] + A[d][3])) + t0 * T0[d];
+            A[d][0] = A[d][1] = A[d][2] = A[d][3] = a;
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+        f[n][0] = (T) 0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const T dx = A[0][p + 2 * q] * k[0], dy = A[1][p + 2 * q] * k[1], dz = A[2][p + 2 * q] * k[2];
+            f[1 + 2 * p + 4 * q][0] += dx;
+            f[2 * p + 4 * q][0] -= dx;
+            f[p + 2 + 4 * q][0] += dy;
+            f[p + 4 * q][0] -= dy;
+            f[p + 2 * q + 4][0] += dz;
+            f[p + 2 * q][0] -= dz;
+        }
+}
+
 // lane l <- lane l-1 inside each row of 16 lanes (DPP row_shr:1); row starts get 0
 __device__ __forceinline__ double dpp_row_shr1(double v)
 {

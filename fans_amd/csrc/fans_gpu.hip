@@ -22,6 +22,7 @@
 #include "common.h"
 #include "element_kernels.h"
 #include "fft_kernels.h"
+#include "fft_reg_kernels.h"
 #include "green_kernels.h"
 #include "vec_kernels.h"
 
@@ -44,11 +45,12 @@ enum {
     KF_REDUCE,
     KF_VECTOR,
     KF_ELEMENT,
+    KF_CG_FFT_Z_FWD,  // CG update fused into the forward z pass
     KF_COUNT
 };
-const char *kFamilyNames[KF_COUNT] = {"fft_z_r2c",    "fft_y_fwd",    "fft_x_green", "fft_y_inv",
+const char *kFamilyNames[KF_COUNT] = {"fft_z_r2c",    "fft_y_fwd",    "fft_x_green",  "fft_y_inv",
                                       "fft_z_c2r",    "assemble_lin", "assemble_res", "cg_update",
-                                      "reduce",       "vector",       "element_pass"};
+                                      "reduce",       "vector",       "element_pass", "cg_update_fft_z_r2c"};
 
 int nparams(int kind)
 {
@@ -105,6 +107,18 @@ bool make_len(int n, int tws, FftLen &L)
     return m == 1;
 }
 
+// lines per block of the compile-time z kernels (0: generic kernels)
+int zct_lines(int h, int M)
+{
+    switch (M) {
+    case 128: return h == 1 ? zct_lpb<1, 128>() : zct_lpb<3, 128>();
+    case 64: return h == 1 ? zct_lpb<1, 64>() : zct_lpb<3, 64>();
+    case 32: return h == 1 ? zct_lpb<1, 32>() : zct_lpb<3, 32>();
+    case 16: return h == 1 ? zct_lpb<1, 16>() : zct_lpb<3, 16>();
+    default: return 0;
+    }
+}
+
 template <typename T>
 struct Fft3D {
     int           nx = 0, ny = 0, nz = 0, h = 0, M = 0, Mc = 0;
@@ -122,6 +136,7 @@ struct Fft3D {
         return v;
     }
     int64_t       nblk_z = 0;
+    bool          zct    = false;  // compile-time z kernels in use
     bool          ready  = false;
 
     hipError_t init(int nx_, int ny_, int nz_, int h_)
@@ -149,17 +164,24 @@ struct Fft3D {
         const size_t cb = sizeof(cplx<T>);
         if ((int64_t) M * h > FFT_MAX_ELEMS || ny > FFT_MAX_ELEMS || (int64_t) nx * h > FFT_MAX_ELEMS)
             return hipErrorInvalidValue;
-        lpb          = std::max(1, std::min(16, FFT_MAX_ELEMS / (M * h)));
-        smz          = cb * M * (lpb * h + 1);
-        const int Kc = Mc * h;
-        bc           = pow2_cols(FFT_MAX_ELEMS / ny, std::min(Kc, 16));
-        smy          = cb * ny * (bc + 1);
-        bk           = std::max(1, std::min(Mc, FFT_MAX_ELEMS / (nx * h)));
-        smx          = cb * nx * (bk * h + 1);
-        bcx          = pow2_cols(FFT_MAX_ELEMS / nx, 16);
+        // ~40 KiB of LDS per workgroup keeps four workgroups resident per CU
+        const int lds_elems = (int) (40 * 1024 / cb);
+        const int lim       = std::min(FFT_MAX_ELEMS, lds_elems);
+        lpb                 = std::max(1, std::min(16, lim / (M * h)));
+        smz                 = cb * M * (lpb * h + 1);
+        const int Kc        = Mc * h;
+        bc                  = pow2_cols(lim / ny, std::min(Kc, 16));
+        smy                 = cb * ny * (bc + 1);
+        bk                  = std::max(1, std::min(Mc, lim / (nx * h)));
+        smx                 = cb * nx * (bk * h + 1);
+        bcx                 = pow2_cols(lim / nx, 16);
         smx_plain    = cb * nx * (bcx + 1);
         nblk_z       = ((int64_t) nx * ny + lpb - 1) / lpb;
-        ready        = true;
+        const int lct = zct_lines(h, M);
+        zct           = lct > 0 && ((int64_t) nx * ny) % lct == 0;
+        if (zct)
+            nblk_z = (int64_t) nx * ny / lct;
+        ready = true;
         return hipSuccess;
     }
     void release()
@@ -188,6 +210,7 @@ struct fans_gpu_ctx {
     int                         nphases = 0;
     std::vector<int>            phase_kind;
     bool                        all_linear = true, has_j2 = false, all_iso = false;
+    bool                        cubic = false;  // hx == hy == hz
     double                      kref[36]{};
     double                      keref[576]{};
     std::vector<Phase<double>>  ph64;
@@ -236,6 +259,7 @@ struct fans_gpu_ctx {
 
     // timing
     bool                                                timing = false;
+    int                                                 fft_chunks = 1;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
     std::vector<hipEvent_t>                             ev_pool;
     double                                              fam_ms[KF_COUNT]{};
@@ -312,67 +336,208 @@ void harvest_timing(fans_gpu_ctx *c)
     c->ev_used.clear();
 }
 
+// forget the timing records from index `from` on (launches that returned at entry)
+void drop_timing(fans_gpu_ctx *c, size_t from)
+{
+    for (size_t k = from; k < c->ev_used.size(); ++k) {
+        c->ev_pool.push_back(c->ev_used[k].second.first);
+        c->ev_pool.push_back(c->ev_used[k].second.second);
+    }
+    if (from < c->ev_used.size())
+        c->ev_used.resize(from);
+}
+
 int grid_vec(int64_t n) { return (int) std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, 2048)); }
+
+// ---------------------------------------------------------------------------
+// power-of-two lengths 16..256 use the register four-step kernels
+// ---------------------------------------------------------------------------
+bool reg_len(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256; }
+
+// columns per block: a power of two (>= 4) with ~36 KiB of LDS, at most 256 threads
+template <typename T, int N1, int N2>
+constexpr int reg_cols()
+{
+    constexpr int n   = N1 * N2;
+    constexpr int lim = (int) (36 * 1024 / (n * sizeof(cplx<T>)));
+    int           c   = 4;
+    while (c * 2 <= lim && c * 2 * N2 <= 256)
+        c *= 2;
+    return c;
+}
+
+template <typename T, bool INV, int N1, int N2>
+void launch_strided_reg(fans_gpu_ctx *c, cplx<T> *spec, const cplx<T> *tw, int64_t sn, int c0, int c1,
+                        int64_t ostride, int nouter, const int *done)
+{
+    constexpr int C   = reg_cols<T, N1, N2>();
+    const int     nbx = (c1 - c0 + C - 1) / C;
+    hipLaunchKernelGGL((k_fft_strided_reg<T, N1, N2, C, INV>), dim3(nbx * nouter), dim3(C * N2), 0, c->stream, spec,
+                       tw, sn, c0, c1, ostride, nbx, done);
+}
+
+template <typename T, bool INV>
+void launch_strided(fans_gpu_ctx *c, const FftLen &L, const cplx<T> *tw, int bc, size_t smem, cplx<T> *spec,
+                    int64_t sn, int c0, int c1, int64_t ostride, int nouter, const int *done)
+{
+    switch (reg_len(L.n) ? L.n : 0) {
+    case 256: launch_strided_reg<T, INV, 16, 16>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;
+    case 128: launch_strided_reg<T, INV, 16, 8>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;
+    case 64: launch_strided_reg<T, INV, 8, 8>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;
+    case 32: launch_strided_reg<T, INV, 8, 4>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;
+    case 16: launch_strided_reg<T, INV, 4, 4>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;
+    default:
+        hipLaunchKernelGGL((k_fft_strided<T, INV>), dim3((c1 - c0 + bc - 1) / bc, nouter), dim3(kThreads), smem,
+                           c->stream, spec, L, tw, sn, c0, c1, ostride, bc, done);
+    }
+}
+
+template <typename T, int H, int N1, int N2>
+void launch_x_green_reg(fans_gpu_ctx *c, cplx<T> *spec, const cplx<T> *tw, const T *G, int Mc, int kz_lo, int kz_hi,
+                        const int *done)
+{
+    constexpr int HP = H == 3 ? 4 : 1;
+    constexpr int KB  = reg_cols<T, N1, N2>() / HP;
+    const int     nbk = (kz_hi - kz_lo + KB - 1) / KB;
+    hipLaunchKernelGGL((k_fft_x_green_reg<T, H, N1, N2, KB>), dim3(nbk * c->ny), dim3(KB * HP * N2), 0, c->stream,
+                       spec, tw, G, c->ny, Mc, kz_lo, kz_hi, nbk, done);
+}
+
+template <typename T, int H>
+void launch_x_green(fans_gpu_ctx *c, Fft3D<T> &P, cplx<T> *spec, const T *G, int kz_lo, int kz_hi, const int *done)
+{
+    switch (reg_len(P.Lx.n) ? P.Lx.n : 0) {
+    case 256: launch_x_green_reg<T, H, 16, 16>(c, spec, P.twx, G, P.Mc, kz_lo, kz_hi, done); break;
+    case 128: launch_x_green_reg<T, H, 16, 8>(c, spec, P.twx, G, P.Mc, kz_lo, kz_hi, done); break;
+    case 64: launch_x_green_reg<T, H, 8, 8>(c, spec, P.twx, G, P.Mc, kz_lo, kz_hi, done); break;
+    case 32: launch_x_green_reg<T, H, 8, 4>(c, spec, P.twx, G, P.Mc, kz_lo, kz_hi, done); break;
+    case 16: launch_x_green_reg<T, H, 4, 4>(c, spec, P.twx, G, P.Mc, kz_lo, kz_hi, done); break;
+    default: {
+        dim3 g((kz_hi - kz_lo + P.bk - 1) / P.bk, c->ny);
+        hipLaunchKernelGGL((k_fft_x_green<T, H>), g, dim3(kThreads), P.smx, c->stream, spec, P.Lx, P.twx, G, c->ny,
+                           P.Mc, kz_lo, kz_hi, P.bk, done);
+    }
+    }
+}
 
 // ---------------------------------------------------------------------------
 // FFT convolution: out = G * r  with the requested epilogue
 // ---------------------------------------------------------------------------
-template <typename T>
-int convolve_dev(fans_gpu_ctx *c, Fft3D<T> &P, const T *rin, cplx<T> *spec, const T *Gp, int epi, EpiArgs<T> ep,
-                 const int *done)
+int reduce(fans_gpu_ctx *c, const double *partial, int nblk, int nq, int stage);
+
+// z passes: compile-time kernels where M = nz/2 is a power of two in 16..128
+// and the line count divides evenly, else the generic ones
+template <typename T, int H, int PRO>
+void launch_z_r2c(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const ProArgs<T> &pa, const int *done)
 {
-    const int     h = c->h, Kc = P.Mc * h;
     const int64_t nlines = (int64_t) c->nx * c->ny;
-    {
-        Timed t(c, KF_FFT_Z_FWD);
-        if (h == 1)
-            hipLaunchKernelGGL((k_fft_z_r2c<T, 1>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, rin, spec, P.Lz,
-                               P.twz, c->nz, nlines, P.lpb, done);
-        else
-            hipLaunchKernelGGL((k_fft_z_r2c<T, 3>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, rin, spec, P.Lz,
-                               P.twz, c->nz, nlines, P.lpb, done);
+#define ZF(MM)                                                                                                  \
+    hipLaunchKernelGGL((k_fft_z_r2c_ct<T, H, MM, PRO>), dim3((unsigned) (nlines / zct_lpb<H, MM>())),           \
+                       dim3(zct_threads<H, MM>()), 0, c->stream, rin, spec, P.twz, pa, done)
+    switch (P.zct ? P.M : 0) {
+    case 128: ZF(128); break;
+    case 64: ZF(64); break;
+    case 32: ZF(32); break;
+    case 16: ZF(16); break;
+    default:
+        hipLaunchKernelGGL((k_fft_z_r2c<T, H, PRO>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, rin, spec,
+                           P.Lz, P.twz, c->nz, nlines, P.lpb, pa, done);
     }
-    {
-        Timed t(c, KF_FFT_Y_FWD);
-        hipLaunchKernelGGL((k_fft_strided<T, false>), dim3((Kc + P.bc - 1) / P.bc, c->nx), dim3(kThreads), P.smy,
-                           c->stream, spec, P.Ly, P.twy, (int64_t) Kc, Kc, (int64_t) c->ny * Kc, P.bc, done);
+#undef ZF
+}
+
+template <typename T, int H, int EPI>
+void launch_z_c2r(fans_gpu_ctx *c, Fft3D<T> &P, const cplx<T> *spec, const EpiArgs<T> &ep, const int *done)
+{
+    const int64_t nlines = (int64_t) c->nx * c->ny;
+#define ZI(MM)                                                                                                  \
+    hipLaunchKernelGGL((k_fft_z_c2r_ct<T, H, MM, EPI>), dim3((unsigned) (nlines / zct_lpb<H, MM>())),           \
+                       dim3(zct_threads<H, MM>()), 0, c->stream, spec, P.twz, ep, done)
+    switch (P.zct ? P.M : 0) {
+    case 128: ZI(128); break;
+    case 64: ZI(64); break;
+    case 32: ZI(32); break;
+    case 16: ZI(16); break;
+    default:
+        hipLaunchKernelGGL((k_fft_z_c2r<T, H, EPI>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, spec, P.Lz,
+                           P.twz, c->nz, nlines, P.lpb, ep, done);
     }
+#undef ZI
+}
+
+// ---------------------------------------------------------------------------
+// FFT convolution: out = G * r  with the requested epilogue.  With `pro` the
+// forward z pass first applies the previous CG update to r (and u) and the
+// RED_UPDATE reduction follows it.
+// ---------------------------------------------------------------------------
+template <typename T>
+int convolve_dev(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const T *Gp, int epi, EpiArgs<T> ep,
+                 const int *done, const ProArgs<T> *pro = nullptr)
+{
+    const int h = c->h, Kc = P.Mc * h;
     {
-        Timed t(c, KF_FFT_X_GREEN);
-        dim3  g((P.Mc + P.bk - 1) / P.bk, c->ny);
-        if (h == 1)
-            hipLaunchKernelGGL((k_fft_x_green<T, 1>), g, dim3(kThreads), P.smx, c->stream, spec, P.Lx, P.twx, Gp,
-                               c->ny, P.Mc, P.bk, done);
-        else
-            hipLaunchKernelGGL((k_fft_x_green<T, 3>), g, dim3(kThreads), P.smx, c->stream, spec, P.Lx, P.twx, Gp,
-                               c->ny, P.Mc, P.bk, done);
+        Timed            t(c, pro ? KF_CG_FFT_Z_FWD : KF_FFT_Z_FWD);
+        const ProArgs<T> pa = pro ? *pro : ProArgs<T>{};
+        if (h == 1) {
+            if (pro)
+                launch_z_r2c<T, 1, PRO_CG>(c, P, rin, spec, pa, done);
+            else
+                launch_z_r2c<T, 1, PRO_NONE>(c, P, rin, spec, pa, done);
+        } else {
+            if (pro)
+                launch_z_r2c<T, 3, PRO_CG>(c, P, rin, spec, pa, done);
+            else
+                launch_z_r2c<T, 3, PRO_NONE>(c, P, rin, spec, pa, done);
+        }
     }
-    {
-        Timed t(c, KF_FFT_Y_INV);
-        hipLaunchKernelGGL((k_fft_strided<T, true>), dim3((Kc + P.bc - 1) / P.bc, c->nx), dim3(kThreads), P.smy,
-                           c->stream, spec, P.Ly, P.twy, (int64_t) Kc, Kc, (int64_t) c->ny * Kc, P.bc, done);
+    HIPCHK(c, hipGetLastError());
+    if (pro) {
+        const int rc = reduce(c, pro->partial, (int) P.nblk_z, 2, RED_UPDATE);
+        if (rc)
+            return rc;
+    }
+    // The y / x+Green / y^-1 passes only couple values of equal kz, so they can
+    // run on kz chunks one after the other; a chunk that fits the Infinity
+    // Cache keeps the middle passes out of HBM.
+    const int nch = std::max(1, std::min(c->fft_chunks, P.Mc));
+    for (int q = 0; q < nch; ++q) {
+        const int kz_lo = (int) ((int64_t) P.Mc * q / nch), kz_hi = (int) ((int64_t) P.Mc * (q + 1) / nch);
+        const int c0 = kz_lo * h, c1 = kz_hi * h;
+        {
+            Timed t(c, KF_FFT_Y_FWD);
+            launch_strided<T, false>(c, P.Ly, P.twy, P.bc, P.smy, spec, (int64_t) Kc, c0, c1, (int64_t) c->ny * Kc,
+                                     c->nx, done);
+        }
+        {
+            Timed t(c, KF_FFT_X_GREEN);
+            if (h == 1)
+                launch_x_green<T, 1>(c, P, spec, Gp, kz_lo, kz_hi, done);
+            else
+                launch_x_green<T, 3>(c, P, spec, Gp, kz_lo, kz_hi, done);
+        }
+        {
+            Timed t(c, KF_FFT_Y_INV);
+            launch_strided<T, true>(c, P.Ly, P.twy, P.bc, P.smy, spec, (int64_t) Kc, c0, c1, (int64_t) c->ny * Kc,
+                                    c->nx, done);
+        }
     }
     {
         Timed t(c, KF_FFT_Z_INV);
-#define ZINV(HH, EE)                                                                                              \
-    hipLaunchKernelGGL((k_fft_z_c2r<T, HH, EE>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, spec, P.Lz, \
-                       P.twz, c->nz, nlines, P.lpb, ep, done)
         if (h == 1) {
             if (epi == EPI_STORE)
-                ZINV(1, EPI_STORE);
+                launch_z_c2r<T, 1, EPI_STORE>(c, P, spec, ep, done);
             else if (epi == EPI_CG)
-                ZINV(1, EPI_CG);
+                launch_z_c2r<T, 1, EPI_CG>(c, P, spec, ep, done);
             else
-                ZINV(1, EPI_FP);
+                launch_z_c2r<T, 1, EPI_FP>(c, P, spec, ep, done);
         } else {
             if (epi == EPI_STORE)
-                ZINV(3, EPI_STORE);
+                launch_z_c2r<T, 3, EPI_STORE>(c, P, spec, ep, done);
             else if (epi == EPI_CG)
-                ZINV(3, EPI_CG);
+                launch_z_c2r<T, 3, EPI_CG>(c, P, spec, ep, done);
             else
-                ZINV(3, EPI_FP);
+                launch_z_c2r<T, 3, EPI_FP>(c, P, spec, ep, done);
         }
-#undef ZINV
     }
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -393,6 +558,7 @@ AsmArgs<T> asm_args(fans_gpu_ctx *c)
     a.ntz     = c->ntz;
     a.geom    = c->geom;
     a.ms      = c->ms;
+    a.nphases = c->nphases;
     a.js      = J2State{c->js, c->N};
     a.partial = c->partial;
     a.measure = c->d.measure;
@@ -404,14 +570,17 @@ template <typename T, int MODE>
 int launch_assemble(fans_gpu_ctx *c, AsmArgs<T> a)
 {
     Timed t(c, MODE == ASM_CGLIN ? KF_ASSEMBLE_LIN : KF_ASSEMBLE_RES);
-#define ASM(HH, GG)                                                                                          \
-    do {                                                                                                     \
-        if (c->all_iso)                                                                                      \
-            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, true>), dim3(c->nblk_asm), dim3(kThreads), 0,  \
-                               c->stream, a);                                                                \
-        else                                                                                                 \
-            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, false>), dim3(c->nblk_asm), dim3(kThreads), 0, \
-                               c->stream, a);                                                                \
+#define ASM(HH, GG)                                                                                             \
+    do {                                                                                                        \
+        if (c->all_iso && c->cubic)                                                                             \
+            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_ISO_CUBIC>), dim3(c->nblk_asm), dim3(kThreads), \
+                               0, c->stream, a);                                                                \
+        else if (c->all_iso)                                                                                    \
+            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_ISO>), dim3(c->nblk_asm), dim3(kThreads), 0,    \
+                               c->stream, a);                                                                   \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_GENERAL>), dim3(c->nblk_asm), dim3(kThreads),   \
+                               0, c->stream, a);                                                                \
     } while (0)
     if (c->h == 1) {
         if (c->ngp == 8)
@@ -444,7 +613,7 @@ int residual64(fans_gpu_ctx *c, const double *uin, double *rout, const double *d
 int reduce(fans_gpu_ctx *c, const double *partial, int nblk, int nq, int stage)
 {
     Timed t(c, KF_REDUCE);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, c->stream, partial, nblk, nq, stage, c->sc, c->is,
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kReduceThreads), 0, c->stream, partial, nblk, nq, stage, c->sc, c->is,
                        c->hist);
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -648,21 +817,36 @@ void host_sym_pinv(int n, const double *A, double *P)
 
 // Linear CG fast path (solverCG.h:84-114, islinear && !mixed); every scalar
 // lives on the device, the host only polls the done flag once per chunk.
-template <typename T, typename TU>
-int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext, T *rnew, TU *u, cplx<T> *spec,
+// Linear CG (solverCG.h:84-117) with device-side scalars.  Iteration i runs
+//   z r2c [+ the update of iteration i-1: r -= alpha K d, u -= alpha d]
+//   -> RED_UPDATE (err, deltamid; sets done) -> y, x+Green, y^-1
+//   -> z c2r (s = -G r, <r, s>) -> RED_DELTA (beta)
+//   -> assembly (d = s + beta d, K d, <d, K d>) -> RED_ALPHA
+// so the update costs no pass of its own.  After RED_UPDATE sets `done` every
+// later kernel returns at entry; u and r then hold the final iterate.
+template <typename T>
+int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext, T *rnew, T *u, cplx<T> *spec,
                    const T *Gp, const Phase<T> *ph, int *iters_out)
 {
-    const int *done = c->is + IS_DONE;
+    const int *done  = c->is + IS_DONE;
     int        chunk = 4, rc;
     int        is[IS_COUNT];
+    bool       first = true;
     for (;;) {
-        for (int k = 0; k < chunk; ++k) {
-            EpiArgs<T> ep{};
+        // timing mode checks `done` after every iteration so that launches that
+        // returned at entry are not counted in the per-kernel durations
+        const int n = c->timing ? 1 : chunk;
+        for (int k = 0; k < n; ++k) {
+            const size_t     mark    = c->ev_used.size();
+            const bool       had_pro = !first;
+            const ProArgs<T> pa{rnew, u, dcur, s, c->sc + SC_ALPHA, c->d.measure, c->partial};
+            EpiArgs<T>       ep{};
             ep.out     = s;
             ep.r       = r;
             ep.partial = c->partial;
-            if ((rc = convolve_dev<T>(c, P, r, spec, Gp, EPI_CG, ep, done)))
+            if ((rc = convolve_dev<T>(c, P, r, spec, Gp, EPI_CG, ep, done, first ? nullptr : &pa)))
                 return rc;
+            first = false;
             if ((rc = reduce(c, c->partial, (int) P.nblk_z, 1, RED_DELTA)))
                 return rc;
             AsmArgs<T> a = asm_args<T>(c);
@@ -677,17 +861,17 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext
                 return rc;
             if ((rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ALPHA)))
                 return rc;
-            {
-                Timed t(c, KF_CG_UPDATE);
-                hipLaunchKernelGGL((k_cg_update<T, TU>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, r, rnew, u,
-                                   dnext, s, c->Nh, c->sc, c->is, c->partial);
-                HIPCHK(c, hipGetLastError());
-            }
-            if ((rc = reduce(c, c->partial, c->nblk_vec, 2, RED_UPDATE)))
-                return rc;
             std::swap(dcur, dnext);
+            if (c->timing) {
+                if ((rc = read_scalars(c, nullptr, is)))
+                    return rc;
+                // done was set by this iteration's RED_UPDATE (the fused update
+                // and that reduction did work) or before it (nothing did)
+                if (is[IS_DONE])
+                    drop_timing(c, had_pro ? mark + 2 : mark);
+            }
         }
-        if ((rc = read_scalars(c, nullptr, is)))
+        if (!c->timing && (rc = read_scalars(c, nullptr, is)))
             return rc;
         if (is[IS_DONE])
             break;
@@ -723,7 +907,7 @@ int solve_cg_fp64(fans_gpu_ctx *c)
     if (linear) {
         double *dcur = c->dA, *dnext = c->dB;
         int     it   = 0;
-        if ((rc = cg_linear_loop<double, double>(c, c->fft64, c->r, c->s, dcur, dnext, c->rnew, c->u, c->spec64,
+        if ((rc = cg_linear_loop<double>(c, c->fft64, c->r, c->s, dcur, dnext, c->rnew, c->u, c->spec64,
                                                  c->G64, c->dph64, &it)))
             return rc;
         c->iter = it;
@@ -879,7 +1063,7 @@ int solve_cg_mixed(fans_gpu_ctx *c)
             return rc;
         float *dcur = c->dA32, *dnext = c->dB32;
         int    it   = 0;
-        if ((rc = cg_linear_loop<float, float>(c, c->fft32, c->r32, c->s32, dcur, dnext, c->rnew32, c->x32, c->spec32,
+        if ((rc = cg_linear_loop<float>(c, c->fft32, c->r32, c->s32, dcur, dnext, c->rnew32, c->x32, c->spec32,
                                                c->G32, c->dph32, &it)))
             return rc;
         inner += it;
@@ -938,7 +1122,8 @@ int alloc_fields(fans_gpu_ctx *c)
         HIPCHK(c, hipMalloc(&c->spec32, sizeof(cplx<float>) * c->Nf * c->h));
     }
     // partial sums: the largest grid of any reducing kernel times its quantities
-    int64_t need = 2 * std::max<int64_t>({(int64_t) c->nblk_asm, (int64_t) c->nblk_vec, c->fft64.nblk_z});
+    int64_t need = 2 * std::max<int64_t>({(int64_t) c->nblk_asm, (int64_t) c->nblk_vec, c->fft64.nblk_z,
+                                          c->fft32.nblk_z});
     need         = std::max<int64_t>(need, (int64_t) 6 * c->nblk_el);
     c->partial_len = need;
     HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * need));
@@ -1144,6 +1329,7 @@ int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *d)
     c->geom.w       = lx * ly * lz / c->ngp;
     c->geom.q0      = 0.5 - std::sqrt(3.) / 6.;
     c->geom.q1      = 0.5 + std::sqrt(3.) / 6.;
+    c->cubic        = lx == ly && ly == lz;
     *out            = c;
     if (d->fe_type == FANS_FE_BBAR && c->h == 3)
         return fail(c, FANS_GPU_EUNSUP, "FE_type BBAR is not implemented on the GPU");
@@ -1853,23 +2039,20 @@ static int fft_standalone(int dev, int nx, int ny, int nz, int h, const double *
     }
     const int64_t nr = (int64_t) nx * ny * nz * h, nc = (int64_t) nx * ny * (nz / 2 + 1) * h;
     auto         &P  = c->fft64;
-    const int64_t nlines = (int64_t) nx * ny;
     const int     Kc     = P.Mc * h;
     if (fwd) {
         rc = hipMemcpy(c->dB, in, sizeof(double) * nr, hipMemcpyHostToDevice) == hipSuccess ? 0 : FANS_GPU_EHIP;
         if (!rc) {
+            const ProArgs<double> pa{};
             if (h == 1)
-                hipLaunchKernelGGL((k_fft_z_r2c<double, 1>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, c->dB,
-                                   c->spec64, P.Lz, P.twz, nz, nlines, P.lpb, nullptr);
+                launch_z_r2c<double, 1, PRO_NONE>(c, P, c->dB, c->spec64, pa, nullptr);
             else
-                hipLaunchKernelGGL((k_fft_z_r2c<double, 3>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, c->dB,
-                                   c->spec64, P.Lz, P.twz, nz, nlines, P.lpb, nullptr);
-            hipLaunchKernelGGL((k_fft_strided<double, false>), dim3((Kc + P.bc - 1) / P.bc, nx), dim3(kThreads), P.smy,
-                               c->stream, c->spec64, P.Ly, P.twy, (int64_t) Kc, Kc, (int64_t) ny * Kc, P.bc, nullptr);
+                launch_z_r2c<double, 3, PRO_NONE>(c, P, c->dB, c->spec64, pa, nullptr);
+            launch_strided<double, false>(c, P.Ly, P.twy, P.bc, P.smy, c->spec64, (int64_t) Kc, 0, Kc,
+                                          (int64_t) ny * Kc, nx, nullptr);
             // x pass: strided with columns = the whole (y, kz, h) block
-            hipLaunchKernelGGL((k_fft_strided<double, false>), dim3((ny * Kc + P.bcx - 1) / P.bcx, 1), dim3(kThreads),
-                               P.smx_plain, c->stream, c->spec64, P.Lx, P.twx, (int64_t) ny * Kc, ny * Kc,
-                               (int64_t) 0, P.bcx, nullptr);
+            launch_strided<double, false>(c, P.Lx, P.twx, P.bcx, P.smx_plain, c->spec64, (int64_t) ny * Kc, 0,
+                                          ny * Kc, (int64_t) 0, 1, nullptr);
             rc = hipMemcpy(out, c->spec64, sizeof(cplx<double>) * nc / 1, hipMemcpyDeviceToHost) == hipSuccess
                      ? 0
                      : FANS_GPU_EHIP;
@@ -1878,19 +2061,16 @@ static int fft_standalone(int dev, int nx, int ny, int nz, int h, const double *
         rc = hipMemcpy(c->spec64, in, sizeof(cplx<double>) * nc, hipMemcpyHostToDevice) == hipSuccess ? 0
                                                                                                       : FANS_GPU_EHIP;
         if (!rc) {
-            hipLaunchKernelGGL((k_fft_strided<double, true>), dim3((ny * Kc + P.bcx - 1) / P.bcx, 1), dim3(kThreads),
-                               P.smx_plain, c->stream, c->spec64, P.Lx, P.twx, (int64_t) ny * Kc, ny * Kc,
-                               (int64_t) 0, P.bcx, nullptr);
-            hipLaunchKernelGGL((k_fft_strided<double, true>), dim3((Kc + P.bc - 1) / P.bc, nx), dim3(kThreads), P.smy,
-                               c->stream, c->spec64, P.Ly, P.twy, (int64_t) Kc, Kc, (int64_t) ny * Kc, P.bc, nullptr);
+            launch_strided<double, true>(c, P.Lx, P.twx, P.bcx, P.smx_plain, c->spec64, (int64_t) ny * Kc, 0,
+                                         ny * Kc, (int64_t) 0, 1, nullptr);
+            launch_strided<double, true>(c, P.Ly, P.twy, P.bc, P.smy, c->spec64, (int64_t) Kc, 0, Kc,
+                                         (int64_t) ny * Kc, nx, nullptr);
             EpiArgs<double> ep{};
             ep.out = c->dB;
             if (h == 1)
-                hipLaunchKernelGGL((k_fft_z_c2r<double, 1, EPI_STORE>), dim3(P.nblk_z), dim3(kThreads), P.smz,
-                                   c->stream, c->spec64, P.Lz, P.twz, nz, nlines, P.lpb, ep, nullptr);
+                launch_z_c2r<double, 1, EPI_STORE>(c, P, c->spec64, ep, nullptr);
             else
-                hipLaunchKernelGGL((k_fft_z_c2r<double, 3, EPI_STORE>), dim3(P.nblk_z), dim3(kThreads), P.smz,
-                                   c->stream, c->spec64, P.Lz, P.twz, nz, nlines, P.lpb, ep, nullptr);
+                launch_z_c2r<double, 3, EPI_STORE>(c, P, c->spec64, ep, nullptr);
             rc = hipMemcpy(out, c->dB, sizeof(double) * nr, hipMemcpyDeviceToHost) == hipSuccess ? 0 : FANS_GPU_EHIP;
         }
     }
@@ -1920,6 +2100,12 @@ int fans_gpu_set_option(fans_gpu_ctx *c, int32_t opt, int64_t value)
         return FANS_GPU_EINVAL;
     if (opt == FANS_OPT_TIMING) {
         c->timing = value != 0;
+        return 0;
+    }
+    if (opt == FANS_OPT_FFT_CHUNKS) {
+        if (value < 1)
+            return fail(c, FANS_GPU_EINVAL, "FFT chunk count must be >= 1");
+        c->fft_chunks = (int) value;
         return 0;
     }
     return fail(c, FANS_GPU_EINVAL, "unknown option %d", opt);

@@ -95,43 +95,104 @@ __device__ void lds_fft(cplx<T> *buf, const FftLen &L, const cplx<T> *__restrict
     }
 }
 
+// ---- CG update fused into the forward z pass --------------------------------
+// PRO_CG: the forward z pass of CG iteration i first finishes iteration i-1
+// (solverCG.h:106-117): r -= alpha rnew, u -= alpha d, and emits per block the
+// partials |r| (norm of the measure) and <r, s_prev>, which the reduction turns
+// into the error and deltamid.  Only the updated r enters the transform.
+enum { PRO_NONE = 0, PRO_CG = 1 };
+
+template <typename T>
+struct ProArgs {
+    const T      *rnew;     // K d of the previous iteration
+    T            *u;        // displacement (or the inner correction)
+    const T      *d;        // previous search direction
+    const T      *s;        // previous preconditioned residual
+    const double *alpha;    // device scalar
+    int           measure;  // 0 L1, 1 L2, 2 Linf
+    double       *partial;  // [2][gridDim.x]
+};
+
+template <typename T, int PRO>
+__device__ __forceinline__ T z_load(T *__restrict__ in, int64_t gi, const ProArgs<T> &pa, T alpha, double &a0,
+                                    double &a1)
+{
+    if constexpr (PRO == PRO_CG) {
+        const T rv = in[gi] - alpha * pa.rnew[gi];
+        in[gi]     = rv;
+        pa.u[gi] -= alpha * pa.d[gi];
+        acc_norm(a0, rv, pa.measure);
+        a1 += (double) rv * (double) pa.s[gi];
+        return rv;
+    } else {
+        return in[gi];
+    }
+}
+
+template <int PRO, typename T>
+__device__ __forceinline__ void z_pro_partials(const ProArgs<T> &pa, double a0, double a1, double *red)
+{
+    if constexpr (PRO == PRO_CG) {
+        double     v[2]  = {a0, a1};
+        const bool mx[2] = {pa.measure == 2, false};
+        block_reduce<2>(v, mx, red);
+        if (threadIdx.x == 0) {
+            pa.partial[blockIdx.x]             = v[0];
+            pa.partial[gridDim.x + blockIdx.x] = v[1];
+        }
+    }
+}
+
+// r2c post-processing of the half-length transform Z (in LDS, column col):
+// X[k] = (Z[k] + conj Z[M-k]) / 2 + W^k (Z[k] - conj Z[M-k]) / (2i)
+template <typename T>
+__device__ __forceinline__ cplx<T> r2c_untangle(const cplx<T> *A, int ld, int col, int k, int M,
+                                                const cplx<T> *__restrict__ tw)
+{
+    const cplx<T> zk = A[(k == M ? 0 : k) * ld + col];
+    const cplx<T> zc = conjg(A[(k == 0 ? 0 : M - k) * ld + col]);
+    const cplx<T> E  = cscale(zk + zc, (T) 0.5);
+    const cplx<T> D  = cscale(zk - zc, (T) 0.5);
+    return E + cmul(tw[k], cplx<T>{D.y, -D.x});
+}
+
 // ---- z pass, forward r2c ----------------------------------------------------
 // LM: complex length M = nz/2 with tws = 2 into the length-nz table `tw`.
 // Block b handles lines [b*lpb, b*lpb + lpb) of nz*H reals each.
-template <typename T, int H>
-__global__ __launch_bounds__(256) void k_fft_z_r2c(const T *__restrict__ in, cplx<T> *__restrict__ out, FftLen LM,
+template <typename T, int H, int PRO>
+__global__ __launch_bounds__(256) void k_fft_z_r2c(T *__restrict__ in, cplx<T> *__restrict__ out, FftLen LM,
                                                    const cplx<T> *__restrict__ tw, int nz, int64_t nlines, int lpb,
-                                                   const int *__restrict__ done)
+                                                   ProArgs<T> pa, const int *__restrict__ done)
 {
     if (done && *done)
         return;
     extern __shared__ char smem_raw[];
+    __shared__ double red[8];
     cplx<T>      *A     = reinterpret_cast<cplx<T> *>(smem_raw);
     const int     M     = nz / 2;
     const int64_t line0 = (int64_t) blockIdx.x * lpb;
     const int     nl    = (int) ((nlines - line0) < lpb ? (nlines - line0) : lpb);
     const int     ncols = nl * H, ld = ncols + 1;
-    const T      *src   = in + line0 * nz * H;
     const int     nzh   = nz * H;
+    const int64_t g0    = line0 * nzh;
+    T             alpha = (T) 0;
+    double        a0 = 0.0, a1 = 0.0;
+    if constexpr (PRO == PRO_CG)
+        alpha = (T) *pa.alpha;
     for (int e = threadIdx.x; e < nl * nzh; e += blockDim.x) {
         const int l = e / nzh, rem = e - l * nzh, z = rem / H, c = rem - z * H;
         T        *d = reinterpret_cast<T *>(&A[(z >> 1) * ld + l * H + c]);
-        d[z & 1]    = src[e];
+        d[z & 1]    = z_load<T, PRO>(in, g0 + e, pa, alpha, a0, a1);
     }
     __syncthreads();
     lds_fft<T, false>(A, LM, tw, ncols, ld);
     const int Mc  = M + 1;
     cplx<T>  *dst = out + line0 * Mc * H;
     for (int e = threadIdx.x; e < nl * Mc * H; e += blockDim.x) {
-        const int     l = e / (Mc * H), rem = e - l * (Mc * H), k = rem / H, c = rem - k * H;
-        const int     col = l * H + c;
-        const cplx<T> zk  = A[(k == M ? 0 : k) * ld + col];
-        const cplx<T> zc  = conjg(A[(k == 0 ? 0 : M - k) * ld + col]);
-        const cplx<T> E   = cscale(zk + zc, (T) 0.5);
-        const cplx<T> D   = cscale(zk - zc, (T) 0.5);
-        const cplx<T> O   = cplx<T>{D.y, -D.x};  // (zk - zc) / (2i)
-        dst[e]            = E + cmul(tw[k], O);
+        const int l = e / (Mc * H), rem = e - l * (Mc * H), k = rem / H, c = rem - k * H;
+        dst[e]      = r2c_untangle(A, ld, l * H + c, k, M, tw);
     }
+    z_pro_partials<PRO>(pa, a0, a1, red);
 }
 
 // ---- z pass, inverse c2r with fused epilogues -------------------------------
@@ -144,6 +205,45 @@ struct EpiArgs {
     double       *u;        // EPI_FP: u -= result (FP64 displacement)
     double       *partial;  // EPI_CG: one partial sum per block
 };
+
+// c2r pre-processing of one (line, component): `line` points at kz = 0 of the
+// component (stride H); the half-length transform input is
+// Z[k] = (X[k] + conj X[M-k]) + i W^-k (X[k] - conj X[M-k])
+template <typename T, int H>
+__device__ __forceinline__ cplx<T> c2r_tangle(const cplx<T> *__restrict__ line, int k, int M,
+                                              const cplx<T> *__restrict__ tw)
+{
+    const cplx<T> xk = line[k * H];
+    const cplx<T> xm = conjg(line[(M - k) * H]);
+    const cplx<T> WD = cmul(conjg(tw[k]), xk - xm);
+    return (xk + xm) + cplx<T>{-WD.y, WD.x};
+}
+
+template <typename T, int EPI>
+__device__ __forceinline__ void z_epilogue(const EpiArgs<T> &ep, int64_t gi, T val, double &acc)
+{
+    if constexpr (EPI == EPI_STORE) {
+        ep.out[gi] = val;
+    } else if constexpr (EPI == EPI_CG) {
+        const T sv = -val;
+        ep.out[gi] = sv;
+        acc += (double) ep.r[gi] * (double) sv;
+    } else {
+        ep.u[gi] -= (double) val;
+    }
+}
+
+template <int EPI, typename T>
+__device__ __forceinline__ void z_epi_partials(const EpiArgs<T> &ep, double acc, double *red)
+{
+    if constexpr (EPI == EPI_CG) {
+        double     v[1]  = {acc};
+        const bool mx[1] = {false};
+        block_reduce<1>(v, mx, red);
+        if (threadIdx.x == 0)
+            ep.partial[blockIdx.x] = v[0];
+    }
+}
 
 template <typename T, int H, int EPI>
 __global__ __launch_bounds__(256) void k_fft_z_c2r(const cplx<T> *__restrict__ in, FftLen LM,
@@ -161,13 +261,8 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const cplx<T> *__restrict__ i
     const int         ncols = nl * H, ld = ncols + 1;
     const cplx<T>    *src   = in + line0 * Mc * H;
     for (int e = threadIdx.x; e < nl * M * H; e += blockDim.x) {
-        const int     l = e / (M * H), rem = e - l * (M * H), k = rem / H, c = rem - k * H;
-        const cplx<T> xk = src[(l * Mc + k) * H + c];
-        const cplx<T> xm = conjg(src[(l * Mc + (M - k)) * H + c]);
-        const cplx<T> D  = xk - xm;
-        const cplx<T> W  = conjg(tw[k]);
-        const cplx<T> WD = cmul(W, D);
-        A[k * ld + l * H + c] = (xk + xm) + cplx<T>{-WD.y, WD.x};  // + i * W * D
+        const int l = e / (M * H), rem = e - l * (M * H), k = rem / H, c = rem - k * H;
+        A[k * ld + l * H + c] = c2r_tangle<T, H>(src + l * Mc * H + c, k, M, tw);
     }
     __syncthreads();
     lds_fft<T, true>(A, LM, tw, ncols, ld);
@@ -176,42 +271,175 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const cplx<T> *__restrict__ i
     double        acc = 0.0;
     for (int e = threadIdx.x; e < nl * nzh; e += blockDim.x) {
         const int     l = e / nzh, rem = e - l * nzh, z = rem / H, c = rem - z * H;
-        const cplx<T> v   = A[(z >> 1) * ld + l * H + c];
-        const T       val = (z & 1) ? v.y : v.x;
-        const int64_t gi  = g0 + e;
-        if constexpr (EPI == EPI_STORE) {
-            ep.out[gi] = val;
-        } else if constexpr (EPI == EPI_CG) {
-            const T sv = -val;
-            ep.out[gi] = sv;
-            acc += (double) ep.r[gi] * (double) sv;
-        } else {
-            ep.u[gi] -= (double) val;
+        const cplx<T> v = A[(z >> 1) * ld + l * H + c];
+        z_epilogue<T, EPI>(ep, g0 + e, (z & 1) ? v.y : v.x, acc);
+    }
+    z_epi_partials<EPI>(ep, acc, red);
+}
+
+// ---- z passes with compile-time shape ---------------------------------------
+// For M = nz/2 in {16, 32, 64, 128} the line count per block, the thread count
+// and every index are compile-time constants (no runtime divisions), and the
+// thread count equals the butterfly count of the first radix stage.
+template <int M>
+constexpr int zct_radix1()
+{
+    return M >= 32 ? 8 : 4;
+}
+template <int H, int M>
+constexpr int zct_lpb()
+{
+    int l = 1;
+    while ((M / zct_radix1<M>()) * (2 * l) * H <= 256)
+        l *= 2;
+    return l;
+}
+template <int H, int M>
+constexpr int zct_threads()
+{
+    return (M / zct_radix1<M>()) * zct_lpb<H, M>() * H;
+}
+
+template <typename T, int R, bool INV, int N, int NC, int LD, int NS, int NT>
+__device__ __forceinline__ void stockham_stage_ct(cplx<T> *__restrict__ buf, const cplx<T> *__restrict__ tw)
+{
+    constexpr int nb = N / R, total = nb * NC, MB = (total + NT - 1) / NT;
+    constexpr int tstep = 2 * (N / (NS * R));  // into the length-2N table
+    cplx<T>       v[MB][R];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int w = threadIdx.x + b * NT;
+        if (total % NT == 0 || w < total) {
+            const int col = w % NC, j = w / NC;
+#pragma unroll
+            for (int t = 0; t < R; ++t)
+                v[b][t] = buf[(j + t * nb) * LD + col];
         }
     }
-    if constexpr (EPI == EPI_CG) {
-        double     v[1]  = {acc};
-        const bool mx[1] = {false};
-        block_reduce<1>(v, mx, red);
-        if (threadIdx.x == 0)
-            ep.partial[blockIdx.x] = v[0];
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int w = threadIdx.x + b * NT;
+        if (total % NT == 0 || w < total) {
+            const int col = w % NC, j = w / NC, k = j % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int t = 1; t < R; ++t) {
+                    const cplx<T> wv = tw[t * k * tstep];
+                    v[b][t]          = INV ? cmulc(v[b][t], wv) : cmul(v[b][t], wv);
+                }
+            }
+            dft<T, R, INV>(v[b]);
+            const int o = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int t = 0; t < R; ++t)
+                buf[(o + t * NS) * LD + col] = v[b][t];
+        }
+    }
+    __syncthreads();
+}
+
+// radix schedules: 128 = 8*4*4, 64 = 8*8, 32 = 8*4, 16 = 4*4
+template <typename T, bool INV, int N, int NC, int LD, int NT>
+__device__ __forceinline__ void lds_fft_ct(cplx<T> *buf, const cplx<T> *__restrict__ tw)
+{
+    if constexpr (N == 128) {
+        stockham_stage_ct<T, 8, INV, N, NC, LD, 1, NT>(buf, tw);
+        stockham_stage_ct<T, 4, INV, N, NC, LD, 8, NT>(buf, tw);
+        stockham_stage_ct<T, 4, INV, N, NC, LD, 32, NT>(buf, tw);
+    } else if constexpr (N == 64) {
+        stockham_stage_ct<T, 8, INV, N, NC, LD, 1, NT>(buf, tw);
+        stockham_stage_ct<T, 8, INV, N, NC, LD, 8, NT>(buf, tw);
+    } else if constexpr (N == 32) {
+        stockham_stage_ct<T, 8, INV, N, NC, LD, 1, NT>(buf, tw);
+        stockham_stage_ct<T, 4, INV, N, NC, LD, 8, NT>(buf, tw);
+    } else {
+        static_assert(N == 16, "unsupported compile-time length");
+        stockham_stage_ct<T, 4, INV, N, NC, LD, 1, NT>(buf, tw);
+        stockham_stage_ct<T, 4, INV, N, NC, LD, 4, NT>(buf, tw);
     }
 }
 
+// forward: lines [b*LPB, b*LPB + LPB); the host launches nlines / LPB blocks
+template <typename T, int H, int M, int PRO>
+__global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_r2c_ct(T *__restrict__ in, cplx<T> *__restrict__ out,
+                                                                      const cplx<T> *__restrict__ tw, ProArgs<T> pa,
+                                                                      const int *__restrict__ done)
+{
+    if (done && *done)
+        return;
+    constexpr int NT = zct_threads<H, M>(), LPB = zct_lpb<H, M>();
+    constexpr int NZH = 2 * M * H, NC = LPB * H, LD = NC + 1, MC = M + 1;
+    __shared__ cplx<T> A[M * LD];
+    __shared__ double  red[2 * (NT / 64 + 1)];
+    const int64_t      line0 = (int64_t) blockIdx.x * LPB;
+    const int64_t      g0    = line0 * NZH;
+    T                  alpha = (T) 0;
+    double             a0 = 0.0, a1 = 0.0;
+    if constexpr (PRO == PRO_CG)
+        alpha = (T) *pa.alpha;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < LPB * NZH; e += NT) {
+        const int l = e / NZH, rem = e - l * NZH, z = rem / H, c = rem - z * H;
+        T        *d = reinterpret_cast<T *>(&A[(z >> 1) * LD + l * H + c]);
+        d[z & 1]    = z_load<T, PRO>(in, g0 + e, pa, alpha, a0, a1);
+    }
+    __syncthreads();
+    lds_fft_ct<T, false, M, NC, LD, NT>(A, tw);
+    cplx<T> *dst = out + line0 * MC * H;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < LPB * MC * H; e += NT) {
+        const int l = e / (MC * H), rem = e - l * (MC * H), k = rem / H, c = rem - k * H;
+        dst[e]      = r2c_untangle(A, LD, l * H + c, k, M, tw);
+    }
+    z_pro_partials<PRO>(pa, a0, a1, red);
+}
+
+template <typename T, int H, int M, int EPI>
+__global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cplx<T> *__restrict__ in,
+                                                                      const cplx<T> *__restrict__ tw, EpiArgs<T> ep,
+                                                                      const int *__restrict__ done)
+{
+    if (done && *done)
+        return;
+    constexpr int NT = zct_threads<H, M>(), LPB = zct_lpb<H, M>();
+    constexpr int NZH = 2 * M * H, NC = LPB * H, LD = NC + 1, MC = M + 1;
+    __shared__ cplx<T> A[M * LD];
+    __shared__ double  red[NT / 64 + 1];
+    const int64_t      line0 = (int64_t) blockIdx.x * LPB;
+    const cplx<T>     *src   = in + line0 * MC * H;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < LPB * M * H; e += NT) {
+        const int l = e / (M * H), rem = e - l * (M * H), k = rem / H, c = rem - k * H;
+        A[k * LD + l * H + c] = c2r_tangle<T, H>(src + l * MC * H + c, k, M, tw);
+    }
+    __syncthreads();
+    lds_fft_ct<T, true, M, NC, LD, NT>(A, tw);
+    const int64_t g0  = line0 * NZH;
+    double        acc = 0.0;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < LPB * NZH; e += NT) {
+        const int     l = e / NZH, rem = e - l * NZH, z = rem / H, c = rem - z * H;
+        const cplx<T> v = A[(z >> 1) * LD + l * H + c];
+        z_epilogue<T, EPI>(ep, g0 + e, (z & 1) ? v.y : v.x, acc);
+    }
+    z_epi_partials<EPI>(ep, acc, red);
+}
+
 // ---- batched strided complex pass (y or x axis) ----------------------------
-// data[o*ostride + i*sn + b] for i < L.n, b < Kc; block (bx, o) owns columns
-// [bx*bc, bx*bc + bc).
+// data[o*ostride + i*sn + b] for i < L.n, b in [c0, c1); block (bx, o) owns
+// columns [c0 + bx*bc, c0 + bx*bc + bc).
 template <typename T, bool INV>
 __global__ __launch_bounds__(256) void k_fft_strided(cplx<T> *__restrict__ data, FftLen L,
-                                                     const cplx<T> *__restrict__ tw, int64_t sn, int Kc,
+                                                     const cplx<T> *__restrict__ tw, int64_t sn, int c0, int c1,
                                                      int64_t ostride, int bc, const int *__restrict__ done)
 {
     if (done && *done)
         return;
     extern __shared__ char smem_raw[];
     cplx<T>  *A     = reinterpret_cast<cplx<T> *>(smem_raw);
-    const int b0    = blockIdx.x * bc;
-    const int ncols = (Kc - b0) < bc ? (Kc - b0) : bc, ld = ncols + 1;
+    const int b0    = c0 + blockIdx.x * bc;
+    const int ncols = (c1 - b0) < bc ? (c1 - b0) : bc, ld = ncols + 1;
     cplx<T>  *base  = data + (int64_t) blockIdx.y * ostride + b0;
     for (int w = threadIdx.x; w < L.n * ncols; w += blockDim.x) {
         const int i = w / ncols, c = w - i * ncols;
@@ -226,12 +454,13 @@ __global__ __launch_bounds__(256) void k_fft_strided(cplx<T> *__restrict__ data,
 }
 
 // ---- x pass with the Green operator between forward and inverse ------------
-// spectrum [x][y][kz][H]; block (bx, y) owns kz in [bx*bk, bx*bk+bk) and all H.
+// spectrum [x][y][kz][H]; block (bx, y) owns kz in [kz_lo + bx*bk, ... + bk) and all H.
 // G packed per frequency [x][y][kz][H(H+1)/2] = (00 01 02 11 12 22) or (0).
 template <typename T, int H>
 __global__ __launch_bounds__(256) void k_fft_x_green(cplx<T> *__restrict__ data, FftLen L,
                                                      const cplx<T> *__restrict__ tw, const T *__restrict__ G, int ny,
-                                                     int Mc, int bk, const int *__restrict__ done)
+                                                     int Mc, int kz_lo, int kz_hi, int bk,
+                                                     const int *__restrict__ done)
 {
     if (done && *done)
         return;
@@ -239,8 +468,8 @@ __global__ __launch_bounds__(256) void k_fft_x_green(cplx<T> *__restrict__ data,
     extern __shared__ char smem_raw[];
     cplx<T>      *A     = reinterpret_cast<cplx<T> *>(smem_raw);
     const int     y     = blockIdx.y;
-    const int     kz0   = blockIdx.x * bk;
-    const int     nk    = (Mc - kz0) < bk ? (Mc - kz0) : bk;
+    const int     kz0   = kz_lo + blockIdx.x * bk;
+    const int     nk    = (kz_hi - kz0) < bk ? (kz_hi - kz0) : bk;
     const int     ncols = nk * H, ld = ncols + 1;
     const int64_t sn    = (int64_t) ny * Mc * H;
     cplx<T>      *base  = data + ((int64_t) y * Mc + kz0) * H;

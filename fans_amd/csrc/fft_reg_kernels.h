@@ -1,0 +1,230 @@
+// Register four-step FFT kernels for power-of-two transform lengths.
+//
+// A length N = N1 * N2 transform of one column is done by N2 cooperating
+// threads, thread j holding the N1 values x[j + N2 m]:
+//   1. an N1-point DFT in registers over m,
+//   2. the twiddle W_N^(j k1),
+//   3. one transpose through LDS (rows swizzled so neither the writes nor the
+//      reads of a wave collide in LDS banks),
+//   4. N1/N2 N2-point DFTs in registers.
+// Thread j ends with X[j + N2 q + N1 k2] in v[q N2 + k2] -- the same index set
+// an inverse transform of the same shape takes as input, so a forward
+// transform, a pointwise operation and an inverse transform chain without a
+// second exchange (used by the x pass around the Green operator).
+#pragma once
+#include "fft_butterflies.h"
+
+namespace fans {
+
+// in-register DFT of length 16 as 4 x 4 with constant internal twiddles
+template <typename T, bool INV>
+__device__ __forceinline__ void dft16(cplx<T> (&v)[16])
+{
+    const T c = (T) 0.92387953251128675613, s = (T) 0.38268343236508977173, r = (T) 0.70710678118654752440;
+    cplx<T> y[4][4];
+#pragma unroll
+    for (int i2 = 0; i2 < 4; ++i2) {
+        cplx<T> a[4] = {v[i2], v[i2 + 4], v[i2 + 8], v[i2 + 12]};
+        dft4<T, INV>(a);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1)
+            y[i2][k1] = a[k1];
+    }
+    // y[i2][k1] *= W16^(i2 k1); forward W16^p = (cos, -sin)(2 pi p / 16)
+    auto tw = [&](cplx<T> a, T wr, T wi) { return INV ? cplx<T>{a.x * wr + a.y * wi, a.y * wr - a.x * wi}
+                                                       : cplx<T>{a.x * wr - a.y * wi, a.x * wi + a.y * wr}; };
+    // (wr, wi) below are the forward twiddles; the lambda conjugates for INV
+    y[1][1] = tw(y[1][1], c, -s);
+    y[1][2] = tw(y[1][2], r, -r);
+    y[1][3] = tw(y[1][3], s, -c);
+    y[2][1] = tw(y[2][1], r, -r);
+    y[2][2] = tw(y[2][2], (T) 0, (T) -1);
+    y[2][3] = tw(y[2][3], -r, -r);
+    y[3][1] = tw(y[3][1], s, -c);
+    y[3][2] = tw(y[3][2], -r, -r);
+    y[3][3] = tw(y[3][3], -c, s);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        cplx<T> a[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+        dft4<T, INV>(a);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2)
+            v[k1 + 4 * k2] = a[k2];
+    }
+}
+
+template <typename T, int N, bool INV>
+__device__ __forceinline__ void dft_reg(cplx<T> (&v)[N])
+{
+    if constexpr (N == 2)
+        dft<T, 2, INV>(v);
+    else if constexpr (N == 4)
+        dft4<T, INV>(v);
+    else if constexpr (N == 8)
+        dft8<T, INV>(v);
+    else
+        dft16<T, INV>(v);
+}
+
+// Four-step transform of one column (see the file comment).  `tile` is the
+// block's LDS tile with row stride `ld` (complex), `col` this column; `tw` the
+// length-N table W_N^e = exp(-2 pi i e / N).  Ends synchronised.
+template <typename T, int N1, int N2, bool INV>
+__device__ __forceinline__ void fourstep(cplx<T> (&v)[N1], int j, cplx<T> *tile, int ld, int col,
+                                         const cplx<T> *__restrict__ tw)
+{
+    dft_reg<T, N1, INV>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) {
+        const cplx<T> w = tw[j * k1];
+        v[k1]           = INV ? cmulc(v[k1], w) : cmul(v[k1], w);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1)
+        tile[(k1 * N2 + ((j + k1) & (N2 - 1))) * ld + col] = v[k1];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N1 / N2; ++q) {
+        const int k1 = j + N2 * q;
+        cplx<T>   w[N2];
+#pragma unroll
+        for (int i2 = 0; i2 < N2; ++i2)
+            w[i2] = tile[(k1 * N2 + ((i2 + k1) & (N2 - 1))) * ld + col];
+        dft_reg<T, N2, INV>(w);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2)
+            v[q * N2 + k2] = w[k2];
+    }
+    __syncthreads();
+}
+
+// register order of the four-step output -> input order of the next transform
+// of the same shape: v_in[m] = v_out[q N2 + k2] with m = q + (N1/N2) k2
+template <typename T, int N1, int N2>
+__device__ __forceinline__ void fourstep_out_to_in(cplx<T> (&v)[N1])
+{
+    cplx<T> t[N1];
+#pragma unroll
+    for (int q = 0; q < N1 / N2; ++q)
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2)
+            t[q + (N1 / N2) * k2] = v[q * N2 + k2];
+#pragma unroll
+    for (int m = 0; m < N1; ++m)
+        v[m] = t[m];
+}
+
+// ---- plain strided pass (y axis) ------------------------------------------
+// data[o*ostride + i*sn + b], i < N1*N2, b in [c0, c1); C columns per block,
+// block = C * N2 threads, column fastest in the lane order.  1-D grid of
+// nbx * n_outer blocks, XCD-mapped so that neighbouring column groups (which
+// share cache lines) run on the same XCD.
+template <typename T, int N1, int N2, int C, bool INV>
+__global__ __launch_bounds__(C * N2) void k_fft_strided_reg(cplx<T> *__restrict__ data,
+                                                            const cplx<T> *__restrict__ tw, int64_t sn, int c0,
+                                                            int c1, int64_t ostride, int nbx,
+                                                            const int *__restrict__ done)
+{
+    if (done && *done)
+        return;
+    constexpr int N  = N1 * N2;
+    constexpr int LD = C + 1;
+    __shared__ cplx<T> tile[N * LD];
+    const int          t = threadIdx.x, col = t % C, j = t / C;
+    const int          lb = xcd_tile_index(blockIdx.x, gridDim.x), bx = lb % nbx, bo = lb / nbx;
+    const int          b  = c0 + bx * C + col;
+    const bool         ok = b < c1;
+    cplx<T>           *base = data + (int64_t) bo * ostride + b;
+    cplx<T>            v[N1];
+#pragma unroll
+    for (int m = 0; m < N1; ++m)
+        v[m] = ok ? base[(int64_t) (j + N2 * m) * sn] : cplx<T>{(T) 0, (T) 0};
+    fourstep<T, N1, N2, INV>(v, j, tile, LD, col, tw);
+    if (ok)
+#pragma unroll
+        for (int q = 0; q < N1 / N2; ++q)
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2)
+                base[(int64_t) (j + N2 * q + N1 * k2) * sn] = v[q * N2 + k2];
+}
+
+// lane <- lane (quad base + s) for every lane of a quad (DPP quad_perm broadcast)
+template <int S>
+__device__ __forceinline__ double quad_bcast(double x)
+{
+    constexpr int ctrl = S * 0x55;
+    const int     lo   = __builtin_amdgcn_mov_dpp(__double2loint(x), ctrl, 0xf, 0xf, true);
+    const int     hi   = __builtin_amdgcn_mov_dpp(__double2hiint(x), ctrl, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+template <int S>
+__device__ __forceinline__ float quad_bcast(float x)
+{
+    constexpr int ctrl = S * 0x55;
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), ctrl, 0xf, 0xf, true));
+}
+
+// ---- x pass: forward FFT, Green multiply, inverse FFT ---------------------
+// spectrum [x][y][kz][H]; for H = 3 the columns of a block are (kz, c) with c
+// padded to 4 so the three components of one frequency sit in one lane quad;
+// the Green multiply exchanges them with DPP.  Logical block (bk, y) of the
+// XCD-mapped 1-D grid (nbk * ny blocks) owns kz in [kz_lo + bk*KB, ... + KB).
+template <typename T, int H, int N1, int N2, int KB>
+__global__ __launch_bounds__(KB *(H == 3 ? 4 : 1) * N2) void k_fft_x_green_reg(
+    cplx<T> *__restrict__ data, const cplx<T> *__restrict__ tw, const T *__restrict__ G, int ny, int Mc, int kz_lo,
+    int kz_hi, int nbk, const int *__restrict__ done)
+{
+    if (done && *done)
+        return;
+    constexpr int N  = N1 * N2;
+    constexpr int HP = H == 3 ? 4 : 1;
+    constexpr int C  = KB * HP;
+    constexpr int LD = C + 1;
+    constexpr int NP = H * (H + 1) / 2;
+    __shared__ cplx<T> tile[N * LD];
+    const int          t = threadIdx.x, col = t % C, j = t / C;
+    const int          kk = col / HP, c = col % HP;
+    const int          lb = xcd_tile_index(blockIdx.x, gridDim.x);
+    const int          kz = kz_lo + (lb % nbk) * KB + kk;
+    const int          y  = lb / nbk;
+    const bool         ok = kz < kz_hi && c < H;
+    const int64_t      sn = (int64_t) ny * Mc * H;
+    cplx<T>           *base = data + ((int64_t) y * Mc + kz) * H + c;
+    cplx<T>            v[N1];
+#pragma unroll
+    for (int m = 0; m < N1; ++m)
+        v[m] = ok ? base[(int64_t) (j + N2 * m) * sn] : cplx<T>{(T) 0, (T) 0};
+    fourstep<T, N1, N2, false>(v, j, tile, LD, col, tw);
+    const int kzg = kz < kz_hi ? kz : kz_lo;  // keep G reads in range for idle lanes
+#pragma unroll
+    for (int q = 0; q < N1 / N2; ++q)
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+            const int     row = j + N2 * q + N1 * k2;  // x frequency
+            const T      *g   = G + (((int64_t) row * ny + y) * Mc + kzg) * NP;
+            cplx<T>      &a   = v[q * N2 + k2];
+            if constexpr (H == 1) {
+                a = cscale(a, g[0]);
+            } else {
+                // row c of the packed symmetric 3x3 (00 01 02 11 12 22)
+                const int i0 = c == 0 ? 0 : (c == 1 ? 1 : 2);
+                const int i1 = c == 0 ? 1 : (c == 1 ? 3 : 4);
+                const int i2 = c == 0 ? 2 : (c == 1 ? 4 : 5);
+                const T   ga = g[c < 3 ? i0 : 0], gb = g[c < 3 ? i1 : 0], gc = g[c < 3 ? i2 : 0];
+                const cplx<T> v0{quad_bcast<0>(a.x), quad_bcast<0>(a.y)};
+                const cplx<T> v1{quad_bcast<1>(a.x), quad_bcast<1>(a.y)};
+                const cplx<T> v2{quad_bcast<2>(a.x), quad_bcast<2>(a.y)};
+                a = cplx<T>{ga * v0.x + gb * v1.x + gc * v2.x, ga * v0.y + gb * v1.y + gc * v2.y};
+            }
+        }
+    fourstep_out_to_in<T, N1, N2>(v);
+    fourstep<T, N1, N2, true>(v, j, tile, LD, col, tw);
+    if (ok)
+#pragma unroll
+        for (int q = 0; q < N1 / N2; ++q)
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2)
+                base[(int64_t) (j + N2 * q + N1 * k2) * sn] = v[q * N2 + k2];
+}
+
+}  // namespace fans

@@ -51,10 +51,14 @@ __device__ __forceinline__ void record_error(double *sc, int *is, double *hist, 
     is[IS_DONE]          = !(it < is[IS_NIT] && e > sc[SC_TOL]);
 }
 
-__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ partial, int nblk, int nq, int stage,
-                                                double *sc, int *is, double *hist)
+// one block of kReduceThreads; each thread folds its strided partials into
+// four independent accumulators (fixed order) before the block reduction
+constexpr int kReduceThreads = 1024;
+
+__global__ __launch_bounds__(kReduceThreads) void k_reduce(const double *__restrict__ partial, int nblk, int nq,
+                                                           int stage, double *sc, int *is, double *hist)
 {
-    __shared__ double sm[4 * 8];
+    __shared__ double sm[2 * (kReduceThreads / 64)];
     const int measure = is[IS_MEASURE];
     if (stage != RED_ERR_INIT && stage != RED_SUM && is[IS_DONE])
         return;
@@ -65,13 +69,20 @@ __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ parti
         for (int k = 0; k < 2; ++k) {
             const bool norm_slot = (k + q == 0) && (stage == RED_ERR_INIT || stage == RED_UPDATE || stage == RED_ERR_NEXT);
             mx[k]                = norm_slot && measure == 2;
-            double a             = 0.0;
-            if (k < nqq)
-                for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-                    const double p = partial[(int64_t) (q + k) * nblk + b];
-                    a              = mx[k] ? fmax(a, p) : a + p;
-                }
-            v[k] = a;
+            double a[4] = {0.0, 0.0, 0.0, 0.0};
+            if (k < nqq) {
+                const double *src = partial + (int64_t) (q + k) * nblk;
+                int           b   = threadIdx.x;
+                for (; b + 3 * kReduceThreads < nblk; b += 4 * kReduceThreads)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const double p = src[b + j * kReduceThreads];
+                        a[j]           = mx[k] ? fmax(a[j], p) : a[j] + p;
+                    }
+                for (; b < nblk; b += kReduceThreads)
+                    a[0] = mx[k] ? fmax(a[0], src[b]) : a[0] + src[b];
+            }
+            v[k] = mx[k] ? fmax(fmax(a[0], a[1]), fmax(a[2], a[3])) : (a[0] + a[1]) + (a[2] + a[3]);
         }
         block_reduce<2>(v, mx, sm);
         if (threadIdx.x == 0) {
@@ -101,51 +112,6 @@ __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ parti
             }
         }
         __syncthreads();
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void acc_norm(double &acc, T v, int measure)
-{
-    const double a = fabs((double) v);
-    if (measure == 2)
-        acc = fmax(acc, a);
-    else if (measure == 1)
-        acc += a * a;
-    else
-        acc += a;
-}
-
-// r -= alpha rnew; u -= alpha d; partials: |r| (measure), <r, s>
-template <typename T, typename TU>
-__global__ __launch_bounds__(256) void k_cg_update(T *__restrict__ r, const T *__restrict__ rnew, TU *__restrict__ u,
-                                                   const T *__restrict__ d, const T *__restrict__ s, int64_t n,
-                                                   const double *__restrict__ sc, const int *__restrict__ is,
-                                                   double *__restrict__ partial)
-{
-    __shared__ double sm[8];
-    if (is[IS_DONE])
-        return;
-    const int    measure = is[IS_MEASURE];
-    const T      alpha   = (T) sc[SC_ALPHA];
-    const double alphad  = sc[SC_ALPHA];
-    double       a0 = 0.0, a1 = 0.0;
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
-        const T rv = r[i] - alpha * rnew[i];
-        r[i]       = rv;
-        if constexpr (sizeof(TU) == sizeof(T))
-            u[i] -= alpha * d[i];
-        else
-            u[i] -= alphad * (double) d[i];
-        acc_norm(a0, rv, measure);
-        a1 += (double) rv * (double) s[i];
-    }
-    double     v[2]  = {a0, a1};
-    const bool mx[2] = {measure == 2, false};
-    block_reduce<2>(v, mx, sm);
-    if (threadIdx.x == 0) {
-        partial[blockIdx.x]             = v[0];
-        partial[gridDim.x + blockIdx.x] = v[1];
     }
 }
 

@@ -23,6 +23,7 @@ ERR_TYPE = {"absolute": 0, "relative": 1}
 PRECISION = {"fp64": 0, "mixed": 1}
 FIELD = {"u": 0, "r": 1, "u_prev": 2, "plastic_strain": 3, "hardening": 4}
 OPT_TIMING = 1
+OPT_FFT_CHUNKS = 2
 ERRORS = {-1: "EINVAL", -2: "ENODEV", -3: "EHIP", -4: "ESTATE", -5: "EUNSUP"}
 
 
@@ -241,6 +242,9 @@ class GpuSolver:
 
     def set_timing(self, on=True):
         self._check(self.L.fans_gpu_set_option(self.ptr, OPT_TIMING, int(bool(on))))
+
+    def set_fft_chunks(self, k):
+        self._check(self.L.fans_gpu_set_option(self.ptr, OPT_FFT_CHUNKS, int(k)))
 
     def get_field(self, name):
         if name in ("plastic_strain",):

@@ -170,10 +170,11 @@ int fans_gpu_get_reference_stiffness(fans_gpu_ctx *ctx, double *out, int32_t n_s
 int fans_gpu_fft_r2c(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t h, const double *in, double *out_cplx);
 int fans_gpu_fft_c2r(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t h, const double *in_cplx, double *out);
 
-/* options: FANS_OPT_TIMING=1 brackets every launch with HIP events on the
- * context's stream (no reference counterpart; replaces the clock() counters of
- * solver.h:286-297) */
-enum { FANS_OPT_TIMING = 1 };
+/* options (no reference counterpart): FANS_OPT_TIMING=1 brackets every launch
+ * with HIP events on the context's stream (replaces the clock() counters of
+ * solver.h:286-297); FANS_OPT_FFT_CHUNKS=k runs the y / x+Green / y^-1 passes
+ * of the convolution on k kz-chunks in turn (k >= 1). */
+enum { FANS_OPT_TIMING = 1, FANS_OPT_FFT_CHUNKS = 2 };
 int fans_gpu_set_option(fans_gpu_ctx *ctx, int32_t opt, int64_t value);
 /* per kernel family of the last solve: out[2k] = total ms, out[2k+1] = launches */
 int         fans_gpu_kernel_times(fans_gpu_ctx *ctx, double *out, int32_t cap, int32_t *count);

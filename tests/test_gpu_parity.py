@@ -37,15 +37,29 @@ def test_fft_matches_numpy(shape):
     assert relerr(y, refi) < 1e-13
 
 
-@pytest.mark.parametrize("maker,fe", [(cases.thermal, "HEX8"), (cases.thermal, "HEX8R"), (cases.elastic, "HEX8"),
-                                      (cases.elastic, "HEX8R")])
-def test_green_and_operators(maker, fe):
-    p = maker(n=12, fe=fe, dims=(12, 10, 8))
+# HEX8R has hourglass modes: the reference stiffness symbol (a sum of terms
+# that cancel, solver.h:173-186) vanishes on the lines k = (., pi, pi) and
+# permutations, so G = pinv(K(k)) amplifies rounding by
+# kappa = max|G| / min|G| -- 352 at 12x10x8, 7e5 at 32x64x16, 1e7 and more
+# on the larger grids.  HEX8R is checked where kappa stays below 1e6 (with a
+# 1e-10 budget), HEX8 (kappa <= 7e3) on every grid at 1e-12.
+_OPERATOR_CASES = [(m, fe, dims) for m in (cases.thermal, cases.elastic)
+                   for fe, grids in (("HEX8", [(12, 10, 8), (32, 64, 16), (16, 128, 32), (256, 32, 8)]),
+                                     ("HEX8R", [(12, 10, 8), (32, 64, 16)]))
+                   for dims in grids]
+
+
+@pytest.mark.parametrize("maker,fe,dims", _OPERATOR_CASES)
+def test_green_and_operators(maker, fe, dims):
+    """Generic (12x10x8) and power-of-two grids (register four-step FFT paths
+    for every length 16..256 along x and y, compile-time z passes)."""
+    p = maker(n=12, fe=fe, dims=dims)
     say(f"operators {maker.__name__} {fe} dims={p.dims}")
     o = oracle.Oracle(p)
     g = gpu.GpuSolver(p)
     assert relerr(g.reference_stiffness(), o.reference_stiffness()) < 1e-15
-    assert relerr(g.green(), o.green()) < 1e-12
+    tol_g = 1e-12 if fe == "HEX8" else 1e-10
+    assert relerr(g.green(), o.green()) < tol_g
     rng = np.random.default_rng(3)
     u = rng.standard_normal(p.dims + (p.howmany,))
     g0 = rng.standard_normal(p.n_str) * 0.1
@@ -54,7 +68,7 @@ def test_green_and_operators(maker, fe):
     assert relerr(g.residual(u), o.residual(u)) < 1e-12
     assert relerr(g.residual(u, linear_kd=True), o.residual(u, linear_kd=True)) < 1e-12
     r = rng.standard_normal(p.dims + (p.howmany,))
-    assert relerr(g.convolve(r), o.convolve(r)) < 1e-12
+    assert relerr(g.convolve(r), o.convolve(r)) < tol_g
     g.close()
 
 
