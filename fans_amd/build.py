@@ -52,5 +52,31 @@ def build(verbose=False):
     return LIB
 
 
+HOST_SRC = [os.path.join(HERE, "host", f) for f in ("main.cpp", "reader.cpp", "h5dyn.cpp")]
+CLI = os.path.join(HERE, "FANS")
+HDF5_PREFIX = os.environ.get("FANS_HDF5_PREFIX", "/opt/conda")
+
+
+def build_cli(verbose=False):
+    """The drop-in CLI `fans_amd/FANS in.json out.h5`: C++ host (g++) over the C
+    ABI; the serial HDF5 headers come from FANS_HDF5_PREFIX (default
+    /opt/conda), the library is loaded at run time (host/h5dyn.cpp)."""
+    if not os.path.exists(LIB):
+        build(verbose)
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+           f"-I{os.path.join(HDF5_PREFIX, 'include')}", "-o", CLI + ".tmp"] + HOST_SRC + [
+        f"-L{HERE}", "-lfans_gpu", "-Wl,-rpath,$ORIGIN", "-ldl"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed to build the FANS CLI")
+    os.replace(CLI + ".tmp", CLI)
+    return CLI
+
+
 if __name__ == "__main__":
     print(build(verbose="--verbose" in sys.argv))
+    print(build_cli(verbose="--verbose" in sys.argv))

@@ -44,6 +44,8 @@ struct AsmArgs {
     double         *partial;   // [2][gridDim] : slot 0 = norm/dot, slot 1 = dot
     int             measure;   // 0 L1, 1 L2, 2 Linf  (ASM_RESID)
     const int      *done;
+    const T        *vk;        // per-voxel isotropic moduli (nullable, ISO only):
+    const T        *vg;        // bulk K and shear G, or conductivity k in vk
 };
 
 template <typename T, int H>
@@ -193,7 +195,12 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
         T             f[8][H];
         if constexpr (ISO) {
             if constexpr (H == 3) {
-                const T lw = PL[ph][0] * w, mw = PL[ph][1] * w;
+                T lam = PL[ph][0], mu = PL[ph][1];
+                if (a.vk) {
+                    mu  = a.vg[el];
+                    lam = a.vk[el] - (T) (2.0 / 3.0) * mu;
+                }
+                const T lw = lam * w, mw = mu * w;
                 T       T0[3][3];
                 if constexpr (MODE == ASM_RESID) {
                     const T r = (T) kSqrtHalf, tr = g0[0] + g0[1] + g0[2];
@@ -214,7 +221,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
                 element_forces_iso3_modal<T, NGP, CUBIC>(reinterpret_cast<const T(&)[8][3]>(u), a.geom, lw, mw, T0,
                                             reinterpret_cast<T(&)[8][3]>(f));
             } else {
-                const T kw    = PL[ph][0] * w;
+                const T kw    = (a.vk ? a.vk[el] : PL[ph][0]) * w;
                 const T T0[3] = {kw * g0[0], kw * g0[1], kw * g0[2]};
                 element_forces_iso1_modal<T, NGP, CUBIC>(reinterpret_cast<const T(&)[8][1]>(u), a.geom, kw, T0,
                                             reinterpret_cast<T(&)[8][1]>(f));

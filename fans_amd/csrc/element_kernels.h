@@ -699,7 +699,11 @@ __device__ __forceinline__ float dpp_row_shr1(float v)
 // element-local pass (one thread per element): homogenized stress, fields,
 // committing the J2 state.  Reads nodal u directly (no relative gather).
 // ---------------------------------------------------------------------------
-enum { ELEM_HOMOG = 0, ELEM_FIELDS = 1, ELEM_COMMIT = 2 };
+// ELEM_SENS: per-element integrals of eps_a^T dC/dp eps_b for the isotropic
+// moduli p (bulk K, shear G; thermal: conductivity), eps_a from (u, g0) and
+// eps_b from (ub, g0b) -- the sensitivity of v . Sigma_bar (times the volume)
+// when (u, g0) solves for g0 and (ub, g0b) for the macro strain v.
+enum { ELEM_HOMOG = 0, ELEM_FIELDS = 1, ELEM_COMMIT = 2, ELEM_SENS = 3 };
 
 template <typename T>
 struct ElemArgs {
@@ -715,7 +719,31 @@ struct ElemArgs {
     double         *stress_elem;
     double         *strain_gp;
     double         *stress_gp;
+    const T        *vk;            // per-voxel moduli (nullable): K (thermal: k) ...
+    const T        *vg;            // ... and G, replacing the phase table's
+    const T        *ub;            // SENS: second displacement field and gradient
+    const T        *g0b;
+    double         *sens;          // SENS: [N][H == 1 ? 1 : 2]
 };
+
+// isotropic stress with per-voxel moduli: K, G (mechanics, Mandel) or k (thermal)
+template <typename T, int H>
+__device__ __forceinline__ void voxel_iso_stress(T K, T G, const T (&e)[H == 1 ? 3 : 6], T (&s)[H == 1 ? 3 : 6])
+{
+    if constexpr (H == 1) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            s[i] = K * e[i];
+    } else {
+        const T lam = K - (T) (2.0 / 3.0) * G, b1 = lam * (e[0] + e[1] + e[2]), b2 = (T) 2 * G;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            s[i] = b1 + b2 * e[i];
+#pragma unroll
+        for (int i = 3; i < 6; ++i)
+            s[i] = b2 * e[i];
+    }
+}
 
 template <typename T, int H, int NGP, int MODE>
 __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
@@ -746,6 +774,58 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
 #pragma unroll
         for (int k = 0; k < NS; ++k)
             g0[k] = a.g0[k];
+        const T vK = a.vk ? a.vk[el] : (T) 0, vG = (a.vk && H == 3) ? a.vg[el] : (T) 0;
+        if constexpr (MODE == ELEM_SENS) {
+            // second field at the same corners
+            T ub[8][H];
+#pragma unroll
+            for (int cr = 0; cr < 8; ++cr) {
+                const int64_t n =
+                    ((int64_t) ((cr & 1) ? x1 : x) * a.ny + ((cr & 2) ? y1 : y)) * a.nz + ((cr & 4) ? z1 : z);
+#pragma unroll
+                for (int c = 0; c < H; ++c)
+                    ub[cr][c] = a.ub[n * H + c];
+            }
+            T Hx[2][2][H], Hy[2][2][H], Hz[2][2][H];
+            grad_factors<T, H, NGP>(ub, a.geom, Hx, Hy, Hz);
+            T g0b[NS];
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+                g0b[k] = a.g0b[k];
+            double sk = 0.0, sg = 0.0;
+#pragma unroll
+            for (int gp = 0; gp < NGP; ++gp) {
+                const int i = gp & 1, j = (gp >> 1) & 1, k = gp >> 2;
+                T         ga[H][3], gb[H][3];
+#pragma unroll
+                for (int c = 0; c < H; ++c) {
+                    ga[c][0] = Gx[j][k][c], ga[c][1] = Gy[i][k][c], ga[c][2] = Gz[i][j][c];
+                    gb[c][0] = Hx[j][k][c], gb[c][1] = Hy[i][k][c], gb[c][2] = Hz[i][j][c];
+                }
+                T ea[NS], eb[NS];
+                grad_to_strain<T, H>(ga, g0, ea);
+                grad_to_strain<T, H>(gb, g0b, eb);
+                double dot = 0.0;
+#pragma unroll
+                for (int c = 0; c < NS; ++c)
+                    dot += (double) ea[c] * (double) eb[c];
+                if constexpr (H == 1) {
+                    sk += dot;  // dC/dk = I
+                } else {
+                    const double tra = (double) ea[0] + ea[1] + ea[2], trb = (double) eb[0] + eb[1] + eb[2];
+                    sk += tra * trb;                             // dC/dK = m m^T
+                    sg += 2.0 * dot - (2.0 / 3.0) * tra * trb;  // dC/dG = 2 I - 2/3 m m^T
+                }
+            }
+            const double w = a.geom.w;
+            if constexpr (H == 1) {
+                a.sens[el] = w * sk;
+            } else {
+                a.sens[2 * el]     = w * sk;
+                a.sens[2 * el + 1] = w * sg;
+            }
+            return;
+        }
         double eavg[NS], savg[NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k)
@@ -781,7 +861,10 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
                     a.js.st[((int64_t) 6 * NGP + gp) * a.js.nel + el] = qo;
                 }
             } else {
-                material_point<T, H>(P, e, s, a.js, el, gp, NGP);
+                if (a.vk)
+                    voxel_iso_stress<T, H>(vK, vG, e, s);
+                else
+                    material_point<T, H>(P, e, s, a.js, el, gp, NGP);
 #pragma unroll
                 for (int c = 0; c < NS; ++c) {
                     eavg[c] += (double) e[c] / NGP;

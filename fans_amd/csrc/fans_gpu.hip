@@ -225,6 +225,12 @@ struct fans_gpu_ctx {
     cplx<double> *spec64 = nullptr;
     double       *G64 = nullptr, *keref_d = nullptr;
     float        *G32 = nullptr;
+    int           gxq = 0, gyq = 0;  // stored kx / ky counts of the Green operator
+    // per-voxel isotropic moduli (K, G; thermal: k in vk), replacing the phase table
+    bool          voxel_moduli = false, user_ref = false;
+    double       *vk64 = nullptr, *vg64 = nullptr;
+    float        *vk32 = nullptr, *vg32 = nullptr;
+    double       *ub_d = nullptr, *sens_d = nullptr;
     double       *js = nullptr;
     double       *g0d = nullptr;
     float        *g0f = nullptr;
@@ -336,6 +342,29 @@ void harvest_timing(fans_gpu_ctx *c)
     c->ev_used.clear();
 }
 
+// Is the component-major reference element stiffness symmetric under the
+// mirror of axis `ax` (0 = x, 1 = y)?  The mirror maps corner a to a ^ (1 << ax)
+// and flips the sign of displacement component ax (mechanics); then
+// G(-k_ax) = R G(k_ax) R and only half of that axis' frequencies need storing.
+bool keref_mirror_symmetric(const double *ke, int h, int ax)
+{
+    const int ne = 8 * h;
+    double    mx = 0.0;
+    for (int k = 0; k < ne * ne; ++k)
+        mx = std::max(mx, std::fabs(ke[k]));
+    for (int i = 0; i < h; ++i)
+        for (int a = 0; a < 8; ++a)
+            for (int j = 0; j < h; ++j)
+                for (int b = 0; b < 8; ++b) {
+                    const double ri = (h == 3 && i == ax) ? -1.0 : 1.0, rj = (h == 3 && j == ax) ? -1.0 : 1.0;
+                    const double v  = ke[(8 * i + a) * ne + 8 * j + b];
+                    const double m  = ke[(8 * i + (a ^ (1 << ax))) * ne + 8 * j + (b ^ (1 << ax))];
+                    if (std::fabs(v - ri * rj * m) > 1e-13 * mx)
+                        return false;
+                }
+    return true;
+}
+
 // forget the timing records from index `from` on (launches that returned at entry)
 void drop_timing(fans_gpu_ctx *c, size_t from)
 {
@@ -354,12 +383,15 @@ int grid_vec(int64_t n) { return (int) std::max<int64_t>(1, std::min<int64_t>((n
 // ---------------------------------------------------------------------------
 bool reg_len(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256; }
 
+#ifndef FANS_REG_LDS_KIB
+#define FANS_REG_LDS_KIB 36
+#endif
 // columns per block: a power of two (>= 4) with ~36 KiB of LDS, at most 256 threads
 template <typename T, int N1, int N2>
 constexpr int reg_cols()
 {
     constexpr int n   = N1 * N2;
-    constexpr int lim = (int) (36 * 1024 / (n * sizeof(cplx<T>)));
+    constexpr int lim = (int) (FANS_REG_LDS_KIB * 1024 / (n * sizeof(cplx<T>)));
     int           c   = 4;
     while (c * 2 <= lim && c * 2 * N2 <= 256)
         c *= 2;
@@ -393,36 +425,34 @@ void launch_strided(fans_gpu_ctx *c, const FftLen &L, const cplx<T> *tw, int bc,
 }
 
 template <typename T, int H, int N1, int N2>
-void launch_x_green_reg(fans_gpu_ctx *c, cplx<T> *spec, const cplx<T> *tw, const T *G, int Mc, int kz_lo, int kz_hi,
-                        const int *done)
+void launch_x_green_reg(fans_gpu_ctx *c, cplx<T> *spec, const cplx<T> *tw, const GreenView<T> &gv, int kz_lo,
+                        int kz_hi, const int *done)
 {
-    constexpr int HP = H == 3 ? 4 : 1;
+    constexpr int HP  = H == 3 ? 4 : 1;
     constexpr int KB  = reg_cols<T, N1, N2>() / HP;
     const int     nbk = (kz_hi - kz_lo + KB - 1) / KB;
     hipLaunchKernelGGL((k_fft_x_green_reg<T, H, N1, N2, KB>), dim3(nbk * c->ny), dim3(KB * HP * N2), 0, c->stream,
-                       spec, tw, G, c->ny, Mc, kz_lo, kz_hi, nbk, done);
+                       spec, tw, gv, kz_lo, kz_hi, nbk, done);
 }
 
 template <typename T, int H>
 void launch_x_green(fans_gpu_ctx *c, Fft3D<T> &P, cplx<T> *spec, const T *G, int kz_lo, int kz_hi, const int *done)
 {
+    const GreenView<T> gv{G, c->nx, c->ny, c->gxq, c->gyq, P.Mc};
     switch (reg_len(P.Lx.n) ? P.Lx.n : 0) {
-    case 256: launch_x_green_reg<T, H, 16, 16>(c, spec, P.twx, G, P.Mc, kz_lo, kz_hi, done); break;
-    case 128: launch_x_green_reg<T, H, 16, 8>(c, spec, P.twx, G, P.Mc, kz_lo, kz_hi, done); break;
-    case 64: launch_x_green_reg<T, H, 8, 8>(c, spec, P.twx, G, P.Mc, kz_lo, kz_hi, done); break;
-    case 32: launch_x_green_reg<T, H, 8, 4>(c, spec, P.twx, G, P.Mc, kz_lo, kz_hi, done); break;
-    case 16: launch_x_green_reg<T, H, 4, 4>(c, spec, P.twx, G, P.Mc, kz_lo, kz_hi, done); break;
+    case 256: launch_x_green_reg<T, H, 16, 16>(c, spec, P.twx, gv, kz_lo, kz_hi, done); break;
+    case 128: launch_x_green_reg<T, H, 16, 8>(c, spec, P.twx, gv, kz_lo, kz_hi, done); break;
+    case 64: launch_x_green_reg<T, H, 8, 8>(c, spec, P.twx, gv, kz_lo, kz_hi, done); break;
+    case 32: launch_x_green_reg<T, H, 8, 4>(c, spec, P.twx, gv, kz_lo, kz_hi, done); break;
+    case 16: launch_x_green_reg<T, H, 4, 4>(c, spec, P.twx, gv, kz_lo, kz_hi, done); break;
     default: {
         dim3 g((kz_hi - kz_lo + P.bk - 1) / P.bk, c->ny);
-        hipLaunchKernelGGL((k_fft_x_green<T, H>), g, dim3(kThreads), P.smx, c->stream, spec, P.Lx, P.twx, G, c->ny,
-                           P.Mc, kz_lo, kz_hi, P.bk, done);
+        hipLaunchKernelGGL((k_fft_x_green<T, H>), g, dim3(kThreads), P.smx, c->stream, spec, P.Lx, P.twx, gv, kz_lo,
+                           kz_hi, P.bk, done);
     }
     }
 }
 
-// ---------------------------------------------------------------------------
-// FFT convolution: out = G * r  with the requested epilogue
-// ---------------------------------------------------------------------------
 int reduce(fans_gpu_ctx *c, const double *partial, int nblk, int nq, int stage);
 
 // z passes: compile-time kernels where M = nz/2 is a power of two in 16..128
@@ -563,6 +593,15 @@ AsmArgs<T> asm_args(fans_gpu_ctx *c)
     a.partial = c->partial;
     a.measure = c->d.measure;
     a.done    = nullptr;
+    if (c->voxel_moduli) {
+        if constexpr (sizeof(T) == sizeof(double)) {
+            a.vk = reinterpret_cast<const T *>(c->vk64);
+            a.vg = reinterpret_cast<const T *>(c->vg64);
+        } else {
+            a.vk = reinterpret_cast<const T *>(c->vk32);
+            a.vg = reinterpret_cast<const T *>(c->vg32);
+        }
+    }
     return a;
 }
 
@@ -674,6 +713,8 @@ int homog_stress(fans_gpu_ctx *c, double *out)
     a.js      = J2State{c->js, c->N};
     a.u       = c->u;
     a.g0      = c->g0d;
+    a.vk      = c->voxel_moduli ? c->vk64 : nullptr;
+    a.vg      = c->voxel_moduli ? c->vg64 : nullptr;
     a.partial = c->partial;
     {
         Timed t(c, KF_ELEMENT);
@@ -720,6 +761,8 @@ int commit_state(fans_gpu_ctx *c)
     a.js     = J2State{c->js, c->N};
     a.u      = c->u;
     a.g0     = c->g0d;
+    a.vk      = c->voxel_moduli ? c->vk64 : nullptr;
+    a.vg      = c->voxel_moduli ? c->vg64 : nullptr;
     Timed t(c, KF_ELEMENT);
     if (c->ngp == 8)
         hipLaunchKernelGGL((k_element_pass<double, 3, 8, ELEM_COMMIT>), dim3(c->nblk_el), dim3(kThreads), 0,
@@ -929,14 +972,34 @@ int solve_cg_fp64(fans_gpu_ctx *c)
             return rc;
         HIPCHK(c, hipMemcpyAsync(c->sc + SC_DELTAMID, &sc[SC_RES0], sizeof(double), hipMemcpyHostToDevice,
                                  c->stream));
-        EpiArgs<double> ep{};
-        ep.out     = c->s;
-        ep.r       = c->r;
-        ep.partial = c->partial;
-        if ((rc = convolve_dev<double>(c, c->fft64, c->r, c->spec64, c->G64, EPI_CG, ep, nullptr)))
-            return rc;
-        if ((rc = reduce(c, c->partial, (int) c->fft64.nblk_z, 1, RED_DELTA)))
-            return rc;
+        if (c->d.precision == FANS_PREC_MIXED) {
+            // mixed form of the nonlinear solve: the Green preconditioner s = -G r
+            // in FP32 (FFTs and Green multiply); residuals, the J2 return map,
+            // dot products and the line search stay FP64
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r32,
+                               c->r, c->Nh);
+            EpiArgs<float> ep{};
+            ep.out     = c->s32;
+            ep.r       = c->r32;
+            ep.partial = c->partial;
+            if ((rc = convolve_dev<float>(c, c->fft32, c->r32, c->spec32, c->G32, EPI_CG, ep, nullptr)))
+                return rc;
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->s,
+                               c->s32, c->Nh);
+            hipLaunchKernelGGL((k_dot<double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r, c->s, c->Nh,
+                               c->d.measure, c->partial);
+            if ((rc = reduce(c, c->partial, c->nblk_vec, 1, RED_DELTA)))
+                return rc;
+        } else {
+            EpiArgs<double> ep{};
+            ep.out     = c->s;
+            ep.r       = c->r;
+            ep.partial = c->partial;
+            if ((rc = convolve_dev<double>(c, c->fft64, c->r, c->spec64, c->G64, EPI_CG, ep, nullptr)))
+                return rc;
+            if ((rc = reduce(c, c->partial, (int) c->fft64.nblk_z, 1, RED_DELTA)))
+                return rc;
+        }
         hipLaunchKernelGGL((k_direction<double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, d, c->s, c->sc,
                            c->Nh);
         // --- LineSearchSecant ---
@@ -1032,8 +1095,10 @@ int solve_fp_fp64(fans_gpu_ctx *c)
 int solve_cg_mixed(fans_gpu_ctx *c)
 {
     int rc;
+    // nonlinear materials or mixed boundary conditions: the nonlinear CG with
+    // an FP32 Green preconditioner (solve_cg_fp64 checks the precision)
     if (!(c->all_linear && !c->mixed_active))
-        return fail(c, FANS_GPU_EUNSUP, "mixed precision form supports linear materials without mixed BCs");
+        return solve_cg_fp64(c);
     // outer residual
     if ((rc = init_scalars(c, c->tol, c->tol_type, c->d.n_it)))
         return rc;
@@ -1137,7 +1202,8 @@ void free_all(fans_gpu_ctx *c)
                     (void *) c->spec64, (void *) c->G64, (void *) c->keref_d, (void *) c->G32, (void *) c->js,
                     (void *) c->g0d, (void *) c->g0f, (void *) c->partial, (void *) c->sc, (void *) c->is,
                     (void *) c->hist, (void *) c->r32, (void *) c->s32, (void *) c->dA32, (void *) c->dB32,
-                    (void *) c->rnew32, (void *) c->x32, (void *) c->spec32})
+                    (void *) c->rnew32, (void *) c->x32, (void *) c->spec32, (void *) c->vk64, (void *) c->vg64,
+                    (void *) c->vk32, (void *) c->vg32, (void *) c->ub_d, (void *) c->sens_d})
         if (p)
             (void) hipFree(p);
     c->fft64.release();
@@ -1382,6 +1448,41 @@ int fans_gpu_set_microstructure(fans_gpu_ctx *c, const uint16_t *ms, int64_t cou
     return 0;
 }
 
+// reference element stiffness (from c->kref) and the Green operator on the device
+int rebuild_green(fans_gpu_ctx *c)
+{
+    ref_element_stiffness(c);
+    const int ne = 8 * c->h, npk = c->h * (c->h + 1) / 2;
+    if (!c->keref_d)
+        HIPCHK(c, hipMalloc(&c->keref_d, sizeof(double) * 576));
+    HIPCHK(c, hipMemcpy(c->keref_d, c->keref, sizeof(double) * ne * ne, hipMemcpyHostToDevice));
+    if (!c->G64)
+        HIPCHK(c, hipMalloc(&c->G64, sizeof(double) * c->Nf * npk));
+    // quarter storage when the reference element is mirror symmetric in x and y
+    const bool quarter = c->nx % 2 == 0 && c->ny % 2 == 0 && keref_mirror_symmetric(c->keref, c->h, 0) &&
+                         keref_mirror_symmetric(c->keref, c->h, 1);
+    c->gxq            = quarter ? c->nx / 2 + 1 : c->nx;
+    c->gyq            = quarter ? c->ny / 2 + 1 : c->ny;
+    const int64_t ngs = (int64_t) c->gxq * c->gyq * (c->nz / 2 + 1);  // stored frequencies
+    const int     gb  = (int) ((ngs + kThreads - 1) / kThreads);
+    if (c->h == 1)
+        hipLaunchKernelGGL((k_build_green<1>), dim3(gb), dim3(kThreads), 0, c->stream, c->keref_d, c->G64, c->nx,
+                           c->ny, c->nz, c->gxq, c->gyq);
+    else
+        hipLaunchKernelGGL((k_build_green<3>), dim3(gb), dim3(kThreads), 0, c->stream, c->keref_d, c->G64, c->nx,
+                           c->ny, c->nz, c->gxq, c->gyq);
+    HIPCHK(c, hipGetLastError());
+    if (c->d.precision == FANS_PREC_MIXED) {
+        if (!c->G32)
+            HIPCHK(c, hipMalloc(&c->G32, sizeof(float) * c->Nf * npk));
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_vec(ngs * npk)), dim3(kThreads), 0, c->stream,
+                           c->G32, c->G64, ngs * npk);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, const int32_t *nphase,
                            const int32_t *phase_ids, const double *params, int64_t n_params, const double *ref)
 {
@@ -1439,6 +1540,7 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
             return fail(c, FANS_GPU_EINVAL, "MaterialManager: Phase %d not assigned", p);
     // reference stiffness (MaterialManager.h:177-205)
     const int ns = c->nstr;
+    c->user_ref  = ref != nullptr;
     if (ref) {
         std::memcpy(c->kref, ref, sizeof(double) * ns * ns);
     } else {
@@ -1536,29 +1638,9 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
         HIPCHK(c, hipMalloc(&c->js, bytes));
         HIPCHK(c, hipMemset(c->js, 0, bytes));
     }
-    // reference element stiffness + Green operator
-    ref_element_stiffness(c);
-    const int ne = 8 * c->h, npk = c->h * (c->h + 1) / 2;
-    if (!c->keref_d)
-        HIPCHK(c, hipMalloc(&c->keref_d, sizeof(double) * 576));
-    HIPCHK(c, hipMemcpy(c->keref_d, c->keref, sizeof(double) * ne * ne, hipMemcpyHostToDevice));
-    if (!c->G64)
-        HIPCHK(c, hipMalloc(&c->G64, sizeof(double) * c->Nf * npk));
-    const int gb = (int) ((c->Nf + kThreads - 1) / kThreads);
-    if (c->h == 1)
-        hipLaunchKernelGGL((k_build_green<1>), dim3(gb), dim3(kThreads), 0, c->stream, c->keref_d, c->G64, c->nx,
-                           c->ny, c->nz);
-    else
-        hipLaunchKernelGGL((k_build_green<3>), dim3(gb), dim3(kThreads), 0, c->stream, c->keref_d, c->G64, c->nx,
-                           c->ny, c->nz);
-    HIPCHK(c, hipGetLastError());
-    if (c->d.precision == FANS_PREC_MIXED) {
-        if (!c->G32)
-            HIPCHK(c, hipMalloc(&c->G32, sizeof(float) * c->Nf * npk));
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_vec(c->Nf * npk)), dim3(kThreads), 0, c->stream,
-                           c->G32, c->G64, c->Nf * npk);
-        HIPCHK(c, hipGetLastError());
-    }
+    c->voxel_moduli = false;  // new phase tables replace per-voxel moduli
+    if (const int rcg = rebuild_green(c))
+        return rcg;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->all_iso = true;
     for (int p = 0; p < c->nphases; ++p)
@@ -1828,6 +1910,8 @@ int fans_gpu_postprocess(fans_gpu_ctx *c, double *strain_elem, double *stress_el
     a.js          = J2State{c->js, c->N};
     a.u           = c->u;
     a.g0          = c->g0d;
+    a.vk      = c->voxel_moduli ? c->vk64 : nullptr;
+    a.vg      = c->voxel_moduli ? c->vg64 : nullptr;
     a.strain_elem = de;
     a.stress_elem = ds;
     a.strain_gp   = dge;
@@ -2011,10 +2095,136 @@ int fans_gpu_get_green(fans_gpu_ctx *c, double *out, int64_t count)
 {
     if (!c || !out)
         return FANS_GPU_EINVAL;
-    const int64_t n = c->Nf * c->h * (c->h + 1) / 2;
+    const int     np = c->h * (c->h + 1) / 2, Mc = c->nz / 2 + 1;
+    const int64_t n  = c->Nf * np;
     if (count != n || !c->G64)
         return fail(c, FANS_GPU_EINVAL, "green size mismatch or not built");
-    HIPCHK(c, hipMemcpy(out, c->G64, sizeof(double) * n, hipMemcpyDeviceToHost));
+    std::vector<double> st((size_t) c->gxq * c->gyq * Mc * np);
+    HIPCHK(c, hipMemcpy(st.data(), c->G64, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
+    // expand the stored frequencies to the full [nx][ny][nz/2+1] layout
+    for (int ix = 0; ix < c->nx; ++ix)
+        for (int iy = 0; iy < c->ny; ++iy) {
+            const bool   fx = ix >= c->gxq, fy = iy >= c->gyq;
+            const int    qx = fx ? c->nx - ix : ix, qy = fy ? c->ny - iy : iy;
+            const double sx = fx ? -1.0 : 1.0, sy = fy ? -1.0 : 1.0;
+            for (int iz = 0; iz < Mc; ++iz) {
+                const double *g = &st[(((size_t) qx * c->gyq + qy) * Mc + iz) * np];
+                double       *o = out + (((int64_t) ix * c->ny + iy) * Mc + iz) * np;
+                if (np == 1) {
+                    o[0] = g[0];
+                } else {
+                    const double sg[6] = {1.0, sx * sy, sx, 1.0, sy, 1.0};
+                    for (int k = 0; k < 6; ++k)
+                        o[k] = g[k] * sg[k];
+                }
+            }
+        }
+    return 0;
+}
+
+int fans_gpu_set_voxel_moduli(fans_gpu_ctx *c, const double *k, const double *g, int64_t count)
+{
+    if (!c || !k || (c->h == 3 && !g))
+        return FANS_GPU_EINVAL;
+    if (!c->have_mat)
+        return fail(c, FANS_GPU_ESTATE, "set_materials must precede set_voxel_moduli");
+    if (count != c->N)
+        return fail(c, FANS_GPU_EINVAL, "voxel moduli: expected %lld values, got %lld", (long long) c->N,
+                    (long long) count);
+    if (!c->all_iso)
+        return fail(c, FANS_GPU_EINVAL, "per-voxel moduli need LinearElasticIsotropic / LinearThermalIsotropic phases");
+    for (int64_t i = 0; i < count; ++i)
+        if (!(k[i] > 0.0) || (c->h == 3 && !(g[i] > 0.0)))
+            return fail(c, FANS_GPU_EINVAL, "voxel moduli must be positive (voxel %lld)", (long long) i);
+    const size_t b64 = sizeof(double) * c->N, b32 = sizeof(float) * c->N;
+    if (!c->vk64) {
+        HIPCHK(c, hipMalloc(&c->vk64, b64));
+        HIPCHK(c, hipMalloc(&c->vg64, b64));
+        HIPCHK(c, hipMalloc(&c->vk32, b32));
+        HIPCHK(c, hipMalloc(&c->vg32, b32));
+    }
+    HIPCHK(c, hipMemcpy(c->vk64, k, b64, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->vg64, c->h == 3 ? g : k, b64, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_vec(c->N)), dim3(kThreads), 0, c->stream, c->vk32,
+                       c->vk64, c->N);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_vec(c->N)), dim3(kThreads), 0, c->stream, c->vg32,
+                       c->vg64, c->N);
+    HIPCHK(c, hipGetLastError());
+    c->voxel_moduli = true;
+    // reference stiffness as if every voxel were its own phase:
+    // LinearElasticIsotropic (max + min) / 2 of lambda and mu (LinearElastic.h:55-69),
+    // LinearThermalIsotropic mean conductivity (LinearThermal.h:19-45)
+    if (!c->user_ref) {
+        std::memset(c->kref, 0, sizeof(c->kref));
+        if (c->h == 1) {
+            double m = 0.0;
+            for (int64_t i = 0; i < count; ++i)
+                m += k[i];
+            m /= (double) count;
+            for (int i = 0; i < 3; ++i)
+                c->kref[i * 3 + i] = m;
+        } else {
+            double lmax = -INFINITY, lmin = INFINITY, mmax = -INFINITY, mmin = INFINITY;
+            for (int64_t i = 0; i < count; ++i) {
+                const double lam = k[i] - (2.0 / 3.0) * g[i];
+                lmax = std::max(lmax, lam), lmin = std::min(lmin, lam);
+                mmax = std::max(mmax, g[i]), mmin = std::min(mmin, g[i]);
+            }
+            const double lr = (lmax + lmin) / 2, mr = (mmax + mmin) / 2;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    c->kref[i * 6 + j] = lr;
+            for (int i = 0; i < 6; ++i)
+                c->kref[i * 6 + i] += 2 * mr;
+        }
+    }
+    return rebuild_green(c);
+}
+
+int fans_gpu_sensitivity(fans_gpu_ctx *c, const double *ub, const double *g0b, double *sens, int64_t count)
+{
+    if (!c || !ub || !g0b || !sens)
+        return FANS_GPU_EINVAL;
+    const int nq = c->h == 1 ? 1 : 2;
+    if (count != c->N * nq)
+        return fail(c, FANS_GPU_EINVAL, "sensitivity: expected %lld values", (long long) (c->N * nq));
+    if (!c->all_iso)
+        return fail(c, FANS_GPU_EUNSUP, "sensitivities are implemented for isotropic linear phases");
+    if (!c->ub_d) {
+        HIPCHK(c, hipMalloc(&c->ub_d, sizeof(double) * (c->Nh + 16)));
+        HIPCHK(c, hipMalloc(&c->sens_d, sizeof(double) * c->N * 2));
+    }
+    HIPCHK(c, hipMemcpy(c->ub_d, ub, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->ub_d + c->Nh, g0b, sizeof(double) * c->nstr, hipMemcpyHostToDevice));
+    ElemArgs<double> a{};
+    a.nx     = c->nx;
+    a.ny     = c->ny;
+    a.nz     = c->nz;
+    a.geom   = c->geom;
+    a.ms     = c->ms;
+    a.phases = c->dph64;
+    a.u      = c->u;
+    a.g0     = c->g0d;
+    a.ub     = c->ub_d;
+    a.g0b    = c->ub_d + c->Nh;
+    a.sens   = c->sens_d;
+#define EP(HH, GG)                                                                                                  \
+    hipLaunchKernelGGL((k_element_pass<double, HH, GG, ELEM_SENS>), dim3(c->nblk_el), dim3(kThreads), 0, c->stream, \
+                       a)
+    if (c->h == 1) {
+        if (c->ngp == 8)
+            EP(1, 8);
+        else
+            EP(1, 1);
+    } else {
+        if (c->ngp == 8)
+            EP(3, 8);
+        else
+            EP(3, 1);
+    }
+#undef EP
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpy(sens, c->sens_d, sizeof(double) * count, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -27,6 +27,29 @@ struct FftLen {
     int rad[14];  // radices in application order
 };
 
+// Green operator storage, G[(kxq * gyq + kyq) * Mc + kz][H(H+1)/2] packed
+// (00 01 02 11 12 22), 1/N folded in.  Full storage: gxq = nx, gyq = ny.
+// Quarter storage (the reference element stiffness is symmetric under the x
+// and y mirrors and n_x, n_y are even): only kx <= nx/2 and ky <= ny/2 are
+// stored, and G(-kx) = Rx G(kx) Rx with Rx = diag(-1, 1, 1) flips the xy and
+// xz entries (G(-ky): the xy and yz entries).
+template <typename T>
+struct GreenView {
+    const T *G;
+    int      nx, ny, gxq, gyq, Mc;
+};
+
+// stored block of frequency (kx, ky, kz) and the mirror signs sx, sy
+template <typename T, int H>
+__device__ __forceinline__ const T *green_at(const GreenView<T> &gv, int kx, int ky, int kz, T &sx, T &sy)
+{
+    const bool fx = kx >= gv.gxq, fy = ky >= gv.gyq;
+    sx            = fx ? (T) -1 : (T) 1;
+    sy            = fy ? (T) -1 : (T) 1;
+    const int kxq = fx ? gv.nx - kx : kx, kyq = fy ? gv.ny - ky : ky;
+    return gv.G + (((int64_t) kxq * gv.gyq + kyq) * gv.Mc + kz) * (H * (H + 1) / 2);
+}
+
 // Register budget of the in-place stages: a thread holds at most FFT_MAXV
 // complex values; callers keep n * ncols <= FFT_MAXV * 240 so every radix
 // (including 3 and 5, where MAXV/R rounds down) fits.
@@ -458,13 +481,12 @@ __global__ __launch_bounds__(256) void k_fft_strided(cplx<T> *__restrict__ data,
 // G packed per frequency [x][y][kz][H(H+1)/2] = (00 01 02 11 12 22) or (0).
 template <typename T, int H>
 __global__ __launch_bounds__(256) void k_fft_x_green(cplx<T> *__restrict__ data, FftLen L,
-                                                     const cplx<T> *__restrict__ tw, const T *__restrict__ G, int ny,
-                                                     int Mc, int kz_lo, int kz_hi, int bk,
-                                                     const int *__restrict__ done)
+                                                     const cplx<T> *__restrict__ tw, GreenView<T> gv, int kz_lo,
+                                                     int kz_hi, int bk, const int *__restrict__ done)
 {
     if (done && *done)
         return;
-    constexpr int NP = H * (H + 1) / 2;
+    const int ny = gv.ny, Mc = gv.Mc;
     extern __shared__ char smem_raw[];
     cplx<T>      *A     = reinterpret_cast<cplx<T> *>(smem_raw);
     const int     y     = blockIdx.y;
@@ -481,13 +503,14 @@ __global__ __launch_bounds__(256) void k_fft_x_green(cplx<T> *__restrict__ data,
     lds_fft<T, false>(A, L, tw, ncols, ld);
     for (int w = threadIdx.x; w < L.n * nk; w += blockDim.x) {
         const int i = w / nk, kk = w - i * nk;
-        const T  *g = G + (((int64_t) i * ny + y) * Mc + kz0 + kk) * NP;
+        T         sx, sy;
+        const T  *g = green_at<T, H>(gv, i, y, kz0 + kk, sx, sy);
         cplx<T>  *v = A + i * ld + kk * H;
         if constexpr (H == 1) {
             v[0] = cscale(v[0], g[0]);
         } else {
             const cplx<T> a = v[0], b = v[1], c = v[2];
-            const T       g00 = g[0], g01 = g[1], g02 = g[2], g11 = g[3], g12 = g[4], g22 = g[5];
+            const T       g00 = g[0], g01 = g[1] * sx * sy, g02 = g[2] * sx, g11 = g[3], g12 = g[4] * sy, g22 = g[5];
             v[0]                = cplx<T>{g00 * a.x + g01 * b.x + g02 * c.x, g00 * a.y + g01 * b.y + g02 * c.y};
             v[1]                = cplx<T>{g01 * a.x + g11 * b.x + g12 * c.x, g01 * a.y + g11 * b.y + g12 * c.y};
             v[2]                = cplx<T>{g02 * a.x + g12 * b.x + g22 * c.x, g02 * a.y + g12 * b.y + g22 * c.y};

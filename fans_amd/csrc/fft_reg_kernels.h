@@ -13,6 +13,7 @@
 // second exchange (used by the x pass around the Green operator).
 #pragma once
 #include "fft_butterflies.h"
+#include "fft_kernels.h"
 
 namespace fans {
 
@@ -171,8 +172,8 @@ __device__ __forceinline__ float quad_bcast(float x)
 // XCD-mapped 1-D grid (nbk * ny blocks) owns kz in [kz_lo + bk*KB, ... + KB).
 template <typename T, int H, int N1, int N2, int KB>
 __global__ __launch_bounds__(KB *(H == 3 ? 4 : 1) * N2) void k_fft_x_green_reg(
-    cplx<T> *__restrict__ data, const cplx<T> *__restrict__ tw, const T *__restrict__ G, int ny, int Mc, int kz_lo,
-    int kz_hi, int nbk, const int *__restrict__ done)
+    cplx<T> *__restrict__ data, const cplx<T> *__restrict__ tw, GreenView<T> gv, int kz_lo, int kz_hi, int nbk,
+    const int *__restrict__ done)
 {
     if (done && *done)
         return;
@@ -180,8 +181,8 @@ __global__ __launch_bounds__(KB *(H == 3 ? 4 : 1) * N2) void k_fft_x_green_reg(
     constexpr int HP = H == 3 ? 4 : 1;
     constexpr int C  = KB * HP;
     constexpr int LD = C + 1;
-    constexpr int NP = H * (H + 1) / 2;
     __shared__ cplx<T> tile[N * LD];
+    const int          ny = gv.ny, Mc = gv.Mc;
     const int          t = threadIdx.x, col = t % C, j = t / C;
     const int          kk = col / HP, c = col % HP;
     const int          lb = xcd_tile_index(blockIdx.x, gridDim.x);
@@ -201,16 +202,21 @@ __global__ __launch_bounds__(KB *(H == 3 ? 4 : 1) * N2) void k_fft_x_green_reg(
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2) {
             const int     row = j + N2 * q + N1 * k2;  // x frequency
-            const T      *g   = G + (((int64_t) row * ny + y) * Mc + kzg) * NP;
-            cplx<T>      &a   = v[q * N2 + k2];
+            T             sx, sy;
+            const T      *g = green_at<T, H>(gv, row, y, kzg, sx, sy);
+            cplx<T>      &a = v[q * N2 + k2];
             if constexpr (H == 1) {
                 a = cscale(a, g[0]);
             } else {
-                // row c of the packed symmetric 3x3 (00 01 02 11 12 22)
+                // row c of the packed symmetric 3x3 (00 01 02 11 12 22) with the
+                // mirror signs of the xy (sx sy), xz (sx) and yz (sy) entries
                 const int i0 = c == 0 ? 0 : (c == 1 ? 1 : 2);
                 const int i1 = c == 0 ? 1 : (c == 1 ? 3 : 4);
                 const int i2 = c == 0 ? 2 : (c == 1 ? 4 : 5);
-                const T   ga = g[c < 3 ? i0 : 0], gb = g[c < 3 ? i1 : 0], gc = g[c < 3 ? i2 : 0];
+                const T   s0 = c == 0 ? (T) 1 : (c == 1 ? sx * sy : sx);
+                const T   s1 = c == 0 ? sx * sy : (c == 1 ? (T) 1 : sy);
+                const T   s2 = c == 0 ? sx : (c == 1 ? sy : (T) 1);
+                const T   ga = g[c < 3 ? i0 : 0] * s0, gb = g[c < 3 ? i1 : 0] * s1, gc = g[c < 3 ? i2 : 0] * s2;
                 const cplx<T> v0{quad_bcast<0>(a.x), quad_bcast<0>(a.y)};
                 const cplx<T> v1{quad_bcast<1>(a.x), quad_bcast<1>(a.y)};
                 const cplx<T> v2{quad_bcast<2>(a.x), quad_bcast<2>(a.y)};

@@ -64,18 +64,19 @@ __device__ __forceinline__ void jacobi_eig(double (&a)[N][N], double (&V)[N][N])
 }
 
 // keref: component-major (8H)x(8H) reference element stiffness (matmodel.h:237-253)
-// G: [nx][ny][nz/2+1][H(H+1)/2] packed (00 01 02 11 12 22)
+// G: [gxq][gyq][nz/2+1][H(H+1)/2] packed (00 01 02 11 12 22) for the stored
+// frequencies kx < gxq, ky < gyq (see GreenView)
 template <int H>
 __global__ __launch_bounds__(256) void k_build_green(const double *__restrict__ keref, double *__restrict__ G, int nx,
-                                                     int ny, int nz)
+                                                     int ny, int nz, int gxq, int gyq)
 {
     constexpr int NP = H * (H + 1) / 2, NE = 8 * H;
     const int     Mc = nz / 2 + 1;
-    const int64_t nf = (int64_t) nx * ny * Mc;
+    const int64_t nf = (int64_t) gxq * gyq * Mc;
     const int64_t f  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nf)
         return;
-    const int iz = (int) (f % Mc), iy = (int) ((f / Mc) % ny), ix = (int) (f / ((int64_t) Mc * ny));
+    const int iz = (int) (f % Mc), iy = (int) ((f / Mc) % gyq), ix = (int) (f / ((int64_t) Mc * gyq));
     double   *g  = G + f * NP;
     if (ix == 0 && iy == 0 && iz == 0) {
 #pragma unroll

@@ -57,6 +57,7 @@ EXPORTS = [
     "fans_gpu_postprocess", "fans_gpu_get_field", "fans_gpu_set_field", "fans_gpu_residual", "fans_gpu_convolve",
     "fans_gpu_get_green", "fans_gpu_get_reference_stiffness", "fans_gpu_fft_r2c", "fans_gpu_fft_c2r",
     "fans_gpu_set_option", "fans_gpu_kernel_times", "fans_gpu_kernel_family_name", "fans_gpu_reset_state",
+    "fans_gpu_set_voxel_moduli", "fans_gpu_sensitivity",
 ]
 
 
@@ -96,6 +97,8 @@ def lib():
         L.fans_gpu_kernel_times.argtypes = [vp, dp, C.c_int32, ip]
         L.fans_gpu_kernel_family_name.restype = C.c_char_p
         L.fans_gpu_kernel_family_name.argtypes = [C.c_int32]
+        L.fans_gpu_set_voxel_moduli.argtypes = [vp, dp, dp, i64]
+        L.fans_gpu_sensitivity.argtypes = [vp, dp, dp, dp, i64]
         _LIB = L
     return _LIB
 
@@ -303,6 +306,29 @@ class GpuSolver:
         out = np.zeros((nx, ny, nz // 2 + 1, npk))
         self._check(self.L.fans_gpu_get_green(self.ptr, _dp(out), out.size))
         return out
+
+    def set_voxel_moduli(self, k, g=None):
+        """Per-voxel isotropic moduli [x][y][z]: bulk k and shear g (mechanics) or
+        conductivity k (thermal); rebuilds the reference stiffness and Green operator."""
+        N = int(np.prod(self.p.dims))
+        kk = np.ascontiguousarray(np.asarray(k, dtype=np.float64).reshape(-1))
+        gg = None if g is None else np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(-1))
+        if kk.size != N or (gg is not None and gg.size != N):
+            raise ValueError(f"voxel moduli need {N} values each")
+        self._check(self.L.fans_gpu_set_voxel_moduli(self.ptr, _dp(kk), None if gg is None else _dp(gg), N))
+
+    def sensitivity(self, ub, g0b):
+        """Per-element Gauss integrals of eps_a^T dC/dp eps_b (p = K, G; thermal:
+        k) between the current solution and (ub, g0b); [N, 2] or [N, 1]."""
+        N = int(np.prod(self.p.dims))
+        nq = 1 if self.p.howmany == 1 else 2
+        ubb = np.ascontiguousarray(np.asarray(ub, dtype=np.float64).reshape(-1))
+        gb = np.ascontiguousarray(np.asarray(g0b, dtype=np.float64).reshape(-1))
+        if ubb.size != N * self.p.howmany or gb.size != self.p.n_str:
+            raise ValueError("sensitivity: field or gradient size mismatch")
+        out = np.empty(N * nq)
+        self._check(self.L.fans_gpu_sensitivity(self.ptr, _dp(ubb), _dp(gb), _dp(out), out.size))
+        return out.reshape(N, nq)
 
     def reference_stiffness(self):
         out = np.zeros((self.nstr, self.nstr))

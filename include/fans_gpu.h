@@ -124,6 +124,27 @@ int fans_gpu_set_materials(fans_gpu_ctx *ctx, int32_t n_groups, const int32_t *k
 int fans_gpu_set_gradient(fans_gpu_ctx *ctx, const double *g0, int32_t n_str);
 int fans_gpu_get_gradient(fans_gpu_ctx *ctx, double *g0, int32_t n_str);
 
+/* Per-voxel isotropic moduli replacing the phase table: mechanics k = bulk and
+ * g = shear modulus per voxel, thermal k = conductivity (g unused, may be NULL);
+ * [x][y][z] order, n_x*n_y*n_z values each.  No counterpart in the reference,
+ * whose uint16 phase ids with per-phase parameters cannot express per-voxel
+ * moduli (SIMP densities, pyfans differentiation); the reference stiffness
+ * follows the rules of LinearElastic.h:55-69 / LinearThermal.h:19-45 as if
+ * every voxel were a phase (unless set_materials got ref_material), and the
+ * Green operator is rebuilt.  Requires isotropic linear phases. */
+int fans_gpu_set_voxel_moduli(fans_gpu_ctx *ctx, const double *k, const double *g, int64_t count);
+
+/* Sensitivity of the homogenized response to per-voxel moduli.  For the
+ * context's current solution (u, g0) and a second solution ub for the macro
+ * gradient g0b (host arrays, [N][h] and n_str), writes per element the Gauss
+ * integrals sens[e] = (int_e eps_a^T dC/dK eps_b, int_e eps_a^T dC/dG eps_b)
+ * (thermal: int_e eps_a . eps_b, one value), eps_a = B u + g0, eps_b = B ub + g0b.
+ * When both solve their load cases, d(g0b . Sigma_bar(g0)) / dK_e = sens[e][0] / V
+ * (V = L_x L_y L_z): the adjoint of v . Sigma_bar is the solve for macro gradient
+ * v, the operator being self-adjoint.  No counterpart in the reference (pyfans
+ * differentiation). */
+int fans_gpu_sensitivity(fans_gpu_ctx *ctx, const double *ub, const double *g0b, double *sens, int64_t count);
+
 /* Solver::enableMixedBC / disableMixedBC (solver.h:71-78, mixedBCs.h:181-226):
  * strain path [n_steps][n_E], stress path [n_steps][n_F] */
 int fans_gpu_enable_mixed_bc(fans_gpu_ctx *ctx, int32_t n_E, const int32_t *idx_E, int32_t n_F, const int32_t *idx_F,

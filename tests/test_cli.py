@@ -1,0 +1,149 @@
+"""The drop-in CLI `fans_amd/FANS in.json out.h5` (C++ host over the C ABI).
+
+CPU: usage / version / input-error exit codes and messages of src/main.cpp and
+src/reader.cpp:63-183, and the missing-device error.  GPU: the reference's own
+test inputs (test/input_files/test_LinearThermal.json, test_LinearElastic.json
+on sphere32) run end to end; the HDF5 result layout (reader.h:330-351) and
+values are checked against the oracle.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from fans_amd import hdf5io
+from fans_amd.problem import Problem
+from tests.conftest import GOLDEN
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "fans_amd", "FANS")
+DSET = "/sphere/32x32x32/ms"
+
+
+def _need_cli():
+    if not os.path.exists(CLI):
+        pytest.skip("fans_amd/FANS is not built (python -m fans_amd.build)")
+
+
+def _run(*args, env=None):
+    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=300, env=env)
+
+
+@pytest.fixture()
+def case_dir(tmp_path, sphere32_ms):
+    ms_file = tmp_path / "sphere32.h5"
+    hdf5io.write_microstructure(str(ms_file), DSET, sphere32_ms)
+
+    def make(name, **edits):
+        j = json.load(open(os.path.join(GOLDEN, f"{name}.json")))
+        j["microstructure"]["filepath"] = str(ms_file)
+        j["microstructure"]["datasetname"] = DSET
+        j.update(edits)
+        p = tmp_path / f"{name}.json"
+        json.dump(j, open(p, "w"))
+        return str(p), j
+    return make
+
+
+def test_usage_and_version():
+    _need_cli()
+    r = _run()
+    assert r.returncode == 10 and "USAGE" in r.stderr
+    r = _run("--version")
+    assert r.returncode == 0 and "C ABI 1" in r.stdout
+
+
+@pytest.mark.parametrize("edit,msg", [
+    ({"FE_type": "HEX20"}, "FE_type must be one of"),
+    ({"strain_type": "medium"}, "strain_type must be either"),
+    ({"linesearch_parameters": {"max_iter": 0}}, "linesearch_parameters"),
+    ({"macroscale_loading": [[[1.0, 0.0]]]}, "Invalid length of loading vector"),
+    ({"materials": []}, "'materials' must be non-empty array"),
+])
+def test_input_errors_exit_10(case_dir, edit, msg):
+    _need_cli()
+    path, _ = case_dir("test_LinearElastic", **edit)
+    r = _run(path, path + ".h5")
+    assert r.returncode == 10, r.stderr
+    assert msg in r.stderr
+
+
+def test_missing_property_exit_10(case_dir):
+    _need_cli()
+    path, j = case_dir("test_LinearElastic")
+    del j["materials"][0]["material_properties"]["shear_modulus"]
+    json.dump(j, open(path, "w"))
+    r = _run(path, path + ".h5")
+    assert r.returncode == 10 and "Missing material property 'shear_modulus'" in r.stderr
+
+
+def test_no_device_message(case_dir):
+    _need_cli()
+    from tests.conftest import gpu_available
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    path, _ = case_dir("test_LinearElastic")
+    r = _run(path, path + ".h5", env={**os.environ, "HIP_VISIBLE_DEVICES": ""})
+    assert r.returncode == 1 and "no HIP device" in r.stderr
+
+
+def _res(out, j, field, load=0, step=0):
+    prefix = j.get("results_prefix", "")
+    return hdf5io.read_dataset(out, f"{DSET}_results/{prefix}/load{load}/time_step{step}/{field}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["test_LinearThermal", "test_LinearElastic"])
+def test_cli_end_to_end(case_dir, sphere32_ms, name):
+    _need_cli()
+    path, j = case_dir(name)
+    out = path + ".h5"
+    print(f"[gpu-case] cli {name}", flush=True)
+    r = _run(path, out)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    p = Problem.from_json(path, ms=sphere32_ms)
+    o = oracle.Oracle(p)
+    lc = p.load_cases[0]
+    o.set_gradient(lc.g0_path[0])
+    it = o.solve()
+    n_str, h = p.n_str, p.howmany
+    s_avg, perm = _res(out, j, "stress_average")
+    assert perm is None and s_avg.shape == (n_str,)
+    so = o.homogenized_stress()
+    assert np.max(np.abs(s_avg - so)) <= 1e-10 * np.max(np.abs(so))
+    e_avg, _ = _res(out, j, "strain_average")
+    assert np.allclose(e_avg, lc.g0_path[0], atol=1e-12)
+    err, _ = _res(out, j, "absolute_error")
+    assert abs(len(err) - (it + 1)) <= 2 and err[-1] <= p.tol
+    msd, perm = _res(out, j, "microstructure", )
+    assert perm == "zyx" and msd.shape == (32, 32, 32, 1)
+    assert np.array_equal(msd[..., 0].transpose(2, 1, 0), sphere32_ms)
+    uf, perm = _res(out, j, "displacement_fluctuation")
+    assert perm == "zyx" and uf.shape == (32, 32, 32, h)
+    ut, _ = _res(out, j, "displacement")
+    # displacement = fluctuation + g0 . X with X relative to the cell centre (solver.h:586-648)
+    x = (np.arange(32) / 32.0 - 0.5)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    g = e_avg
+    if h == 1:
+        lin = g[0] * X + g[1] * Y + g[2] * Z
+        assert np.max(np.abs(ut[..., 0] - uf[..., 0] - lin)) < 1e-12
+    else:
+        r2 = np.sqrt(0.5)
+        G = np.array([[g[0], r2 * g[3], r2 * g[4]], [r2 * g[3], g[1], r2 * g[5]], [r2 * g[4], r2 * g[5], g[2]]])
+        for c in range(3):
+            lin = G[c, 0] * X + G[c, 1] * Y + G[c, 2] * Z
+            assert np.max(np.abs(ut[..., c] - uf[..., c] - lin)) < 1e-12
+    fo = o.element_fields()
+    for fld in ("stress", "strain"):
+        v, perm = _res(out, j, fld)
+        assert perm == "zyx" and v.shape == (32, 32, 32, n_str)
+        ref = fo[fld].reshape(32, 32, 32, n_str).transpose(2, 1, 0, 3)
+        assert np.max(np.abs(v - ref)) <= 1e-8 * np.max(np.abs(ref)), fld
+    C, _ = _res(out, j, "homogenized_tangent")
+    Co = o.homogenized_tangent()
+    assert C.shape == (n_str, n_str)
+    assert np.max(np.abs(C - Co)) <= 1e-6 * np.max(np.abs(Co))

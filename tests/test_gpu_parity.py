@@ -162,3 +162,27 @@ def test_j2_mixed_bc_steps():
     epg, qg = g.get_state()
     assert relerr(qg, qo) < 1e-6
     assert qo.max() > 0  # the matrix yielded
+
+
+def test_j2_mixed_precision_steps():
+    """Mixed form of the nonlinear path (FP32 Green preconditioner; FP64
+    residual, J2 return map, dots, line search): the converged load steps
+    reach the FP64 oracle's homogenized stress and macro strain to 1e-8."""
+    p = cases.j2(n=12, seed=1)
+    mbc = cases.uniaxial_mixed(3, 0.02)
+    say("j2 mixed bc 12^3 3 steps, mixed precision")
+    o = oracle.Oracle(p)
+    g = gpu.GpuSolver(p, precision="mixed")
+    for t in range(mbc["n_steps"]):
+        o.enable_mixed_bc(mbc, t)
+        g.enable_mixed_bc(mbc, t)
+        o.solve()
+        g.solve()
+        assert g.error_history()[-1] <= p.tol
+        assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-8, t
+        assert relerr(g.macro_gradient(), o.macro_gradient()) < 1e-8
+        o.extrapolate_displacement()
+        g.extrapolate_displacement()
+    _, qo = o.get_state()
+    _, qg = g.get_state()
+    assert relerr(qg, qo) < 1e-6

@@ -145,6 +145,38 @@ def test_green_inverts_reference_operator(h):
     assert np.max(np.abs(G[0, 0, 0])) == 0.0
 
 
+@pytest.mark.parametrize("h", [1, 3])
+def test_green_mirror_symmetry(h):
+    """The GPU stores the Green operator for kx <= nx/2, ky <= ny/2 only when the
+    reference element is mirror symmetric.  Pin the property it relies on with
+    the oracle's directly computed operator (solver.h:144-204): for an
+    isotropic reference, G(-kx) = Rx G(kx) Rx and G(-ky) = Ry G(ky) Ry, where
+    Rx flips the xy and xz entries and Ry the xy and yz entries."""
+    dims = (8, 6, 10)
+    ms = (np.arange(np.prod(dims)).reshape(dims) % 3 == 0).astype(np.uint16)
+    if h == 1:
+        groups = [MaterialGroup("LinearThermalIsotropic", [0, 1], {"conductivity": [1.0, 7.0]})]
+    else:
+        groups = [MaterialGroup("LinearElasticIsotropic", [0, 1],
+                                {"bulk_modulus": [3.0, 40.0], "shear_modulus": [1.0, 25.0]})]
+    p = Problem(dims=dims, howmany=h, ms=ms, groups=groups, L=(1.0, 0.8, 1.3))
+    G = oracle.Oracle(p).green()
+    nx, ny = dims[0], dims[1]
+    mx = np.roll(G[::-1], 1, axis=0)          # G(-kx): index (nx - kx) % nx
+    my = np.roll(G[:, ::-1], 1, axis=1)
+    if h == 1:
+        sx = sy = np.ones(1)
+    else:  # packed 00 01 02 11 12 22
+        sx = np.array([1, -1, -1, 1, 1, 1.0])
+        sy = np.array([1, -1, 1, 1, -1, 1.0])
+    scale = np.max(np.abs(G))
+    assert np.max(np.abs(mx - G * sx)) < 1e-13 * scale
+    assert np.max(np.abs(my - G * sy)) < 1e-13 * scale
+    # and a plausible mistake (unflipped xy entry) is caught
+    if h == 3:
+        assert np.max(np.abs(mx - G)) > 1e-3 * scale
+
+
 def test_homogeneous_rve_one_step():
     """Homogeneous RVE: the Green operator solves the problem exactly; u stays 0 and
     the homogenized stress is C g0 (Q-free closed form)."""
