@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -266,6 +267,11 @@ struct fans_gpu_ctx {
     // timing
     bool                                                timing = false;
     int                                                 fft_chunks = 1;
+    // captured CG iterations (hipGraph), keyed by precision and the direction
+    // buffer current at the first captured iteration
+    bool                                                use_graphs = true;
+    std::vector<std::pair<const void *, hipGraphExec_t>> graphs;
+    std::array<int, 6>                                  graph_sig{};
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
     std::vector<hipEvent_t>                             ev_pool;
     double                                              fam_ms[KF_COUNT]{};
@@ -365,6 +371,41 @@ bool keref_mirror_symmetric(const double *ke, int h, int ax)
     return true;
 }
 
+// captured iterations hold kernel arguments (buffers, material flags, Green
+// storage); forget them when any of those change
+void drop_graphs(fans_gpu_ctx *c)
+{
+    for (auto &g : c->graphs)
+        (void) hipGraphExecDestroy(g.second);
+    c->graphs.clear();
+}
+
+// after new materials: keep the captured iterations when every by-value
+// kernel argument they hold is unchanged (tables and the Green operator are
+// read through fixed pointers), e.g. for a batch of RVEs on one context
+void refresh_graphs(fans_gpu_ctx *c)
+{
+    const std::array<int, 6> sig = {c->all_iso, c->cubic, c->voxel_moduli, c->gxq, c->gyq, c->nphases};
+    if (sig != c->graph_sig) {
+        drop_graphs(c);
+        c->graph_sig = sig;
+    }
+}
+
+// Copies and clears ordered on the context's stream (non-blocking, so other
+// host threads' legacy-stream calls neither wait for it nor disturb a capture
+// in progress on it), then waited for.
+hipError_t copy_on(fans_gpu_ctx *c, void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+}
+hipError_t fill_on(fans_gpu_ctx *c, void *dst, size_t bytes)
+{
+    hipError_t e = hipMemsetAsync(dst, 0, bytes, c->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+}
+
 // forget the timing records from index `from` on (launches that returned at entry)
 void drop_timing(fans_gpu_ctx *c, size_t from)
 {
@@ -383,15 +424,15 @@ int grid_vec(int64_t n) { return (int) std::max<int64_t>(1, std::min<int64_t>((n
 // ---------------------------------------------------------------------------
 bool reg_len(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256; }
 
-#ifndef FANS_REG_LDS_KIB
-#define FANS_REG_LDS_KIB 36
-#endif
-// columns per block: a power of two (>= 4) with ~36 KiB of LDS, at most 256 threads
+// columns per block: a power of two (>= 4) within ~72 KiB of LDS (two
+// workgroups per CU), at most 256 threads; at 256^3 FP64 this gives 16 columns
+// (256-byte row segments) for the y pass and 4 frequencies for the x pass
+constexpr int kRegLdsKiB = 72;
 template <typename T, int N1, int N2>
 constexpr int reg_cols()
 {
     constexpr int n   = N1 * N2;
-    constexpr int lim = (int) (FANS_REG_LDS_KIB * 1024 / (n * sizeof(cplx<T>)));
+    constexpr int lim = (int) (kRegLdsKiB * 1024 / (n * sizeof(cplx<T>)));
     int           c   = 4;
     while (c * 2 <= lim && c * 2 * N2 <= 256)
         c *= 2;
@@ -875,36 +916,82 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext
     int        chunk = 4, rc;
     int        is[IS_COUNT];
     bool       first = true;
+    // one CG iteration (with the update of the previous one unless `first`)
+    auto iteration = [&](bool with_pro) -> int {
+        const ProArgs<T> pa{rnew, u, dcur, s, c->sc + SC_ALPHA, c->d.measure, c->partial};
+        EpiArgs<T>       ep{};
+        ep.out     = s;
+        ep.r       = r;
+        ep.partial = c->partial;
+        int e;
+        if ((e = convolve_dev<T>(c, P, r, spec, Gp, EPI_CG, ep, done, with_pro ? &pa : nullptr)))
+            return e;
+        if ((e = reduce(c, c->partial, (int) P.nblk_z, 1, RED_DELTA)))
+            return e;
+        AsmArgs<T> a = asm_args<T>(c);
+        a.phases     = ph;
+        a.in         = s;
+        a.din        = dcur;
+        a.dout       = dnext;
+        a.out        = rnew;
+        a.scal       = c->sc;  // SC_BETA == 0
+        a.done       = done;
+        if ((e = launch_assemble<T, ASM_CGLIN>(c, a)))
+            return e;
+        if ((e = reduce(c, c->partial, c->nblk_asm, 1, RED_ALPHA)))
+            return e;
+        std::swap(dcur, dnext);
+        return 0;
+    };
+    // After the first iteration every iteration has the same launches; a pair
+    // of iterations returns dcur/dnext to their roles, so kGraphIters (even)
+    // iterations are captured once into a hipGraph and replayed (one launch
+    // per replay instead of ~10 per iteration).  Timing mode stays eager.
+    constexpr int kGraphIters = 8;
+    if (c->use_graphs && !c->timing) {
+        if ((rc = iteration(false)))
+            return rc;
+        first = false;
+        hipGraphExec_t exec = nullptr;
+        for (auto &g : c->graphs)
+            if (g.first == (const void *) dcur)
+                exec = g.second;
+        if (!exec) {
+            hipGraph_t graph = nullptr;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int crc = 0;
+            T  *d0  = dcur;
+            for (int k = 0; k < kGraphIters && !crc; ++k)
+                crc = iteration(true);
+            const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+            if (crc)
+                return crc;
+            HIPCHK(c, ec);
+            HIPCHK(c, hipGraphInstantiateWithFlags(&exec, graph, 0));
+            (void) hipGraphDestroy(graph);
+            c->graphs.push_back({(const void *) d0, exec});
+        }
+        for (;;) {
+            HIPCHK(c, hipGraphLaunch(exec, c->stream));
+            if ((rc = read_scalars(c, nullptr, is)))
+                return rc;
+            if (is[IS_DONE])
+                break;
+        }
+        if (iters_out)
+            *iters_out = is[IS_ITER];
+        return 0;
+    }
     for (;;) {
         // timing mode checks `done` after every iteration so that launches that
         // returned at entry are not counted in the per-kernel durations
         const int n = c->timing ? 1 : chunk;
         for (int k = 0; k < n; ++k) {
-            const size_t     mark    = c->ev_used.size();
-            const bool       had_pro = !first;
-            const ProArgs<T> pa{rnew, u, dcur, s, c->sc + SC_ALPHA, c->d.measure, c->partial};
-            EpiArgs<T>       ep{};
-            ep.out     = s;
-            ep.r       = r;
-            ep.partial = c->partial;
-            if ((rc = convolve_dev<T>(c, P, r, spec, Gp, EPI_CG, ep, done, first ? nullptr : &pa)))
+            const size_t mark    = c->ev_used.size();
+            const bool   had_pro = !first;
+            if ((rc = iteration(had_pro)))
                 return rc;
             first = false;
-            if ((rc = reduce(c, c->partial, (int) P.nblk_z, 1, RED_DELTA)))
-                return rc;
-            AsmArgs<T> a = asm_args<T>(c);
-            a.phases     = ph;
-            a.in         = s;
-            a.din        = dcur;
-            a.dout       = dnext;
-            a.out        = rnew;
-            a.scal       = c->sc;  // SC_BETA == 0
-            a.done       = done;
-            if ((rc = launch_assemble<T, ASM_CGLIN>(c, a)))
-                return rc;
-            if ((rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ALPHA)))
-                return rc;
-            std::swap(dcur, dnext);
             if (c->timing) {
                 if ((rc = read_scalars(c, nullptr, is)))
                     return rc;
@@ -1206,6 +1293,7 @@ void free_all(fans_gpu_ctx *c)
                     (void *) c->vk32, (void *) c->vg32, (void *) c->ub_d, (void *) c->sens_d})
         if (p)
             (void) hipFree(p);
+    drop_graphs(c);
     c->fft64.release();
     c->fft32.release();
     for (auto &p : c->ev_used) {
@@ -1399,8 +1487,8 @@ int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *d)
     *out            = c;
     if (d->fe_type == FANS_FE_BBAR && c->h == 3)
         return fail(c, FANS_GPU_EUNSUP, "FE_type BBAR is not implemented on the GPU");
-    if (hipStreamCreate(&c->stream) != hipSuccess)
-        return fail(c, FANS_GPU_EHIP, "hipStreamCreate failed");
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(c, FANS_GPU_EHIP, "hipStreamCreateWithFlags failed");
     // launch geometry
     c->nty      = (c->ny + NOUT - 1) / NOUT;
     c->ntz      = (c->nz + NOUT - 1) / NOUT;
@@ -1443,7 +1531,7 @@ int fans_gpu_set_microstructure(fans_gpu_ctx *c, const uint16_t *ms, int64_t cou
             return fail(c, FANS_GPU_EUNSUP, "phase ids above 255 are not supported");
         m8[i] = (uint8_t) ms[i];
     }
-    HIPCHK(c, hipMemcpy(c->ms, m8.data(), c->N, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c, c->ms, m8.data(), c->N, hipMemcpyHostToDevice));
     c->have_ms = true;
     return 0;
 }
@@ -1455,7 +1543,7 @@ int rebuild_green(fans_gpu_ctx *c)
     const int ne = 8 * c->h, npk = c->h * (c->h + 1) / 2;
     if (!c->keref_d)
         HIPCHK(c, hipMalloc(&c->keref_d, sizeof(double) * 576));
-    HIPCHK(c, hipMemcpy(c->keref_d, c->keref, sizeof(double) * ne * ne, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c, c->keref_d, c->keref, sizeof(double) * ne * ne, hipMemcpyHostToDevice));
     if (!c->G64)
         HIPCHK(c, hipMalloc(&c->G64, sizeof(double) * c->Nf * npk));
     // quarter storage when the reference element is mirror symmetric in x and y
@@ -1630,13 +1718,13 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
         HIPCHK(c, hipMalloc(&c->dph64, sizeof(Phase<double>) * 256));
     if (!c->dph32)
         HIPCHK(c, hipMalloc(&c->dph32, sizeof(Phase<float>) * 256));
-    HIPCHK(c, hipMemcpy(c->dph64, c->ph64.data(), sizeof(Phase<double>) * c->nphases, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->dph32, c->ph32.data(), sizeof(Phase<float>) * c->nphases, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c, c->dph64, c->ph64.data(), sizeof(Phase<double>) * c->nphases, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c, c->dph32, c->ph32.data(), sizeof(Phase<float>) * c->nphases, hipMemcpyHostToDevice));
     // J2 state (zero, Solver ctor solver.h:139)
     if (c->has_j2 && !c->js) {
         const size_t bytes = sizeof(double) * 7 * c->ngp * c->N;
         HIPCHK(c, hipMalloc(&c->js, bytes));
-        HIPCHK(c, hipMemset(c->js, 0, bytes));
+        HIPCHK(c, fill_on(c, c->js, bytes));
     }
     c->voxel_moduli = false;  // new phase tables replace per-voxel moduli
     if (const int rcg = rebuild_green(c))
@@ -1647,6 +1735,7 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
         if (c->phase_kind[p] != (c->h == 1 ? FANS_MODEL_THERMAL_ISO : FANS_MODEL_ELASTIC_ISO))
             c->all_iso = false;
     c->have_mat = true;
+    refresh_graphs(c);
     return 0;
 }
 
@@ -2012,7 +2101,7 @@ int fans_gpu_get_field(fans_gpu_ctx *c, int32_t which, double *host, int64_t cou
             return 0;
         }
         std::vector<double> st((size_t) 7 * ng * c->N);
-        HIPCHK(c, hipMemcpy(st.data(), c->js, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_on(c, st.data(), c->js, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
         for (int64_t e = 0; e < c->N; ++e)
             for (int g = 0; g < ng; ++g) {
                 if (which == FANS_FIELD_PLASTIC_STRAIN)
@@ -2026,7 +2115,7 @@ int fans_gpu_get_field(fans_gpu_ctx *c, int32_t which, double *host, int64_t cou
         return fail(c, FANS_GPU_EINVAL, "unknown field %d", which);
     if (count != n)
         return fail(c, FANS_GPU_EINVAL, "field size mismatch (expected %lld)", (long long) n);
-    HIPCHK(c, hipMemcpy(host, src, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_on(c, host, src, sizeof(double) * n, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -2039,7 +2128,7 @@ int fans_gpu_set_field(fans_gpu_ctx *c, int32_t which, const double *host, int64
         return fail(c, FANS_GPU_EINVAL, "field %d is not writable", which);
     if (count != c->Nh)
         return fail(c, FANS_GPU_EINVAL, "field size mismatch");
-    HIPCHK(c, hipMemcpy(dst, host, sizeof(double) * count, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c, dst, host, sizeof(double) * count, hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -2051,11 +2140,11 @@ int fans_gpu_residual(fans_gpu_ctx *c, const double *u, double *r, int32_t linea
         return fail(c, FANS_GPU_ESTATE, "materials not set");
     if (linear_kd && !c->all_linear)
         return fail(c, FANS_GPU_EINVAL, "linear matvec needs linear materials");
-    HIPCHK(c, hipMemcpy(c->dB, u, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c, c->dB, u, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
     int rc;
     if (linear_kd) {
         // rnew = K d with d = s + 0 * d: feed s = u and beta = 0
-        HIPCHK(c, hipMemcpy(c->s, u, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
+        HIPCHK(c, copy_on(c, c->s, u, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
         if ((rc = init_scalars(c, 0.0, 0, 1)))
             return rc;
         AsmArgs<double> a = asm_args<double>(c);
@@ -2071,7 +2160,7 @@ int fans_gpu_residual(fans_gpu_ctx *c, const double *u, double *r, int32_t linea
     }
     if (rc)
         return rc;
-    HIPCHK(c, hipMemcpy(r, c->rnew, sizeof(double) * c->Nh, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_on(c, r, c->rnew, sizeof(double) * c->Nh, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -2081,13 +2170,13 @@ int fans_gpu_convolve(fans_gpu_ctx *c, const double *r, double *out)
         return FANS_GPU_EINVAL;
     if (!c->have_mat)
         return fail(c, FANS_GPU_ESTATE, "materials not set");
-    HIPCHK(c, hipMemcpy(c->dB, r, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c, c->dB, r, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
     EpiArgs<double> ep{};
     ep.out = c->rnew;
     int rc = convolve_dev<double>(c, c->fft64, c->dB, c->spec64, c->G64, EPI_STORE, ep, nullptr);
     if (rc)
         return rc;
-    HIPCHK(c, hipMemcpy(out, c->rnew, sizeof(double) * c->Nh, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_on(c, out, c->rnew, sizeof(double) * c->Nh, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -2100,7 +2189,7 @@ int fans_gpu_get_green(fans_gpu_ctx *c, double *out, int64_t count)
     if (count != n || !c->G64)
         return fail(c, FANS_GPU_EINVAL, "green size mismatch or not built");
     std::vector<double> st((size_t) c->gxq * c->gyq * Mc * np);
-    HIPCHK(c, hipMemcpy(st.data(), c->G64, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_on(c, st.data(), c->G64, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
     // expand the stored frequencies to the full [nx][ny][nz/2+1] layout
     for (int ix = 0; ix < c->nx; ++ix)
         for (int iy = 0; iy < c->ny; ++iy) {
@@ -2143,8 +2232,8 @@ int fans_gpu_set_voxel_moduli(fans_gpu_ctx *c, const double *k, const double *g,
         HIPCHK(c, hipMalloc(&c->vk32, b32));
         HIPCHK(c, hipMalloc(&c->vg32, b32));
     }
-    HIPCHK(c, hipMemcpy(c->vk64, k, b64, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->vg64, c->h == 3 ? g : k, b64, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c, c->vk64, k, b64, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c, c->vg64, c->h == 3 ? g : k, b64, hipMemcpyHostToDevice));
     hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_vec(c->N)), dim3(kThreads), 0, c->stream, c->vk32,
                        c->vk64, c->N);
     hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_vec(c->N)), dim3(kThreads), 0, c->stream, c->vg32,
@@ -2178,7 +2267,10 @@ int fans_gpu_set_voxel_moduli(fans_gpu_ctx *c, const double *k, const double *g,
                 c->kref[i * 6 + i] += 2 * mr;
         }
     }
-    return rebuild_green(c);
+    if (const int rcg = rebuild_green(c))
+        return rcg;
+    refresh_graphs(c);
+    return 0;
 }
 
 int fans_gpu_sensitivity(fans_gpu_ctx *c, const double *ub, const double *g0b, double *sens, int64_t count)
@@ -2194,8 +2286,8 @@ int fans_gpu_sensitivity(fans_gpu_ctx *c, const double *ub, const double *g0b, d
         HIPCHK(c, hipMalloc(&c->ub_d, sizeof(double) * (c->Nh + 16)));
         HIPCHK(c, hipMalloc(&c->sens_d, sizeof(double) * c->N * 2));
     }
-    HIPCHK(c, hipMemcpy(c->ub_d, ub, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->ub_d + c->Nh, g0b, sizeof(double) * c->nstr, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c, c->ub_d, ub, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c, c->ub_d + c->Nh, g0b, sizeof(double) * c->nstr, hipMemcpyHostToDevice));
     ElemArgs<double> a{};
     a.nx     = c->nx;
     a.ny     = c->ny;
@@ -2224,7 +2316,7 @@ int fans_gpu_sensitivity(fans_gpu_ctx *c, const double *ub, const double *g0b, d
     }
 #undef EP
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpy(sens, c->sens_d, sizeof(double) * count, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_on(c, sens, c->sens_d, sizeof(double) * count, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -2251,7 +2343,7 @@ static int fft_standalone(int dev, int nx, int ny, int nz, int h, const double *
     auto         &P  = c->fft64;
     const int     Kc     = P.Mc * h;
     if (fwd) {
-        rc = hipMemcpy(c->dB, in, sizeof(double) * nr, hipMemcpyHostToDevice) == hipSuccess ? 0 : FANS_GPU_EHIP;
+        rc = copy_on(c, c->dB, in, sizeof(double) * nr, hipMemcpyHostToDevice) == hipSuccess ? 0 : FANS_GPU_EHIP;
         if (!rc) {
             const ProArgs<double> pa{};
             if (h == 1)
@@ -2263,12 +2355,12 @@ static int fft_standalone(int dev, int nx, int ny, int nz, int h, const double *
             // x pass: strided with columns = the whole (y, kz, h) block
             launch_strided<double, false>(c, P.Lx, P.twx, P.bcx, P.smx_plain, c->spec64, (int64_t) ny * Kc, 0,
                                           ny * Kc, (int64_t) 0, 1, nullptr);
-            rc = hipMemcpy(out, c->spec64, sizeof(cplx<double>) * nc / 1, hipMemcpyDeviceToHost) == hipSuccess
+            rc = copy_on(c, out, c->spec64, sizeof(cplx<double>) * nc / 1, hipMemcpyDeviceToHost) == hipSuccess
                      ? 0
                      : FANS_GPU_EHIP;
         }
     } else {
-        rc = hipMemcpy(c->spec64, in, sizeof(cplx<double>) * nc, hipMemcpyHostToDevice) == hipSuccess ? 0
+        rc = copy_on(c, c->spec64, in, sizeof(cplx<double>) * nc, hipMemcpyHostToDevice) == hipSuccess ? 0
                                                                                                       : FANS_GPU_EHIP;
         if (!rc) {
             launch_strided<double, true>(c, P.Lx, P.twx, P.bcx, P.smx_plain, c->spec64, (int64_t) ny * Kc, 0,
@@ -2281,7 +2373,7 @@ static int fft_standalone(int dev, int nx, int ny, int nz, int h, const double *
                 launch_z_c2r<double, 1, EPI_STORE>(c, P, c->spec64, ep, nullptr);
             else
                 launch_z_c2r<double, 3, EPI_STORE>(c, P, c->spec64, ep, nullptr);
-            rc = hipMemcpy(out, c->dB, sizeof(double) * nr, hipMemcpyDeviceToHost) == hipSuccess ? 0 : FANS_GPU_EHIP;
+            rc = copy_on(c, out, c->dB, sizeof(double) * nr, hipMemcpyDeviceToHost) == hipSuccess ? 0 : FANS_GPU_EHIP;
         }
     }
     if (!rc && hipGetLastError() != hipSuccess)
@@ -2316,6 +2408,29 @@ int fans_gpu_set_option(fans_gpu_ctx *c, int32_t opt, int64_t value)
         if (value < 1)
             return fail(c, FANS_GPU_EINVAL, "FFT chunk count must be >= 1");
         c->fft_chunks = (int) value;
+        drop_graphs(c);
+        return 0;
+    }
+    if (opt == FANS_OPT_GRAPHS) {
+        c->use_graphs = value != 0;
+        return 0;
+    }
+    if (opt == FANS_OPT_ASM_PLANES) {
+        // x planes per assembly workgroup (tile shape of the stencil pass)
+        if (value < 1 || value > c->nx)
+            return fail(c, FANS_GPU_EINVAL, "assembly planes per block must be in [1, n_x]");
+        c->cx               = (int) value;
+        const int     nbx   = (c->nx + c->cx - 1) / c->cx;
+        c->nblk_asm         = nbx * c->nty * c->ntz;
+        const int64_t need  = 2 * (int64_t) c->nblk_asm;
+        if (need > c->partial_len) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipFree(c->partial));
+            c->partial = nullptr;
+            HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * need));
+            c->partial_len = need;
+        }
+        drop_graphs(c);
         return 0;
     }
     return fail(c, FANS_GPU_EINVAL, "unknown option %d", opt);

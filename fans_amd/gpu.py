@@ -24,6 +24,8 @@ PRECISION = {"fp64": 0, "mixed": 1}
 FIELD = {"u": 0, "r": 1, "u_prev": 2, "plastic_strain": 3, "hardening": 4}
 OPT_TIMING = 1
 OPT_FFT_CHUNKS = 2
+OPT_GRAPHS = 3
+OPT_ASM_PLANES = 4
 ERRORS = {-1: "EINVAL", -2: "ENODEV", -3: "EHIP", -4: "ESTATE", -5: "EUNSUP"}
 
 
@@ -164,13 +166,22 @@ class GpuSolver:
             raise FansGpuError(rc, msg or "fans_gpu_create failed")
         if timing:
             self._check(self.L.fans_gpu_set_option(self.ptr, OPT_TIMING, 1))
+        self.stats = Stats()
+        self.load(problem)
+
+    def load(self, problem: Problem):
+        """Microstructure and material tables of `problem` (same grid, physics and
+        solver settings as the context was created with); fresh solver state."""
+        if tuple(problem.dims) != self.dims or problem.howmany != self.h:
+            raise ValueError("load(): grid and howmany must match the context")
+        self.p = problem
         ms = np.ascontiguousarray(problem.ms, dtype=np.uint16).reshape(-1)
         self._check(self.L.fans_gpu_set_microstructure(self.ptr, ms.ctypes.data_as(C.POINTER(C.c_uint16)), ms.size))
         kinds, nph, pids, params = problem.flat_materials()
         self._ref = None if problem.ref_material is None else np.ascontiguousarray(problem.ref_material, np.float64)
         self._check(self.L.fans_gpu_set_materials(self.ptr, len(kinds), _ip(kinds), _ip(nph), _ip(pids), _dp(params),
                                                   params.size, _dp(self._ref)))
-        self.stats = Stats()
+        self._check(self.L.fans_gpu_reset_state(self.ptr))
 
     def _err(self):
         if self.ptr:
@@ -248,6 +259,14 @@ class GpuSolver:
 
     def set_fft_chunks(self, k):
         self._check(self.L.fans_gpu_set_option(self.ptr, OPT_FFT_CHUNKS, int(k)))
+
+    def set_asm_planes(self, p):
+        """x planes per assembly workgroup (stencil tile depth)."""
+        self._check(self.L.fans_gpu_set_option(self.ptr, OPT_ASM_PLANES, int(p)))
+
+    def set_graphs(self, on=True):
+        """Replay captured CG iterations (hipGraph; default) or launch eagerly."""
+        self._check(self.L.fans_gpu_set_option(self.ptr, OPT_GRAPHS, int(bool(on))))
 
     def get_field(self, name):
         if name in ("plastic_strain",):

@@ -14,22 +14,22 @@ def random_spheres(n, count, rmin, rmax, seed=0, n_inclusion_phases=1, max_tries
     """Phase 0 matrix; inclusions get ids 1..n_inclusion_phases (cycled)."""
     n = np.array([n, n, n] if np.isscalar(n) else n, dtype=np.int64)
     rng = np.random.default_rng(seed)
-    centers, radii = [], []
+    C = np.zeros((max(count, 1), 3))
+    R = np.zeros(max(count, 1))
+    m = 0
     tries = 0
-    while len(centers) < count and tries < max_tries:
+    while m < count and tries < max_tries:
         tries += 1
         c = rng.uniform(0.0, 1.0, size=3) * n
         r = rng.uniform(rmin, rmax)
-        ok = True
-        for c2, r2 in zip(centers, radii):
-            d = np.abs(c - c2)
+        if m:
+            d = np.abs(C[:m] - c)
             d = np.minimum(d, n - d)
-            if np.sqrt((d * d).sum()) < r + r2:
-                ok = False
-                break
-        if ok:
-            centers.append(c)
-            radii.append(r)
+            if np.any(np.sqrt((d * d).sum(axis=1)) < r + R[:m]):
+                continue
+        C[m], R[m] = c, r
+        m += 1
+    centers, radii = list(C[:m]), list(R[:m])
     ms = np.zeros(tuple(n), dtype=dtype)
     for k, (c, r) in enumerate(zip(centers, radii)):
         phase = 1 + (k % n_inclusion_phases)

@@ -186,3 +186,28 @@ def test_j2_mixed_precision_steps():
     _, qo = o.get_state()
     _, qg = g.get_state()
     assert relerr(qg, qo) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replay_matches_eager(precision):
+    """Captured CG iterations (hipGraph replays, 8 iterations each, host sync
+    between replays) reproduce the eager launches bit for bit, over two
+    solves in a row with the same context."""
+    p = cases.elastic(n=32, seed=8, count=8, rmin=3, rmax=6)
+    say(f"graph vs eager 32^3 {precision}")
+    res = []
+    for graphs in (False, True):
+        g = gpu.GpuSolver(p, precision=precision)
+        g.set_graphs(graphs)
+        out = []
+        for g0 in ([1.0, 0, 0, 0, 0, 0], [0, 0.3, 0, 0, 0.5, 0]):
+            g.reset_state()
+            g.set_gradient(g0)
+            g.solve()
+            out.append((int(g.stats.inner_iterations), g.homogenized_stress().copy(), g.get_u().copy()))
+        g.close()
+        res.append(out)
+    for (ie, se, ue), (ig, sg, ug) in zip(*res):
+        assert ie == ig
+        assert np.array_equal(se, sg)
+        assert np.array_equal(ue, ug)
