@@ -168,10 +168,10 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
 
     fetch_plane(X0 - 1);
     commit_plane(X0 - 1, 0);
+    fetch_plane(X0);
     int ph_next = phase_of(X0 - 1);
     for (int i = 0; X0 - 1 + i < X1; ++i) {
         const int xe = X0 - 1 + i;
-        fetch_plane(xe + 1);
         commit_plane(xe + 1, (i + 1) & 1);
         const int ph = ph_next;
         if (xe + 1 < X1)
@@ -233,6 +233,10 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
             };
             element_forces<T, H, NGP>(u, a.geom, g0, mat, f);
         }
+        // the loads of the plane after next fly during the exchange and the
+        // stores (the element's registers are free again here)
+        if (xe + 2 <= X1)
+            fetch_plane(xe + 2);
         // z: node (ey,ez) <- az=0 corners of element ez + az=1 corners of element ez-1
         T P2[2][2][H];
 #pragma unroll

@@ -40,6 +40,8 @@ struct Geom {
     double ilx, ily, ilz;  // 1 / element edge lengths
     double w;              // v_e / n_gp
     double q0, q1;         // Gauss point coordinates 0.5 -+ sqrt(3)/6
+    int    bbar;           // B-bar element (mechanics, 8 Gauss points)
+    int    pad;
 };
 
 constexpr double kSqrtHalf = 7.071067811865476e-01;  // literal of matmodel.h:288
@@ -321,6 +323,29 @@ __device__ __forceinline__ void grad_factors_T(const T (&Sx)[2][2][H], const T (
     }
 }
 
+// B-bar (include/matmodel.h:113-140): the volumetric rows of B at every Gauss
+// point are replaced by those at the element centre, which for the trilinear
+// element equal their mean over the 2x2x2 Gauss points.  Applied to the
+// gradient at Gauss point (i,j,k): the normal components shift by
+// (tr_c - tr_gp) / 3 with tr_c the Gauss mean of the trace.  The transpose
+// maps a stress sigma_gp to B^T (dev sigma_gp + m mean_gp(tr sigma) / 3).
+template <typename T>
+__device__ __forceinline__ T bbar_mean_trace(const T (&Gx)[2][2][3], const T (&Gy)[2][2][3], const T (&Gz)[2][2][3])
+{
+    return (T) 0.25 * (((Gx[0][0][0] + Gx[1][1][0]) + (Gx[0][1][0] + Gx[1][0][0])) +
+                       ((Gy[0][0][1] + Gy[1][1][1]) + (Gy[0][1][1] + Gy[1][0][1])) +
+                       ((Gz[0][0][2] + Gz[1][1][2]) + (Gz[0][1][2] + Gz[1][0][2])));
+}
+
+template <typename T>
+__device__ __forceinline__ void bbar_project_grad(T (&gr)[3][3], T trc)
+{
+    const T d = (trc - (gr[0][0] + gr[1][1] + gr[2][2])) * (T) (1.0 / 3.0);
+    gr[0][0] += d;
+    gr[1][1] += d;
+    gr[2][2] += d;
+}
+
 // Full element: f = sum_gp B^T sigma(B u + g0) * w.  MatF(gp, e, s) evaluates
 // the constitutive law at Gauss point gp (index i + 2j + 4k).
 template <typename T, int H, int NGP, typename MatF>
@@ -338,6 +363,12 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][H], const Geom &g
 #pragma unroll
             for (int c = 0; c < H; ++c)
                 Sx[a][b][c] = Sy[a][b][c] = Sz[a][b][c] = (T) 0;
+    constexpr bool CAN_BBAR = H == 3 && NGP == 8;
+    const bool     bbar     = CAN_BBAR && g.bbar;
+    T              trc = (T) 0, trs = (T) 0;
+    if constexpr (CAN_BBAR)
+        if (bbar)
+            trc = bbar_mean_trace<T>(Gx, Gy, Gz);
 #pragma unroll
     for (int gp = 0; gp < NGP; ++gp) {
         const int i = gp & 1, j = (gp >> 1) & 1, k = gp >> 2;
@@ -348,9 +379,21 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][H], const Geom &g
             gr[c][1] = Gy[i][k][c];
             gr[c][2] = Gz[i][j][c];
         }
+        if constexpr (CAN_BBAR)
+            if (bbar)
+                bbar_project_grad<T>(gr, trc);
         T e[NS], s[NS], Tt[H][3];
         grad_to_strain<T, H>(gr, g0, e);
         mat(gp, e, s);
+        if constexpr (CAN_BBAR)
+            if (bbar) {
+                const T t = s[0] + s[1] + s[2];
+                trs += t;
+                const T t3 = t * (T) (1.0 / 3.0);
+                s[0] -= t3;
+                s[1] -= t3;
+                s[2] -= t3;
+            }
         stress_to_T<T, H>(s, Tt);
 #pragma unroll
         for (int c = 0; c < H; ++c) {
@@ -359,6 +402,20 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][H], const Geom &g
             Sz[i][j][c] += Tt[c][2];
         }
     }
+    if constexpr (CAN_BBAR)
+        if (bbar) {
+            // m mean(tr sigma) / 3 at each of the 8 points: every factor entry
+            // collects two points
+            const T v = trs * (T) (2.0 / 24.0);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    Sx[a][b][0] += v;
+                    Sy[a][b][1] += v;
+                    Sz[a][b][2] += v;
+                }
+        }
     const T w = (T) g.w;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -589,8 +646,11 @@ __device__ __forceinline__ void element_forces_iso3_modal(const T (&u)[8][3], co
             }
     }
     if constexpr (NGP == 8) {
+        // B-bar keeps only the element-mean trace: the trace energy of the
+        // higher modes drops by K, which leaves lambda - K = -2 mu / 3
         const T third = (T) (1.0 / 3.0);
-        const T L1 = L0 * third, M1 = M0 * third, L2 = L1 * third, M2 = M1 * third;
+        const T Lh = g.bbar ? (T) (-2.0 / 3.0) * M0 : L0;
+        const T L1 = Lh * third, M1 = M0 * third, L2 = L1 * third, M2 = M1 * third;
         // xi: d/dy mode 1, d/dz mode 1; eta: d/dx mode 1, d/dz mode 2; zeta: d/dx mode 2, d/dy mode 2
         modal_pair<T, 1, 2>(A[1][1], A[2][1], L1, M1);
         modal_pair<T, 0, 2>(A[0][1], A[2][2], L1, M1);
@@ -770,6 +830,11 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
         const Phase<T> &P = a.phases[a.ms[el]];
         T               Gx[2][2][H], Gy[2][2][H], Gz[2][2][H];
         grad_factors<T, H, NGP>(u, a.geom, Gx, Gy, Gz);
+        constexpr bool CAN_BBAR = H == 3 && NGP == 8;
+        const bool     bbar     = CAN_BBAR && a.geom.bbar;
+        T              trc      = (T) 0;
+        if constexpr (CAN_BBAR)
+            trc = bbar_mean_trace<T>(Gx, Gy, Gz);
         T g0[NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k)
@@ -788,6 +853,9 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
             }
             T Hx[2][2][H], Hy[2][2][H], Hz[2][2][H];
             grad_factors<T, H, NGP>(ub, a.geom, Hx, Hy, Hz);
+            T trcb = (T) 0;
+            if constexpr (CAN_BBAR)
+                trcb = bbar_mean_trace<T>(Hx, Hy, Hz);
             T g0b[NS];
 #pragma unroll
             for (int k = 0; k < NS; ++k)
@@ -802,6 +870,11 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
                     ga[c][0] = Gx[j][k][c], ga[c][1] = Gy[i][k][c], ga[c][2] = Gz[i][j][c];
                     gb[c][0] = Hx[j][k][c], gb[c][1] = Hy[i][k][c], gb[c][2] = Hz[i][j][c];
                 }
+                if constexpr (CAN_BBAR)
+                    if (bbar) {
+                        bbar_project_grad<T>(ga, trc);
+                        bbar_project_grad<T>(gb, trcb);
+                    }
                 T ea[NS], eb[NS];
                 grad_to_strain<T, H>(ga, g0, ea);
                 grad_to_strain<T, H>(gb, g0b, eb);
@@ -840,6 +913,9 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
                 gr[c][1] = Gy[i][k][c];
                 gr[c][2] = Gz[i][j][c];
             }
+            if constexpr (CAN_BBAR)
+                if (bbar)
+                    bbar_project_grad<T>(gr, trc);
             T e[NS], s[NS];
             grad_to_strain<T, H>(gr, g0, e);
             if constexpr (MODE == ELEM_COMMIT) {

@@ -1392,12 +1392,20 @@ void ref_element_stiffness(fans_gpu_ctx *c)
                              {xm, xm, xp}, {xp, xm, xp}, {xm, xp, xp}, {xp, xp, xp}};
     const double ve       = (c->d.L[0] / c->nx) * (c->d.L[1] / c->ny) * (c->d.L[2] / c->nz);
     double       tmp[576] = {0};
+    double       Bc[6 * 24];
+    host_B(c, 0.5, 0.5, 0.5, Bc);
     for (int p = 0; p < c->ngp; ++p) {
         double B[6 * 24];
         if (c->ngp == 1)
             host_B(c, 0.5, 0.5, 0.5, B);
         else
             host_B(c, xi[p][0], xi[p][1], xi[p][2], B);
+        if (c->geom.bbar)  // volumetric rows from the centre (matmodel.h:118-131)
+            for (int j = 0; j < ne; ++j) {
+                const double dv = (Bc[j] + Bc[ne + j] + Bc[2 * ne + j] - B[j] - B[ne + j] - B[2 * ne + j]) / 3.0;
+                for (int i = 0; i < 3; ++i)
+                    B[i * ne + j] += dv;
+            }
         double CB[6 * 24];
         for (int i = 0; i < ns; ++i)
             for (int j = 0; j < ne; ++j) {
@@ -1483,10 +1491,9 @@ int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *d)
     c->geom.w       = lx * ly * lz / c->ngp;
     c->geom.q0      = 0.5 - std::sqrt(3.) / 6.;
     c->geom.q1      = 0.5 + std::sqrt(3.) / 6.;
+    c->geom.bbar    = d->fe_type == FANS_FE_BBAR && c->h == 3;
     c->cubic        = lx == ly && ly == lz;
     *out            = c;
-    if (d->fe_type == FANS_FE_BBAR && c->h == 3)
-        return fail(c, FANS_GPU_EUNSUP, "FE_type BBAR is not implemented on the GPU");
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(c, FANS_GPU_EHIP, "hipStreamCreateWithFlags failed");
     // launch geometry

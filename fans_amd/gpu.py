@@ -13,7 +13,8 @@ import numpy as np
 from .problem import MODEL_KIND, Problem
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfans_gpu.so")
+# FANS_GPU_LIB selects another build of the library (e.g. a variant for A/B timing)
+LIB_PATH = os.environ.get("FANS_GPU_LIB") or os.path.join(_HERE, "libfans_gpu.so")
 _LIB = None
 
 FE = {"HEX8": 0, "HEX8R": 1, "BBAR": 2}

@@ -24,6 +24,8 @@ MODEL_PARAMS = {
     "J2PlasticityNew_LinearIsotropicHardening": ("bulk_modulus", "shear_modulus", "yield_stress",
                                                  "isotropic_hardening_parameter"),
     "PseudoPlasticLinearHardening": ("bulk_modulus", "shear_modulus", "yield_stress", "hardening_parameter"),
+    "SaintVenantKirchhoff": ("bulk_modulus", "shear_modulus"),
+    "CompressibleNeoHookean": ("bulk_modulus", "shear_modulus"),
 }
 MODEL_KIND = {
     "LinearThermalIsotropic": 1,
@@ -32,7 +34,11 @@ MODEL_KIND = {
     "LinearElasticTriclinic": 4,
     "J2PlasticityNew_LinearIsotropicHardening": 5,
     "PseudoPlasticLinearHardening": 6,
+    "SaintVenantKirchhoff": 7,
+    "CompressibleNeoHookean": 8,
 }
+# createMatmodel<3,9> (include/setup.h:62-73): finite strain, F and P row major
+LARGE_STRAIN_MODELS = {"SaintVenantKirchhoff", "CompressibleNeoHookean"}
 THERMAL_MODELS = {"LinearThermalIsotropic", "LinearThermalTriclinic"}
 LINEAR_MODELS = {"LinearThermalIsotropic", "LinearThermalTriclinic", "LinearElasticIsotropic",
                  "LinearElasticTriclinic"}
@@ -96,7 +102,13 @@ class Problem:
 
     @property
     def n_str(self) -> int:
-        return 3 if self.howmany == 1 else 6
+        if self.howmany == 1:
+            return 3
+        return 9 if self.strain_type == "large" else 6
+
+    @property
+    def large_strain(self) -> bool:
+        return self.howmany == 3 and self.strain_type == "large"
 
     @property
     def all_linear(self) -> bool:
@@ -112,6 +124,9 @@ class Problem:
             thermal = g.matmodel in THERMAL_MODELS
             if thermal != (self.howmany == 1):
                 raise ValueError(f"{g.matmodel} is not a valid matmodel for this problem type")
+            if self.howmany == 3 and (g.matmodel in LARGE_STRAIN_MODELS) != self.large_strain:
+                kind = "large" if self.large_strain else "small"
+                raise ValueError(f"{g.matmodel} is not a valid {kind} strain material model")
             g.param_matrix()  # missing / inconsistent material properties raise here
         seen = set()
         for g in self.groups:
@@ -158,6 +173,8 @@ class Problem:
             howmany, n_str = 1, 3
         elif problem_type == "mechanical" and strain_type == "small":
             howmany, n_str = 3, 6
+        elif problem_type == "mechanical" and strain_type == "large":
+            howmany, n_str = 3, 9
         else:
             raise ValueError(f"{problem_type}/{strain_type} is not a supported problem type")
         fe_type = j.get("FE_type", "HEX8")

@@ -28,6 +28,8 @@ MODEL = {
     "LinearElasticTriclinic": 4,
     "J2PlasticityNew_LinearIsotropicHardening": 5,
     "PseudoPlasticLinearHardening": 6,
+    "SaintVenantKirchhoff": 7,
+    "CompressibleNeoHookean": 8,
 }
 
 
@@ -37,7 +39,7 @@ class OrDesc(C.Structure):
         ("howmany", C.c_int32), ("fe_type", C.c_int32), ("method", C.c_int32),
         ("measure", C.c_int32), ("err_type", C.c_int32), ("n_it", C.c_int32),
         ("ls_max_iter", C.c_int32), ("L", C.c_double * 3), ("tol", C.c_double),
-        ("ls_tol", C.c_double),
+        ("ls_tol", C.c_double), ("large_strain", C.c_int32),
     ]
 
 
@@ -140,9 +142,10 @@ class Oracle:
         d.L[:] = list(problem.L)
         d.tol = problem.tol
         d.ls_tol = problem.ls_tol
+        d.large_strain = int(problem.n_str == 9)
         self.L = lib()
         self.h = problem.howmany
-        self.nstr = 3 if self.h == 1 else 6
+        self.nstr = problem.n_str
         self.dims = tuple(problem.dims)
         self.N = int(np.prod(self.dims))
         self.ptr = self.L.or_create(C.byref(d))

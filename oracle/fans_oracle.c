@@ -35,7 +35,7 @@
 typedef double complex cplx;
 
 #define MAXH 3
-#define MAXSTR 6
+#define MAXSTR 9
 #define MAXE 24 /* 8 * howmany */
 
 /* ------------------------------------------------------------------------ */
@@ -196,12 +196,12 @@ void or_fft_c2r(int nx, int ny, int nz, int h, const double *in_c, double *out)
 /* small dense helpers                                                       */
 /* ------------------------------------------------------------------------ */
 
-/* eigen-decomposition of a symmetric n x n (n <= 6) matrix by cyclic Jacobi
+/* eigen-decomposition of a symmetric n x n (n <= 9) matrix by cyclic Jacobi
  * rotations; used for the pseudo-inverse of solver.h:89-96 (JacobiSVD with an
  * absolute cutoff on singular values = |eigenvalues| for symmetric input) */
 static void sym_eig(int n, const double *A, double *lam, double *V)
 {
-    double a[36];
+    double a[81];
     memcpy(a, A, sizeof(double) * n * n);
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++)
@@ -248,7 +248,7 @@ static void sym_eig(int n, const double *A, double *lam, double *V)
 
 static void sym_pinv(int n, const double *A, double cutoff, double *P)
 {
-    double lam[6], V[36];
+    double lam[9], V[81];
     sym_eig(n, A, lam, V);
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++) {
@@ -258,6 +258,43 @@ static void sym_pinv(int n, const double *A, double cutoff, double *P)
                     acc += V[i * n + k] * V[j * n + k] / lam[k];
             P[i * n + j] = acc;
         }
+}
+
+/* inverse of a general n x n (n <= 9) matrix by Gauss-Jordan elimination with
+ * partial pivoting (the complete orthogonal decomposition pinv of
+ * mixedBCs.h:43 for a nonsingular, non-symmetric block); nonzero if singular */
+static int gauss_inverse(int n, const double *A, double *Ai)
+{
+    double a[9][18];
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < 2 * n; j++)
+            a[i][j] = j < n ? A[i * n + j] : (j - n == i);
+    for (int c = 0; c < n; c++) {
+        int piv = c;
+        for (int r = c + 1; r < n; r++)
+            if (fabs(a[r][c]) > fabs(a[piv][c]))
+                piv = r;
+        if (a[piv][c] == 0.0)
+            return -1;
+        for (int j = 0; j < 2 * n; j++) {
+            double t    = a[c][j];
+            a[c][j]     = a[piv][j];
+            a[piv][j]   = t;
+        }
+        double d = a[c][c];
+        for (int j = 0; j < 2 * n; j++)
+            a[c][j] /= d;
+        for (int r = 0; r < n; r++)
+            if (r != c) {
+                double f = a[r][c];
+                for (int j = 0; j < 2 * n; j++)
+                    a[r][j] -= f * a[c][j];
+            }
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            Ai[i * n + j] = a[i][n + j];
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -276,6 +313,9 @@ struct or_solver {
     ptrdiff_t N, Nf;
     double    le[3], ve;
     double    Bint[8][MAXSTR * MAXE]; /* B_int[p]: nstr x 8h, row major */
+    /* nonzero pattern of B_int (skipping exact zeros leaves every sum unchanged) */
+    int       nzr[8][MAXSTR], nzr_col[8][MAXSTR][MAXE];
+    int       nzc[8][MAXE], nzc_row[8][MAXE][MAXSTR];
     uint16_t *ms;
     int       ngroups;
     or_group *groups;
@@ -313,6 +353,8 @@ int or_model_nparams(int kind)
     case OR_MODEL_ELASTIC_TRI: return 21;
     case OR_MODEL_J2_NEW: return 4;
     case OR_MODEL_PSEUDO_PLASTIC: return 4;
+    case OR_MODEL_SVK: return 2;
+    case OR_MODEL_NEO_HOOKE: return 2;
     default: return -1;
     }
 }
@@ -378,6 +420,15 @@ static void compute_B(const or_solver *S, double x, double y, double z, double *
                 out[i * ncol + q] = b[i][q];
         return;
     }
+    if (S->nstr == 9) {
+        /* LargeStrainMechModel::Compute_B (LargeStrainMechModel.h:205-222):
+         * row 3i+J = dN/dX_J acting on u_i */
+        for (int i = 0; i < 3; i++)
+            for (int J = 0; J < 3; J++)
+                for (int q = 0; q < 8; q++)
+                    out[(3 * i + J) * ncol + 3 * q + i] = b[J][q];
+        return;
+    }
     const double sh = 7.071067811865476e-01;
     for (int q = 0; q < 8; q++) {
         out[0 * ncol + 3 * q + 0] = b[0][q];
@@ -429,13 +480,15 @@ or_solver *or_create(const or_desc *d)
         return NULL;
     if (d->n_x < 1 || d->n_y < 1 || d->n_z < 2 || (d->n_z % 2) != 0)
         return NULL;
+    if (d->large_strain && d->howmany != 3)
+        return NULL;
     or_solver *S = (or_solver *) calloc(1, sizeof(or_solver));
     S->cfg       = *d;
     S->nx        = d->n_x;
     S->ny        = d->n_y;
     S->nz        = d->n_z;
     S->h         = d->howmany;
-    S->nstr      = d->howmany == 1 ? 3 : 6;
+    S->nstr      = d->howmany == 1 ? 3 : (d->large_strain ? 9 : 6);
     S->ngp       = d->fe_type == OR_FE_HEX8R ? 1 : 8;
     S->N         = (ptrdiff_t) S->nx * S->ny * S->nz;
     S->Nf        = (ptrdiff_t) S->nx * S->ny * (S->nz / 2 + 1);
@@ -457,6 +510,21 @@ or_solver *or_create(const or_desc *d)
     S->ms        = (uint16_t *) calloc((size_t) S->N, sizeof(uint16_t));
     S->err_all   = (double *) calloc((size_t) d->n_it + 1, sizeof(double));
     construct_B(S);
+    for (int p = 0; p < S->ngp; p++) {
+        int ne = 8 * S->h;
+        for (int i = 0; i < S->nstr; i++) {
+            S->nzr[p][i] = 0;
+            for (int j = 0; j < ne; j++)
+                if (S->Bint[p][i * ne + j] != 0.0)
+                    S->nzr_col[p][i][S->nzr[p][i]++] = j;
+        }
+        for (int j = 0; j < ne; j++) {
+            S->nzc[p][j] = 0;
+            for (int i = 0; i < S->nstr; i++)
+                if (S->Bint[p][i * ne + j] != 0.0)
+                    S->nzc_row[p][j][S->nzc[p][j]++] = i;
+        }
+    }
     return S;
 }
 
@@ -558,13 +626,19 @@ static void element_stiffness(const or_solver *S, const double *C, double *Ke)
 }
 
 /* model->get_reference_stiffness() averaged over groups (MaterialManager.h:198-204) */
+static void large_reference_stiffness(const or_group *g, double *C);
+
 static void group_reference_stiffness(const or_solver *S, const or_group *g, double *C)
 {
     int n = S->nstr;
     memset(C, 0, sizeof(double) * n * n);
+    if (g->kind == OR_MODEL_SVK || g->kind == OR_MODEL_NEO_HOOKE) {
+        large_reference_stiffness(g, C);
+        return;
+    }
     if (g->kind == OR_MODEL_THERMAL_ISO || g->kind == OR_MODEL_THERMAL_TRI || g->kind == OR_MODEL_ELASTIC_TRI) {
         /* LinearThermal.h:42-45 (kappa_average), LinearElastic.h:266-274 */
-        double Ci[36];
+        double Ci[81];
         for (int i = 0; i < g->nphase; i++) {
             linear_C(S->nstr, g->kind, g->params + (size_t) i * g->np, Ci);
             for (int k = 0; k < n * n; k++)
@@ -667,6 +741,10 @@ int or_set_materials(or_solver *S, int n_groups, const int32_t *kind, const int3
         int thermal = kind[g] == OR_MODEL_THERMAL_ISO || kind[g] == OR_MODEL_THERMAL_TRI;
         if (thermal != (S->h == 1))
             return -2;
+        /* createMatmodel<3,9> accepts only the finite strain models (setup.h:62-73) */
+        int large = kind[g] == OR_MODEL_SVK || kind[g] == OR_MODEL_NEO_HOOKE;
+        if (large != (S->nstr == 9))
+            return -2;
         for (int i = 0; i < nphase[g]; i++)
             maxp = phase_ids[tot + i] > maxp ? phase_ids[tot + i] : maxp;
         tot += nphase[g];
@@ -710,7 +788,7 @@ int or_set_materials(or_solver *S, int n_groups, const int32_t *kind, const int3
         memcpy(S->kref, ref_material, sizeof(double) * n * n);
     } else {
         memset(S->kref, 0, sizeof(S->kref));
-        double C[36];
+        double C[81];
         for (int g = 0; g < n_groups; g++) {
             group_reference_stiffness(S, &S->groups[g], C);
             for (int k = 0; k < n * n; k++)
@@ -726,7 +804,7 @@ int or_set_materials(or_solver *S, int n_groups, const int32_t *kind, const int3
         int           k;
         const double *pp = phase_params(S, p, &k);
         if (model_is_linear(k)) {
-            double C[36];
+            double C[81];
             linear_C(S->nstr, k, pp, C);
             element_stiffness(S, C, S->Ke + (size_t) p * ne * ne);
         }
@@ -841,9 +919,173 @@ static void pseudo_point(const double *p, const double *e, double *s)
         s[i] = buf2 * dev[i];
 }
 
+/* ------------------------------------------------------------------------ */
+/* finite strain (LargeStrainMechModel.h, SaintVenantKirchhoff.h,             */
+/* CompressibleNeoHookean.h): eps holds F row major, sigma holds P = F S      */
+/* ------------------------------------------------------------------------ */
+static int mandel_index(int a, int b)
+{
+    if (a == b)
+        return a;
+    int lo = a < b ? a : b, hi = a < b ? b : a;
+    return lo == 0 ? (hi == 1 ? 3 : 4) : 5;
+}
+
+static void inv3(const double A[3][3], double Ai[3][3], double *det)
+{
+    double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    double d   = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+    Ai[0][0]   = c00 / d;
+    Ai[1][0]   = c01 / d;
+    Ai[2][0]   = c02 / d;
+    Ai[0][1]   = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / d;
+    Ai[1][1]   = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / d;
+    Ai[2][1]   = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / d;
+    Ai[0][2]   = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / d;
+    Ai[1][2]   = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / d;
+    Ai[2][2]   = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / d;
+    *det       = d;
+}
+
+/* second Piola-Kirchhoff stress S(F) of the two hyperelastic models; returns
+ * nonzero for det F <= 0 (CompressibleNeoHookean.h:40-42 throws there) */
+static int large_S(int kind, const double *p, const double F[3][3], double S[3][3])
+{
+    const double mu = p[1], lam = p[0] - (2.0 / 3.0) * p[1];
+    double       C[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            C[a][b] = F[0][a] * F[0][b] + F[1][a] * F[1][b] + F[2][a] * F[2][b];
+    if (kind == OR_MODEL_SVK) {
+        /* S = lam tr(E) I + 2 mu E, E = (C - I) / 2 (SaintVenantKirchhoff.h:34-39) */
+        double trE = 0.5 * (C[0][0] + C[1][1] + C[2][2] - 3.0);
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                S[a][b] = mu * (C[a][b] - (a == b)) + (a == b ? lam * trE : 0.0);
+        return 0;
+    }
+    /* S = lam log(J) C^-1 + mu (I - C^-1) (CompressibleNeoHookean.h:35-49) */
+    double Fa[3][3], Fi[3][3], J, Ci[3][3], dC;
+    memcpy(Fa, F, sizeof(Fa));
+    inv3(Fa, Fi, &J);
+    if (!(J > 0.0))
+        return -1;
+    inv3(C, Ci, &dC);
+    const double lj = log(J);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            S[a][b] = lam * lj * Ci[a][b] + mu * ((a == b) - Ci[a][b]);
+    return 0;
+}
+
+/* LargeStrainMechModel::get_sigma (LargeStrainMechModel.h:193-199) */
+static void large_point(int kind, const double *p, const double *e, double *P)
+{
+    double F[3][3], S[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            F[a][b] = e[3 * a + b];
+    if (large_S(kind, p, F, S)) {
+        for (int k = 0; k < 9; k++)
+            P[k] = NAN;
+        return;
+    }
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            P[3 * a + b] = F[a][0] * S[0][b] + F[a][1] * S[1][b] + F[a][2] * S[2][b];
+}
+
+/* material tangent dS/dE (Mandel, 6x6) of the two models at F
+ * (SaintVenantKirchhoff.h:41-47, CompressibleNeoHookean.h:51-87) */
+static void large_material_tangent(int kind, const double *p, const double F[3][3], double *Cm)
+{
+    const double mu = p[1], lam = p[0] - (2.0 / 3.0) * p[1];
+    memset(Cm, 0, sizeof(double) * 36);
+    if (kind == OR_MODEL_SVK) {
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                Cm[a * 6 + b] = lam;
+        for (int a = 0; a < 6; a++)
+            Cm[a * 6 + a] += 2.0 * mu;
+        return;
+    }
+    double C[3][3], Ci[3][3], Fa[3][3], Fi[3][3], J, dC;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            C[a][b] = F[0][a] * F[0][b] + F[1][a] * F[1][b] + F[2][a] * F[2][b];
+    memcpy(Fa, F, sizeof(Fa));
+    inv3(Fa, Fi, &J);
+    inv3(C, Ci, &dC);
+    static const int ij[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+    const double     r2       = sqrt(2.0);
+    double           cv[6];
+    for (int a = 0; a < 6; a++)
+        cv[a] = Ci[ij[a][0]][ij[a][1]] * (a < 3 ? 1.0 : r2);
+    const double c2 = mu - lam * log(J);
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) {
+            int          ia = ij[a][0], ja = ij[a][1], kb = ij[b][0], lb = ij[b][1];
+            const double pp2 =
+                (Ci[ia][kb] * Ci[ja][lb] + Ci[ia][lb] * Ci[ja][kb]) * (a < 3 ? 1.0 : r2) * (b < 3 ? 1.0 : r2);
+            Cm[a * 6 + b] = lam * cv[a] * cv[b] + c2 * pp2;
+        }
+}
+
+/* dP/dF (9x9, rows iJ, columns kL) from S and dS/dE
+ * (LargeStrainMechModel.h:101-176).  Like the reference, the sum over the
+ * symmetric pair (P,Q) runs over P <= Q only, so the columns of the off-diagonal
+ * F_kL carry half of the tensor's dE_PQ/dF_kL contribution. */
+static void large_spatial_tangent(const double F[3][3], const double S[3][3], const double *Cm, double *A)
+{
+    memset(A, 0, sizeof(double) * 81);
+    const double r2 = sqrt(2.0);
+    for (int i = 0; i < 3; i++)
+        for (int J = 0; J < 3; J++)
+            for (int k = 0; k < 3; k++)
+                for (int L = 0; L < 3; L++) {
+                    double a = (i == k) ? S[L][J] : 0.0;
+                    for (int M = 0; M < 3; M++) {
+                        int mj = mandel_index(M, J);
+                        for (int P = 0; P < 3; P++)
+                            for (int Q = P; Q < 3; Q++) {
+                                int    pq = mandel_index(P, Q);
+                                double c  = Cm[mj * 6 + pq] / ((mj >= 3 ? r2 : 1.0) * (pq >= 3 ? r2 : 1.0));
+                                double de = (Q == L ? 0.5 * F[k][P] : 0.0) + (P == L ? 0.5 * F[k][Q] : 0.0);
+                                a += F[i][M] * c * de;
+                            }
+                    }
+                    A[(3 * i + J) * 9 + 3 * k + L] = a;
+                }
+}
+
+/* get_reference_stiffness of the two models: dP/dF at F = I averaged over the
+ * group's phases (SaintVenantKirchhoff.h:49-66, CompressibleNeoHookean.h:89-104) */
+static void large_reference_stiffness(const or_group *g, double *C)
+{
+    const double Fi[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    memset(C, 0, sizeof(double) * 81);
+    for (int ph = 0; ph < g->nphase; ph++) {
+        const double *p = g->params + (size_t) ph * g->np;
+        double        S[3][3], Cm[36], A[81];
+        large_S(g->kind, p, Fi, S);
+        large_material_tangent(g->kind, p, Fi, Cm);
+        large_spatial_tangent(Fi, S, Cm, A);
+        for (int k = 0; k < 81; k++)
+            C[k] += A[k];
+    }
+    for (int k = 0; k < 81; k++)
+        C[k] /= g->nphase;
+}
+
 void or_material_point(int kind, const double *p, const double *e, const double *ep_t, double q_t, double *sigma,
                        double *ep, double *q)
 {
+    if (kind == OR_MODEL_SVK || kind == OR_MODEL_NEO_HOOKE) {
+        large_point(kind, p, e, sigma);
+        return;
+    }
     if (kind == OR_MODEL_J2_NEW) {
         j2new_point(p, e, ep_t, q_t, sigma, ep, q);
     } else if (kind == OR_MODEL_PSEUDO_PLASTIC) {
@@ -879,8 +1121,10 @@ static void element_strain_stress(or_solver *S, const double *ue, int phase, ptr
     for (int p = 0; p < S->ngp; p++)
         for (int i = 0; i < ns; i++) {
             double a = 0;
-            for (int j = 0; j < ne; j++)
+            for (int k = 0; k < S->nzr[p][i]; k++) {
+                int j = S->nzr_col[p][i][k];
                 a += S->Bint[p][i * ne + j] * ue[j];
+            }
             eps[p * ns + i] = a + S->g0[i];
         }
     int           kind;
@@ -904,8 +1148,10 @@ static void element_residual(or_solver *S, const double *ue, int phase, ptrdiff_
     for (int j = 0; j < ne; j++) {
         double a = 0;
         for (int p = 0; p < S->ngp; p++)
-            for (int i = 0; i < ns; i++)
+            for (int k = 0; k < S->nzc[p][j]; k++) {
+                int i = S->nzc_row[p][j][k];
                 a += S->Bint[p][i * ne + j] * sig[p * ns + i];
+            }
         res[j] = a * S->ve / S->ngp;
     }
 }
@@ -1052,6 +1298,9 @@ void or_homogenized_stress(or_solver *S, double *out)
     int       h = S->h, ns = S->nstr;
     ptrdiff_t N = S->N;
     double    acc[MAXSTR] = {0};
+    /* x planes in parallel, summed in plane order afterwards */
+    double   *plane = (double *) calloc((size_t) S->nx * ns, sizeof(double));
+#pragma omp parallel for schedule(static)
     for (int ix = 0; ix < S->nx; ix++)
         for (int iy = 0; iy < S->ny; iy++)
             for (int iz = 0; iz < S->nz; iz++) {
@@ -1068,9 +1317,13 @@ void or_homogenized_stress(or_solver *S, double *out)
                     double av = 0;
                     for (int p = 0; p < S->ngp; p++)
                         av += sig[p * ns + j] / S->ngp;
-                    acc[j] += av;
+                    plane[(size_t) ix * ns + j] += av;
                 }
             }
+    for (int ix = 0; ix < S->nx; ix++)
+        for (int j = 0; j < ns; j++)
+            acc[j] += plane[(size_t) ix * ns + j];
+    free(plane);
     for (int j = 0; j < ns; j++)
         out[j] = acc[j] / (double) N;
 }
@@ -1123,13 +1376,20 @@ int or_enable_mixed_bc(or_solver *S, int nE, const int32_t *idxE, int nF, const 
     if (nF) {
         /* M = pinv(Q_F^T C0 Q_F): a principal submatrix of the SPD reference stiffness */
         double A[81];
+        int    sym = 1;
         for (int i = 0; i < nF; i++)
-            for (int j = 0; j < nF; j++)
+            for (int j = 0; j < nF; j++) {
                 A[i * nF + j] = S->kref[S->idxF[i] * n + S->idxF[j]];
-        sym_pinv(nF, A, 1e-300, S->M);
+                if (S->kref[S->idxF[i] * n + S->idxF[j]] != S->kref[S->idxF[j] * n + S->idxF[i]])
+                    sym = 0;
+            }
+        if (sym || gauss_inverse(nF, A, S->M))
+            sym_pinv(nF, A, 1e-300, S->M);
     }
     if (step == 0) {
         memset(S->gvec, 0, sizeof(S->gvec));
+        if (n == 9) /* finite strain starts from F = I (mixedBCs.h:191-197) */
+            S->gvec[0] = S->gvec[4] = S->gvec[8] = 1.0;
         memcpy(S->gvec_prev, S->gvec, sizeof(S->gvec));
     } else {
         double delta[9];
@@ -1273,7 +1533,7 @@ int or_homogenized_tangent(or_solver *S, double pert, double *out)
     memcpy(g0, S->g0, sizeof(g0));
     S->err_type = OR_ERR_RELATIVE;
     S->tol      = fmax(1e-6, S->tol);
-    double C[36];
+    double C[81];
     for (int i = 0; i < n; i++) {
         if (S->all_linear) {
             memset(pe, 0, sizeof(pe));

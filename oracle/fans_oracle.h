@@ -18,6 +18,8 @@
  *   include/solver.h:707-778         homogenized stress / tangent
  *   include/solverCG.h, solverFP.h   CG (linear + secant line search), FP
  *   include/mixedBCs.h               mixed strain/stress control
+ *   include/LargeStrainMechModel.h   finite strain kinematics (F, P = F S),
+ *   material_models/SaintVenantKirchhoff.h, CompressibleNeoHookean.h
  *
  * Parity pins (see tests/test_oracle_*.py): numpy.fft for the FFT, brute-force
  * assembled sparse operators and scipy direct solves on tiny grids, closed
@@ -49,7 +51,9 @@ enum {
     OR_MODEL_ELASTIC_ISO     = 3, /* [bulk shear]                                    */
     OR_MODEL_ELASTIC_TRI     = 4, /* [C11..C16 C22..C26 C33..C36 C44..C46 C55 C56 C66] */
     OR_MODEL_J2_NEW          = 5, /* [bulk shear yield_stress iso_hardening]         */
-    OR_MODEL_PSEUDO_PLASTIC  = 6  /* [bulk shear yield_stress hardening]             */
+    OR_MODEL_PSEUDO_PLASTIC  = 6, /* [bulk shear yield_stress hardening]             */
+    OR_MODEL_SVK             = 7, /* SaintVenantKirchhoff [bulk shear] (large strain) */
+    OR_MODEL_NEO_HOOKE       = 8  /* CompressibleNeoHookean [bulk shear] (large strain) */
 };
 
 typedef struct {
@@ -64,6 +68,7 @@ typedef struct {
     double  L[3];
     double  tol;
     double  ls_tol;
+    int32_t large_strain; /* mechanics with strain_type "large": n_str = 9 (F row major) */
 } or_desc;
 
 typedef struct or_solver or_solver;

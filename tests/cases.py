@@ -55,3 +55,36 @@ def uniaxial_mixed(n_steps, emax):
     return dict(strain_indices=[0, 3, 4, 5], stress_indices=[1, 2],
                 strain_path=np.stack([steps, 0 * steps, 0 * steps, 0 * steps], 1),
                 stress_path=np.zeros((n_steps, 2)), n_steps=n_steps)
+
+
+def _spd(n, rng, shift):
+    a = rng.standard_normal((n, n))
+    return a @ a.T + shift * np.eye(n)
+
+
+def general(model, n=12, seed=0, fe="HEX8", tol=1e-10, n_it=300, dims=None, count=3, rmin=2.0, rmax=3.0):
+    """Two phases of one of the per-Gauss-point models (triclinic thermal or
+    elastic with random SPD tensors, PseudoPlastic or J2 matrix with an elastic
+    inclusion)."""
+    dims = dims or (n, n, n)
+    rng = np.random.default_rng(100 + seed)
+    ms = random_spheres(dims, count, rmin, rmax, seed=seed)
+    h = 1 if model == "LinearThermalTriclinic" else 3
+    if model == "LinearThermalTriclinic":
+        Ks = [_spd(3, rng, 1.0), _spd(3, rng, 5.0)]
+        keys = [(i, j) for i in range(3) for j in range(i, 3)]
+        props = {f"K_{i + 1}{j + 1}": [float(K[i, j]) for K in Ks] for i, j in keys}
+        groups = [MaterialGroup(model, [0, 1], props)]
+    elif model == "LinearElasticTriclinic":
+        Cs = [_spd(6, rng, 3.0), _spd(6, rng, 20.0)]
+        keys = [(i, j) for i in range(6) for j in range(i, 6)]
+        props = {f"C_{i + 1}{j + 1}": [float(C[i, j]) for C in Cs] for i, j in keys}
+        groups = [MaterialGroup(model, [0, 1], props)]
+    elif model == "PseudoPlasticLinearHardening":
+        groups = [MaterialGroup(model, [0], {"bulk_modulus": [62.5], "shear_modulus": [28.8462],
+                                             "yield_stress": [0.1], "hardening_parameter": [0.5]}),
+                  MaterialGroup("LinearElasticIsotropic", [1], {"bulk_modulus": [222.222],
+                                                                "shear_modulus": [166.6667]})]
+    else:
+        return j2(n=n, seed=seed, fe=fe, tol=tol, n_it=n_it, count=count, rmin=rmin, rmax=rmax)
+    return Problem(dims=dims, howmany=h, ms=ms, groups=groups, fe_type=fe, method="cg", tol=tol, n_it=n_it)

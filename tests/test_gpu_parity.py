@@ -45,7 +45,8 @@ def test_fft_matches_numpy(shape):
 # 1e-10 budget), HEX8 (kappa <= 7e3) on every grid at 1e-12.
 _OPERATOR_CASES = [(m, fe, dims) for m in (cases.thermal, cases.elastic)
                    for fe, grids in (("HEX8", [(12, 10, 8), (32, 64, 16), (16, 128, 32), (256, 32, 8)]),
-                                     ("HEX8R", [(12, 10, 8), (32, 64, 16)]))
+                                     ("HEX8R", [(12, 10, 8), (32, 64, 16)]),
+                                     ("BBAR", [(12, 10, 8), (16, 16, 16)]))
                    for dims in grids]
 
 
@@ -69,6 +70,118 @@ def test_green_and_operators(maker, fe, dims):
     assert relerr(g.residual(u, linear_kd=True), o.residual(u, linear_kd=True)) < 1e-12
     r = rng.standard_normal(p.dims + (p.howmany,))
     assert relerr(g.convolve(r), o.convolve(r)) < tol_g
+    g.close()
+
+
+_GENERAL_CASES = [(m, fe, dims) for m in ("LinearThermalTriclinic", "LinearElasticTriclinic",
+                                          "PseudoPlasticLinearHardening", "J2")
+                  for fe in ("HEX8", "BBAR", "HEX8R") for dims in ((12, 10, 8), (40, 24, 16))
+                  if not (fe == "BBAR" and m == "LinearThermalTriclinic")]
+
+
+@pytest.mark.parametrize("model,fe,dims", _GENERAL_CASES)
+def test_general_model_operators(model, fe, dims):
+    """Per-Gauss-point constitutive path (triclinic tensors, PseudoPlastic and
+    J2 return maps with yielding Gauss points) on the residual, and the linear
+    K d operator for the linear models."""
+    p = cases.general(model, fe=fe, dims=dims)
+    say(f"general operators {model} {fe} dims={p.dims}")
+    o = oracle.Oracle(p)
+    g = gpu.GpuSolver(p)
+    assert relerr(g.reference_stiffness(), o.reference_stiffness()) < 1e-15
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(p.dims + (p.howmany,)) * 0.002
+    g0 = rng.standard_normal(p.n_str) * 0.01
+    o.set_gradient(g0)
+    g.set_gradient(g0)
+    assert relerr(g.residual(u), o.residual(u)) < 1e-12
+    if model.startswith("Linear"):
+        assert relerr(g.residual(u, linear_kd=True), o.residual(u, linear_kd=True)) < 1e-12
+    g.close()
+
+
+def test_pseudo_plastic_mixed_bc_golden(sphere32_ms):
+    """test_MixedBCs.json load case 2 (hydrostatic stress -0.05/-0.1, nonlinear
+    CG with the secant line search and the mixed-BC controller, PseudoPlastic
+    matrix): the macro strains the reference stored as load case 3 of that
+    file, reached to 1e-8 relative."""
+    p = cases.reference_input("test_MixedBCs", sphere32_ms)
+    lc = p.load_cases[1]
+    golden = p.load_cases[2].g0_path
+    say("pseudo-plastic mixed bc golden sphere32")
+    g = gpu.GpuSolver(p)
+    for t in range(lc.n_steps):
+        g.enable_mixed_bc(lc.mbc, t)
+        g.solve()
+        mg = g.macro_gradient()
+        assert np.max(np.abs(mg[:3] - golden[t][:3])) < 1e-8 * np.abs(golden[t][:3]).max(), (t, mg, golden[t])
+        assert np.max(np.abs(mg[3:])) < 1e-12
+        assert np.max(np.abs(g.homogenized_stress() - np.array(lc.mbc["stress_path"][t]))) < 1e-9
+        g.extrapolate_displacement()
+    g.close()
+
+
+def test_pseudo_plastic_uniaxial_steps():
+    """PseudoPlastic matrix past yield under uniaxial strain with zero lateral
+    stress (the control law of test_MixedBCs.json load case 1)."""
+    p = cases.general("PseudoPlasticLinearHardening", n=12, tol=1e-9)
+    mbc = cases.uniaxial_mixed(3, 0.01)
+    say("pseudo-plastic uniaxial 12^3 3 steps")
+    o = oracle.Oracle(p)
+    g = gpu.GpuSolver(p)
+    for t in range(mbc["n_steps"]):
+        o.enable_mixed_bc(mbc, t)
+        g.enable_mixed_bc(mbc, t)
+        it_o, it_g = o.solve(), g.solve()
+        assert abs(it_o - it_g) <= 2, (t, it_o, it_g)
+        assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-8, t
+        assert relerr(g.macro_gradient(), o.macro_gradient()) < 1e-8
+        o.extrapolate_displacement()
+        g.extrapolate_displacement()
+    g.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_bbar_elastic_solve(precision):
+    """B-bar elements with a nearly incompressible matrix: the solve against
+    the oracle (iterations within +-2 in FP64) and the element fields."""
+    E, nu = 1.0, 0.49
+    K, G = E / (3 * (1 - 2 * nu)), E / (2 * (1 + nu))
+    p = cases.elastic(n=16, seed=6, fe="BBAR", K=(K, 20 * K), G=(G, 20 * G), count=3, rmin=2.5, rmax=4.0,
+                      tol=1e-10, n_it=2000)
+    say(f"bbar elastic 16^3 {precision}")
+    g0 = np.array([0.001, -0.002, 0.003, 0.0015, -0.0025, 0.001])
+    o, g, it_o, it_g = _solve_pair(p, g0, precision=precision)
+    if precision == "fp64":
+        assert abs(it_o - it_g) <= 2
+        assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-10
+        fo, fg = o.element_fields(), g.element_fields()
+        for k in ("strain", "stress", "stress_average", "strain_average"):
+            assert relerr(fg[k], fo[k]) < 1e-8, k
+    else:
+        assert g.error_history()[-1] <= p.tol
+        assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-8
+    g.close()
+
+
+def test_bbar_j2_mixed_bc_steps():
+    p = cases.j2(n=12, seed=1, fe="BBAR")
+    mbc = cases.uniaxial_mixed(3, 0.02)
+    say("bbar j2 mixed bc 12^3 3 steps")
+    o = oracle.Oracle(p)
+    g = gpu.GpuSolver(p)
+    for t in range(mbc["n_steps"]):
+        o.enable_mixed_bc(mbc, t)
+        g.enable_mixed_bc(mbc, t)
+        it_o, it_g = o.solve(), g.solve()
+        assert abs(it_o - it_g) <= 2, (t, it_o, it_g)
+        assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-8, t
+        o.extrapolate_displacement()
+        g.extrapolate_displacement()
+    _, qo = o.get_state()
+    _, qg = g.get_state()
+    assert relerr(qg, qo) < 1e-6
+    assert qo.max() > 0
     g.close()
 
 

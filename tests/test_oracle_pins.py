@@ -96,6 +96,93 @@ def test_element_stiffness(h, fe):
             assert rel(Ke, Kref) < 1e-8
 
 
+def test_bbar_B_matrices():
+    """B-bar (matmodel.h:113-140) against the HEX8 and HEX8R B matrices, which
+    the test above pins to differentiated shape functions: shear rows and the
+    deviatoric part of the normal rows are HEX8's, the trace row is the centre
+    (HEX8R) one at every Gauss point, and that equals the Gauss mean of the
+    HEX8 trace rows.  Thermal B-bar is HEX8."""
+    L = (1.0, 0.8, 1.3)
+    small = dict(n=4, L=L, count=1, rmin=1.0, rmax=1.0)
+    Bh = oracle.Oracle(cases.elastic(fe="HEX8", **small)).B()
+    Bc = oracle.Oracle(cases.elastic(fe="HEX8R", **small)).B()[0]
+    Bb = oracle.Oracle(cases.elastic(fe="BBAR", **small)).B()
+    assert Bb.shape == Bh.shape == (8, 6, 24)
+    assert np.array_equal(Bb[:, 3:], Bh[:, 3:])
+    tr_h, tr_b, tr_c = Bh[:, :3].sum(1), Bb[:, :3].sum(1), Bc[:3].sum(0)
+    for gp in range(8):
+        assert np.max(np.abs(tr_b[gp] - tr_c)) < 1e-12
+        assert np.max(np.abs((Bb[gp, :3] - tr_b[gp] / 3) - (Bh[gp, :3] - tr_h[gp] / 3))) < 1e-12
+    assert np.max(np.abs(tr_h.mean(0) - tr_c)) < 1e-12
+    Th = oracle.Oracle(cases.thermal(fe="HEX8", **small)).B()
+    Tb = oracle.Oracle(cases.thermal(fe="BBAR", **small)).B()
+    assert np.array_equal(Th, Tb)
+
+
+def test_bbar_stiffness_energy_identity():
+    """For sigma = K tr(eps) m + 2 G dev(eps), B-bar removes the bulk energy of
+    the trace's deviation from its element mean:
+        Ke_HEX8 - Ke_BBAR = K w sum_gp (b_gp - b_mean)^T (b_gp - b_mean),
+    b_gp = trace row of the HEX8 B at Gauss point gp, w = v_e / 8."""
+    L = (1.0, 0.9, 1.1)
+    small = dict(n=4, L=L, count=1, rmin=1.0, rmax=1.0)
+    ph = cases.elastic(fe="HEX8", **small)
+    oh = oracle.Oracle(ph)
+    ob = oracle.Oracle(cases.elastic(fe="BBAR", **small))
+    b = oh.B()[:, :3].sum(1)
+    db = b - b.mean(0)
+    w = np.prod(np.array(L) / 4) / 8
+    params = ph.groups[0].param_matrix()
+    for phase in range(2):
+        K = params[phase][0]
+        expect = K * w * db.T @ db
+        diff = oh.phase_stiffness(phase) - ob.phase_stiffness(phase)
+        assert rel(diff, expect) < 1e-12
+        # and the identity is not trivially satisfied
+        assert np.abs(expect).max() > 1e-3 * np.abs(oh.phase_stiffness(phase)).max()
+    kh, kb = oh.ref_element_stiffness(), ob.ref_element_stiffness()
+    assert np.max(np.abs(kh - kb)) > 0
+
+
+@pytest.mark.parametrize("comp", [0, 3])
+def test_bbar_laminate_closed_form(comp):
+    """Laminate with interfaces on voxel faces: the exact solution has a
+    constant strain per element, so B-bar changes nothing and reproduces the
+    closed form."""
+    dims = (8, 4, 4)
+    ms = laminate(dims, axis=0, widths=(5, 3), phases=(0, 1))
+    K, G = (2.0, 30.0), (1.0, 12.0)
+    p = Problem(dims=dims, howmany=3, ms=ms, fe_type="BBAR",
+                groups=[MaterialGroup("LinearElasticIsotropic", [0, 1], {"bulk_modulus": list(K),
+                                                                          "shear_modulus": list(G)})], tol=1e-13,
+                n_it=500)
+    o = oracle.Oracle(p)
+    e = np.zeros(6)
+    e[comp] = 1.0
+    o.set_gradient(e)
+    o.solve()
+    lam = [K[i] - 2 / 3 * G[i] for i in range(2)]
+    assert rel(o.homogenized_stress(), elastic_laminate([5 / 8, 3 / 8], lam, G, e)) < 1e-10
+
+
+def test_bbar_relieves_volumetric_locking():
+    """Ke_BBAR <= Ke_HEX8 in the energy sense, so the homogenized shear
+    stiffness with B-bar lies below HEX8's.  With a nearly incompressible
+    matrix (nu = 0.4999) around a stiff inclusion the gap is several percent:
+    the volumetric locking of full integration that B-bar relieves."""
+    E, nu = 1.0, 0.4999
+    K, G = E / (3 * (1 - 2 * nu)), E / (2 * (1 + nu))
+    res = {}
+    for fe in ("HEX8", "BBAR"):
+        p = cases.elastic(n=8, fe=fe, K=(K, 50 * K), G=(G, 50 * G), count=1, rmin=2.5, rmax=2.5, seed=1,
+                          tol=1e-12, n_it=3000)
+        o = oracle.Oracle(p)
+        o.set_gradient([0.0, 0.0, 0.0, 1.0, 0.0, 0.0])
+        o.solve()
+        res[fe] = o.homogenized_stress()[3]
+    assert 0.02 < (res["HEX8"] - res["BBAR"]) / res["HEX8"] < 0.2
+
+
 @pytest.mark.parametrize("maker,fe", [(cases.thermal, "HEX8"), (cases.elastic, "HEX8")])
 def test_residual_against_sparse_assembly(maker, fe):
     p = maker(n=5, fe=fe, dims=(4, 5, 6), count=2, rmin=1.0, rmax=2.0)
@@ -376,3 +463,142 @@ def test_reference_pytest_invariants(sphere32_ms):
         assert np.all(np.linalg.eigvalsh(C) > 0)
         # test_homogenization_consistency.py:91-99
         assert np.allclose(ef["stress_average"], C @ ef["strain_average"], rtol=1e-4, atol=1e-1)
+
+
+# ----------------------------------------------------------------------------- finite strain
+LARGE_KG = {"bulk_modulus": [62.5, 222.222], "shear_modulus": [28.8462, 166.6667]}
+
+
+def large_problem(model, dims=(6, 4, 8), ms=None, tol=1e-10, n_it=300, fe="HEX8", L=(1.0, 1.0, 1.0)):
+    if ms is None:
+        ms = np.zeros(dims, dtype=np.uint16)
+        ms[1:3, 1:3, 2:5] = 1
+    return Problem(dims=tuple(ms.shape), howmany=3, ms=ms, groups=[MaterialGroup(model, [0, 1], dict(LARGE_KG))],
+                   strain_type="large", fe_type=fe, tol=tol, n_it=n_it, L=L)
+
+
+def hyperelastic_energy(model, K, G, F):
+    """Stored energies whose F-gradients are the models' P: Saint-Venant
+    Kirchhoff W = lam/2 tr(E)^2 + mu E:E; compressible Neo-Hooke
+    W = mu/2 (tr C - 3) - mu ln J + lam/2 (ln J)^2."""
+    lam, mu = K - 2 * G / 3, G
+    C = F.T @ F
+    if model == "SaintVenantKirchhoff":
+        E = 0.5 * (C - np.eye(3))
+        return 0.5 * lam * np.trace(E) ** 2 + mu * np.sum(E * E)
+    lj = np.log(np.linalg.det(F))
+    return 0.5 * mu * (np.trace(C) - 3) - mu * lj + 0.5 * lam * lj ** 2
+
+
+def test_large_strain_B_matrix():
+    """LargeStrainMechModel::Compute_B: B_F u_e of a linear displacement field
+    u = H X is H row major (F_iJ - delta_iJ = du_i/dX_J) at every Gauss point;
+    a transposed layout fails for the non-symmetric H used here."""
+    L = (1.0, 0.8, 1.3)
+    o = oracle.Oracle(large_problem("SaintVenantKirchhoff", L=L))
+    B = o.B()
+    assert B.shape == (8, 9, 24)
+    le = np.array(L) / np.array((6, 4, 8))
+    H = np.array([[0.1, 0.7, -0.3], [0.2, -0.4, 0.9], [-0.6, 0.05, 0.3]])
+    X = np.array([[(a & 1) * le[0], ((a >> 1) & 1) * le[1], ((a >> 2) & 1) * le[2]] for a in range(8)])
+    ue = (X @ H.T).reshape(-1)
+    for gp in range(8):
+        assert rel(B[gp] @ ue, H.reshape(-1)) < 1e-12
+        assert np.max(np.abs(B[gp] @ np.tile([1.0, 2.0, 3.0], 8))) < 1e-12
+
+
+@pytest.mark.parametrize("model", ["SaintVenantKirchhoff", "CompressibleNeoHookean"])
+def test_hyperelastic_points(model):
+    """P(F) = dW/dF (central differences of the stored energy), objectivity
+    P(QF) = Q P(F), and the small-strain limit P(I + eps H) / eps -> the
+    linear elastic stress of sym(H) with the same K, G."""
+    K, G = 62.5, 28.8462
+    prm = np.array([K, G])
+    rng = np.random.default_rng(4)
+    for _ in range(5):
+        F = np.eye(3) + 0.2 * rng.standard_normal((3, 3))
+        P, _, _ = oracle.material_point(model, prm, F.reshape(-1))
+        P = P.reshape(3, 3)
+        h = 1e-6
+        dW = np.zeros((3, 3))
+        for i in range(3):
+            for j in range(3):
+                Fp, Fm = F.copy(), F.copy()
+                Fp[i, j] += h
+                Fm[i, j] -= h
+                dW[i, j] = (hyperelastic_energy(model, K, G, Fp) - hyperelastic_energy(model, K, G, Fm)) / (2 * h)
+        assert rel(P, dW) < 1e-7
+        a = rng.standard_normal(3)
+        a /= np.linalg.norm(a)
+        th = 0.7
+        A = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+        Q = np.eye(3) + np.sin(th) * A + (1 - np.cos(th)) * A @ A
+        PQ, _, _ = oracle.material_point(model, prm, (Q @ F).reshape(-1))
+        assert rel(PQ.reshape(3, 3), Q @ P) < 1e-12
+    H = rng.standard_normal((3, 3))
+    eps = 1e-7
+    P, _, _ = oracle.material_point(model, prm, (np.eye(3) + eps * H).reshape(-1))
+    S = 0.5 * (H + H.T)
+    r2 = np.sqrt(2)
+    e6 = np.array([S[0, 0], S[1, 1], S[2, 2], r2 * S[0, 1], r2 * S[0, 2], r2 * S[1, 2]])
+    s6, _, _ = oracle.material_point("LinearElasticIsotropic", prm, e6)
+    lin = np.array([[s6[0], s6[3] / r2, s6[4] / r2], [s6[3] / r2, s6[1], s6[5] / r2], [s6[4] / r2, s6[5] / r2, s6[2]]])
+    assert rel(P.reshape(3, 3) / eps, lin) < 1e-5
+
+
+@pytest.mark.parametrize("model", ["SaintVenantKirchhoff", "CompressibleNeoHookean"])
+def test_large_strain_reference_stiffness(model):
+    """get_reference_stiffness of the finite strain models: dP/dF at F = I
+    (central differences of P) averaged over the phases, with the reference's
+    sum over P <= Q only (LargeStrainMechModel.h:142-170), which halves the
+    columns of the off-diagonal F_kL."""
+    o = oracle.Oracle(large_problem(model))
+    kref = o.reference_stiffness()
+    exact = np.zeros((9, 9))
+    for K, G in zip(LARGE_KG["bulk_modulus"], LARGE_KG["shear_modulus"]):
+        for c in range(9):
+            h = 1e-6
+            Fp, Fm = np.eye(3).reshape(-1).copy(), np.eye(3).reshape(-1).copy()
+            Fp[c] += h
+            Fm[c] -= h
+            Pp, _, _ = oracle.material_point(model, [K, G], Fp)
+            Pm, _, _ = oracle.material_point(model, [K, G], Fm)
+            exact[:, c] += (Pp - Pm) / (2 * h) / 2
+    half = np.array([1.0 if c in (0, 4, 8) else 0.5 for c in range(9)])
+    assert rel(kref, exact * half[None, :]) < 1e-8
+    assert np.allclose(kref, kref.T)
+
+
+@pytest.mark.parametrize("model", ["SaintVenantKirchhoff", "CompressibleNeoHookean"])
+def test_large_strain_homogeneous(model):
+    """Homogeneous RVE: the fluctuation stays zero, no iteration is needed and
+    P_bar = P(F_bar)."""
+    ms = np.zeros((6, 4, 8), dtype=np.uint16)
+    p = Problem(dims=ms.shape, howmany=3, ms=ms, strain_type="large", tol=1e-10, n_it=100,
+                groups=[MaterialGroup(model, [0], {"bulk_modulus": [62.5], "shear_modulus": [28.8462]})])
+    o = oracle.Oracle(p)
+    F = np.array([1.1, 0.05, 0.0, -0.02, 0.95, 0.03, 0.01, 0.0, 1.2])
+    o.set_gradient(F)
+    assert o.solve() == 0
+    P, _, _ = oracle.material_point(model, [62.5, 28.8462], F)
+    assert rel(o.homogenized_stress(), P) < 1e-13
+
+
+def test_large_strain_mixed_bc_golden(sphere32_ms):
+    """test_MixedBCs_LargeStrain.json (CompressibleNeoHookean on sphere32,
+    nonlinear CG, line search, mixed BCs starting from F = I): load case 2
+    prescribes P_bar = (0, ..., 74.757449712464) on every component, the
+    homogenized stress that the reference printed for load case 1's last step;
+    load case 3 of that file stores the deformation gradient the reference
+    reached, diag(0.832542244829, 0.832542244829, 2.000000000001)."""
+    with open(os.path.join(GOLDEN, "test_MixedBCs_LargeStrain.json")) as f:
+        j = json.load(f)
+    p = Problem.from_json(j, ms=sphere32_ms)
+    assert p.n_str == 9
+    lc = p.load_cases[1]
+    golden = p.load_cases[2].g0_path[0]
+    o = oracle.Oracle(p)
+    o.enable_mixed_bc(lc.mbc, 0)
+    o.solve()
+    assert np.max(np.abs(o.macro_gradient() - golden)) < 1e-10
+    assert np.max(np.abs(o.homogenized_stress() - np.array(lc.mbc["stress_path"][0]))) < 1e-9

@@ -56,15 +56,15 @@ def formula_sensitivity(p, fa, fb):
     return np.stack([sK, sG], axis=1)
 
 
-@pytest.mark.parametrize("h", [1, 3])
-def test_sensitivity_formula_matches_fd(h):
+@pytest.mark.parametrize("h,fe", [(1, "HEX8"), (3, "HEX8"), (3, "BBAR")])
+def test_sensitivity_formula_matches_fd(h, fe):
     rng = np.random.default_rng(7)
     dims = (4, 4, 6)
     K = rng.uniform(1.0, 5.0, dims)
     G = rng.uniform(0.5, 3.0, dims) if h == 3 else None
     g0 = np.array([1.0, 0.2, -0.1]) if h == 1 else np.array([1.0, 0.1, -0.2, 0.0, 0.3, 0.0])
     v = np.array([0.3, -1.0, 0.5]) if h == 1 else np.array([0.0, 0.0, 0.0, 0.0, 0.0, 1.0])
-    p = per_voxel_problem(K, G)
+    p = per_voxel_problem(K, G, fe=fe)
     fa = oracle_solution(p, g0).element_fields()
     fb = oracle_solution(p, v).element_fields()
     sens = formula_sensitivity(p, fa, fb) / np.prod(p.L)
@@ -79,7 +79,7 @@ def test_sensitivity_formula_matches_fd(h):
                     K2.reshape(-1)[e] += sgn * hstep
                 else:
                     G2.reshape(-1)[e] += sgn * hstep
-                vals.append(v @ oracle_solution(per_voxel_problem(K2, G2), g0).homogenized_stress())
+                vals.append(v @ oracle_solution(per_voxel_problem(K2, G2, fe=fe), g0).homogenized_stress())
             fd = (vals[0] - vals[1]) / (2 * hstep)
             assert abs(fd - sens[e, q]) <= 1e-6 * np.max(np.abs(sens[:, q])), (e, q, fd, sens[e, q])
     # and a plausible mistake (dC/dG without the -2/3 m m^T term) is caught
@@ -103,21 +103,21 @@ def smoothed_density(n, corr=8.0, seed=0):
 
 # ---------------------------------------------------------------------------- GPU
 @pytest.mark.gpu
-@pytest.mark.parametrize("h", [1, 3])
-def test_voxel_moduli_forward_and_sensitivity(h):
+@pytest.mark.parametrize("h,fe", [(1, "HEX8"), (3, "HEX8"), (3, "BBAR")])
+def test_voxel_moduli_forward_and_sensitivity(h, fe):
     from fans_amd import gpu
     from fans_amd.pyfans import voxel_problem
-    print(f"[gpu-case] voxel moduli h={h}", flush=True)
+    print(f"[gpu-case] voxel moduli h={h} {fe}", flush=True)
     rng = np.random.default_rng(3)
     dims = (8, 8, 8)
     K = rng.uniform(1.0, 5.0, dims)
     G = rng.uniform(0.5, 3.0, dims) if h == 3 else None
     g0 = np.array([1.0, 0.2, -0.1]) if h == 1 else np.array([1.0, 0.1, -0.2, 0.0, 0.3, 0.0])
     v = np.array([0.3, -1.0, 0.5]) if h == 1 else np.array([0.0, 0.0, 0.0, 0.2, 0.0, 1.0])
-    p = per_voxel_problem(K, G, tol=1e-10)
+    p = per_voxel_problem(K, G, fe=fe, tol=1e-10)
     o = oracle_solution(p, g0)
     ob = oracle_solution(p, v)
-    s = gpu.GpuSolver(voxel_problem(dims, howmany=h, tol=1e-10))
+    s = gpu.GpuSolver(voxel_problem(dims, howmany=h, fe_type=fe, tol=1e-10))
     s.set_voxel_moduli(K, G)
     assert np.max(np.abs(s.reference_stiffness() - o.reference_stiffness())) < 1e-14
     s.set_gradient(v)
