@@ -56,17 +56,18 @@ __device__ __forceinline__ T shfl_up16(T v)
 
 // VAR: ASM_GENERAL = per-Gauss-point constitutive path (any model);
 // ASM_ISO = every phase LinearElasticIsotropic (H=3) or LinearThermalIsotropic
-// (H=1), modal element arithmetic; ASM_ISO_CUBIC = the same on cubic voxels.
-enum { ASM_GENERAL = 0, ASM_ISO = 1, ASM_ISO_CUBIC = 2 };
+// (H=1), modal element arithmetic; ASM_ISO_CUBIC = the same on cubic voxels;
+// ASM_LARGE = the per-Gauss-point path with finite strain (F and P, n_str 9).
+enum { ASM_GENERAL = 0, ASM_ISO = 1, ASM_ISO_CUBIC = 2, ASM_LARGE = 3 };
 
 template <typename T, int H, int NGP, int MODE, int VAR>
 __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
 {
-    constexpr bool ISO = VAR != ASM_GENERAL, CUBIC = VAR == ASM_ISO_CUBIC;
+    constexpr bool ISO = VAR == ASM_ISO || VAR == ASM_ISO_CUBIC, CUBIC = VAR == ASM_ISO_CUBIC;
     if (a.done && *a.done)
         return;
     constexpr int NL = TILE + 1;  // node tile edge
-    constexpr int NS = H == 1 ? 3 : 6;
+    constexpr int NS = nstr_of<H, VAR == ASM_LARGE>();
     __shared__ T      Un[2][NL * NL * H];
     __shared__ T      Yb[4][TILE * 2 * H];  // last row of each wave for the y exchange
     __shared__ double red[8];
@@ -229,9 +230,9 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
         } else {
             const Phase<T> &P   = a.phases[ph];
             auto            mat = [&](int gp, const T(&e)[NS], T(&s)[NS]) {
-                material_point<T, H>(P, e, s, a.js, el, gp, NGP);
+                material_point_ns<T, H, NS>(P, e, s, a.js, el, gp, NGP);
             };
-            element_forces<T, H, NGP>(u, a.geom, g0, mat, f);
+            element_forces<T, H, NGP, NS>(u, a.geom, g0, mat, f);
         }
         // the loads of the plane after next fly during the exchange and the
         // stores (the element's registers are free again here)

@@ -22,13 +22,24 @@ enum {
     MODEL_ELASTIC_ISO    = 3,
     MODEL_ELASTIC_TRI    = 4,
     MODEL_J2_NEW         = 5,
-    MODEL_PSEUDO_PLASTIC = 6
+    MODEL_PSEUDO_PLASTIC = 6,
+    MODEL_SVK            = 7,  // SaintVenantKirchhoff (finite strain)
+    MODEL_NEO_HOOKE      = 8   // CompressibleNeoHookean (finite strain)
 };
+
+// Strain / stress vector length: thermal gradient (3), small strain Mandel (6)
+// or, with finite strain, F and P row major (9).
+template <int H, bool LARGE>
+constexpr int nstr_of()
+{
+    return H == 1 ? 3 : (LARGE ? 9 : 6);
+}
 
 // Per-phase material table.  p[] holds, by kind:
 //   THERMAL_ISO  k                      THERMAL_TRI  K (3x3 row major)
 //   ELASTIC_ISO  lambda mu              ELASTIC_TRI  C (6x6 row major)
 //   J2_NEW       K G sigma_y H_iso      PSEUDO       K G sigma_y h eps_crit E_s
+//   SVK          lambda mu              NEO_HOOKE    lambda mu
 template <typename T>
 struct Phase {
     int kind;
@@ -113,6 +124,59 @@ __device__ __forceinline__ void pseudo_point(const T *p, const T (&e)[6], T (&s)
         s[i] = buf2 * dev[i];
 }
 
+// First Piola-Kirchhoff stress P = F S(F) of the finite strain models
+// (LargeStrainMechModel.h:193-199); F and P row major.
+//   SaintVenantKirchhoff   S = lam tr(E) I + 2 mu E, E = (F^T F - I) / 2
+//                          (SaintVenantKirchhoff.h:34-39)
+//   CompressibleNeoHookean S = lam ln(J) C^-1 + mu (I - C^-1), C = F^T F
+//                          (CompressibleNeoHookean.h:35-49); J <= 0 gives NaN
+//                          (the reference throws there)
+template <typename T>
+__device__ __forceinline__ void large_point(int kind, const T *p, const T (&e)[9], T (&s)[9])
+{
+    const T lam = p[0], mu = p[1];
+    T       C[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            C[a][b] = e[a] * e[b] + e[3 + a] * e[3 + b] + e[6 + a] * e[6 + b];
+    T S[3][3];
+    if (kind == MODEL_SVK) {
+        const T htr = (T) 0.5 * (C[0][0] + C[1][1] + C[2][2] - (T) 3);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                S[a][b] = mu * (C[a][b] - (a == b ? (T) 1 : (T) 0)) + (a == b ? lam * htr : (T) 0);
+    } else {
+        // C^-1 from the cofactors of the symmetric C; det C = J^2
+        const T c00 = C[1][1] * C[2][2] - C[1][2] * C[1][2];
+        const T c01 = C[0][2] * C[1][2] - C[0][1] * C[2][2];
+        const T c02 = C[0][1] * C[1][2] - C[0][2] * C[1][1];
+        const T c11 = C[0][0] * C[2][2] - C[0][2] * C[0][2];
+        const T c12 = C[0][1] * C[0][2] - C[0][0] * C[1][2];
+        const T c22 = C[0][0] * C[1][1] - C[0][1] * C[0][1];
+        const T detC = C[0][0] * c00 + C[0][1] * c01 + C[0][2] * c02;
+        const T J    = e[0] * (e[4] * e[8] - e[5] * e[7]) - e[1] * (e[3] * e[8] - e[5] * e[6]) +
+                    e[2] * (e[3] * e[7] - e[4] * e[6]);
+        const T ic   = (T) 1 / detC;
+        const T Ci[3][3] = {{c00 * ic, c01 * ic, c02 * ic}, {c01 * ic, c11 * ic, c12 * ic},
+                            {c02 * ic, c12 * ic, c22 * ic}};
+        const T lj = J > (T) 0 ? log(J) : (T) NAN;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                S[a][b] = lam * lj * Ci[a][b] + mu * ((a == b ? (T) 1 : (T) 0) - Ci[a][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            s[3 * a + b] = e[3 * a] * S[0][b] + e[3 * a + 1] * S[1][b] + e[3 * a + 2] * S[2][b];
+}
+
 // State of J2 points, structure of arrays: comp c (0..5 plastic strain, 6 q)
 // of Gauss point g of element e at st[(c * ngp + g) * nel + e].
 struct J2State {
@@ -177,6 +241,17 @@ __device__ __forceinline__ void material_point(const Phase<T> &P, const T (&e)[H
     }
 }
 
+// material_point for any strain vector length (finite strain: NS = 9)
+template <typename T, int H, int NS>
+__device__ __forceinline__ void material_point_ns(const Phase<T> &P, const T (&e)[NS], T (&s)[NS], const J2State &js,
+                                                  int64_t el, int gp, int ngp)
+{
+    if constexpr (NS == 9)
+        large_point<T>(P.kind, P.p, e, s);
+    else
+        material_point<T, H>(P, e, s, js, el, gp, ngp);
+}
+
 // ---------------------------------------------------------------------------
 // element kernel core: u[8][H] (corner values) -> f[8][H] = B^T sigma(B u + g0) w
 // ---------------------------------------------------------------------------
@@ -186,11 +261,18 @@ __device__ __forceinline__ T lerp1(T a, T b, T t)
     return a + t * (b - a);
 }
 
-// strain (Mandel) from displacement gradient g[c][d] = d u_c / d x_d
-template <typename T, int H>
-__device__ __forceinline__ void grad_to_strain(const T (&g)[H][3], const T *g0, T (&e)[H == 1 ? 3 : 6])
+// strain (Mandel) from displacement gradient g[c][d] = d u_c / d x_d; with
+// finite strain (NS = 9) F = g + g0 row major (LargeStrainMechModel.h:205-222)
+template <typename T, int H, int NS = (H == 1 ? 3 : 6)>
+__device__ __forceinline__ void grad_to_strain(const T (&g)[H][3], const T *g0, T (&e)[NS])
 {
-    if constexpr (H == 1) {
+    if constexpr (NS == 9) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                e[3 * c + d] = g[c][d] + g0[3 * c + d];
+    } else if constexpr (H == 1) {
         e[0] = g[0][0] + g0[0];
         e[1] = g[0][1] + g0[1];
         e[2] = g[0][2] + g0[2];
@@ -206,10 +288,17 @@ __device__ __forceinline__ void grad_to_strain(const T (&g)[H][3], const T *g0, 
 }
 
 // traction-like tensor T[c][d] with f_a[c] = sum_d dN_a/dx_d T[c][d]
-template <typename T, int H>
-__device__ __forceinline__ void stress_to_T(const T (&s)[H == 1 ? 3 : 6], T (&Tt)[H][3])
+// (finite strain: T = P)
+template <typename T, int H, int NS = (H == 1 ? 3 : 6)>
+__device__ __forceinline__ void stress_to_T(const T (&s)[NS], T (&Tt)[H][3])
 {
-    if constexpr (H == 1) {
+    if constexpr (NS == 9) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                Tt[c][d] = s[3 * c + d];
+    } else if constexpr (H == 1) {
         Tt[0][0] = s[0];
         Tt[0][1] = s[1];
         Tt[0][2] = s[2];
@@ -323,36 +412,45 @@ __device__ __forceinline__ void grad_factors_T(const T (&Sx)[2][2][H], const T (
     }
 }
 
-// B-bar (include/matmodel.h:113-140): the volumetric rows of B at every Gauss
-// point are replaced by those at the element centre, which for the trilinear
-// element equal their mean over the 2x2x2 Gauss points.  Applied to the
-// gradient at Gauss point (i,j,k): the normal components shift by
-// (tr_c - tr_gp) / 3 with tr_c the Gauss mean of the trace.  The transpose
-// maps a stress sigma_gp to B^T (dev sigma_gp + m mean_gp(tr sigma) / 3).
-template <typename T>
-__device__ __forceinline__ T bbar_mean_trace(const T (&Gx)[2][2][3], const T (&Gy)[2][2][3], const T (&Gz)[2][2][3])
+// B-bar (include/matmodel.h:113-140): rows 0..2 of B at every Gauss point are
+// shifted by the difference between their centre sum and their own sum over
+// 3; the centre value of the trilinear element equals the mean over the 2x2x2
+// Gauss points.  Rows 0..2 are the normal strains (gradient entries (r, r));
+// with finite strain the reference applies the same rows, F_11 F_12 F_13
+// (entries (0, r)).  Applied to the gradient at Gauss point (i,j,k) those
+// entries shift by (t_c - t_gp) / 3 with t the row sum; the transpose maps a
+// stress s_gp to B^T (s_gp + (mean_gp(t(s)) - t(s_gp)) / 3 on rows 0..2).
+template <int NS>
+__device__ constexpr int bbar_comp(int r)
 {
-    return (T) 0.25 * (((Gx[0][0][0] + Gx[1][1][0]) + (Gx[0][1][0] + Gx[1][0][0])) +
-                       ((Gy[0][0][1] + Gy[1][1][1]) + (Gy[0][1][1] + Gy[1][0][1])) +
-                       ((Gz[0][0][2] + Gz[1][1][2]) + (Gz[0][1][2] + Gz[1][0][2])));
+    return NS == 9 ? 0 : r;
 }
 
-template <typename T>
+template <typename T, int NS>
+__device__ __forceinline__ T bbar_mean_trace(const T (&Gx)[2][2][3], const T (&Gy)[2][2][3], const T (&Gz)[2][2][3])
+{
+    constexpr int c0 = bbar_comp<NS>(0), c1 = bbar_comp<NS>(1), c2 = bbar_comp<NS>(2);
+    return (T) 0.25 * (((Gx[0][0][c0] + Gx[1][1][c0]) + (Gx[0][1][c0] + Gx[1][0][c0])) +
+                       ((Gy[0][0][c1] + Gy[1][1][c1]) + (Gy[0][1][c1] + Gy[1][0][c1])) +
+                       ((Gz[0][0][c2] + Gz[1][1][c2]) + (Gz[0][1][c2] + Gz[1][0][c2])));
+}
+
+template <typename T, int NS>
 __device__ __forceinline__ void bbar_project_grad(T (&gr)[3][3], T trc)
 {
-    const T d = (trc - (gr[0][0] + gr[1][1] + gr[2][2])) * (T) (1.0 / 3.0);
-    gr[0][0] += d;
-    gr[1][1] += d;
-    gr[2][2] += d;
+    constexpr int c0 = bbar_comp<NS>(0), c1 = bbar_comp<NS>(1), c2 = bbar_comp<NS>(2);
+    const T       d  = (trc - (gr[c0][0] + gr[c1][1] + gr[c2][2])) * (T) (1.0 / 3.0);
+    gr[c0][0] += d;
+    gr[c1][1] += d;
+    gr[c2][2] += d;
 }
 
 // Full element: f = sum_gp B^T sigma(B u + g0) * w.  MatF(gp, e, s) evaluates
 // the constitutive law at Gauss point gp (index i + 2j + 4k).
-template <typename T, int H, int NGP, typename MatF>
+template <typename T, int H, int NGP, int NS = (H == 1 ? 3 : 6), typename MatF>
 __device__ __forceinline__ void element_forces(const T (&u)[8][H], const Geom &g, const T *g0, MatF &&mat,
                                                T (&f)[8][H])
 {
-    constexpr int NS = H == 1 ? 3 : 6;
     T             Gx[2][2][H], Gy[2][2][H], Gz[2][2][H];
     grad_factors<T, H, NGP>(u, g, Gx, Gy, Gz);
     T Sx[2][2][H], Sy[2][2][H], Sz[2][2][H];
@@ -368,7 +466,7 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][H], const Geom &g
     T              trc = (T) 0, trs = (T) 0;
     if constexpr (CAN_BBAR)
         if (bbar)
-            trc = bbar_mean_trace<T>(Gx, Gy, Gz);
+            trc = bbar_mean_trace<T, NS>(Gx, Gy, Gz);
 #pragma unroll
     for (int gp = 0; gp < NGP; ++gp) {
         const int i = gp & 1, j = (gp >> 1) & 1, k = gp >> 2;
@@ -381,9 +479,9 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][H], const Geom &g
         }
         if constexpr (CAN_BBAR)
             if (bbar)
-                bbar_project_grad<T>(gr, trc);
+                bbar_project_grad<T, NS>(gr, trc);
         T e[NS], s[NS], Tt[H][3];
-        grad_to_strain<T, H>(gr, g0, e);
+        grad_to_strain<T, H, NS>(gr, g0, e);
         mat(gp, e, s);
         if constexpr (CAN_BBAR)
             if (bbar) {
@@ -394,7 +492,7 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][H], const Geom &g
                 s[1] -= t3;
                 s[2] -= t3;
             }
-        stress_to_T<T, H>(s, Tt);
+        stress_to_T<T, H, NS>(s, Tt);
 #pragma unroll
         for (int c = 0; c < H; ++c) {
             Sx[j][k][c] += Tt[c][0];
@@ -406,14 +504,15 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][H], const Geom &g
         if (bbar) {
             // m mean(tr sigma) / 3 at each of the 8 points: every factor entry
             // collects two points
-            const T v = trs * (T) (2.0 / 24.0);
+            const T       v  = trs * (T) (2.0 / 24.0);
+            constexpr int c0 = bbar_comp<NS>(0), c1 = bbar_comp<NS>(1), c2 = bbar_comp<NS>(2);
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    Sx[a][b][0] += v;
-                    Sy[a][b][1] += v;
-                    Sz[a][b][2] += v;
+                    Sx[a][b][c0] += v;
+                    Sy[a][b][c1] += v;
+                    Sz[a][b][c2] += v;
                 }
         }
     const T w = (T) g.w;
@@ -805,10 +904,9 @@ __device__ __forceinline__ void voxel_iso_stress(T K, T G, const T (&e)[H == 1 ?
     }
 }
 
-template <typename T, int H, int NGP, int MODE>
+template <typename T, int H, int NGP, int MODE, int NS = (H == 1 ? 3 : 6)>
 __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
 {
-    constexpr int NS = H == 1 ? 3 : 6;
     __shared__ double red[NS * 4];
     const int64_t     nel = (int64_t) a.nx * a.ny * a.nz;
     const int64_t     el  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -834,7 +932,7 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
         const bool     bbar     = CAN_BBAR && a.geom.bbar;
         T              trc      = (T) 0;
         if constexpr (CAN_BBAR)
-            trc = bbar_mean_trace<T>(Gx, Gy, Gz);
+            trc = bbar_mean_trace<T, NS>(Gx, Gy, Gz);
         T g0[NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k)
@@ -855,7 +953,7 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
             grad_factors<T, H, NGP>(ub, a.geom, Hx, Hy, Hz);
             T trcb = (T) 0;
             if constexpr (CAN_BBAR)
-                trcb = bbar_mean_trace<T>(Hx, Hy, Hz);
+                trcb = bbar_mean_trace<T, NS>(Hx, Hy, Hz);
             T g0b[NS];
 #pragma unroll
             for (int k = 0; k < NS; ++k)
@@ -872,8 +970,8 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
                 }
                 if constexpr (CAN_BBAR)
                     if (bbar) {
-                        bbar_project_grad<T>(ga, trc);
-                        bbar_project_grad<T>(gb, trcb);
+                        bbar_project_grad<T, NS>(ga, trc);
+                        bbar_project_grad<T, NS>(gb, trcb);
                     }
                 T ea[NS], eb[NS];
                 grad_to_strain<T, H>(ga, g0, ea);
@@ -915,9 +1013,9 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
             }
             if constexpr (CAN_BBAR)
                 if (bbar)
-                    bbar_project_grad<T>(gr, trc);
+                    bbar_project_grad<T, NS>(gr, trc);
             T e[NS], s[NS];
-            grad_to_strain<T, H>(gr, g0, e);
+            grad_to_strain<T, H, NS>(gr, g0, e);
             if constexpr (MODE == ELEM_COMMIT) {
                 if (P.kind == MODEL_J2_NEW) {
                     double ed[6], epin[6], sd[6], epo[6], qo, pd[4];
@@ -937,10 +1035,12 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
                     a.js.st[((int64_t) 6 * NGP + gp) * a.js.nel + el] = qo;
                 }
             } else {
-                if (a.vk)
+                if constexpr (NS == 9)
+                    material_point_ns<T, H, NS>(P, e, s, a.js, el, gp, NGP);
+                else if (a.vk)
                     voxel_iso_stress<T, H>(vK, vG, e, s);
                 else
-                    material_point<T, H>(P, e, s, a.js, el, gp, NGP);
+                    material_point_ns<T, H, NS>(P, e, s, a.js, el, gp, NGP);
 #pragma unroll
                 for (int c = 0; c < NS; ++c) {
                     eavg[c] += (double) e[c] / NGP;

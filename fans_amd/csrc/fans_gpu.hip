@@ -62,10 +62,13 @@ int nparams(int kind)
     case FANS_MODEL_ELASTIC_TRI: return 21;
     case FANS_MODEL_J2_NEW: return 4;
     case FANS_MODEL_PSEUDO_PLASTIC: return 4;
+    case FANS_MODEL_SVK: return 2;
+    case FANS_MODEL_NEO_HOOKE: return 2;
     default: return -1;
     }
 }
 bool model_linear(int kind) { return kind >= 1 && kind <= 4; }
+bool model_large(int kind) { return kind == FANS_MODEL_SVK || kind == FANS_MODEL_NEO_HOOKE; }
 
 // ---- host twiddles (long double, then rounded) ------------------------------
 template <typename T>
@@ -212,7 +215,7 @@ struct fans_gpu_ctx {
     std::vector<int>            phase_kind;
     bool                        all_linear = true, has_j2 = false, all_iso = false;
     bool                        cubic = false;  // hx == hy == hz
-    double                      kref[36]{};
+    double                      kref[81]{};
     double                      keref[576]{};
     std::vector<Phase<double>>  ph64;
     std::vector<Phase<float>>   ph32;
@@ -662,7 +665,19 @@ int launch_assemble(fans_gpu_ctx *c, AsmArgs<T> a)
             hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_GENERAL>), dim3(c->nblk_asm), dim3(kThreads),   \
                                0, c->stream, a);                                                                \
     } while (0)
-    if (c->h == 1) {
+    if (c->nstr == 9) {
+        // finite strain: only the residual (the models are nonlinear)
+        if constexpr (MODE == ASM_RESID) {
+            if (c->ngp == 8)
+                hipLaunchKernelGGL((k_assemble<T, 3, 8, MODE, ASM_LARGE>), dim3(c->nblk_asm), dim3(kThreads), 0,
+                                   c->stream, a);
+            else
+                hipLaunchKernelGGL((k_assemble<T, 3, 1, MODE, ASM_LARGE>), dim3(c->nblk_asm), dim3(kThreads), 0,
+                                   c->stream, a);
+        } else {
+            return fail(c, FANS_GPU_EINVAL, "linear matvec needs linear materials");
+        }
+    } else if (c->h == 1) {
         if (c->ngp == 8)
             ASM(1, 8);
         else
@@ -762,7 +777,14 @@ int homog_stress(fans_gpu_ctx *c, double *out)
 #define EP(HH, GG)                                                                                              \
     hipLaunchKernelGGL((k_element_pass<double, HH, GG, ELEM_HOMOG>), dim3(c->nblk_el), dim3(kThreads), 0, \
                        c->stream, a)
-        if (c->h == 1) {
+        if (c->nstr == 9) {
+            if (c->ngp == 8)
+                hipLaunchKernelGGL((k_element_pass<double, 3, 8, ELEM_HOMOG, 9>), dim3(c->nblk_el), dim3(kThreads),
+                                   0, c->stream, a);
+            else
+                hipLaunchKernelGGL((k_element_pass<double, 3, 1, ELEM_HOMOG, 9>), dim3(c->nblk_el), dim3(kThreads),
+                                   0, c->stream, a);
+        } else if (c->h == 1) {
             if (c->ngp == 8)
                 EP(1, 8);
             else
@@ -843,6 +865,40 @@ int mixed_update(fans_gpu_ctx *c)
 }
 
 // symmetric pseudo-inverse of a small SPD block (Jacobi, host)
+// inverse of a nonsingular, non-symmetric n x n block (n <= 9) by Gauss-Jordan
+// elimination with partial pivoting: what the complete orthogonal
+// decomposition pinv of mixedBCs.h:43 gives for such a block; nonzero if singular
+int host_gauss_inverse(int n, const double *A, double *Ai)
+{
+    double a[9][18];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < 2 * n; ++j)
+            a[i][j] = j < n ? A[i * n + j] : (double) (j - n == i);
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+            if (std::fabs(a[r][col]) > std::fabs(a[piv][col]))
+                piv = r;
+        if (a[piv][col] == 0.0)
+            return -1;
+        for (int j = 0; j < 2 * n; ++j)
+            std::swap(a[col][j], a[piv][j]);
+        const double d = a[col][col];
+        for (int j = 0; j < 2 * n; ++j)
+            a[col][j] /= d;
+        for (int r = 0; r < n; ++r)
+            if (r != col) {
+                const double f = a[r][col];
+                for (int j = 0; j < 2 * n; ++j)
+                    a[r][j] -= f * a[col][j];
+            }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            Ai[i * n + j] = a[i][n + j];
+    return 0;
+}
+
 void host_sym_pinv(int n, const double *A, double *P)
 {
     double a[81], V[81];
@@ -1276,7 +1332,7 @@ int alloc_fields(fans_gpu_ctx *c)
     // partial sums: the largest grid of any reducing kernel times its quantities
     int64_t need = 2 * std::max<int64_t>({(int64_t) c->nblk_asm, (int64_t) c->nblk_vec, c->fft64.nblk_z,
                                           c->fft32.nblk_z});
-    need         = std::max<int64_t>(need, (int64_t) 6 * c->nblk_el);
+    need         = std::max<int64_t>(need, (int64_t) c->nstr * c->nblk_el);
     c->partial_len = need;
     HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * need));
     return 0;
@@ -1364,6 +1420,13 @@ void host_B(const fans_gpu_ctx *c, double x, double y, double z, double *out)
     b[2][7]        = x * y / lz;
     const int ncol = 8 * c->h;
     std::memset(out, 0, sizeof(double) * c->nstr * ncol);
+    if (c->nstr == 9) {  // F_iJ rows (LargeStrainMechModel.h:205-222)
+        for (int i = 0; i < 3; ++i)
+            for (int J = 0; J < 3; ++J)
+                for (int q = 0; q < 8; ++q)
+                    out[(3 * i + J) * ncol + 3 * q + i] = b[J][q];
+        return;
+    }
     if (c->h == 1) {
         for (int i = 0; i < 3; ++i)
             for (int q = 0; q < 8; ++q)
@@ -1392,10 +1455,10 @@ void ref_element_stiffness(fans_gpu_ctx *c)
                              {xm, xm, xp}, {xp, xm, xp}, {xm, xp, xp}, {xp, xp, xp}};
     const double ve       = (c->d.L[0] / c->nx) * (c->d.L[1] / c->ny) * (c->d.L[2] / c->nz);
     double       tmp[576] = {0};
-    double       Bc[6 * 24];
+    double       Bc[9 * 24];
     host_B(c, 0.5, 0.5, 0.5, Bc);
     for (int p = 0; p < c->ngp; ++p) {
-        double B[6 * 24];
+        double B[9 * 24];
         if (c->ngp == 1)
             host_B(c, 0.5, 0.5, 0.5, B);
         else
@@ -1406,7 +1469,7 @@ void ref_element_stiffness(fans_gpu_ctx *c)
                 for (int i = 0; i < 3; ++i)
                     B[i * ne + j] += dv;
             }
-        double CB[6 * 24];
+        double CB[9 * 24];
         for (int i = 0; i < ns; ++i)
             for (int j = 0; j < ne; ++j) {
                 double a = 0;
@@ -1467,6 +1530,8 @@ int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *d)
         return FANS_GPU_EINVAL;
     if (d->n_it < 0 || d->device < 0 || d->device >= nd)
         return FANS_GPU_EINVAL;
+    if ((d->large_strain != 0 && d->large_strain != 1) || (d->large_strain && d->howmany != 3))
+        return FANS_GPU_EINVAL;
     fans_gpu_ctx *c = new fans_gpu_ctx();
     c->d            = *d;
     if (hipSetDevice(d->device) != hipSuccess) {
@@ -1477,7 +1542,7 @@ int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *d)
     c->ny       = d->n_y;
     c->nz       = d->n_z;
     c->h        = d->howmany;
-    c->nstr     = d->howmany == 1 ? 3 : 6;
+    c->nstr     = d->howmany == 1 ? 3 : (d->large_strain ? 9 : 6);
     c->ngp      = d->fe_type == FANS_FE_HEX8R ? 1 : 8;
     c->N        = (int64_t) c->nx * c->ny * c->nz;
     c->Nh       = c->N * c->h;
@@ -1593,6 +1658,10 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
         const bool thermal = kind[g] <= 2;
         if (thermal != (c->h == 1))
             return fail(c, FANS_GPU_EINVAL, "material model kind %d does not match howmany=%d", kind[g], c->h);
+        // createMatmodel<3,6> / <3,9> (setup.h:21-73)
+        if (c->h == 3 && model_large(kind[g]) != (c->nstr == 9))
+            return fail(c, FANS_GPU_EINVAL, "material model kind %d is not a valid %s strain material model", kind[g],
+                        c->nstr == 9 ? "large" : "small");
         for (int i = 0; i < nphase[g]; ++i)
             maxp = std::max(maxp, phase_ids[tot + i]);
         tot += nphase[g];
@@ -1643,10 +1712,29 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
         ofs  = 0;
         pofs = 0;
         for (int g = 0; g < ng; ++g) {
-            double       C[36] = {0};
+            double       C[81] = {0};
             const int    npg   = nphase[g], npk = nparams(kind[g]);
             const double *gp   = params + pofs;
-            if (kind[g] == FANS_MODEL_ELASTIC_ISO) {
+            if (model_large(kind[g])) {
+                // get_reference_stiffness of both finite strain models: dP/dF at
+                // F = I averaged over the phases (SaintVenantKirchhoff.h:49-66,
+                // CompressibleNeoHookean.h:89-104).  There S = 0 and dS/dE is the
+                // isotropic lam m m^T + 2 mu I, so the tensor is
+                // lam d_iJ d_kL + mu (d_ik d_JL + d_iL d_Jk); the reference's sum over
+                // the symmetric pair (P, Q) with P <= Q only
+                // (LargeStrainMechModel.h:142-170) halves the off-diagonal F_kL columns.
+                for (int q = 0; q < npg; ++q) {
+                    const double mu = gp[q * npk + 1], lam = gp[q * npk] - (2.0 / 3.0) * mu;
+                    for (int i = 0; i < 3; ++i)
+                        for (int J = 0; J < 3; ++J)
+                            for (int k = 0; k < 3; ++k)
+                                for (int L = 0; L < 3; ++L) {
+                                    const double t = lam * (i == J) * (k == L) +
+                                                     mu * ((i == k) * (J == L) + (i == L) * (J == k));
+                                    C[(3 * i + J) * 9 + 3 * k + L] += t * (k == L ? 1.0 : 0.5) / npg;
+                                }
+                }
+            } else if (kind[g] == FANS_MODEL_ELASTIC_ISO) {
                 double lmax = -INFINITY, lmin = INFINITY, mmax = -INFINITY, mmin = INFINITY;
                 for (int i = 0; i < npg; ++i) {
                     const double lam = gp[i * npk] - (2.0 / 3.0) * gp[i * npk + 1];
@@ -1674,7 +1762,7 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
                 for (int i = 0; i < 6; ++i)
                     C[i * 6 + i] += 2.0 * Gb;
             } else {
-                double Ci[36];
+                double Ci[81];
                 for (int i = 0; i < npg; ++i) {
                     linear_C(ns, kind[g], gp + i * npk, Ci);
                     for (int k = 0; k < ns * ns; ++k)
@@ -1715,6 +1803,11 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
                 P.p[i] = q[i];
             P.p[4] = std::sqrt(2. / 3.) * q[2] / (2. * q[1]);
             P.p[5] = (3. * q[1]) / (3. * q[1] + q[3]);
+            break;
+        case FANS_MODEL_SVK:
+        case FANS_MODEL_NEO_HOOKE:  // lambda = K - 2/3 G, mu = G (SaintVenantKirchhoff.h:26-31)
+            P.p[0] = q[0] - (2.0 / 3.0) * q[1];
+            P.p[1] = q[1];
             break;
         }
         c->ph32[p].kind = P.kind;
@@ -1806,14 +1899,20 @@ int fans_gpu_enable_mixed_bc(fans_gpu_ctx *c, int32_t nE, const int32_t *iE, int
     c->step   = step;
     if (nF) {  // M = pinv(Q_F^T C0 Q_F) (mixedBCs.h:252-253)
         std::vector<double> A(nF * nF);
+        bool                sym = true;
         for (int i = 0; i < nF; ++i)
-            for (int j = 0; j < nF; ++j)
+            for (int j = 0; j < nF; ++j) {
                 A[i * nF + j] = c->kref[iF[i] * n + iF[j]];
+                sym           = sym && c->kref[iF[i] * n + iF[j]] == c->kref[iF[j] * n + iF[i]];
+            }
         c->M.assign(nF * nF, 0.0);
-        host_sym_pinv(nF, A.data(), c->M.data());
+        if (sym || host_gauss_inverse(nF, A.data(), c->M.data()))
+            host_sym_pinv(nF, A.data(), c->M.data());
     }
     if (step == 0) {
         std::memset(c->gvec, 0, sizeof(c->gvec));
+        if (n == 9)  // finite strain starts from F = I (mixedBCs.h:191-197)
+            c->gvec[0] = c->gvec[4] = c->gvec[8] = 1.0;
         std::memcpy(c->gvec_prev, c->gvec, sizeof(c->gvec));
     } else {
         double delta[9];
@@ -2015,7 +2114,14 @@ int fans_gpu_postprocess(fans_gpu_ctx *c, double *strain_elem, double *stress_el
 #define EP(HH, GG)                                                                                               \
     hipLaunchKernelGGL((k_element_pass<double, HH, GG, ELEM_FIELDS>), dim3(c->nblk_el), dim3(kThreads), 0, \
                        c->stream, a)
-    if (c->h == 1) {
+    if (c->nstr == 9) {
+        if (ng == 8)
+            hipLaunchKernelGGL((k_element_pass<double, 3, 8, ELEM_FIELDS, 9>), dim3(c->nblk_el), dim3(kThreads), 0,
+                               c->stream, a);
+        else
+            hipLaunchKernelGGL((k_element_pass<double, 3, 1, ELEM_FIELDS, 9>), dim3(c->nblk_el), dim3(kThreads), 0,
+                               c->stream, a);
+    } else if (c->h == 1) {
         if (ng == 8)
             EP(1, 8);
         else

@@ -41,7 +41,8 @@ class Desc(C.Structure):
         ("n_x", C.c_int32), ("n_y", C.c_int32), ("n_z", C.c_int32), ("howmany", C.c_int32),
         ("fe_type", C.c_int32), ("method", C.c_int32), ("measure", C.c_int32), ("err_type", C.c_int32),
         ("n_it", C.c_int32), ("ls_max_iter", C.c_int32), ("L", C.c_double * 3), ("tol", C.c_double),
-        ("ls_tol", C.c_double), ("precision", C.c_int32), ("device", C.c_int32), ("reserved", C.c_int32 * 6),
+        ("ls_tol", C.c_double), ("precision", C.c_int32), ("device", C.c_int32),
+        ("large_strain", C.c_int32), ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -141,13 +142,14 @@ class GpuSolver:
         self.p = problem
         self.L = lib()
         self.h = problem.howmany
-        self.nstr = 3 if self.h == 1 else 6
+        self.nstr = problem.n_str
         self.dims = tuple(problem.dims)
         self.N = int(np.prod(self.dims))
         self.ngp = 1 if problem.fe_type == "HEX8R" else 8
         d = Desc()
         d.n_x, d.n_y, d.n_z = self.dims
         d.howmany = self.h
+        d.large_strain = int(self.nstr == 9)
         d.fe_type = FE[problem.fe_type]
         d.method = METHOD[problem.method]
         d.measure = MEASURE[problem.measure]

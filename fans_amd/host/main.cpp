@@ -72,6 +72,8 @@ int main(int argc, char *argv[])
         reader.OpenResultsFile(argv[2]);
         if (reader.howmany == 1)
             runSolver<1, 3>(reader, precision);
+        else if (reader.n_str == 9)
+            runSolver<3, 9>(reader, precision);
         else
             runSolver<3, 6>(reader, precision);
         reader.CloseResultsFile();

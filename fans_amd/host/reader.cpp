@@ -39,12 +39,16 @@ const std::map<std::string, std::vector<std::string>> &model_keys()
         k["J2PlasticityNew_LinearIsotropicHardening"] = {"bulk_modulus", "shear_modulus", "yield_stress",
                                                          "isotropic_hardening_parameter"};
         k["PseudoPlasticLinearHardening"] = {"bulk_modulus", "shear_modulus", "yield_stress", "hardening_parameter"};
+        k["SaintVenantKirchhoff"]         = {"bulk_modulus", "shear_modulus"};
+        k["CompressibleNeoHookean"]       = {"bulk_modulus", "shear_modulus"};
         return k;
     }();
     return m;
 }
 
 bool thermal_model(const std::string &m) { return m == "LinearThermalIsotropic" || m == "LinearThermalTriclinic"; }
+// createMatmodel<3,9> (include/setup.h:62-73)
+bool large_strain_model(const std::string &m) { return m == "SaintVenantKirchhoff" || m == "CompressibleNeoHookean"; }
 
 // MixedBC::from_json (include/mixedBCs.h:51-133)
 MixedBC mixed_from_json(const json &jc, int n_str)
@@ -158,8 +162,8 @@ void Reader::ReadInputFile(const char *input_fn)
         howmany = 1, n_str = 3;
     } else if (problemType == "mechanical" && strain_type == "small") {
         howmany = 3, n_str = 6;
-    } else if (problemType == "mechanical") {
-        throw std::invalid_argument("large strain (n_str = 9) models are not implemented on the GPU");
+    } else if (problemType == "mechanical" && strain_type == "large") {
+        howmany = 3, n_str = 9;  // F_11 F_12 ... F_33 (reader.cpp:141-142)
     } else {
         throw std::invalid_argument(problemType + " is not a valid problem type");
     }
@@ -228,6 +232,9 @@ void Reader::ReadInputFile(const char *input_fn)
             throw std::invalid_argument(g.matmodel + " is not a supported matmodel");
         if (thermal_model(g.matmodel) != (howmany == 1))
             throw std::invalid_argument(g.matmodel + " is not a valid matmodel for this problem type");
+        if (howmany == 3 && large_strain_model(g.matmodel) != (n_str == 9))
+            throw std::invalid_argument(g.matmodel + (n_str == 9 ? " is not a valid large strain material model"
+                                                                 : " is not a valid small strain material model"));
         g.keys                 = kit->second;
         const json &props      = mg["material_properties"];
         const size_t np        = g.phases.size();

@@ -42,6 +42,10 @@ inline int model_kind(const std::string &m)
         return FANS_MODEL_J2_NEW;
     if (m == "PseudoPlasticLinearHardening")
         return FANS_MODEL_PSEUDO_PLASTIC;
+    if (m == "SaintVenantKirchhoff")
+        return FANS_MODEL_SVK;
+    if (m == "CompressibleNeoHookean")
+        return FANS_MODEL_NEO_HOOKE;
     throw std::invalid_argument(m + " is not a supported matmodel");
 }
 
@@ -105,7 +109,8 @@ class Solver {
     {
         fans_gpu_desc d{};
         d.n_x = reader.dims[0], d.n_y = reader.dims[1], d.n_z = reader.dims[2];
-        d.howmany     = howmany;
+        d.howmany      = howmany;
+        d.large_strain = n_str == 9;
         d.fe_type     = reader.FE_type == "HEX8R" ? FANS_FE_HEX8R : (reader.FE_type == "BBAR" ? FANS_FE_BBAR
                                                                                                 : FANS_FE_HEX8);
         d.method      = reader.method == "fp" ? FANS_METHOD_FP : FANS_METHOD_CG;
@@ -228,7 +233,12 @@ void Solver<howmany, n_str>::postprocess(Reader &reader, int load_idx, int time_
             const double y = iy * dy - Ly2;
             for (int iz = 0; iz < reader.dims[2]; ++iz, ++n) {
                 const double z = iz * dz - Lz2;
-                if (howmany == 3) {
+                if (howmany == 3 && n_str == 9) {
+                    // u = (F - I) X with F = strain_average (solver.h:623-638)
+                    for (int i = 0; i < 3; ++i)
+                        ut[3 * n + i] = u[3 * n + i] + (ea[3 * i] - (i == 0)) * x + (ea[3 * i + 1] - (i == 1)) * y +
+                                        (ea[3 * i + 2] - (i == 2)) * z;
+                } else if (howmany == 3) {
                     const double g11 = ea[0], g22 = ea[1], g33 = ea[2];
                     const double g12 = ea[3] * rs2, g13 = ea[4] * rs2, g23 = ea[5] * rs2;
                     ut[3 * n]     = u[3 * n] + g11 * x + g12 * y + g13 * z;

@@ -16,7 +16,8 @@
  * Field layouts are the reference's in-memory layouts without ghost planes or
  * FFT padding: nodal fields [x][y][z][howmany], microstructure [x][y][z]
  * (Reader::ms after ReadMS, reader.cpp:380-399), element fields [x][y][z][n_str],
- * Gauss point fields [x][y][z][n_gp][n_str].
+ * Gauss point fields [x][y][z][n_gp][n_str].  n_str is 3 (thermal), 6 (small
+ * strain, Mandel) or 9 (finite strain: F_11 F_12 ... F_33 and P likewise).
  */
 #ifndef FANS_GPU_H
 #define FANS_GPU_H
@@ -60,8 +61,12 @@ enum {
     FANS_MODEL_J2_NEW         = 5, /* J2PlasticityNew_LinearIsotropicHardening
                                       [bulk_modulus shear_modulus yield_stress
                                        isotropic_hardening_parameter]                     */
-    FANS_MODEL_PSEUDO_PLASTIC = 6  /* PseudoPlasticLinearHardening
+    FANS_MODEL_PSEUDO_PLASTIC = 6, /* PseudoPlasticLinearHardening
                                       [bulk_modulus shear_modulus yield_stress hardening_parameter] */
+    FANS_MODEL_SVK            = 7, /* SaintVenantKirchhoff [bulk_modulus shear_modulus]
+                                      (finite strain, setup.h:62-73)                       */
+    FANS_MODEL_NEO_HOOKE      = 8  /* CompressibleNeoHookean [bulk_modulus shear_modulus]
+                                      (finite strain)                                      */
 };
 /* fields for fans_gpu_get_field / fans_gpu_set_field */
 enum {
@@ -86,7 +91,9 @@ typedef struct {
     double  ls_tol;          /* linesearch_parameters.tol (default 1e-2)             */
     int32_t precision;       /* FANS_PREC_*                                          */
     int32_t device;          /* HIP device ordinal                                   */
-    int32_t reserved[6];     /* must be zero                                         */
+    int32_t large_strain;    /* 1: strain_type "large" (main.cpp:71-72): n_str = 9,
+                                gradient and stress are F and P, row major           */
+    int32_t reserved[5];     /* must be zero                                         */
 } fans_gpu_desc;
 
 typedef struct {

@@ -147,3 +147,41 @@ def test_cli_end_to_end(case_dir, sphere32_ms, name):
     Co = o.homogenized_tangent()
     assert C.shape == (n_str, n_str)
     assert np.max(np.abs(C - Co)) <= 1e-6 * np.max(np.abs(Co))
+
+
+@pytest.mark.gpu
+def test_cli_large_strain(case_dir, sphere32_ms):
+    """test_CompressibleNeoHookean.json (finite strain, n_str = 9) cut to its
+    first two shear steps: F_bar and P_bar per step against the oracle, the
+    total displacement u + (F - I) X of solver.h:623-638 and the 9-component
+    slabs."""
+    _need_cli()
+    j0 = json.load(open(os.path.join(GOLDEN, "test_CompressibleNeoHookean.json")))
+    steps = j0["macroscale_loading"][0][:2]
+    path, j = case_dir("test_CompressibleNeoHookean", macroscale_loading=[steps])
+    out = path + ".h5"
+    print("[gpu-case] cli large strain", flush=True)
+    r = _run(path, out)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    p = Problem.from_json(path, ms=sphere32_ms)
+    o = oracle.Oracle(p)
+    for t, F in enumerate(steps):
+        o.set_gradient(F)
+        o.solve()
+        P_avg, _ = _res(out, j, "stress_average", step=t)
+        F_avg, _ = _res(out, j, "strain_average", step=t)
+        assert P_avg.shape == (9,) and np.allclose(F_avg, F, atol=1e-12)
+        Po = o.homogenized_stress()
+        assert np.max(np.abs(P_avg - Po)) <= 1e-9 * np.max(np.abs(Po)), t
+        if t == 0:
+            uf, perm = _res(out, j, "displacement_fluctuation", step=t)
+            ut, _ = _res(out, j, "displacement", step=t)
+            x = (np.arange(32) / 32.0 - 0.5)
+            Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+            Fm = np.asarray(F).reshape(3, 3) - np.eye(3)
+            for c in range(3):
+                lin = Fm[c, 0] * X + Fm[c, 1] * Y + Fm[c, 2] * Z
+                assert np.max(np.abs(ut[..., c] - uf[..., c] - lin)) < 1e-12
+            s, perm = _res(out, j, "stress", step=t)
+            assert perm == "zyx" and s.shape == (32, 32, 32, 9)
+        o.extrapolate_displacement()

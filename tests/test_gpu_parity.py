@@ -324,3 +324,86 @@ def test_graph_replay_matches_eager(precision):
         assert ie == ig
         assert np.array_equal(se, sg)
         assert np.array_equal(ue, ug)
+
+
+# ----------------------------------------------------------------------------- finite strain
+def _large(model, dims=(12, 10, 8), fe="HEX8", tol=1e-10, n_it=300, seed=0):
+    from fans_amd.microstructure import random_spheres
+    from fans_amd.problem import MaterialGroup, Problem
+    ms = random_spheres(dims, 3, 2.0, 3.0, seed=seed)
+    kg = {"bulk_modulus": [62.5, 222.222], "shear_modulus": [28.8462, 166.6667]}
+    return Problem(dims=dims, howmany=3, ms=ms, groups=[MaterialGroup(model, [0, 1], kg)], strain_type="large",
+                   fe_type=fe, tol=tol, n_it=n_it)
+
+
+@pytest.mark.parametrize("model,fe,dims", [(m, fe, d) for m in ("SaintVenantKirchhoff", "CompressibleNeoHookean")
+                                          for fe in ("HEX8", "HEX8R", "BBAR") for d in ((12, 10, 8), (40, 24, 16))])
+def test_large_strain_operators(model, fe, dims):
+    """Finite strain (n_str = 9, F and P row major): the dP/dF reference
+    stiffness, the Green operator built from it, and the residual at a finite
+    macro deformation gradient."""
+    p = _large(model, dims=dims, fe=fe)
+    say(f"large strain operators {model} {fe} dims={p.dims}")
+    o = oracle.Oracle(p)
+    g = gpu.GpuSolver(p)
+    assert relerr(g.reference_stiffness(), o.reference_stiffness()) < 1e-15
+    tol_g = 1e-10 if fe == "HEX8R" else 1e-12
+    assert relerr(g.green(), o.green()) < tol_g
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(p.dims + (3,)) * 0.002
+    F = np.eye(3).reshape(-1) + rng.standard_normal(9) * 0.1
+    o.set_gradient(F)
+    g.set_gradient(F)
+    assert relerr(g.residual(u), o.residual(u)) < 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_large_strain_solve(precision):
+    """Nonlinear CG with the secant line search on a Neo-Hookean RVE under a
+    finite shear + stretch: stress, deformation gradient and element fields
+    against the oracle."""
+    p = _large("CompressibleNeoHookean", dims=(16, 16, 16), tol=1e-10, seed=2)
+    say(f"large strain solve 16^3 {precision}")
+    F = np.array([1.05, 0.2, 0.0, 0.0, 0.97, 0.05, 0.0, 0.0, 1.1])
+    o, g, it_o, it_g = _solve_pair(p, F, precision=precision)
+    if precision == "fp64":
+        assert abs(it_o - it_g) <= 2
+        assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-10
+        fo, fg = o.element_fields(), g.element_fields()
+        for k in ("strain", "stress", "stress_average", "strain_average"):
+            assert relerr(fg[k], fo[k]) < 1e-8, k
+    else:
+        assert g.error_history()[-1] <= p.tol
+        assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-8
+    g.close()
+
+
+def test_large_strain_mixed_bc_golden(sphere32_ms):
+    """test_MixedBCs_LargeStrain.json on sphere32: load case 1 (F_33 ramped to
+    2.0 in 10 steps with P_11 = P_22 = 0) ends at P_33 = 74.757449712464 and
+    F_11 = F_22 = 0.832542244829, the values the reference stored as load cases
+    2 and 3 of that file; load case 2 (all nine components stress controlled,
+    fresh solver) reaches F = diag(0.832542244829, 0.832542244829,
+    2.000000000001)."""
+    p = cases.reference_input("test_MixedBCs_LargeStrain", sphere32_ms)
+    golden_F = p.load_cases[2].g0_path[0]
+    golden_P = np.array(p.load_cases[1].mbc["stress_path"][0])
+    lc = p.load_cases[0]
+    say("large strain mixed bc golden sphere32, 10 steps")
+    g = gpu.GpuSolver(p)
+    for t in range(lc.n_steps):
+        g.enable_mixed_bc(lc.mbc, t)
+        g.solve()
+        assert g.error_history()[-1] <= p.tol, t
+        g.extrapolate_displacement() if t + 1 < lc.n_steps else None
+    F, P = g.macro_gradient(), g.homogenized_stress()
+    assert abs(F[0] - golden_F[0]) < 1e-10 and abs(F[4] - golden_F[4]) < 1e-10
+    assert abs(P[8] - golden_P[8]) < 1e-9 * golden_P[8]
+    assert np.max(np.abs(P[:8])) < 1e-9
+    g.close()
+    g = gpu.GpuSolver(p)
+    g.enable_mixed_bc(p.load_cases[1].mbc, 0)
+    g.solve()
+    assert np.max(np.abs(g.macro_gradient() - golden_F)) < 1e-10
+    g.close()
