@@ -229,7 +229,9 @@ def main():
                         "time_to_solution_speedup": round(t_64 / t_mx, 3), "homog_stress_rel_err": rel,
                         "converged": rmx["converged"], "solve_ms": round(t_mx, 2), "fp64_solve_ms": round(t_64, 2),
                         "iteration_hbm_frac": round(B_ALG_MIXED * N * sum(rmx["iters"]) /
-                                                    (sum(rmx["solve_ms"]) * 1e-3) / HBM_PEAK, 4)}
+                                                    (sum(rmx["solve_ms"]) * 1e-3) / HBM_PEAK, 4),
+                        "kernel_ms_per_inner_iteration": {k: round(v[0] / max(rmx["fam_iters"], 1), 4)
+                                                          for k, v in rmx["fam"].items() if v[1]}}
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(p, g0)
