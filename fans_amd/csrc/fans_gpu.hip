@@ -428,25 +428,32 @@ int grid_vec(int64_t n) { return (int) std::max<int64_t>(1, std::min<int64_t>((n
 bool reg_len(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256; }
 
 // columns per block: a power of two (>= 4) within ~72 KiB of LDS (two
-// workgroups per CU), at most 256 threads; at 256^3 FP64 this gives 16 columns
-// (256-byte row segments) for the y pass and 4 frequencies for the x pass
+// workgroups per CU) and at most MAXT threads; at 256^3 FP64 this gives 16
+// columns (256-byte row segments) for the y pass and 4 frequencies for the x
+// pass.  The FP32 y pass takes 512 threads (32 columns, 256-byte segments:
+// 0.095 -> 0.085 ms at 256^3); the FP32 x pass stays at 256 (wider was slower).
 constexpr int kRegLdsKiB = 72;
-template <typename T, int N1, int N2>
+template <typename T, int N1, int N2, int MAXT = 256>
 constexpr int reg_cols()
 {
     constexpr int n   = N1 * N2;
     constexpr int lim = (int) (kRegLdsKiB * 1024 / (n * sizeof(cplx<T>)));
     int           c   = 4;
-    while (c * 2 <= lim && c * 2 * N2 <= 256)
+    while (c * 2 <= lim && c * 2 * N2 <= MAXT)
         c *= 2;
     return c;
+}
+template <typename T>
+constexpr int strided_max_threads()
+{
+    return sizeof(T) == 4 ? 512 : 256;
 }
 
 template <typename T, bool INV, int N1, int N2>
 void launch_strided_reg(fans_gpu_ctx *c, cplx<T> *spec, const cplx<T> *tw, int64_t sn, int c0, int c1,
                         int64_t ostride, int nouter, const int *done)
 {
-    constexpr int C   = reg_cols<T, N1, N2>();
+    constexpr int C   = reg_cols<T, N1, N2, strided_max_threads<T>()>();
     const int     nbx = (c1 - c0 + C - 1) / C;
     hipLaunchKernelGGL((k_fft_strided_reg<T, N1, N2, C, INV>), dim3(nbx * nouter), dim3(C * N2), 0, c->stream, spec,
                        tw, sn, c0, c1, ostride, nbx, done);
@@ -1258,16 +1265,13 @@ int solve_cg_mixed(fans_gpu_ctx *c)
     const double        eta   = 2e-5;
     while (!done && inner < c->d.n_it) {
         // inner FP32 CG on K x = r, x = 0 (its own scalars/history)
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r32, c->r,
-                           c->Nh);
-        if ((rc = fill<float>(c, c->x32, 0.f, c->Nh)) || (rc = fill<float>(c, c->dA32, 0.f, c->Nh)))
-            return rc;
         const double err_out = sc[SC_ERR];
         if ((rc = init_scalars(c, std::max(eta * err_out, 0.25 * c->tol), FANS_ERR_ABSOLUTE, c->d.n_it - inner)))
             return rc;
-        hipLaunchKernelGGL((k_dot<float>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r32, (const float *) nullptr,
-                           c->Nh, c->d.measure, c->partial);
-        if ((rc = reduce(c, c->partial + c->nblk_vec, c->nblk_vec, 1, RED_ERR_INIT)))
+        hipLaunchKernelGGL(k_inner_start, dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r32, c->x32, c->dA32,
+                           c->r, c->Nh, c->d.measure, c->partial);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = reduce(c, c->partial, c->nblk_vec, 1, RED_ERR_INIT)))
             return rc;
         float *dcur = c->dA32, *dnext = c->dB32;
         int    it   = 0;

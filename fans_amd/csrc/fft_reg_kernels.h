@@ -187,7 +187,11 @@ __global__ __launch_bounds__(KB *(H == 3 ? 4 : 1) * N2) void k_fft_x_green_reg(
     const int          kk = col / HP, c = col % HP;
     const int          lb = xcd_tile_index(blockIdx.x, gridDim.x);
     const int          kz = kz_lo + (lb % nbk) * KB + kk;
-    const int          y  = lb / nbk;
+    // ky and its mirror ny - ky read the same quarter-stored Green values:
+    // order the rows 0, 1, ny-1, 2, ny-2, ... so that both run back to back
+    // on one XCD and the second finds them in that XCD's L2
+    const int          yt = lb / nbk;
+    const int          y  = yt == 0 ? 0 : ((yt & 1) ? (yt + 1) >> 1 : ny - (yt >> 1));
     const bool         ok = kz < kz_hi && c < H;
     const int64_t      sn = (int64_t) ny * Mc * H;
     cplx<T>           *base = data + ((int64_t) y * Mc + kz) * H + c;

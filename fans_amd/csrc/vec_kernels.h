@@ -136,6 +136,29 @@ __global__ __launch_bounds__(256) void k_dot(const T *__restrict__ a, const T *_
     }
 }
 
+// start of an inner FP32 solve of the mixed form in one pass: r32 = (float) r,
+// x = 0, d = 0 (solverCG's zero search direction) and per block the norm
+// partial of r32 (slot 0); same grid and accumulation order as k_dot's norm
+__global__ __launch_bounds__(256) void k_inner_start(float *__restrict__ r32, float *__restrict__ x,
+                                                     float *__restrict__ d, const double *__restrict__ r, int64_t n,
+                                                     int measure, double *__restrict__ partial)
+{
+    __shared__ double sm[8];
+    double            a0 = 0.0;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        const float v = (float) r[i];
+        r32[i]        = v;
+        x[i]          = 0.f;
+        d[i]          = 0.f;
+        acc_norm(a0, v, measure);
+    }
+    double     v[1]  = {a0};
+    const bool mx[1] = {measure == 2};
+    block_reduce<1>(v, mx, sm);
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = v[0];
+}
+
 // y = a * x + b * y  (a, b from host)
 template <typename T, typename TX>
 __global__ __launch_bounds__(256) void k_axpby(T *__restrict__ y, const TX *__restrict__ x, double a, double b, int64_t n)
