@@ -479,10 +479,10 @@ template <typename T, int H, int N1, int N2>
 void launch_x_green_reg(fans_gpu_ctx *c, cplx<T> *spec, const cplx<T> *tw, const GreenView<T> &gv, int kz_lo,
                         int kz_hi, const int *done)
 {
-    constexpr int HP  = H == 3 ? 4 : 1;
-    constexpr int KB  = reg_cols<T, N1, N2>() / HP;
+    constexpr int C   = reg_cols<T, N1, N2>();
+    constexpr int KB  = H == 3 ? (C == 16 ? 5 : C / 4) : C;
     const int     nbk = (kz_hi - kz_lo + KB - 1) / KB;
-    hipLaunchKernelGGL((k_fft_x_green_reg<T, H, N1, N2, KB>), dim3(nbk * c->ny), dim3(KB * HP * N2), 0, c->stream,
+    hipLaunchKernelGGL((k_fft_x_green_reg<T, H, N1, N2, KB, C>), dim3(nbk * c->ny), dim3(C * N2), 0, c->stream,
                        spec, tw, gv, kz_lo, kz_hi, nbk, done);
 }
 
