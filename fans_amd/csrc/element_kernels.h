@@ -635,10 +635,13 @@ __device__ __forceinline__ void element_forces_iso1(const T (&u)[8][1], const Ge
 //   mode xi       : d/dy, d/dz   (eta: d/dx, d/dz; zeta: d/dx, d/dy)
 //   mode eta zeta : d/dx only    (xi zeta: d/dy; xi eta: d/dz)
 // and the forces are the transposed Hadamard / edge-difference maps applied to
-// the modal stresses.  This is the same operator as the Gauss-point loop (the
-// quadrature is exact for it) at about 360 instead of ~600 FP64 operations per
-// element, with a third of the live registers.  For one Gauss point (HEX8R)
-// only mode 1 remains.
+// the modal stresses.  With 8 Gauss points both maps are one 3-D Walsh
+// transform of the corner values per component (walsh8: the Hadamard modes of
+// the edge differences are Walsh coefficients, and the mixed modes that two
+// derivative directions share are the same coefficient).  This is the same
+// operator as the Gauss-point loop (the quadrature is exact for it) at about
+// 250 instead of ~600 FP64 operations per element.  For one Gauss point
+// (HEX8R) only mode 1 remains.
 // ---------------------------------------------------------------------------
 
 // forward: D[p][q] (p, q in {0,1}) -> A0..A3
@@ -660,6 +663,28 @@ __device__ __forceinline__ void hadamard4_T(T &a0, T &a1, T &a2, T &a3)
     a1 = p0 + p1;  // D10
     a2 = q0 - q1;  // D01
     a3 = q0 + q1;  // D11
+}
+
+// unnormalised 3-D Walsh-Hadamard transform of the 8 corner values (corner /
+// coefficient index i + 2j + 4k): W[ijk] = sum_xyz (-1)^(ix+jy+kz) v[xyz].
+// The Hadamard modes of the edge differences are Walsh coefficients:
+//   d/dx: A0..A3 = -W100, W110, W101, -W111   (d/dy: -W010, W110, W011, -W111;
+//   d/dz: -W001, W101, W011, -W111)
+// so one transform of 24 additions replaces the 3 x (4 differences + 8-addition
+// Hadamard) per component, and the transposed maps are one transform of the
+// mode sums.  The transform is symmetric (its own transpose).
+template <typename T>
+__device__ __forceinline__ void walsh8(T (&v)[8])
+{
+#pragma unroll
+    for (int s = 1; s < 8; s <<= 1)
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+            if ((a & s) == 0) {
+                const T x = v[a], y = v[a + s];
+                v[a]      = x + y;
+                v[a + s]  = x - y;
+            }
 }
 
 // modal block with two derivative directions dA, dB present (third missing):
@@ -697,29 +722,37 @@ __device__ __forceinline__ void element_forces_iso3_modal(const T (&u)[8][3], co
 {
     const T kc   = (T) g.ilx;
     const T k[3] = {CUBIC ? (T) 1 : (T) g.ilx, CUBIC ? (T) 1 : (T) g.ily, CUBIC ? (T) 1 : (T) g.ilz};
-    // A[d][mode][c]; edge differences along d (scaled by 1/h_d), then Hadamard
+    // A[d][mode][c]: Hadamard modes of the edge differences along d, scaled by 1/h_d
     T A[3][4][3];
+    if constexpr (NGP == 8) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < 3; ++c) {
+            T w[8];
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
+            for (int n = 0; n < 8; ++n)
+                w[n] = u[n][c];
+            walsh8(w);
+            A[0][0][c] = -w[1] * k[0], A[0][1][c] = w[3] * k[0], A[0][2][c] = w[5] * k[0], A[0][3][c] = -w[7] * k[0];
+            A[1][0][c] = -w[2] * k[1], A[1][1][c] = w[3] * k[1], A[1][2][c] = w[6] * k[1], A[1][3][c] = -w[7] * k[1];
+            A[2][0][c] = -w[4] * k[2], A[2][1][c] = w[5] * k[2], A[2][2][c] = w[6] * k[2], A[2][3][c] = -w[7] * k[2];
+        }
+    } else {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                A[0][p + 2 * q][c] = (u[1 + 2 * p + 4 * q][c] - u[2 * p + 4 * q][c]) * k[0];
-                A[1][p + 2 * q][c] = (u[p + 2 + 4 * q][c] - u[p + 4 * q][c]) * k[1];
-                A[2][p + 2 * q][c] = (u[p + 2 * q + 4][c] - u[p + 2 * q][c]) * k[2];
-            }
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    A[0][p + 2 * q][c] = (u[1 + 2 * p + 4 * q][c] - u[2 * p + 4 * q][c]) * k[0];
+                    A[1][p + 2 * q][c] = (u[p + 2 + 4 * q][c] - u[p + 4 * q][c]) * k[1];
+                    A[2][p + 2 * q][c] = (u[p + 2 * q + 4][c] - u[p + 2 * q][c]) * k[2];
+                }
+    }
     // Gauss weight w = V / NGP; mode weights NGP / 3^deg; 1/16 from the two
     // factors 1/4 of the bilinear expansion
     const T s  = CUBIC ? (T) NGP / (T) 16 * kc * kc : (T) NGP / (T) 16;
     const T L0 = s * lw, M0 = s * mw;
-    if constexpr (NGP == 8) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                hadamard4(A[d][0][c], A[d][1][c], A[d][2][c], A[d][3][c]);
-    } else {
+    if constexpr (NGP == 1) {
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -757,11 +790,24 @@ __device__ __forceinline__ void element_forces_iso3_modal(const T (&u)[8][3], co
         modal_single<T, 0>(A[0][3], L2, M2);  // eta zeta
         modal_single<T, 1>(A[1][3], L2, M2);  // xi zeta
         modal_single<T, 2>(A[2][3], L2, M2);  // xi eta
+        // transposed maps: the Walsh transform of the mode sums
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
+        for (int c = 0; c < 3; ++c) {
+            T b[8];
+            b[0] = (T) 0;
+            b[1] = -A[0][0][c] * k[0];
+            b[2] = -A[1][0][c] * k[1];
+            b[4] = -A[2][0][c] * k[2];
+            b[3] = A[0][1][c] * k[0] + A[1][1][c] * k[1];
+            b[5] = A[0][2][c] * k[0] + A[2][1][c] * k[2];
+            b[6] = A[1][2][c] * k[1] + A[2][2][c] * k[2];
+            b[7] = -(A[0][3][c] * k[0] + A[1][3][c] * k[1] + A[2][3][c] * k[2]);
+            walsh8(b);
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                hadamard4_T(A[d][0][c], A[d][1][c], A[d][2][c], A[d][3][c]);
+            for (int n = 0; n < 8; ++n)
+                f[n][c] = b[n];
+        }
+        return;
     } else {
 #pragma unroll
         for (int d = 0; d < 3; ++d)
@@ -769,7 +815,7 @@ __device__ __forceinline__ void element_forces_iso3_modal(const T (&u)[8][3], co
             for (int c = 0; c < 3; ++c)
                 A[d][1][c] = A[d][2][c] = A[d][3][c] = A[d][0][c];
     }
-    // transposed edge differences
+    // transposed edge differences (one Gauss point)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
@@ -799,7 +845,32 @@ __device__ __forceinline__ void element_forces_iso1_modal(const T (&u)[8][1], co
 {
     const T kc   = (T) g.ilx;
     const T k[3] = {CUBIC ? (T) 1 : (T) g.ilx, CUBIC ? (T) 1 : (T) g.ily, CUBIC ? (T) 1 : (T) g.ilz};
-    T       A[3][4];
+    const T c0   = CUBIC ? (T) NGP / (T) 16 * kw * kc * kc : (T) NGP / (T) 16 * kw;
+    const T t0   = CUBIC ? (T) NGP * (T) 0.25 * kc : (T) NGP * (T) 0.25;
+    if constexpr (NGP == 8) {
+        // Walsh coefficients -> modes (see walsh8), modal scaling, mode sums -> Walsh
+        T w[8];
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+            w[n] = u[n][0];
+        walsh8(w);
+        const T c1 = c0 * (T) (1.0 / 3.0), c2 = c1 * (T) (1.0 / 3.0);
+        T       b[8];
+        b[0]        = (T) 0;
+        b[1]        = -(c0 * (-w[1] * k[0]) + t0 * T0[0]) * k[0];
+        b[2]        = -(c0 * (-w[2] * k[1]) + t0 * T0[1]) * k[1];
+        b[4]        = -(c0 * (-w[4] * k[2]) + t0 * T0[2]) * k[2];
+        b[3]        = c1 * w[3] * (k[0] * k[0] + k[1] * k[1]);
+        b[5]        = c1 * w[5] * (k[0] * k[0] + k[2] * k[2]);
+        b[6]        = c1 * w[6] * (k[1] * k[1] + k[2] * k[2]);
+        b[7]        = c2 * w[7] * (k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+        walsh8(b);
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+            f[n][0] = b[n];
+        return;
+    }
+    T A[3][4];
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -808,8 +879,6 @@ __device__ __forceinline__ void element_forces_iso1_modal(const T (&u)[8][1], co
             A[1][p + 2 * q] = (u[p + 2 + 4 * q][0] - u[p + 4 * q][0]) * k[1];
             A[2][p + 2 * q] = (u[p + 2 * q + 4][0] - u[p + 2 * q][0]) * k[2];
         }
-    const T c0 = CUBIC ? (T) NGP / (T) 16 * kw * kc * kc : (T) NGP / (T) 16 * kw;
-    const T t0 = CUBIC ? (T) NGP * (T) 0.25 * kc : (T) NGP * (T) 0.25;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         if constexpr (NGP == 8) {
