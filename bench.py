@@ -41,11 +41,12 @@ def family_bytes_per_voxel(nz, h=3, wb=8, green_quarter=True):
         "fft_x_green": 2 * S + Gp,        # 54.6 B at 256^3 with quarter storage
         "fft_y_inv": 2 * S,
         "fft_z_c2r": S + 2 * V,      # read spectrum + r, write s
-        "assemble_lin": 4 * V + 1,   # read s, d, phase; write d, rnew
-        "cg_update": 7 * V,          # read r, rnew, u, d, s; write r, u
-        # CG update fused into the forward z pass: read r, rnew, u, d, s;
-        # write r, u and the spectrum
-        "cg_update_fft_z_r2c": 7 * V + S,
+        # FP64: read s, d, phase, u (owned nodes); write d, rnew, u -- the matvec
+        # pass also applies the previous iteration's u -= alpha d
+        "assemble_lin": 6 * V + 1,
+        # FP64: residual update fused into the forward z pass: read r, rnew, s;
+        # write r and the spectrum
+        "cg_update_fft_z_r2c": 4 * V + S,
     }
 
 

@@ -33,11 +33,15 @@ struct AsmArgs {
     int             nphases;
     J2State         js;
     // ASM_RESID: in = u, out = r; norm/dot partials
-    // ASM_CGLIN: in = s, din = d_cur, dout = d_next (d = s + beta d), out = rnew
+    // ASM_CGLIN: in = s, din = d_cur, dout = d_next (d = s + beta d), out = rnew;
+    //            u (nullable): u -= alpha d_cur at owned nodes -- the solution
+    //            update of the previous iteration (alpha at scal[1]), done here
+    //            because this pass reads d_cur anyway
     const T        *in;
     const T        *din;
     T              *dout;
     T              *out;
+    T              *u;
     const T        *dotv;      // ASM_RESID optional: partial <out, dotv>
     const double   *scal;      // device scalars (beta at scal[0] for CGLIN)
     const T        *g0;        // macroscopic gradient (n_str), device
@@ -96,9 +100,11 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
     const int64_t off_own = (int64_t) wrapy(ey) * nz + wrapz(ez);
     const int64_t off_ext = (int64_t) wrapy(e_a) * nz + wrapz(e_b);
 
-    T beta = (T) 0;
-    if constexpr (MODE == ASM_CGLIN)
-        beta = (T) a.scal[0];
+    T beta = (T) 0, alpha = (T) 0;
+    if constexpr (MODE == ASM_CGLIN) {
+        beta  = (T) a.scal[0];
+        alpha = (T) a.scal[1];
+    }
 
     T      carry[H];
     double acc0 = 0.0, acc1 = 0.0;
@@ -121,7 +127,8 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
 
     // node plane x+1: fetch (global) then commit (d update, LDS); the phase id
     // of the next element plane is loaded one plane ahead
-    T pf_own[MODE == ASM_CGLIN ? 2 : 1][H], pf_ext[MODE == ASM_CGLIN ? 2 : 1][H];
+    T pf_own[MODE == ASM_CGLIN ? 2 : 1][H], pf_ext[MODE == ASM_CGLIN ? 2 : 1][H], pf_u[H];
+    auto owned = [&](int xp) { return xp >= X0 && xp < X1 && own_yz; };
     auto fetch_plane = [&](int xp) {
         const int     xg = xp < 0 ? xp + nx : (xp >= nx ? xp - nx : xp);
         const int64_t pb = xg * plane;
@@ -131,6 +138,11 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
             if constexpr (MODE == ASM_CGLIN)
                 pf_own[1][c] = a.din[(pb + off_own) * H + c];
         }
+        if constexpr (MODE == ASM_CGLIN)
+            if (a.u && owned(xp))
+#pragma unroll
+                for (int c = 0; c < H; ++c)
+                    pf_u[c] = a.u[(pb + off_own) * H + c];
         if (ext >= 0)
 #pragma unroll
             for (int c = 0; c < H; ++c) {
@@ -141,15 +153,18 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
     };
     auto commit_plane = [&](int xp, int buf) {
         const int     xg = xp < 0 ? xp + nx : (xp >= nx ? xp - nx : xp);
-        const bool    wr = (MODE == ASM_CGLIN) && xp >= X0 && xp < X1 && own_yz;
+        const bool    wr = (MODE == ASM_CGLIN) && owned(xp);
         const int64_t n  = xg * plane + off_own;
 #pragma unroll
         for (int c = 0; c < H; ++c) {
             T val = pf_own[0][c];
             if constexpr (MODE == ASM_CGLIN) {
                 val = val + beta * pf_own[1][c];
-                if (wr)
+                if (wr) {
                     a.dout[n * H + c] = val;
+                    if (a.u)
+                        a.u[n * H + c] = pf_u[c] - alpha * pf_own[1][c];
+                }
             }
             Un[buf][(ey * NL + ez) * H + c] = val;
         }

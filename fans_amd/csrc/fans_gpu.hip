@@ -979,6 +979,24 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext
     int        chunk = 4, rc;
     int        is[IS_COUNT];
     bool       first = true;
+    // When the matvec pass of iteration j applies u -= alpha_(j-1) d_(j-1)
+    // (cg_update_u_in_matvec), the update of iteration k-1 is still pending
+    // when the stop test fires in iteration k.  Iteration j writes its
+    // direction into the buffer that was dnext at entry for even j and into
+    // dcur for odd j.
+    T *const d_even = dnext, *const d_odd = dcur;
+    auto     finish = [&]() -> int {
+        const int k = is[IS_ITER];
+        if (cg_update_u_in_matvec<T>() && k >= 1) {
+            T *dl = ((k - 1) % 2 == 0) ? d_even : d_odd;
+            hipLaunchKernelGGL((k_sub_scaled<T>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, u, dl,
+                               c->sc + SC_ALPHA, c->Nh);
+            HIPCHK(c, hipGetLastError());
+        }
+        if (iters_out)
+            *iters_out = k;
+        return 0;
+    };
     // one CG iteration (with the update of the previous one unless `first`)
     auto iteration = [&](bool with_pro) -> int {
         const ProArgs<T> pa{rnew, u, dcur, s, c->sc + SC_ALPHA, c->d.measure, c->partial};
@@ -997,7 +1015,8 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext
         a.din        = dcur;
         a.dout       = dnext;
         a.out        = rnew;
-        a.scal       = c->sc;  // SC_BETA == 0
+        a.u          = cg_update_u_in_matvec<T>() ? u : nullptr;  // u -= alpha d_cur (previous iteration)
+        a.scal       = c->sc;  // SC_BETA == 0, SC_ALPHA == 1
         a.done       = done;
         if ((e = launch_assemble<T, ASM_CGLIN>(c, a)))
             return e;
@@ -1041,9 +1060,7 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext
             if (is[IS_DONE])
                 break;
         }
-        if (iters_out)
-            *iters_out = is[IS_ITER];
-        return 0;
+        return finish();
     }
     for (;;) {
         // timing mode checks `done` after every iteration so that launches that
@@ -1070,9 +1087,7 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext
             break;
         chunk = std::min(chunk * 2, 16);
     }
-    if (iters_out)
-        *iters_out = is[IS_ITER];
-    return 0;
+    return finish();
 }
 
 int fetch_history(fans_gpu_ctx *c, int iters)

@@ -159,6 +159,17 @@ __global__ __launch_bounds__(256) void k_inner_start(float *__restrict__ r32, fl
         partial[blockIdx.x] = v[0];
 }
 
+// y -= a x with the scalar a on the device (the CG solution update left
+// pending when the iteration stops)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sub_scaled(T *__restrict__ y, const T *__restrict__ x,
+                                                    const double *__restrict__ a, int64_t n)
+{
+    const T s = (T) *a;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
+        y[i] -= s * x[i];
+}
+
 // y = a * x + b * y  (a, b from host)
 template <typename T, typename TX>
 __global__ __launch_bounds__(256) void k_axpby(T *__restrict__ y, const TX *__restrict__ x, double a, double b, int64_t n)
