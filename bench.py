@@ -157,6 +157,100 @@ def run_batch_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def j2_problem(n=256, seed=0, tol=1e-8):
+    """J2 plasticity with linear isotropic hardening (matrix E=70, nu=0.33,
+    yield 0.25, hardening 1.0) around elastic spheres (E=400, nu=0.2), the
+    headline sphere law (150 spheres, r ~ U[8,20] at 256^3, seed 0), HEX8,
+    CG L-inf 1e-8 per step."""
+    from fans_amd.microstructure import random_spheres
+    from fans_amd.problem import MaterialGroup, Problem, isotropic_KG
+    s = n / 256.0
+    ms = random_spheres(n, max(1, int(round(150 * s ** 3))), 8 * s, 20 * s, seed=seed)
+    Km, Gm = isotropic_KG(70.0, 0.33)
+    Ki, Gi = isotropic_KG(400.0, 0.2)
+    groups = [MaterialGroup("J2PlasticityNew_LinearIsotropicHardening", [0],
+                            {"bulk_modulus": [Km], "shear_modulus": [Gm], "yield_stress": [0.25],
+                             "isotropic_hardening_parameter": [1.0]}),
+              MaterialGroup("LinearElasticIsotropic", [1], {"bulk_modulus": [Ki], "shear_modulus": [Gi]})]
+    return Problem(dims=(n, n, n), howmany=3, ms=ms, groups=groups, fe_type="HEX8", method="cg",
+                   measure="Linfinity", err_type="absolute", tol=tol, n_it=2000)
+
+
+def run_j2_workload(args):
+    """Uniaxial strain ramp to 2 % in `--j2-steps` of 20 load steps with zero
+    lateral macroscopic stress (mixed BCs), J2 matrix; FP64 and mixed form
+    (FP32 Green preconditioner, FP64 residuals and return map).  One JSON line:
+    voxel-iterations/s of the nonlinear CG (every line-search residual counts
+    as part of its iteration) and the FP64-vs-mixed stress difference."""
+    from fans_amd.gpu import GpuSolver
+    n = args.size
+    p = j2_problem(n)
+    N = n ** 3
+    nsteps = 20
+    emax = 0.02
+    strain = np.linspace(emax / nsteps, emax, nsteps)
+    mbc = dict(strain_indices=[0, 3, 4, 5], stress_indices=[1, 2],
+               strain_path=np.stack([strain, 0 * strain, 0 * strain, 0 * strain], 1),
+               stress_path=np.zeros((nsteps, 2)), n_steps=nsteps)
+    res = {}
+    # the third run repeats the FP64 ramp with every launch bracketed by HIP
+    # events (per-kernel durations; kept out of the timed runs)
+    for prec, timing in (("fp64", False), ("mixed", False), ("fp64", True)):
+        print(f"[bench] J2 {n}^3 {prec}{' (kernel timing)' if timing else ''}: {args.j2_steps} load steps",
+              file=sys.stderr, flush=True)
+        g = GpuSolver(p, precision=prec)
+        g.set_timing(timing)
+        its, ms, fam = [], [], {}
+        for t in range(args.j2_steps):
+            g.enable_mixed_bc(mbc, t)
+            g.solve()
+            its.append(int(g.stats.iterations))
+            ms.append(float(g.stats.solve_ms))
+            print(f"[bench]   step {t}: {its[-1]} iterations, {ms[-1]:.1f} ms", file=sys.stderr, flush=True)
+            if timing:
+                for k, (kms, cnt) in g.kernel_times().items():
+                    a, b = fam.get(k, (0.0, 0))
+                    fam[k] = (a + kms, b + cnt)
+            g.extrapolate_displacement()
+        _, q = g.get_state()
+        res["timing" if timing else prec] = dict(its=its, ms=ms, stress=g.homogenized_stress().copy(),
+                                                 qmax=float(q.max()), fam=fam)
+        g.close()
+    r64, rmx, rt = res["fp64"], res["mixed"], res["timing"]
+    # J2 residual assembly: reads u (24 B) and the phase byte, writes r (24 B);
+    # J2 voxels also read the committed state, 7 doubles at each of 8 Gauss
+    # points (448 B); elastic voxels carry no state
+    f_j2 = float(np.mean(p.ms == 0))
+    res_bpv = 49 + 448 * f_j2
+    ms_res, n_res = rt["fam"].get("assemble_res", (0.0, 0))
+    avg_res = ms_res / max(n_res, 1)
+    ach = res_bpv * N / (avg_res * 1e-3) / 1e9 if n_res else None
+    rel = float(np.max(np.abs(rmx["stress"] - r64["stress"])) / np.max(np.abs(r64["stress"])))
+    t64 = sum(r64["ms"]) * 1e-3
+    out = {"metric": f"voxel-iters/s, J2 {n}^3 nonlinear CG (uniaxial ramp, mixed BCs)",
+           "value": round(N * sum(r64["its"]) / t64, 1), "unit": "voxel-iters/s", "n_gpus": 1,
+           "steps": args.j2_steps, "warmup": 0, "ms_per_step": round(1000 * t64 / args.j2_steps, 2),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{n}^3 J2PlasticityNew linear hardening matrix E=70 nu=0.33 sy=0.25 H=1.0, "
+                                  "150 elastic spheres E=400 nu=0.2 r~U[8,20] seed 0, eps11 ramp to 0.02 in 20 "
+                                  "steps with zero lateral stress, CG L-inf 1e-8", "load_steps": args.j2_steps},
+           "fp64": {"iterations": r64["its"], "step_ms": [round(v, 1) for v in r64["ms"]]},
+           "mixed": {"iterations": rmx["its"], "step_ms": [round(v, 1) for v in rmx["ms"]],
+                     "time_to_solution_speedup": round(sum(r64["ms"]) / sum(rmx["ms"]), 3),
+                     "homog_stress_rel_err": rel},
+           "max_hardening_q": r64["qmax"],
+           "roofline": {"bound": "hbm", "kernel": "assemble_res", "achieved": round(ach, 1) if ach else None,
+                        "peak": 8000.0, "unit": "GB/s", "frac": round(ach / 8000.0, 4) if ach else None,
+                        "traffic": traffic_from_profiles("assemble_res_j2"),
+                        "alg_bytes_per_launch": res_bpv * N, "avg_launch_ms": round(avg_res, 4),
+                        "j2_voxel_fraction": round(f_j2, 4)},
+           "kernel_ms_per_iteration": {k: round(v[0] / max(sum(rt["its"]), 1), 4)
+                                       for k, v in rt["fam"].items() if v[1]},
+           "launches_per_iteration": {k: round(v[1] / max(sum(rt["its"]), 1), 2)
+                                      for k, v in rt["fam"].items() if v[1]}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,8 +259,10 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--workload", choices=["headline", "batch"], default="headline",
-                    help="batch: the 384-RVE uneven-size batch (RVEs/s), not the headline metric")
+    ap.add_argument("--workload", choices=["headline", "batch", "j2"], default="headline",
+                    help="batch: the 384-RVE uneven-size batch (RVEs/s); j2: the J2 load-step ramp; neither is "
+                         "the headline metric")
+    ap.add_argument("--j2-steps", type=int, default=20, help="load steps of the j2 workload (of 20)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (reads 1 GiB, writes 1 GiB) to scale HBM counters")
     args = ap.parse_args()
@@ -179,13 +275,16 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 8}), flush=True)
         return
-    if args.workload == "batch":
-        run_batch_workload(args)
-        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is a design "
               "described in DESIGN.md and is not executed", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        run_batch_workload(args)
+        return
+    if args.workload == "j2":
+        run_j2_workload(args)
+        return
     n = args.size
     p = headline_problem(n)
     N = n ** 3

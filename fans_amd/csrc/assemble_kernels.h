@@ -64,8 +64,11 @@ __device__ __forceinline__ T shfl_up16(T v)
 // ASM_LARGE = the per-Gauss-point path with finite strain (F and P, n_str 9).
 enum { ASM_GENERAL = 0, ASM_ISO = 1, ASM_ISO_CUBIC = 2, ASM_LARGE = 3 };
 
+// The per-Gauss-point FP64 small-strain variants land just above 256 registers
+// (one wave per SIMD) unless asked for two waves, which they reach without
+// spilling; the finite-strain variant would spill, so it keeps one.
 template <typename T, int H, int NGP, int MODE, int VAR>
-__global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_LARGE ? 1 : 2))) void k_assemble(AsmArgs<T> a)
 {
     constexpr bool ISO = VAR == ASM_ISO || VAR == ASM_ISO_CUBIC, CUBIC = VAR == ASM_ISO_CUBIC;
     if (a.done && *a.done)
@@ -243,11 +246,19 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs<T> a)
                                             reinterpret_cast<T(&)[8][1]>(f));
             }
         } else {
-            const Phase<T> &P   = a.phases[ph];
-            auto            mat = [&](int gp, const T(&e)[NS], T(&s)[NS]) {
-                material_point_ns<T, H, NS>(P, e, s, a.js, el, gp, NGP);
-            };
-            element_forces<T, H, NGP, NS>(u, a.geom, g0, mat, f);
+            const Phase<T> &P = a.phases[ph];
+            // the linear matvec sees linear models only: no state to prefetch
+            if constexpr (NS == 9 || MODE == ASM_CGLIN) {
+                auto mat = [&](int gp, const T(&e)[NS], T(&s)[NS]) {
+                    material_point_ns<T, H, NS>(P, e, s, a.js, el, gp, NGP);
+                };
+                element_forces<T, H, NGP, NS>(u, a.geom, g0, mat, f);
+            } else {
+                PointModel<T, H, NGP> pm;
+                pm.init(P, a.js, el);
+                auto mat = [&](int gp, const T(&e)[NS], T(&s)[NS]) { pm.eval(gp, e, s); };
+                element_forces<T, H, NGP, NS>(u, a.geom, g0, mat, f);
+            }
         }
         // the loads of the plane after next fly during the exchange and the
         // stores (the element's registers are free again here)

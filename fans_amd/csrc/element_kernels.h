@@ -86,12 +86,15 @@ __device__ __forceinline__ void j2_point(const double *p, const double (&e)[6], 
     const double sy     = p[2] + q_in * p[3];
     const double phi    = s_t - c23 * sy;
     const double dgamma = fmax(0.0, phi / (2.0 * G + 2.0 / 3.0 * p[3]));
-    const bool   ok     = s_t > 1e-12;
+    // one reciprocal instead of twelve divisions by s_t; below the 1e-12 guard
+    // the updates add exact zeros, which leaves ep_in and s untouched as the
+    // reference's branch does
+    const double inv = s_t > 1e-12 ? 1.0 / s_t : 0.0;
+    const double ge = dgamma * inv, gs = dgamma * 2.0 * G * inv;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
-        ep[i] = ok ? ep_in[i] + dgamma * st[i] / s_t : ep_in[i];
-        if (ok)
-            s[i] = s[i] - dgamma * 2.0 * G * st[i] / s_t;
+        ep[i] = ep_in[i] + ge * st[i];
+        s[i]  = s[i] - gs * st[i];
     }
     qo = q_in + c23 * dgamma;
 }
@@ -240,6 +243,98 @@ __device__ __forceinline__ void material_point(const Phase<T> &P, const T (&e)[H
         }
     }
 }
+
+// One element's constitutive model on the small-strain per-Gauss-point paths.
+// The phase kind and its first six parameters are read once per element (every
+// kind but the triclinic ones needs no more), and the J2 state of Gauss point
+// gp + 1 is requested while point gp is evaluated, so the state loads overlap
+// the previous point's arithmetic instead of each waiting on its own.
+template <typename T, int H, int NGP>
+struct PointModel {
+    const Phase<T> *P;
+    int             kind;
+    T               pr[6];
+    const double   *sb;     // J2 element: its state at Gauss point 0, else null
+    int64_t         nel;    // state stride between Gauss points
+    double          nx[7];  // state of the next Gauss point
+
+    __device__ __forceinline__ void init(const Phase<T> &ph, const J2State &js, int64_t el)
+    {
+        P    = &ph;
+        kind = ph.kind;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+            pr[i] = ph.p[i];
+        nel = js.nel;
+        sb  = (H == 3 && kind == MODEL_J2_NEW) ? js.st + el : nullptr;
+        fetch(0);
+    }
+    __device__ __forceinline__ void fetch(int gp)
+    {
+        if (sb)
+#pragma unroll
+            for (int c = 0; c < 7; ++c)
+                nx[c] = sb[((int64_t) c * NGP + gp) * nel];
+    }
+    __device__ __forceinline__ void eval(int gp, const T (&e)[H == 1 ? 3 : 6], T (&s)[H == 1 ? 3 : 6])
+    {
+        if constexpr (H == 1) {
+            if (kind == MODEL_THERMAL_ISO) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    s[i] = pr[0] * e[i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    s[i] = P->p[3 * i] * e[0] + P->p[3 * i + 1] * e[1] + P->p[3 * i + 2] * e[2];
+            }
+        } else {
+            double cur[7];
+#pragma unroll
+            for (int c = 0; c < 7; ++c)
+                cur[c] = nx[c];
+            if (gp + 1 < NGP)
+                fetch(gp + 1);
+            switch (kind) {
+            case MODEL_ELASTIC_ISO: {
+                const T b1 = pr[0] * (e[0] + e[1] + e[2]), b2 = (T) 2 * pr[1];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    s[i] = b1 + b2 * e[i];
+#pragma unroll
+                for (int i = 3; i < 6; ++i)
+                    s[i] = b2 * e[i];
+            } break;
+            case MODEL_ELASTIC_TRI:
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    T a = 0;
+#pragma unroll
+                    for (int j = 0; j < 6; ++j)
+                        a += P->p[6 * i + j] * e[j];
+                    s[i] = a;
+                }
+                break;
+            case MODEL_J2_NEW: {
+                double ed[6], epin[6], sd[6], epo[6], qo, pd[4];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    ed[i]   = (double) e[i];
+                    epin[i] = cur[i];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    pd[i] = (double) pr[i];
+                j2_point(pd, ed, epin, cur[6], sd, epo, qo);
+#pragma unroll
+                for (int i = 0; i < 6; ++i)
+                    s[i] = (T) sd[i];
+            } break;
+            default: pseudo_point<T>(pr, e, s); break;
+            }
+        }
+    }
+};
 
 // material_point for any strain vector length (finite strain: NS = 9)
 template <typename T, int H, int NS>
@@ -1066,6 +1161,9 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
             }
             return;
         }
+        PointModel<T, H, NGP> pm;
+        if constexpr (MODE != ELEM_COMMIT && NS != 9)
+            pm.init(P, a.js, el);
         double eavg[NS], savg[NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k)
@@ -1109,7 +1207,7 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
                 else if (a.vk)
                     voxel_iso_stress<T, H>(vK, vG, e, s);
                 else
-                    material_point_ns<T, H, NS>(P, e, s, a.js, el, gp, NGP);
+                    pm.eval(gp, e, s);
 #pragma unroll
                 for (int c = 0; c < NS; ++c) {
                     eavg[c] += (double) e[c] / NGP;
