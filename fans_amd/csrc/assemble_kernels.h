@@ -91,8 +91,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
     const int     ny = a.ny, nz = a.nz, nx = a.nx;
     const int64_t plane = (int64_t) ny * nz;
 
-    auto wrapy = [&](int v) { int y = Y0 - 1 + v; y += (y < 0) ? ny : 0; return y >= ny ? y - ny : y; };
-    auto wrapz = [&](int v) { int z = Z0 - 1 + v; z += (z < 0) ? nz : 0; return z >= nz ? z - nz : z; };
+    // periodic node index of tile position v (several wraps on grids below 16)
+    auto wrapy = [&](int v) { return ((Y0 - 1 + v) % ny + ny) % ny; };
+    auto wrapz = [&](int v) { return ((Z0 - 1 + v) % nz + nz) % nz; };
     const int  my_y   = Y0 - 1 + ey, my_z = Z0 - 1 + ez;
     const bool own_yz = ey >= 1 && ez >= 1 && my_y < ny && my_z < nz;
     // node-plane slots of this thread: its own node (ey, ez) and, for 33 threads
@@ -102,6 +103,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
     const int  e_b    = ext < 0 ? 0 : (ext < NL ? ext : TILE);
     const int64_t off_own = (int64_t) wrapy(ey) * nz + wrapz(ez);
     const int64_t off_ext = (int64_t) wrapy(e_a) * nz + wrapz(e_b);
+    // J2 state slot of this thread's element within an x plane of the state
+    // layout: the tile's 16 slots are its 16 element columns in z
+    static_assert(NOUT == kStateTile && TILE == 16, "the J2 state layout follows the stencil tiles");
+    const int64_t splane = (int64_t) ny * a.ntz * 16;
+    const int64_t sl_own = ((int64_t) wrapy(ey) * a.ntz + tz) * 16 + ez;
+    // elements next to an owned node; the others (rows of the last tile past
+    // the grid edge when 15 does not divide it) only feed unowned sums
+    const bool el_needed = Y0 - 1 + max(ey, 1) < ny && Z0 - 1 + max(ez, 1) < nz;
 
     T beta = (T) 0, alpha = (T) 0;
     if constexpr (MODE == ASM_CGLIN) {
@@ -127,6 +136,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
             PL[p][1] = H == 3 ? a.phases[p].p[1] : (T) 0;
         }
     }
+    // per-Gauss-point residual: model kind of every phase, staged once per block
+    constexpr bool PM = !ISO && VAR != ASM_LARGE && MODE == ASM_RESID;
+    __shared__ int PK[PM ? 256 : 1];
+    if constexpr (PM)
+        for (int p = t; p < a.nphases; p += blockDim.x)
+            PK[p] = a.phases[p].kind;
 
     // node plane x+1: fetch (global) then commit (d update, LDS); the phase id
     // of the next element plane is loaded one plane ahead
@@ -188,7 +203,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
     fetch_plane(X0 - 1);
     commit_plane(X0 - 1, 0);
     fetch_plane(X0);
-    int ph_next = phase_of(X0 - 1);
+    int                   ph_next = phase_of(X0 - 1);
+    PointModel<T, H, NGP> pm;
+    bool                  pm_pre = false;  // point 0 of the coming element requested
     for (int i = 0; X0 - 1 + i < X1; ++i) {
         const int xe = X0 - 1 + i;
         commit_plane(xe + 1, (i + 1) & 1);
@@ -250,12 +267,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
             // the linear matvec sees linear models only: no state to prefetch
             if constexpr (NS == 9 || MODE == ASM_CGLIN) {
                 auto mat = [&](int gp, const T(&e)[NS], T(&s)[NS]) {
-                    material_point_ns<T, H, NS>(P, e, s, a.js, el, gp, NGP);
+                    material_point_ns<T, H, NS>(P, e, s);
                 };
                 element_forces<T, H, NGP, NS>(u, a.geom, g0, mat, f);
             } else {
-                PointModel<T, H, NGP> pm;
-                pm.init(P, a.js, el);
+                // the next element of this thread's row is xe + 1 (phase ph_next),
+                // 0 <= xe + 1 < X1 <= nx: no wrap (xe = -1 itself wraps to nx - 1)
+                // an element feeding no owned node reads no state (elastic stand-in)
+                const double *sn = nullptr;
+                if (xe + 1 < X1 && el_needed)
+                    sn = PointModel<T, H, NGP>::state_of(PK[ph_next], a.js, (int64_t) (xe + 1) * splane + sl_own);
+                pm.begin(P, el_needed ? PK[ph] : (int) MODEL_ELASTIC_ISO, a.js, (int64_t) xg * splane + sl_own, sn,
+                         pm_pre);
+                pm_pre = true;
                 auto mat = [&](int gp, const T(&e)[NS], T(&s)[NS]) { pm.eval(gp, e, s); };
                 element_forces<T, H, NGP, NS>(u, a.geom, g0, mat, f);
             }

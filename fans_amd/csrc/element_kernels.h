@@ -180,17 +180,39 @@ __device__ __forceinline__ void large_point(int kind, const T *p, const T (&e)[9
             s[3 * a + b] = e[3 * a] * S[0][b] + e[3 * a + 1] * S[1][b] + e[3 * a + 2] * S[2][b];
 }
 
-// State of J2 points, structure of arrays: comp c (0..5 plastic strain, 6 q)
-// of Gauss point g of element e at st[(c * ngp + g) * nel + e].
+// State of J2 points, structure of arrays over slots laid out like the tiles
+// of the stencil pass: z is cut into tiles of kStateTile elements, and the 16
+// slots of tile tz hold elements z = kStateTile * tz - 1 + lz (lz = 0..15,
+// periodic in z), so that one row of the stencil pass reads whole 128-byte
+// lines.  Elements at a tile seam (and elements wrapped past the last tile)
+// own two slots, which always hold the same values.  Component c (0..5 plastic
+// strain, 6 q) of Gauss point g of slot s = ((x * ny + y) * ntz + tz) * 16 + lz
+// sits at st[(c * ngp + g) * nslot + s].
+constexpr int kStateTile = 15;
 struct J2State {
-    double *st;   // committed (_t) state
-    int64_t nel;  // number of elements
+    double *st;     // committed (_t) state
+    int64_t nslot;  // slots per component and Gauss point: nx * ny * ntz * 16
 };
+
+// the element of slot s; canonical: the one slot of the element that every
+// pass but the state commit (which writes all of them) works on
+__device__ __forceinline__ void state_slot_element(int64_t s, int ny, int nz, int ntz, int &x, int &y, int &z,
+                                                   bool &canonical)
+{
+    const int     lz = (int) (s & 15);
+    const int64_t r  = s >> 4;
+    const int     tz = (int) (r % ntz);
+    const int64_t xy = r / ntz;
+    y                = (int) (xy % ny);
+    x                = (int) (xy / ny);
+    const int zz     = kStateTile * tz - 1 + lz;
+    canonical        = lz >= 1 && zz < nz;
+    z                = ((zz % nz) + nz) % nz;
+}
 
 // sigma = model(eps) for one Gauss point; returns in s
 template <typename T, int H>
-__device__ __forceinline__ void material_point(const Phase<T> &P, const T (&e)[H == 1 ? 3 : 6], T (&s)[H == 1 ? 3 : 6],
-                                               const J2State &js, int64_t el, int gp, int ngp)
+__device__ __forceinline__ void material_point(const Phase<T> &P, const T (&e)[H == 1 ? 3 : 6], T (&s)[H == 1 ? 3 : 6])
 {
     if constexpr (H == 1) {
         if (P.kind == MODEL_THERMAL_ISO) {
@@ -223,22 +245,13 @@ __device__ __forceinline__ void material_point(const Phase<T> &P, const T (&e)[H
                 s[i] = a;
             }
             break;
-        case MODEL_J2_NEW: {
-            double ed[6], epin[6], sd[6], epo[6], qo, pd[4];
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                ed[i]   = (double) e[i];
-                epin[i] = js.st[((int64_t) i * ngp + gp) * js.nel + el];
-            }
-            const double qin = js.st[((int64_t) 6 * ngp + gp) * js.nel + el];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                pd[i] = (double) P.p[i];
-            j2_point(pd, ed, epin, qin, sd, epo, qo);
+        case MODEL_J2_NEW:
+            // J2 points need their state: PointModel evaluates them; the
+            // linear-operator paths that call this never see a J2 phase
 #pragma unroll
             for (int i = 0; i < 6; ++i)
-                s[i] = (T) sd[i];
-        } break;
+                s[i] = (T) NAN;
+            break;
         default: pseudo_point<T>(P.p, e, s); break;
         }
     }
@@ -248,34 +261,52 @@ __device__ __forceinline__ void material_point(const Phase<T> &P, const T (&e)[H
 // The phase kind and its first six parameters are read once per element (every
 // kind but the triclinic ones needs no more), and the J2 state of Gauss point
 // gp + 1 is requested while point gp is evaluated, so the state loads overlap
-// the previous point's arithmetic instead of each waiting on its own.
+// the previous point's arithmetic instead of each waiting on its own.  A caller
+// that walks a row of elements names the next element's state (sn) so that the
+// last Gauss point requests that element's point 0 (begin() then skips it).
 template <typename T, int H, int NGP>
 struct PointModel {
     const Phase<T> *P;
     int             kind;
     T               pr[6];
-    const double   *sb;     // J2 element: its state at Gauss point 0, else null
-    int64_t         nel;    // state stride between Gauss points
+    const double   *sb;     // J2 element: its state slot at Gauss point 0, else null
+    const double   *sn;     // next J2 element of the row (point 0), else null
+    int64_t         nel;    // state stride between Gauss points (slots)
     double          nx[7];  // state of the next Gauss point
 
-    __device__ __forceinline__ void init(const Phase<T> &ph, const J2State &js, int64_t el)
+    __device__ __forceinline__ static const double *state_of(int kind, const J2State &js, int64_t slot)
+    {
+        return (H == 3 && kind == MODEL_J2_NEW) ? js.st + slot : nullptr;
+    }
+    // slot: a state slot of the element (J2State); sn: the next element's state
+    // (from state_of), or null; prefetched: point 0 of this element was
+    // requested by the previous element's last point
+    __device__ __forceinline__ void begin(const Phase<T> &ph, int knd, const J2State &js, int64_t slot,
+                                          const double *next, bool prefetched)
     {
         P    = &ph;
-        kind = ph.kind;
+        kind = knd;
 #pragma unroll
         for (int i = 0; i < 6; ++i)
             pr[i] = ph.p[i];
-        nel = js.nel;
-        sb  = (H == 3 && kind == MODEL_J2_NEW) ? js.st + el : nullptr;
-        fetch(0);
+        nel = js.nslot;
+        sb  = state_of(kind, js, slot);
+        sn  = next;
+        if (!prefetched)
+            fetch(0);
     }
-    __device__ __forceinline__ void fetch(int gp)
+    __device__ __forceinline__ void init(const Phase<T> &ph, const J2State &js, int64_t slot)
     {
-        if (sb)
+        begin(ph, ph.kind, js, slot, nullptr, false);
+    }
+    __device__ __forceinline__ static void load(const double *b, int gp, int64_t nel, double (&v)[7])
+    {
+        if (b)
 #pragma unroll
             for (int c = 0; c < 7; ++c)
-                nx[c] = sb[((int64_t) c * NGP + gp) * nel];
+                v[c] = b[((int64_t) c * NGP + gp) * nel];
     }
+    __device__ __forceinline__ void fetch(int gp) { load(sb, gp, nel, nx); }
     __device__ __forceinline__ void eval(int gp, const T (&e)[H == 1 ? 3 : 6], T (&s)[H == 1 ? 3 : 6])
     {
         if constexpr (H == 1) {
@@ -295,6 +326,8 @@ struct PointModel {
                 cur[c] = nx[c];
             if (gp + 1 < NGP)
                 fetch(gp + 1);
+            else
+                load(sn, 0, nel, nx);
             switch (kind) {
             case MODEL_ELASTIC_ISO: {
                 const T b1 = pr[0] * (e[0] + e[1] + e[2]), b2 = (T) 2 * pr[1];
@@ -338,13 +371,12 @@ struct PointModel {
 
 // material_point for any strain vector length (finite strain: NS = 9)
 template <typename T, int H, int NS>
-__device__ __forceinline__ void material_point_ns(const Phase<T> &P, const T (&e)[NS], T (&s)[NS], const J2State &js,
-                                                  int64_t el, int gp, int ngp)
+__device__ __forceinline__ void material_point_ns(const Phase<T> &P, const T (&e)[NS], T (&s)[NS])
 {
     if constexpr (NS == 9)
         large_point<T>(P.kind, P.p, e, s);
     else
-        material_point<T, H>(P, e, s, js, el, gp, ngp);
+        material_point<T, H>(P, e, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1019,7 +1051,8 @@ __device__ __forceinline__ float dpp_row_shr1(float v)
 }
 
 // ---------------------------------------------------------------------------
-// element-local pass (one thread per element): homogenized stress, fields,
+// element-local pass (one thread per element, or per J2 state slot for the
+// state commit): homogenized stress, fields,
 // committing the J2 state.  Reads nodal u directly (no relative gather).
 // ---------------------------------------------------------------------------
 // ELEM_SENS: per-element integrals of eps_a^T dC/dp eps_b for the isotropic
@@ -1031,6 +1064,7 @@ enum { ELEM_HOMOG = 0, ELEM_FIELDS = 1, ELEM_COMMIT = 2, ELEM_SENS = 3 };
 template <typename T>
 struct ElemArgs {
     int             nx, ny, nz;
+    int             ntz;           // z tiles of the state layout (J2State)
     Geom            geom;
     const uint8_t  *ms;
     const Phase<T> *phases;
@@ -1068,18 +1102,38 @@ __device__ __forceinline__ void voxel_iso_stress(T K, T G, const T (&e)[H == 1 ?
     }
 }
 
+// The state commit walks the slots of the J2 state layout (J2State) and
+// evaluates every slot (an element with two slots twice, with the same
+// result); every other mode walks the elements and reads their canonical slots.
 template <typename T, int H, int NGP, int MODE, int NS = (H == 1 ? 3 : 6)>
 __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
 {
     __shared__ double red[NS * 4];
-    const int64_t     nel = (int64_t) a.nx * a.ny * a.nz;
-    const int64_t     el  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t     tid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     double            sacc[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k)
         sacc[k] = 0.0;
-    if (el < nel) {
-        const int z = (int) (el % a.nz), y = (int) ((el / a.nz) % a.ny), x = (int) (el / ((int64_t) a.nz * a.ny));
+    int     x = 0, y = 0, z = 0;
+    int64_t slot   = tid;
+    bool    active = false;
+    if constexpr (MODE == ELEM_COMMIT) {
+        bool canonical;
+        active = tid < a.js.nslot;
+        if (active)
+            state_slot_element(tid, a.ny, a.nz, a.ntz, x, y, z, canonical);
+    } else {
+        active = tid < (int64_t) a.nx * a.ny * a.nz;
+        if (active) {
+            z                = (int) (tid % a.nz);
+            const int64_t xy = tid / a.nz;
+            y                = (int) (xy % a.ny);
+            x                = (int) (xy / a.ny);
+            slot             = (xy * a.ntz + z / kStateTile) * 16 + z % kStateTile + 1;
+        }
+    }
+    if (active) {
+        const int64_t el = ((int64_t) x * a.ny + y) * a.nz + z;
         const int x1 = x + 1 == a.nx ? 0 : x + 1, y1 = y + 1 == a.ny ? 0 : y + 1, z1 = z + 1 == a.nz ? 0 : z + 1;
         T         u[8][H];
 #pragma unroll
@@ -1163,7 +1217,7 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
         }
         PointModel<T, H, NGP> pm;
         if constexpr (MODE != ELEM_COMMIT && NS != 9)
-            pm.init(P, a.js, el);
+            pm.init(P, a.js, slot);
         double eavg[NS], savg[NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k)
@@ -1189,21 +1243,21 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
 #pragma unroll
                     for (int c = 0; c < 6; ++c) {
                         ed[c]   = (double) e[c];
-                        epin[c] = a.js.st[((int64_t) c * NGP + gp) * a.js.nel + el];
+                        epin[c] = a.js.st[((int64_t) c * NGP + gp) * a.js.nslot + slot];
                     }
-                    const double qin = a.js.st[((int64_t) 6 * NGP + gp) * a.js.nel + el];
+                    const double qin = a.js.st[((int64_t) 6 * NGP + gp) * a.js.nslot + slot];
 #pragma unroll
                     for (int c = 0; c < 4; ++c)
                         pd[c] = (double) P.p[c];
                     j2_point(pd, ed, epin, qin, sd, epo, qo);
 #pragma unroll
                     for (int c = 0; c < 6; ++c)
-                        a.js.st[((int64_t) c * NGP + gp) * a.js.nel + el] = epo[c];
-                    a.js.st[((int64_t) 6 * NGP + gp) * a.js.nel + el] = qo;
+                        a.js.st[((int64_t) c * NGP + gp) * a.js.nslot + slot] = epo[c];
+                    a.js.st[((int64_t) 6 * NGP + gp) * a.js.nslot + slot] = qo;
                 }
             } else {
                 if constexpr (NS == 9)
-                    material_point_ns<T, H, NS>(P, e, s, a.js, el, gp, NGP);
+                    material_point_ns<T, H, NS>(P, e, s);
                 else if (a.vk)
                     voxel_iso_stress<T, H>(vK, vG, e, s);
                 else

@@ -211,6 +211,7 @@ struct fans_gpu_ctx {
 
     // materials
     bool                        have_ms = false, have_mat = false;
+    int                         ms_max  = -1;  // largest phase id of the microstructure
     int                         nphases = 0;
     std::vector<int>            phase_kind;
     bool                        all_linear = true, has_j2 = false, all_iso = false;
@@ -251,6 +252,8 @@ struct fans_gpu_ctx {
 
     // launch geometry
     int     cx = 16, nty = 0, ntz = 0, nblk_asm = 0, nblk_vec = 0, nblk_el = 0;
+    int64_t nslot = 0;      // J2 state slots per component and Gauss point
+    int     nblk_slot = 0;  // workgroups of the state commit (one thread per slot)
     double  g0[9]{};
     Geom    geom{};
 
@@ -640,7 +643,7 @@ AsmArgs<T> asm_args(fans_gpu_ctx *c)
     a.geom    = c->geom;
     a.ms      = c->ms;
     a.nphases = c->nphases;
-    a.js      = J2State{c->js, c->N};
+    a.js      = J2State{c->js, c->nslot};
     a.partial = c->partial;
     a.measure = c->d.measure;
     a.done    = nullptr;
@@ -770,10 +773,11 @@ int homog_stress(fans_gpu_ctx *c, double *out)
     a.nx      = c->nx;
     a.ny      = c->ny;
     a.nz      = c->nz;
+    a.ntz     = c->ntz;
     a.geom    = c->geom;
     a.ms      = c->ms;
     a.phases  = c->dph64;
-    a.js      = J2State{c->js, c->N};
+    a.js      = J2State{c->js, c->nslot};
     a.u       = c->u;
     a.g0      = c->g0d;
     a.vk      = c->voxel_moduli ? c->vk64 : nullptr;
@@ -825,20 +829,21 @@ int commit_state(fans_gpu_ctx *c)
     a.nx     = c->nx;
     a.ny     = c->ny;
     a.nz     = c->nz;
+    a.ntz    = c->ntz;
     a.geom   = c->geom;
     a.ms     = c->ms;
     a.phases = c->dph64;
-    a.js     = J2State{c->js, c->N};
+    a.js     = J2State{c->js, c->nslot};
     a.u      = c->u;
     a.g0     = c->g0d;
     a.vk      = c->voxel_moduli ? c->vk64 : nullptr;
     a.vg      = c->voxel_moduli ? c->vg64 : nullptr;
     Timed t(c, KF_ELEMENT);
     if (c->ngp == 8)
-        hipLaunchKernelGGL((k_element_pass<double, 3, 8, ELEM_COMMIT>), dim3(c->nblk_el), dim3(kThreads), 0,
+        hipLaunchKernelGGL((k_element_pass<double, 3, 8, ELEM_COMMIT>), dim3(c->nblk_slot), dim3(kThreads), 0,
                            c->stream, a);
     else
-        hipLaunchKernelGGL((k_element_pass<double, 3, 1, ELEM_COMMIT>), dim3(c->nblk_el), dim3(kThreads), 0,
+        hipLaunchKernelGGL((k_element_pass<double, 3, 1, ELEM_COMMIT>), dim3(c->nblk_slot), dim3(kThreads), 0,
                            c->stream, a);
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -1588,6 +1593,9 @@ int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *d)
     c->nblk_asm = nbx * c->nty * c->ntz;
     c->nblk_vec = grid_vec(c->Nh);
     c->nblk_el  = (int) ((c->N + kThreads - 1) / kThreads);
+    // J2 state slots (J2State); the state commit walks them
+    c->nslot    = (int64_t) c->nx * c->ny * c->ntz * 16;
+    c->nblk_slot = (int) ((c->nslot + kThreads - 1) / kThreads);
     int rc;
     if ((rc = build_fft<double>(c, c->fft64)))
         return rc;
@@ -1617,10 +1625,12 @@ int fans_gpu_set_microstructure(fans_gpu_ctx *c, const uint16_t *ms, int64_t cou
     if (!c || !ms || count != c->N)
         return fail(c, FANS_GPU_EINVAL, "microstructure must hold n_x*n_y*n_z entries");
     std::vector<uint8_t> m8(c->N);
+    c->ms_max = -1;
     for (int64_t i = 0; i < c->N; ++i) {
         if (ms[i] > 255)
             return fail(c, FANS_GPU_EUNSUP, "phase ids above 255 are not supported");
-        m8[i] = (uint8_t) ms[i];
+        m8[i]     = (uint8_t) ms[i];
+        c->ms_max = std::max(c->ms_max, (int) ms[i]);
     }
     HIPCHK(c, copy_on(c, c->ms, m8.data(), c->N, hipMemcpyHostToDevice));
     c->have_ms = true;
@@ -1841,7 +1851,7 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
     HIPCHK(c, copy_on(c, c->dph32, c->ph32.data(), sizeof(Phase<float>) * c->nphases, hipMemcpyHostToDevice));
     // J2 state (zero, Solver ctor solver.h:139)
     if (c->has_j2 && !c->js) {
-        const size_t bytes = sizeof(double) * 7 * c->ngp * c->N;
+        const size_t bytes = sizeof(double) * 7 * c->ngp * c->nslot;
         HIPCHK(c, hipMalloc(&c->js, bytes));
         HIPCHK(c, fill_on(c, c->js, bytes));
     }
@@ -1963,12 +1973,24 @@ int fans_gpu_disable_mixed_bc(fans_gpu_ctx *c)
     return 0;
 }
 
+// microstructure and materials set, and every phase id of the microstructure
+// has a material (MaterialManager.h:104-107): the kernels index the phase
+// table with the ids unchecked
+int check_ready(fans_gpu_ctx *c)
+{
+    if (!c->have_ms || !c->have_mat)
+        return fail(c, FANS_GPU_ESTATE, "set_microstructure and set_materials must precede this call");
+    if (c->ms_max >= c->nphases)
+        return fail(c, FANS_GPU_EINVAL, "MaterialManager: Phase %d not assigned", c->nphases);
+    return 0;
+}
+
 int fans_gpu_solve(fans_gpu_ctx *c, fans_gpu_stats *st)
 {
     if (!c)
         return FANS_GPU_EINVAL;
-    if (!c->have_ms || !c->have_mat)
-        return fail(c, FANS_GPU_ESTATE, "set_microstructure and set_materials must precede solve");
+    if (int rc = check_ready(c))
+        return rc;
     for (int k = 0; k < KF_COUNT; ++k) {
         c->fam_ms[k] = 0;
         c->fam_n[k]  = 0;
@@ -2029,8 +2051,8 @@ int fans_gpu_homogenized_stress(fans_gpu_ctx *c, double *out, int32_t n)
 {
     if (!c || !out || n != c->nstr)
         return FANS_GPU_EINVAL;
-    if (!c->have_ms || !c->have_mat)
-        return fail(c, FANS_GPU_ESTATE, "materials not set");
+    if (int rc = check_ready(c))
+        return rc;
     return homog_stress(c, out);
 }
 
@@ -2038,6 +2060,8 @@ int fans_gpu_homogenized_tangent(fans_gpu_ctx *c, double pert, double *out, int3
 {
     if (!c || !out || n != c->nstr)
         return FANS_GPU_EINVAL;
+    if (int rc0 = check_ready(c))
+        return rc0;
     const int ns = c->nstr;
     double    unpert[9], g0[9], pe[9], ps[9], C[81];
     int       rc = homog_stress(c, unpert);
@@ -2087,7 +2111,7 @@ int fans_gpu_reset_state(fans_gpu_ctx *c)
     if ((rc = fill<double>(c, c->u, 0.0, c->Nh)) || (rc = fill<double>(c, c->u_prev, 0.0, c->Nh)) ||
         (rc = fill<double>(c, c->r, 0.0, c->Nh)))
         return rc;
-    if (c->js && (rc = fill<double>(c, c->js, 0.0, (int64_t) 7 * c->ngp * c->N)))
+    if (c->js && (rc = fill<double>(c, c->js, 0.0, (int64_t) 7 * c->ngp * c->nslot)))
         return rc;
     c->mixed_active = false;
     std::memset(c->gvec, 0, sizeof(c->gvec));
@@ -2104,8 +2128,8 @@ int fans_gpu_postprocess(fans_gpu_ctx *c, double *strain_elem, double *stress_el
 {
     if (!c)
         return FANS_GPU_EINVAL;
-    if (!c->have_ms || !c->have_mat)
-        return fail(c, FANS_GPU_ESTATE, "materials not set");
+    if (int rc = check_ready(c))
+        return rc;
     const int ns = c->nstr, ng = c->ngp;
     double   *de = nullptr, *ds = nullptr, *dge = nullptr, *dgs = nullptr;
     HIPCHK(c, hipMalloc(&de, sizeof(double) * c->N * ns));
@@ -2118,10 +2142,11 @@ int fans_gpu_postprocess(fans_gpu_ctx *c, double *strain_elem, double *stress_el
     a.nx          = c->nx;
     a.ny          = c->ny;
     a.nz          = c->nz;
+    a.ntz         = c->ntz;
     a.geom        = c->geom;
     a.ms          = c->ms;
     a.phases      = c->dph64;
-    a.js          = J2State{c->js, c->N};
+    a.js          = J2State{c->js, c->nslot};
     a.u           = c->u;
     a.g0          = c->g0d;
     a.vk      = c->voxel_moduli ? c->vk64 : nullptr;
@@ -2232,16 +2257,21 @@ int fans_gpu_get_field(fans_gpu_ctx *c, int32_t which, double *host, int64_t cou
             std::memset(host, 0, sizeof(double) * n);
             return 0;
         }
-        std::vector<double> st((size_t) 7 * ng * c->N);
+        std::vector<double> st((size_t) 7 * ng * c->nslot);
         HIPCHK(c, copy_on(c, st.data(), c->js, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
-        for (int64_t e = 0; e < c->N; ++e)
+        // element (x, y, z) from its canonical slot: tile z / kStateTile, position z % kStateTile + 1
+        for (int64_t e = 0; e < c->N; ++e) {
+            const int     z  = (int) (e % c->nz);
+            const int64_t xy = e / c->nz;
+            const int64_t sl = (xy * c->ntz + z / kStateTile) * 16 + z % kStateTile + 1;
             for (int g = 0; g < ng; ++g) {
                 if (which == FANS_FIELD_PLASTIC_STRAIN)
                     for (int k = 0; k < 6; ++k)
-                        host[(e * ng + g) * 6 + k] = st[((size_t) k * ng + g) * c->N + e];
+                        host[(e * ng + g) * 6 + k] = st[((size_t) k * ng + g) * c->nslot + sl];
                 else
-                    host[e * ng + g] = st[((size_t) 6 * ng + g) * c->N + e];
+                    host[e * ng + g] = st[((size_t) 6 * ng + g) * c->nslot + sl];
             }
+        }
         return 0;
     } else
         return fail(c, FANS_GPU_EINVAL, "unknown field %d", which);
@@ -2268,8 +2298,8 @@ int fans_gpu_residual(fans_gpu_ctx *c, const double *u, double *r, int32_t linea
 {
     if (!c || !u || !r)
         return FANS_GPU_EINVAL;
-    if (!c->have_ms || !c->have_mat)
-        return fail(c, FANS_GPU_ESTATE, "materials not set");
+    if (int rc = check_ready(c))
+        return rc;
     if (linear_kd && !c->all_linear)
         return fail(c, FANS_GPU_EINVAL, "linear matvec needs linear materials");
     HIPCHK(c, copy_on(c, c->dB, u, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
@@ -2409,6 +2439,8 @@ int fans_gpu_sensitivity(fans_gpu_ctx *c, const double *ub, const double *g0b, d
 {
     if (!c || !ub || !g0b || !sens)
         return FANS_GPU_EINVAL;
+    if (int rc = check_ready(c))
+        return rc;
     const int nq = c->h == 1 ? 1 : 2;
     if (count != c->N * nq)
         return fail(c, FANS_GPU_EINVAL, "sensitivity: expected %lld values", (long long) (c->N * nq));
@@ -2424,9 +2456,11 @@ int fans_gpu_sensitivity(fans_gpu_ctx *c, const double *ub, const double *g0b, d
     a.nx     = c->nx;
     a.ny     = c->ny;
     a.nz     = c->nz;
+    a.ntz    = c->ntz;
     a.geom   = c->geom;
     a.ms     = c->ms;
     a.phases = c->dph64;
+    a.js     = J2State{c->js, c->nslot};
     a.u      = c->u;
     a.g0     = c->g0d;
     a.ub     = c->ub_d;

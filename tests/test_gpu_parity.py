@@ -240,6 +240,24 @@ def test_homogenized_tangent_thermal():
     assert np.allclose(Cg, Cg.T)
 
 
+def test_unassigned_phase_id_is_rejected():
+    """A microstructure id without a material fails like MaterialManager.h:104-107
+    before any kernel indexes the phase table with it."""
+    import ctypes as C
+    p = cases.elastic(n=8, seed=1, count=2, rmin=2, rmax=3)
+    say("unassigned phase id 8^3")
+    g = gpu.GpuSolver(p)
+    ms = np.ascontiguousarray(p.ms, dtype=np.uint16).reshape(-1).copy()
+    ms[5] = 2  # phases 0 and 1 have materials
+    g._check(g.L.fans_gpu_set_microstructure(g.ptr, ms.ctypes.data_as(C.POINTER(C.c_uint16)), ms.size))
+    g.set_gradient(np.array([0.01, 0, 0, 0, 0, 0]))
+    with pytest.raises(gpu.FansGpuError, match="not assigned"):
+        g.solve()
+    with pytest.raises(gpu.FansGpuError, match="not assigned"):
+        g.homogenized_stress()
+    g.close()
+
+
 def test_mixed_precision_elastic():
     p = cases.elastic(n=32, seed=5, count=10, rmin=3, rmax=6)
     say("mixed precision elastic 32^3")
