@@ -152,12 +152,17 @@ __device__ __forceinline__ T z_load(T *__restrict__ in, int64_t gi, const ProArg
                                     double &a1)
 {
     if constexpr (PRO == PRO_CG) {
-        const T rv = in[gi] - alpha * pa.rnew[gi];
-        in[gi]     = rv;
+        // r, rnew, s (and FP32 u, d) are not read again before the spectrum
+        // passes that follow: streaming (non-temporal) accesses keep them from
+        // displacing the freshly written spectrum in the Infinity Cache
+        const T rv = __builtin_nontemporal_load(&in[gi]) - alpha * __builtin_nontemporal_load(&pa.rnew[gi]);
+        __builtin_nontemporal_store(rv, &in[gi]);
         if constexpr (!cg_update_u_in_matvec<T>())
-            pa.u[gi] -= alpha * pa.d[gi];
+            __builtin_nontemporal_store(__builtin_nontemporal_load(&pa.u[gi]) -
+                                            alpha * __builtin_nontemporal_load(&pa.d[gi]),
+                                        &pa.u[gi]);
         acc_norm(a0, rv, pa.measure);
-        a1 += (double) rv * (double) pa.s[gi];
+        a1 += (double) rv * (double) __builtin_nontemporal_load(&pa.s[gi]);
         return rv;
     } else {
         return in[gi];
@@ -260,9 +265,11 @@ __device__ __forceinline__ void z_epilogue(const EpiArgs<T> &ep, int64_t gi, T v
     if constexpr (EPI == EPI_STORE) {
         ep.out[gi] = val;
     } else if constexpr (EPI == EPI_CG) {
+        // s is read by the very next pass (the matvec) and stays cached; r is
+        // not read again this iteration
         const T sv = -val;
         ep.out[gi] = sv;
-        acc += (double) ep.r[gi] * (double) sv;
+        acc += (double) __builtin_nontemporal_load(&ep.r[gi]) * (double) sv;
     } else {
         ep.u[gi] -= (double) val;
     }
