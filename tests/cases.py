@@ -36,9 +36,9 @@ def elastic(n=16, seed=0, fe="HEX8", method="cg", tol=1e-10, n_it=300, measure="
                    tol=tol, n_it=n_it)
 
 
-def j2(n=16, seed=0, fe="HEX8", tol=1e-8, n_it=300, count=4, rmin=2.0, rmax=4.0):
+def j2(n=16, seed=0, fe="HEX8", tol=1e-8, n_it=300, count=4, rmin=2.0, rmax=4.0, dims=None):
     """Config-3 materials on a small grid: J2 matrix (E=70, nu=0.33) + elastic spheres (E=400, nu=0.2)."""
-    dims = (n, n, n)
+    dims = dims or (n, n, n)
     ms = random_spheres(dims, count, rmin, rmax, seed=seed)
     Km, Gm = 70.0 / (3 * (1 - 2 * 0.33)), 70.0 / (2 * 1.33)
     Ki, Gi = 400.0 / (3 * (1 - 0.4)), 400.0 / 2.4

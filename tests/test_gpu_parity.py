@@ -295,6 +295,30 @@ def test_j2_mixed_bc_steps():
     assert qo.max() > 0  # the matrix yielded
 
 
+def test_j2_state_across_tile_seams():
+    """J2 steps on a grid that the 15-node stencil tiles do not divide (two y
+    tiles, three z tiles, the last one partial): the state slots of elements at
+    tile seams and of wrapped elements must stay identical through the commits."""
+    p = cases.j2(dims=(10, 18, 32), seed=2, count=5, rmin=2.0, rmax=4.0)
+    mbc = cases.uniaxial_mixed(3, 0.02)
+    say("j2 mixed bc 10x18x32 3 steps")
+    o = oracle.Oracle(p)
+    g = gpu.GpuSolver(p)
+    for t in range(mbc["n_steps"]):
+        o.enable_mixed_bc(mbc, t)
+        g.enable_mixed_bc(mbc, t)
+        it_o, it_g = o.solve(), g.solve()
+        assert abs(it_o - it_g) <= 2, (t, it_o, it_g)
+        assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-8, t
+        o.extrapolate_displacement()
+        g.extrapolate_displacement()
+    epo, qo = o.get_state()
+    epg, qg = g.get_state()
+    assert qo.max() > 0  # the matrix yielded
+    assert relerr(qg, qo) < 1e-6
+    assert relerr(epg, epo) < 1e-6
+
+
 def test_j2_mixed_precision_steps():
     """Mixed form of the nonlinear path (FP32 Green preconditioner; FP64
     residual, J2 return map, dots, line search): the converged load steps
