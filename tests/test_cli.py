@@ -150,6 +150,48 @@ def test_cli_end_to_end(case_dir, sphere32_ms, name):
 
 
 @pytest.mark.gpu
+def test_cli_j2_steps(case_dir, sphere32_ms):
+    """test_J2Plasticity.json with its model swapped for J2PlasticityNew (the
+    reference input's J2ViscoPlastic_NonLinearIsotropicHardening is not built;
+    same moduli and yield stress, no hardening) and three of its strain steps
+    (0, 0.002, 0.004 in e11, the last two past yield): per-step P_bar and the
+    model's plastic_strain / isotropic_hardening_variable slabs
+    (J2PlasticityNew.h:155-208) against the oracle."""
+    _need_cli()
+    j0 = json.load(open(os.path.join(GOLDEN, "test_J2Plasticity.json")))
+    steps = [j0["macroscale_loading"][0][k] for k in (0, 20, 40)]
+    mats = [{"phases": [0], "matmodel": "J2PlasticityNew_LinearIsotropicHardening",
+             "material_properties": {"bulk_modulus": [62.5], "shear_modulus": [28.8462], "yield_stress": [0.1],
+                                     "isotropic_hardening_parameter": [0.0]}},
+            j0["materials"][1]]
+    results = ["stress_average", "strain_average", "plastic_strain", "isotropic_hardening_variable"]
+    path, j = case_dir("test_J2Plasticity", macroscale_loading=[steps], materials=mats, results=results)
+    out = path + ".h5"
+    print("[gpu-case] cli j2 3 steps", flush=True)
+    r = _run(path, out)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    p = Problem.from_json(path, ms=sphere32_ms)
+    o = oracle.Oracle(p)
+    for t, g in enumerate(steps):
+        o.set_gradient(g)
+        o.solve()
+        s_avg, _ = _res(out, j, "stress_average", step=t)
+        so = o.homogenized_stress()
+        assert np.max(np.abs(s_avg - so)) <= 1e-8 * np.max(np.abs(so)), t
+        ep, q = o.get_state()
+        pe, perm = _res(out, j, "plastic_strain", step=t)
+        assert perm == "zyx" and pe.shape == (32, 32, 32, 6)
+        pq, _ = _res(out, j, "isotropic_hardening_variable", step=t)
+        ref_e = ep.mean(axis=1).reshape(32, 32, 32, 6).transpose(2, 1, 0, 3)
+        ref_q = q.mean(axis=1).reshape(32, 32, 32, 1).transpose(2, 1, 0, 3)
+        if t > 0:
+            assert q.max() > 0, t  # past yield
+        assert np.max(np.abs(pe - ref_e)) <= 1e-6 * max(np.max(np.abs(ref_e)), 1e-12), t
+        assert np.max(np.abs(pq - ref_q)) <= 1e-6 * max(np.max(np.abs(ref_q)), 1e-12), t
+        o.extrapolate_displacement()
+
+
+@pytest.mark.gpu
 def test_cli_large_strain(case_dir, sphere32_ms):
     """test_CompressibleNeoHookean.json (finite strain, n_str = 9) cut to its
     first two shear steps: F_bar and P_bar per step against the oracle, the
