@@ -192,6 +192,35 @@ def test_cli_j2_steps(case_dir, sphere32_ms):
 
 
 @pytest.mark.gpu
+def test_cli_mixed_bc_golden(case_dir, sphere32_ms):
+    """test_MixedBCs.json (PseudoPlastic matrix, HEX8) load cases 2 and 3 through the
+    CLI: the stress-controlled hydrostatic steps reach the macro strains the
+    reference stored as load case 3 (1e-8 relative) at the prescribed stress,
+    and load case 3 driven by those strains gives that stress back."""
+    _need_cli()
+    j0 = json.load(open(os.path.join(GOLDEN, "test_MixedBCs.json")))
+    ml = j0["macroscale_loading"]
+    path, j = case_dir("test_MixedBCs", macroscale_loading=[ml[1], ml[2]],
+                       results=["stress_average", "strain_average", "absolute_error"])
+    out = path + ".h5"
+    print("[gpu-case] cli mixed bc golden", flush=True)
+    r = _run(path, out)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    golden = np.array(ml[2])
+    for t in range(len(golden)):
+        target = np.array(ml[1]["stress"][t])
+        e_avg, _ = _res(out, j, "strain_average", load=0, step=t)
+        s_avg, _ = _res(out, j, "stress_average", load=0, step=t)
+        assert np.max(np.abs(e_avg[:3] - golden[t][:3])) < 1e-8 * np.abs(golden[t][:3]).max(), (t, e_avg)
+        assert np.max(np.abs(e_avg[3:])) < 1e-12
+        assert np.max(np.abs(s_avg - target)) < 1e-9, (t, s_avg)
+        e1, _ = _res(out, j, "strain_average", load=1, step=t)
+        s1, _ = _res(out, j, "stress_average", load=1, step=t)
+        assert np.allclose(e1, golden[t], rtol=0, atol=1e-15)
+        assert np.max(np.abs(s1 - target)) < 1e-7, (t, s1)
+
+
+@pytest.mark.gpu
 def test_cli_large_strain(case_dir, sphere32_ms):
     """test_CompressibleNeoHookean.json (finite strain, n_str = 9) cut to its
     first two shear steps: F_bar and P_bar per step against the oracle, the
