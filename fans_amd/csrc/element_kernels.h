@@ -24,7 +24,8 @@ enum {
     MODEL_J2_NEW         = 5,
     MODEL_PSEUDO_PLASTIC = 6,
     MODEL_SVK            = 7,  // SaintVenantKirchhoff (finite strain)
-    MODEL_NEO_HOOKE      = 8   // CompressibleNeoHookean (finite strain)
+    MODEL_NEO_HOOKE      = 8,  // CompressibleNeoHookean (finite strain)
+    MODEL_PSEUDO_NL      = 9   // PseudoPlasticNonLinearHardening
 };
 
 // Strain / stress vector length: thermal gradient (3), small strain Mandel (6)
@@ -40,6 +41,7 @@ constexpr int nstr_of()
 //   ELASTIC_ISO  lambda mu              ELASTIC_TRI  C (6x6 row major)
 //   J2_NEW       K G sigma_y H_iso      PSEUDO       K G sigma_y h eps_crit E_s
 //   SVK          lambda mu              NEO_HOOKE    lambda mu
+//   PSEUDO_NL    K G sigma_y n eps_crit eps_0
 template <typename T>
 struct Phase {
     int kind;
@@ -125,6 +127,56 @@ __device__ __forceinline__ void pseudo_point(const T *p, const T (&e)[6], T (&s)
 #pragma unroll
     for (int i = 3; i < 6; ++i)
         s[i] = buf2 * dev[i];
+}
+
+// PseudoPlasticNonLinearHardening::get_sigma (PseudoPlastic.h:143-168):
+// sigma = K tr(eps) 1 + 2 G dev(eps) up to eps_eq = sqrt(2/3) |dev eps| =
+// eps_crit, beyond it sigma_y (eps_eq / eps_0)^n along dev / |dev| (scaled by
+// sqrt(2/3), Mandel)
+template <typename T>
+__device__ __forceinline__ void pseudo_nl_point(const T *p, const T (&e)[6], T (&s)[6])
+{
+    const T tr = e[0] + e[1] + e[2];
+    T       dev[6];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        dev[i] = e[i] - (T) (1.0 / 3.0) * tr;
+#pragma unroll
+    for (int i = 3; i < 6; ++i)
+        dev[i] = e[i];
+    T nd = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+        nd += dev[i] * dev[i];
+    nd            = sqrt(nd);
+    const T c23   = (T) 0.81649658092772603273;  // sqrt(2/3)
+    const T eq    = c23 * nd;
+    const T buf1  = p[0] * tr;
+    const T buf2  = eq <= p[4] ? (T) 2.0 * p[1] : c23 * p[2] * pow(eq / p[5], p[3]) / nd;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        s[i] = buf1 + buf2 * dev[i];
+#pragma unroll
+    for (int i = 3; i < 6; ++i)
+        s[i] = buf2 * dev[i];
+}
+
+// PseudoPlastic flag of one Gauss point (PseudoPlastic.h:103-110, :153-167):
+// the phase's index in its material group (p[6]), plus the group's phase count
+// (p[7]) where the point is plastic, decided as in pseudo_point / pseudo_nl_point
+template <typename T>
+__device__ __forceinline__ double pseudo_flag(const Phase<T> &P, const T (&e)[6])
+{
+    const T tr = e[0] + e[1] + e[2];
+    T       nd = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const T d = i < 3 ? e[i] - (T) (1.0 / 3.0) * tr : e[i];
+        nd += d * d;
+    }
+    nd                 = sqrt(nd);
+    const bool elastic = P.kind == MODEL_PSEUDO_PLASTIC ? nd <= P.p[4] : (T) 0.81649658092772603273 * nd <= P.p[4];
+    return (double) P.p[6] + (elastic ? 0.0 : (double) P.p[7]);
 }
 
 // First Piola-Kirchhoff stress P = F S(F) of the finite strain models
@@ -252,6 +304,7 @@ __device__ __forceinline__ void material_point(const Phase<T> &P, const T (&e)[H
             for (int i = 0; i < 6; ++i)
                 s[i] = (T) NAN;
             break;
+        case MODEL_PSEUDO_NL: pseudo_nl_point<T>(P.p, e, s); break;
         default: pseudo_point<T>(P.p, e, s); break;
         }
     }
@@ -363,6 +416,7 @@ struct PointModel {
                 for (int i = 0; i < 6; ++i)
                     s[i] = (T) sd[i];
             } break;
+            case MODEL_PSEUDO_NL: pseudo_nl_point<T>(pr, e, s); break;
             default: pseudo_point<T>(pr, e, s); break;
             }
         }
@@ -1076,6 +1130,7 @@ struct ElemArgs {
     double         *stress_elem;
     double         *strain_gp;
     double         *stress_gp;
+    double         *plastic_flag;  // FIELDS (nullable): element mean of the PseudoPlastic flags, else 0
     const T        *vk;            // per-voxel moduli (nullable): K (thermal: k) ...
     const T        *vg;            // ... and G, replacing the phase table's
     const T        *ub;            // SENS: second displacement field and gradient
@@ -1218,7 +1273,10 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
         PointModel<T, H, NGP> pm;
         if constexpr (MODE != ELEM_COMMIT && NS != 9)
             pm.init(P, a.js, slot);
-        double eavg[NS], savg[NS];
+        double eavg[NS], savg[NS], flag = 0.0;
+        bool   pseudo = false;
+        if constexpr (MODE == ELEM_FIELDS && H == 3 && NS == 6)
+            pseudo = a.plastic_flag && !a.vk && (P.kind == MODEL_PSEUDO_PLASTIC || P.kind == MODEL_PSEUDO_NL);
 #pragma unroll
         for (int k = 0; k < NS; ++k)
             eavg[k] = savg[k] = 0.0;
@@ -1268,6 +1326,9 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
                     savg[c] += (double) s[c] / NGP;
                 }
                 if constexpr (MODE == ELEM_FIELDS) {
+                    if constexpr (H == 3 && NS == 6)
+                        if (pseudo)
+                            flag += pseudo_flag<T>(P, e);
                     if (a.strain_gp)
 #pragma unroll
                         for (int c = 0; c < NS; ++c)
@@ -1280,6 +1341,8 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
             }
         }
         if constexpr (MODE == ELEM_FIELDS) {
+            if (a.plastic_flag)
+                a.plastic_flag[el] = pseudo ? flag / NGP : 0.0;
 #pragma unroll
             for (int c = 0; c < NS; ++c) {
                 if (a.strain_elem)

@@ -62,6 +62,7 @@ int nparams(int kind)
     case FANS_MODEL_ELASTIC_TRI: return 21;
     case FANS_MODEL_J2_NEW: return 4;
     case FANS_MODEL_PSEUDO_PLASTIC: return 4;
+    case FANS_MODEL_PSEUDO_PLASTIC_NL: return 5;
     case FANS_MODEL_SVK: return 2;
     case FANS_MODEL_NEO_HOOKE: return 2;
     default: return -1;
@@ -1776,7 +1777,8 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
                         C[i * 6 + j] = lr;
                 for (int i = 0; i < 6; ++i)
                     C[i * 6 + i] += 2 * mr;
-            } else if (kind[g] == FANS_MODEL_J2_NEW || kind[g] == FANS_MODEL_PSEUDO_PLASTIC) {
+            } else if (kind[g] == FANS_MODEL_J2_NEW || kind[g] == FANS_MODEL_PSEUDO_PLASTIC ||
+                       kind[g] == FANS_MODEL_PSEUDO_PLASTIC_NL) {
                 double Kb = 0, Gb = 0;
                 for (int i = 0; i < npg; ++i) {
                     Kb += gp[i * npk];
@@ -1832,6 +1834,16 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
                 P.p[i] = q[i];
             P.p[4] = std::sqrt(2. / 3.) * q[2] / (2. * q[1]);
             P.p[5] = (3. * q[1]) / (3. * q[1] + q[3]);
+            P.p[6] = local_of[p];           // plastic_flag: index in the group ...
+            P.p[7] = nphase[group_of[p]];   // ... and the group's phase count
+            break;
+        case FANS_MODEL_PSEUDO_PLASTIC_NL:  // eps_crit of PseudoPlastic.h:137-140
+            for (int i = 0; i < 4; ++i)
+                P.p[i] = q[i];
+            P.p[4] = q[4] * std::pow(q[2] / (3.0 * q[1] * q[4]), 1.0 / (1.0 - q[3]));
+            P.p[5] = q[4];
+            P.p[6] = local_of[p];
+            P.p[7] = nphase[group_of[p]];
             break;
         case FANS_MODEL_SVK:
         case FANS_MODEL_NEO_HOOKE:  // lambda = K - 2/3 G, mu = G (SaintVenantKirchhoff.h:26-31)
@@ -2272,6 +2284,43 @@ int fans_gpu_get_field(fans_gpu_ctx *c, int32_t which, double *host, int64_t cou
                     host[e * ng + g] = st[((size_t) 6 * ng + g) * c->nslot + sl];
             }
         }
+        return 0;
+    } else if (which == FANS_FIELD_PLASTIC_FLAG) {
+        if (count != c->N)
+            return fail(c, FANS_GPU_EINVAL, "field size mismatch (expected %lld)", (long long) c->N);
+        if (int rc = check_ready(c))
+            return rc;
+        if (c->h != 3 || c->nstr != 6) {
+            std::memset(host, 0, sizeof(double) * c->N);
+            return 0;
+        }
+        double *df = nullptr;
+        HIPCHK(c, hipMalloc(&df, sizeof(double) * c->N));
+        ElemArgs<double> a{};
+        a.nx           = c->nx;
+        a.ny           = c->ny;
+        a.nz           = c->nz;
+        a.ntz          = c->ntz;
+        a.geom         = c->geom;
+        a.ms           = c->ms;
+        a.phases       = c->dph64;
+        a.js           = J2State{c->js, c->nslot};
+        a.u            = c->u;
+        a.g0           = c->g0d;
+        a.vk           = c->voxel_moduli ? c->vk64 : nullptr;
+        a.vg           = c->voxel_moduli ? c->vg64 : nullptr;
+        a.plastic_flag = df;
+        if (c->ngp == 8)
+            hipLaunchKernelGGL((k_element_pass<double, 3, 8, ELEM_FIELDS>), dim3(c->nblk_el), dim3(kThreads), 0,
+                               c->stream, a);
+        else
+            hipLaunchKernelGGL((k_element_pass<double, 3, 1, ELEM_FIELDS>), dim3(c->nblk_el), dim3(kThreads), 0,
+                               c->stream, a);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = copy_on(c, host, df, sizeof(double) * c->N, hipMemcpyDeviceToHost);
+        (void) hipFree(df);
+        HIPCHK(c, e);
         return 0;
     } else
         return fail(c, FANS_GPU_EINVAL, "unknown field %d", which);

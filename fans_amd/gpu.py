@@ -22,7 +22,7 @@ METHOD = {"cg": 0, "fp": 1}
 MEASURE = {"L1": 0, "L2": 1, "Linfinity": 2}
 ERR_TYPE = {"absolute": 0, "relative": 1}
 PRECISION = {"fp64": 0, "mixed": 1}
-FIELD = {"u": 0, "r": 1, "u_prev": 2, "plastic_strain": 3, "hardening": 4}
+FIELD = {"u": 0, "r": 1, "u_prev": 2, "plastic_strain": 3, "hardening": 4, "plastic_flag": 5}
 OPT_TIMING = 1
 OPT_FFT_CHUNKS = 2
 OPT_GRAPHS = 3
@@ -276,6 +276,8 @@ class GpuSolver:
             out = np.zeros((self.N, self.ngp, 6))
         elif name == "hardening":
             out = np.zeros((self.N, self.ngp))
+        elif name == "plastic_flag":
+            out = np.zeros(self.N)
         else:
             out = np.zeros(self.dims + (self.h,))
         self._check(self.L.fans_gpu_get_field(self.ptr, FIELD[name], _dp(out), out.size))
@@ -293,6 +295,10 @@ class GpuSolver:
 
     def get_state(self):
         return self.get_field("plastic_strain"), self.get_field("hardening")
+
+    def plastic_flag(self):
+        """PseudoPlastic::postprocess plastic_flag at the current u (element means)."""
+        return self.get_field("plastic_flag")
 
     def element_fields(self, n_slots=None, gp=True):
         ns, ng = self.nstr, self.ngp

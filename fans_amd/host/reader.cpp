@@ -39,6 +39,8 @@ const std::map<std::string, std::vector<std::string>> &model_keys()
         k["J2PlasticityNew_LinearIsotropicHardening"] = {"bulk_modulus", "shear_modulus", "yield_stress",
                                                          "isotropic_hardening_parameter"};
         k["PseudoPlasticLinearHardening"] = {"bulk_modulus", "shear_modulus", "yield_stress", "hardening_parameter"};
+        k["PseudoPlasticNonLinearHardening"] = {"bulk_modulus", "shear_modulus", "yield_stress", "hardening_exponent",
+                                                "eps_0"};
         k["SaintVenantKirchhoff"]         = {"bulk_modulus", "shear_modulus"};
         k["CompressibleNeoHookean"]       = {"bulk_modulus", "shear_modulus"};
         return k;
@@ -418,6 +420,8 @@ template <>
 hid_t h5_type<int>() { return h5().t_int; }
 template <>
 hid_t h5_type<uint16_t>() { return h5().t_ushort; }
+template <>
+hid_t h5_type<float>() { return h5().t_float; }
 }  // namespace
 
 template <typename T>
@@ -471,5 +475,6 @@ void Reader::writeSlab(const char *field, int load_idx, int time_idx, const T *d
 template void Reader::writeSlab<double>(const char *, int, int, const double *, const std::vector<int> &) const;
 template void Reader::writeSlab<int>(const char *, int, int, const int *, const std::vector<int> &) const;
 template void Reader::writeSlab<uint16_t>(const char *, int, int, const uint16_t *, const std::vector<int> &) const;
+template void Reader::writeSlab<float>(const char *, int, int, const float *, const std::vector<int> &) const;
 
 }  // namespace fans_host

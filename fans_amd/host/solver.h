@@ -42,6 +42,8 @@ inline int model_kind(const std::string &m)
         return FANS_MODEL_J2_NEW;
     if (m == "PseudoPlasticLinearHardening")
         return FANS_MODEL_PSEUDO_PLASTIC;
+    if (m == "PseudoPlasticNonLinearHardening")
+        return FANS_MODEL_PSEUDO_PLASTIC_NL;
     if (m == "SaintVenantKirchhoff")
         return FANS_MODEL_SVK;
     if (m == "CompressibleNeoHookean")
@@ -71,6 +73,15 @@ class MaterialManager {
         for (const auto &g : reader_.materials)
             if (model_kind(g.matmodel) == FANS_MODEL_J2_NEW)
                 return true;
+        return false;
+    }
+    bool has_pseudo_plastic() const
+    {
+        for (const auto &g : reader_.materials) {
+            const int k = model_kind(g.matmodel);
+            if (k == FANS_MODEL_PSEUDO_PLASTIC || k == FANS_MODEL_PSEUDO_PLASTIC_NL)
+                return true;
+        }
         return false;
     }
 
@@ -310,6 +321,15 @@ void Solver<howmany, n_str>::postprocess(Reader &reader, int load_idx, int time_
             if (w_qg)
                 reader.writeSlab("isotropic_hardening_variable_gp", load_idx, time_idx, q.data(), {n_gp});
         }
+    }
+
+    // PseudoPlastic::postprocess (PseudoPlastic.h:56-64): element means of the
+    // Gauss-point plastic flags at the converged u, as float
+    if (howmany == 3 && n_str == 6 && mat_.has_pseudo_plastic() && reader.wants("plastic_flag")) {
+        std::vector<double> fl(N);
+        check(ctx_, fans_gpu_get_field(ctx_, FANS_FIELD_PLASTIC_FLAG, fl.data(), (int64_t) N), "get_field plastic_flag");
+        std::vector<float> ff(fl.begin(), fl.end());
+        reader.writeSlab("plastic_flag", load_idx, time_idx, ff.data(), {1});
     }
 
     if (reader.wants("homogenized_tangent")) {

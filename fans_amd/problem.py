@@ -24,6 +24,8 @@ MODEL_PARAMS = {
     "J2PlasticityNew_LinearIsotropicHardening": ("bulk_modulus", "shear_modulus", "yield_stress",
                                                  "isotropic_hardening_parameter"),
     "PseudoPlasticLinearHardening": ("bulk_modulus", "shear_modulus", "yield_stress", "hardening_parameter"),
+    "PseudoPlasticNonLinearHardening": ("bulk_modulus", "shear_modulus", "yield_stress", "hardening_exponent",
+                                        "eps_0"),
     "SaintVenantKirchhoff": ("bulk_modulus", "shear_modulus"),
     "CompressibleNeoHookean": ("bulk_modulus", "shear_modulus"),
 }
@@ -36,6 +38,7 @@ MODEL_KIND = {
     "PseudoPlasticLinearHardening": 6,
     "SaintVenantKirchhoff": 7,
     "CompressibleNeoHookean": 8,
+    "PseudoPlasticNonLinearHardening": 9,
 }
 # createMatmodel<3,9> (include/setup.h:62-73): finite strain, F and P row major
 LARGE_STRAIN_MODELS = {"SaintVenantKirchhoff", "CompressibleNeoHookean"}

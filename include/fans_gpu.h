@@ -65,8 +65,11 @@ enum {
                                       [bulk_modulus shear_modulus yield_stress hardening_parameter] */
     FANS_MODEL_SVK            = 7, /* SaintVenantKirchhoff [bulk_modulus shear_modulus]
                                       (finite strain, setup.h:62-73)                       */
-    FANS_MODEL_NEO_HOOKE      = 8  /* CompressibleNeoHookean [bulk_modulus shear_modulus]
+    FANS_MODEL_NEO_HOOKE      = 8, /* CompressibleNeoHookean [bulk_modulus shear_modulus]
                                       (finite strain)                                      */
+    FANS_MODEL_PSEUDO_PLASTIC_NL = 9 /* PseudoPlasticNonLinearHardening
+                                      [bulk_modulus shear_modulus yield_stress
+                                       hardening_exponent eps_0]                          */
 };
 /* fields for fans_gpu_get_field / fans_gpu_set_field */
 enum {
@@ -74,7 +77,9 @@ enum {
     FANS_FIELD_R              = 1, /* residual v_r, double [N][h]                      */
     FANS_FIELD_U_PREV         = 2, /* v_u_prev, double [N][h]                          */
     FANS_FIELD_PLASTIC_STRAIN = 3, /* committed plastic strain, double [N][n_gp][6]    */
-    FANS_FIELD_HARDENING      = 4  /* committed isotropic hardening q, double [N][n_gp] */
+    FANS_FIELD_HARDENING      = 4, /* committed isotropic hardening q, double [N][n_gp] */
+    FANS_FIELD_PLASTIC_FLAG   = 5  /* PseudoPlastic::postprocess plastic_flag at the current
+                                      u (PseudoPlastic.h:56-64), double [N]; get only   */
 };
 
 typedef struct {

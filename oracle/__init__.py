@@ -30,6 +30,7 @@ MODEL = {
     "PseudoPlasticLinearHardening": 6,
     "SaintVenantKirchhoff": 7,
     "CompressibleNeoHookean": 8,
+    "PseudoPlasticNonLinearHardening": 9,
 }
 
 
@@ -72,6 +73,7 @@ def lib():
             getattr(L, f).argtypes = [vp, dp]
         L.or_get_state.argtypes = [vp, dp, dp, C.c_int]
         L.or_element_fields.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, ip, C.c_int]
+        L.or_plastic_flag.argtypes = [vp, dp]
         L.or_residual.argtypes = [vp, dp, dp, C.c_int]
         L.or_convolve.argtypes = [vp, dp, dp]
         L.or_get_phase_stiffness.argtypes = [vp, C.c_int, dp]
@@ -253,6 +255,12 @@ class Oracle:
                                  _dp(out["stress_gp"]), _dp(out["stress_average"]), _dp(out["strain_average"]),
                                  _dp(out["phase_stress_average"]), _dp(out["phase_strain_average"]),
                                  _ip(out["phase_counts"]), n_slots)
+        return out
+
+    def plastic_flag(self):
+        """PseudoPlastic::postprocess plastic_flag: element means of the Gauss-point flags."""
+        out = np.zeros(self.N)
+        self.L.or_plastic_flag(self.ptr, _dp(out))
         return out
 
     # --- single operators ---------------------------------------------------------

@@ -353,6 +353,7 @@ int or_model_nparams(int kind)
     case OR_MODEL_ELASTIC_TRI: return 21;
     case OR_MODEL_J2_NEW: return 4;
     case OR_MODEL_PSEUDO_PLASTIC: return 4;
+    case OR_MODEL_PSEUDO_NL: return 5;
     case OR_MODEL_SVK: return 2;
     case OR_MODEL_NEO_HOOKE: return 2;
     default: return -1;
@@ -582,7 +583,8 @@ static void linear_C(int n, int kind, const double *p, double *C)
     } else if (kind == OR_MODEL_THERMAL_TRI) {
         double K[9] = {p[0], p[1], p[2], p[1], p[3], p[4], p[2], p[4], p[5]};
         memcpy(C, K, sizeof(K));
-    } else if (kind == OR_MODEL_ELASTIC_ISO || kind == OR_MODEL_J2_NEW || kind == OR_MODEL_PSEUDO_PLASTIC) {
+    } else if (kind == OR_MODEL_ELASTIC_ISO || kind == OR_MODEL_J2_NEW || kind == OR_MODEL_PSEUDO_PLASTIC ||
+               kind == OR_MODEL_PSEUDO_NL) {
         double lam = p[0] - (2.0 / 3.0) * p[1], mu = p[1];
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++)
@@ -1079,6 +1081,32 @@ static void large_reference_stiffness(const or_group *g, double *C)
         C[k] /= g->nphase;
 }
 
+/* PseudoPlasticNonLinearHardening (PseudoPlastic.h:125-168): deformation
+ * plasticity with the power law sigma_eq = sigma_y (eps_eq / eps_0)^n beyond
+ * eps_crit = eps_0 (sigma_y / (3 G eps_0))^(1 / (1 - n)) (:137-140), where the
+ * elastic branch 3 G eps_eq meets it; eps_eq = sqrt(2/3) |dev eps| (:149) */
+static void pseudo_nl_point(const double *p, const double *e, double *s)
+{
+    const double K = p[0], G = p[1], sy = p[2], n = p[3], e0 = p[4];
+    const double eps_crit = e0 * pow(sy / (3.0 * G * e0), 1.0 / (1.0 - n));
+    const double tr       = e[0] + e[1] + e[2];
+    double       dev[6], nd = 0;
+    for (int i = 0; i < 3; i++)
+        dev[i] = e[i] - (1.0 / 3.0) * tr;
+    for (int i = 3; i < 6; i++)
+        dev[i] = e[i];
+    for (int i = 0; i < 6; i++)
+        nd += dev[i] * dev[i];
+    nd                = sqrt(nd);
+    const double eq   = sqrt(2.0 / 3.0) * nd;
+    const double buf1 = K * tr;
+    const double buf2 = eq <= eps_crit ? 2.0 * G : sqrt(2.0 / 3.0) * sy * pow(eq / e0, n) / nd;
+    for (int i = 0; i < 3; i++)
+        s[i] = buf1 + buf2 * dev[i];
+    for (int i = 3; i < 6; i++)
+        s[i] = buf2 * dev[i];
+}
+
 void or_material_point(int kind, const double *p, const double *e, const double *ep_t, double q_t, double *sigma,
                        double *ep, double *q)
 {
@@ -1090,6 +1118,8 @@ void or_material_point(int kind, const double *p, const double *e, const double 
         j2new_point(p, e, ep_t, q_t, sigma, ep, q);
     } else if (kind == OR_MODEL_PSEUDO_PLASTIC) {
         pseudo_point(p, e, sigma);
+    } else if (kind == OR_MODEL_PSEUDO_NL) {
+        pseudo_nl_point(p, e, sigma);
     } else if (kind == OR_MODEL_ELASTIC_ISO) {
         /* LinearElastic.h:163-173 */
         double lam = p[0] - (2.0 / 3.0) * p[1];
@@ -1587,6 +1617,55 @@ void or_get_state(or_solver *S, double *pe, double *q, int committed)
 }
 
 /* element pass of Solver::postprocess (solver.h:503-574) */
+/* PseudoPlastic::postprocess (PseudoPlastic.h:56-64) with the flags that
+ * get_sigma sets at every Gauss point (PseudoPlasticLinearHardening :103-110 on
+ * |dev eps| <= eps_crit, PseudoPlasticNonLinearHardening :153-167 on
+ * sqrt(2/3) |dev eps| <= eps_crit): the phase's index within its material group
+ * where elastic, that plus the group's phase count where plastic; elements of
+ * other models keep 0.  flag[e] is the mean over the element's Gauss points. */
+void or_plastic_flag(or_solver *S, double *flag)
+{
+    int       h = S->h, ns = S->nstr, ng = S->ngp;
+    ptrdiff_t N = S->N;
+    for (int ix = 0; ix < S->nx; ix++)
+        for (int iy = 0; iy < S->ny; iy++)
+            for (int iz = 0; iz < S->nz; iz++) {
+                ptrdiff_t idx[8];
+                double    ue[MAXE], eps[8 * MAXSTR], sig[8 * MAXSTR];
+                element_nodes(S, ix, iy, iz, idx);
+                int             phase = S->ms[idx[0]];
+                const or_group *G     = &S->groups[S->phase_group[phase]];
+                flag[idx[0]]          = 0.0;
+                if (h != 3 || (G->kind != OR_MODEL_PSEUDO_PLASTIC && G->kind != OR_MODEL_PSEUDO_NL))
+                    continue;
+                for (int a = 0; a < 8; a++) {
+                    ptrdiff_t n = idx[a] >= N ? idx[a] - N : idx[a];
+                    for (int c = 0; c < h; c++)
+                        ue[h * a + c] = S->u[h * n + c];
+                }
+                element_strain_stress(S, ue, phase, idx[0], eps, sig);
+                const int     loc = S->phase_local[phase];
+                const double *p   = G->params + (size_t) loc * G->np;
+                double        acc = 0;
+                for (int gp = 0; gp < ng; gp++) {
+                    const double *e  = eps + gp * ns;
+                    double        tr = e[0] + e[1] + e[2], nd = 0;
+                    for (int i = 0; i < 6; i++) {
+                        double d = i < 3 ? e[i] - tr / 3.0 : e[i];
+                        nd += d * d;
+                    }
+                    nd = sqrt(nd);
+                    int plastic;
+                    if (G->kind == OR_MODEL_PSEUDO_PLASTIC)
+                        plastic = !(nd <= sqrt(2. / 3.) * p[2] / (2. * p[1]));
+                    else
+                        plastic = !(sqrt(2.0 / 3.0) * nd <= p[4] * pow(p[2] / (3.0 * p[1] * p[4]), 1.0 / (1.0 - p[3])));
+                    acc += loc + (plastic ? G->nphase : 0);
+                }
+                flag[idx[0]] = acc / ng;
+            }
+}
+
 void or_element_fields(or_solver *S, double *strain_elem, double *stress_elem, double *strain_gp, double *stress_gp,
                        double *stress_avg, double *strain_avg, double *phase_stress_avg, double *phase_strain_avg,
                        int32_t *phase_counts, int n_slots)

@@ -9,7 +9,8 @@
  * algorithm of FANS 0.6.2:
  *   include/matmodel.h:104-253       B matrices, element residual, reference Ke
  *   include/material_models/ (*.h)  LinearThermal/LinearElastic (isotropic,
- *                                    triclinic), J2PlasticityNew, PseudoPlastic
+ *                                    triclinic), J2PlasticityNew, PseudoPlastic (linear and
+ *                                    power-law hardening)
  *                                    linear hardening
  *   include/MaterialManager.h:177-205 reference stiffness
  *   include/solver.h:144-204         Green operator (fundamental solution)
@@ -53,7 +54,9 @@ enum {
     OR_MODEL_J2_NEW          = 5, /* [bulk shear yield_stress iso_hardening]         */
     OR_MODEL_PSEUDO_PLASTIC  = 6, /* [bulk shear yield_stress hardening]             */
     OR_MODEL_SVK             = 7, /* SaintVenantKirchhoff [bulk shear] (large strain) */
-    OR_MODEL_NEO_HOOKE       = 8  /* CompressibleNeoHookean [bulk shear] (large strain) */
+    OR_MODEL_NEO_HOOKE       = 8, /* CompressibleNeoHookean [bulk shear] (large strain) */
+    OR_MODEL_PSEUDO_NL       = 9  /* PseudoPlasticNonLinearHardening
+                                     [bulk shear yield_stress hardening_exponent eps_0] */
 };
 
 typedef struct {
@@ -105,6 +108,9 @@ void        or_set_u(or_solver *S, const double *in);
 void        or_get_r(or_solver *S, double *out);
 void        or_get_state(or_solver *S, double *plastic_strain, double *q, int committed);
 
+/* PseudoPlastic::postprocess (PseudoPlastic.h:56-64): per element the mean of
+ * the Gauss-point plastic flags (0 for elements of other models) */
+void        or_plastic_flag(or_solver *S, double *flag);
 /* element pass of postprocess (solver.h:503-556): element averages and GP
  * values of strain/stress (any pointer may be NULL) + phase sums */
 void        or_element_fields(or_solver *S, double *strain_elem, double *stress_elem,

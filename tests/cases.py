@@ -80,6 +80,11 @@ def general(model, n=12, seed=0, fe="HEX8", tol=1e-10, n_it=300, dims=None, coun
         keys = [(i, j) for i in range(6) for j in range(i, 6)]
         props = {f"C_{i + 1}{j + 1}": [float(C[i, j]) for C in Cs] for i, j in keys}
         groups = [MaterialGroup(model, [0, 1], props)]
+    elif model == "PseudoPlasticNonLinearHardening":  # test_PseudoPlastic.json's materials
+        groups = [MaterialGroup(model, [0], {"bulk_modulus": [62.5], "shear_modulus": [28.8462],
+                                             "yield_stress": [0.1], "hardening_exponent": [0.2], "eps_0": [0.01]}),
+                  MaterialGroup("LinearElasticIsotropic", [1], {"bulk_modulus": [222.222],
+                                                                "shear_modulus": [166.6667]})]
     elif model == "PseudoPlasticLinearHardening":
         groups = [MaterialGroup(model, [0], {"bulk_modulus": [62.5], "shear_modulus": [28.8462],
                                              "yield_stress": [0.1], "hardening_parameter": [0.5]}),

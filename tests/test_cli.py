@@ -192,6 +192,41 @@ def test_cli_j2_steps(case_dir, sphere32_ms):
 
 
 @pytest.mark.gpu
+def test_cli_pseudo_plastic_input(case_dir, sphere32_ms):
+    """test_PseudoPlastic.json (PseudoPlasticNonLinearHardening matrix, BBAR)
+    cut to three of its 51 steps: per-step P_bar and the plastic_flag slab
+    (float, PseudoPlastic.h:56-64) against the oracle."""
+    _need_cli()
+    j0 = json.load(open(os.path.join(GOLDEN, "test_PseudoPlastic.json")))
+    steps = [j0["macroscale_loading"][0][k] for k in (0, 5, 20)]
+    path, j = case_dir("test_PseudoPlastic", macroscale_loading=[steps],
+                       results=["stress_average", "strain_average", "plastic_flag"])
+    out = path + ".h5"
+    print("[gpu-case] cli pseudo-plastic 3 steps", flush=True)
+    r = _run(path, out)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    p = Problem.from_json(path, ms=sphere32_ms)
+    assert p.fe_type == "BBAR"
+    o = oracle.Oracle(p)
+    for t, g in enumerate(steps):
+        o.set_gradient(g)
+        o.solve()
+        s_avg, _ = _res(out, j, "stress_average", step=t)
+        so = o.homogenized_stress()
+        if t == 0:
+            assert np.max(np.abs(s_avg)) == 0.0
+        else:
+            assert np.max(np.abs(s_avg - so)) <= 1e-8 * np.max(np.abs(so)), t
+        fl, perm = _res(out, j, "plastic_flag", step=t)
+        assert perm == "zyx" and fl.shape == (32, 32, 32, 1)
+        ref = o.plastic_flag().reshape(32, 32, 32, 1).transpose(2, 1, 0, 3).astype(np.float32).astype(np.float64)
+        assert np.array_equal(fl, ref), t
+        if t == 2:
+            assert ref.max() == 1.0  # plastic points in the matrix (one phase: flag 0 / 1)
+        o.extrapolate_displacement()
+
+
+@pytest.mark.gpu
 def test_cli_mixed_bc_golden(case_dir, sphere32_ms):
     """test_MixedBCs.json (PseudoPlastic matrix, HEX8) load cases 2 and 3 through the
     CLI: the stress-controlled hydrostatic steps reach the macro strains the

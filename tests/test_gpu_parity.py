@@ -74,7 +74,7 @@ def test_green_and_operators(maker, fe, dims):
 
 
 _GENERAL_CASES = [(m, fe, dims) for m in ("LinearThermalTriclinic", "LinearElasticTriclinic",
-                                          "PseudoPlasticLinearHardening", "J2")
+                                          "PseudoPlasticLinearHardening", "PseudoPlasticNonLinearHardening", "J2")
                   for fe in ("HEX8", "BBAR", "HEX8R") for dims in ((12, 10, 8), (40, 24, 16))
                   if not (fe == "BBAR" and m == "LinearThermalTriclinic")]
 
@@ -117,6 +117,32 @@ def test_pseudo_plastic_mixed_bc_golden(sphere32_ms):
         assert np.max(np.abs(mg[:3] - golden[t][:3])) < 1e-8 * np.abs(golden[t][:3]).max(), (t, mg, golden[t])
         assert np.max(np.abs(mg[3:])) < 1e-12
         assert np.max(np.abs(g.homogenized_stress() - np.array(lc.mbc["stress_path"][t]))) < 1e-9
+        g.extrapolate_displacement()
+    g.close()
+
+
+@pytest.mark.parametrize("fe", ["HEX8", "BBAR"])
+def test_pseudo_nonlinear_steps_and_flags(fe):
+    """PseudoPlasticNonLinearHardening matrix (test_PseudoPlastic.json's
+    materials) under its own uniaxial-strain path past the threshold: CG
+    iterations, homogenized stress and the plastic_flag element means
+    (PseudoPlastic.h:56-64) against the oracle."""
+    p = cases.general("PseudoPlasticNonLinearHardening", n=12, seed=4, fe=fe, tol=1e-10)
+    say(f"pseudo-plastic power law 12^3 {fe} 3 steps")
+    o = oracle.Oracle(p)
+    g = gpu.GpuSolver(p)
+    for t, a in enumerate((0.0005, 0.002, 0.004)):
+        g0 = np.array([a, -a / 2, -a / 2, 0, 0, 0])
+        o.set_gradient(g0)
+        g.set_gradient(g0)
+        it_o, it_g = o.solve(), g.solve()
+        assert abs(it_o - it_g) <= 2, (t, it_o, it_g)
+        assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-8, t
+        fo, fg = o.plastic_flag(), g.plastic_flag()
+        assert np.max(np.abs(fg - fo)) < 1e-12, t
+        if t == 2:
+            assert fo.max() > 0.5  # plastic Gauss points present
+        o.extrapolate_displacement()
         g.extrapolate_displacement()
     g.close()
 

@@ -417,6 +417,85 @@ def test_j2_uniaxial_strain_closed_form():
         assert abs(q.max() - np.sqrt(2 / 3) * gamma) < 1e-14
 
 
+# ----------------------------------------------------------------------------- power-law pseudo-plasticity
+PNL = dict(K=62.5, G=28.8462, sy=0.1, n=0.2, e0=0.01)  # test_PseudoPlastic.json's matrix
+
+
+def _pnl_params(K=PNL["K"], G=PNL["G"], sy=PNL["sy"], n=PNL["n"], e0=PNL["e0"]):
+    return np.array([K, G, sy, n, e0])
+
+
+def test_pseudo_nonlinear_power_law():
+    """PseudoPlasticNonLinearHardening (PseudoPlastic.h:125-168) as a deformation
+    theory: beyond its threshold the von Mises stress sqrt(3/2)|dev sigma|
+    equals sigma_y (eps_eq / eps_0)^n with eps_eq = sqrt(2/3)|dev eps|, the
+    pressure stays K tr(eps), and the stress deviator is parallel to the strain
+    deviator; below it the law is LinearElasticIsotropic with the same K, G."""
+    rng = np.random.default_rng(3)
+    K, G, sy, n, e0 = (PNL[k] for k in ("K", "G", "sy", "n", "e0"))
+    crit = e0 * (sy / (3 * G * e0)) ** (1 / (1 - n))
+    for scale in (0.2 * crit, 0.9 * crit, 1.5 * crit, 10 * crit, 100 * crit):
+        d = rng.standard_normal(6)
+        d[:3] -= d[:3].mean()
+        eq_dir = np.sqrt(2 / 3) * np.linalg.norm(d)
+        e = d * (scale / eq_dir) + 1e-3 * np.array([1, 1, 1, 0, 0, 0]) * rng.standard_normal()
+        s, _, _ = oracle.material_point("PseudoPlasticNonLinearHardening", _pnl_params(), e)
+        tr = e[:3].sum()
+        de = e - np.r_[np.full(3, tr / 3), np.zeros(3)]
+        ds = s - np.r_[np.full(3, s[:3].mean()), np.zeros(3)]
+        eps_eq = np.sqrt(2 / 3) * np.linalg.norm(de)
+        assert abs(s[:3].mean() - K * tr) < 1e-13 * max(abs(K * tr), 1.0)
+        if eps_eq <= crit:
+            se, _, _ = oracle.material_point("LinearElasticIsotropic", np.array([K, G]), e)
+            assert np.max(np.abs(s - se)) < 1e-13 * np.abs(se).max()
+        else:
+            assert abs(np.sqrt(1.5) * np.linalg.norm(ds) - sy * (eps_eq / e0) ** n) < 1e-12 * sy * (eps_eq / e0) ** n
+            assert np.max(np.abs(np.cross(ds[:3], de[:3]))) < 1e-12 * np.linalg.norm(ds) * np.linalg.norm(de)
+            assert np.max(np.abs(ds * np.linalg.norm(de) - de * np.linalg.norm(ds))) < 1e-12 * np.linalg.norm(ds) * np.linalg.norm(de)
+
+
+def test_pseudo_nonlinear_continuous_at_threshold():
+    """The threshold eps_crit = eps_0 (sigma_y / (3 G eps_0))^(1/(1-n))
+    (PseudoPlastic.h:137-140) is where the elastic 3 G eps_eq meets the power
+    law: the stress is continuous across it for several exponents."""
+    d = np.array([0.7, -0.2, -0.5, 0.3, -0.1, 0.25])
+    for n in (0.1, 0.2, 0.5):
+        K, G, sy, e0 = PNL["K"], PNL["G"], PNL["sy"], PNL["e0"]
+        crit = e0 * (sy / (3 * G * e0)) ** (1 / (1 - n))
+        u = d / (np.sqrt(2 / 3) * np.linalg.norm(d))
+        lo, _, _ = oracle.material_point("PseudoPlasticNonLinearHardening", _pnl_params(n=n), u * crit * (1 - 1e-12))
+        hi, _, _ = oracle.material_point("PseudoPlasticNonLinearHardening", _pnl_params(n=n), u * crit * (1 + 1e-12))
+        assert np.max(np.abs(hi - lo)) < 1e-9 * np.abs(lo).max(), n
+
+
+def test_pseudo_nonlinear_plastic_flag():
+    """PseudoPlastic::postprocess flags (PseudoPlastic.h:56-64, :153-167) on a
+    two-phase PseudoPlasticNonLinearHardening group with a stiff second phase:
+    phase 1 carries local index 1, phase 0 local index 0; a plastic Gauss point
+    adds the group's phase count 2, so elastic / plastic points give 0 / 2 in
+    phase 0 and 1 / 3 in phase 1, element means in between."""
+    from fans_amd.problem import MaterialGroup, Problem
+    n = 8
+    ms = np.zeros((n, n, n), dtype=np.uint8)
+    ms[2:5, 2:5, 2:5] = 1
+    grp = MaterialGroup("PseudoPlasticNonLinearHardening", [0, 1],
+                        {"bulk_modulus": [62.5, 222.0], "shear_modulus": [28.8462, 166.0],
+                         "yield_stress": [0.1, 50.0], "hardening_exponent": [0.2, 0.2], "eps_0": [0.01, 0.01]})
+    p = Problem(dims=(n, n, n), howmany=3, ms=ms, groups=[grp], fe_type="HEX8", method="cg", tol=1e-10, n_it=300)
+    o = oracle.Oracle(p)
+    for g, expect0, expect1 in (([1e-5, 0, 0, 0, 0, 0], 0.0, 1.0), ([0.02, -0.01, -0.01, 0, 0, 0], 2.0, 1.0)):
+        o.set_gradient(np.array(g))
+        o.solve()
+        f = o.plastic_flag().reshape(n, n, n)
+        # element (x, y, z) takes the phase of its node (x, y, z); its Gauss
+        # points give 0 / 2 (phase 0) or 1 / 3 (phase 1), means in eighths
+        f8 = f * 8
+        assert np.allclose(f8, np.round(f8), atol=1e-12)
+        assert np.all((f[ms == 0] >= 0) & (f[ms == 0] <= 2)) and np.all((f[ms == 1] >= 1) & (f[ms == 1] <= 3))
+        assert np.abs(np.median(f[ms == 0]) - expect0) < 1e-12, g
+        assert np.abs(f[3, 3, 3] - expect1) < 1e-12, g
+
+
 # ----------------------------------------------------------------------------- reference goldens
 def test_mixed_bc_golden_from_reference_input(sphere32_ms):
     """test_MixedBCs.json load case 2: hydrostatic stress -0.05/-0.1 on the sphere32
