@@ -162,8 +162,11 @@ def test_pseudo_plastic_uniaxial_steps():
         assert abs(it_o - it_g) <= 2, (t, it_o, it_g)
         assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-8, t
         assert relerr(g.macro_gradient(), o.macro_gradient()) < 1e-8
+        fo, fg = o.plastic_flag(), g.plastic_flag()
+        assert np.max(np.abs(fg - fo)) < 1e-12, t  # PseudoPlastic.h:56-64, :103-110
         o.extrapolate_displacement()
         g.extrapolate_displacement()
+    assert fo.max() > 0.5  # plastic Gauss points at the last step
     g.close()
 
 
