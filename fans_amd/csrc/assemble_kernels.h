@@ -61,14 +61,16 @@ __device__ __forceinline__ T shfl_up16(T v)
 // VAR: ASM_GENERAL = per-Gauss-point constitutive path (any model);
 // ASM_ISO = every phase LinearElasticIsotropic (H=3) or LinearThermalIsotropic
 // (H=1), modal element arithmetic; ASM_ISO_CUBIC = the same on cubic voxels;
-// ASM_LARGE = the per-Gauss-point path with finite strain (F and P, n_str 9).
-enum { ASM_GENERAL = 0, ASM_ISO = 1, ASM_ISO_CUBIC = 2, ASM_LARGE = 3 };
+// ASM_LARGE = the per-Gauss-point path with finite strain (F and P, n_str 9);
+// ASM_GENERAL_PNL = ASM_GENERAL with power-law PseudoPlastic phases present.
+enum { ASM_GENERAL = 0, ASM_ISO = 1, ASM_ISO_CUBIC = 2, ASM_LARGE = 3, ASM_GENERAL_PNL = 4 };
 
-// The per-Gauss-point FP64 small-strain variants land just above 256 registers
-// (one wave per SIMD) unless asked for two waves, which they reach without
-// spilling; the finite-strain variant would spill, so it keeps one.
+// The per-Gauss-point FP64 residual lands just above 256 registers (one wave
+// per SIMD) unless asked for two waves, which it reaches without spilling; the
+// finite-strain and power-law residuals and the FP64 per-Gauss-point linear
+// matvec would spill there, so they keep one.
 template <typename T, int H, int NGP, int MODE, int VAR>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_LARGE ? 1 : 2))) void k_assemble(AsmArgs<T> a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_LARGE || VAR == ASM_GENERAL_PNL || (VAR == ASM_GENERAL && MODE == ASM_CGLIN && sizeof(T) == 8) ? 1 : 2))) void k_assemble(AsmArgs<T> a)
 {
     constexpr bool ISO = VAR == ASM_ISO || VAR == ASM_ISO_CUBIC, CUBIC = VAR == ASM_ISO_CUBIC;
     if (a.done && *a.done)
@@ -204,8 +206,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
     commit_plane(X0 - 1, 0);
     fetch_plane(X0);
     int                   ph_next = phase_of(X0 - 1);
-    PointModel<T, H, NGP> pm;
-    bool                  pm_pre = false;  // point 0 of the coming element requested
+    PointModel<T, H, NGP, VAR == ASM_GENERAL_PNL> pm;
+    bool pm_pre = false;  // point 0 of the coming element requested
     for (int i = 0; X0 - 1 + i < X1; ++i) {
         const int xe = X0 - 1 + i;
         commit_plane(xe + 1, (i + 1) & 1);
@@ -265,10 +267,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
         } else {
             const Phase<T> &P = a.phases[ph];
             // the linear matvec sees linear models only: no state to prefetch
-            if constexpr (NS == 9 || MODE == ASM_CGLIN) {
+            if constexpr (NS == 9) {
                 auto mat = [&](int gp, const T(&e)[NS], T(&s)[NS]) {
                     material_point_ns<T, H, NS>(P, e, s);
                 };
+                element_forces<T, H, NGP, NS>(u, a.geom, g0, mat, f);
+            } else if constexpr (MODE == ASM_CGLIN) {
+                auto mat = [&](int gp, const T(&e)[NS], T(&s)[NS]) { linear_point<T, H>(P, e, s); };
                 element_forces<T, H, NGP, NS>(u, a.geom, g0, mat, f);
             } else {
                 // the next element of this thread's row is xe + 1 (phase ph_next),
@@ -276,7 +281,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
                 // an element feeding no owned node reads no state (elastic stand-in)
                 const double *sn = nullptr;
                 if (xe + 1 < X1 && el_needed)
-                    sn = PointModel<T, H, NGP>::state_of(PK[ph_next], a.js, (int64_t) (xe + 1) * splane + sl_own);
+                    sn = decltype(pm)::state_of(PK[ph_next], a.js, (int64_t) (xe + 1) * splane + sl_own);
                 pm.begin(P, el_needed ? PK[ph] : (int) MODEL_ELASTIC_ISO, a.js, (int64_t) xg * splane + sl_own, sn,
                          pm_pre);
                 pm_pre = true;

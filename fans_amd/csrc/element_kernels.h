@@ -298,13 +298,14 @@ __device__ __forceinline__ void material_point(const Phase<T> &P, const T (&e)[H
             }
             break;
         case MODEL_J2_NEW:
-            // J2 points need their state: PointModel evaluates them; the
-            // linear-operator paths that call this never see a J2 phase
+        case MODEL_PSEUDO_NL:
+            // J2 points need their state, and both these models are nonlinear:
+            // PointModel evaluates them; the linear-operator paths that call
+            // this never see such a phase
 #pragma unroll
             for (int i = 0; i < 6; ++i)
                 s[i] = (T) NAN;
             break;
-        case MODEL_PSEUDO_NL: pseudo_nl_point<T>(P.p, e, s); break;
         default: pseudo_point<T>(P.p, e, s); break;
         }
     }
@@ -317,7 +318,9 @@ __device__ __forceinline__ void material_point(const Phase<T> &P, const T (&e)[H
 // the previous point's arithmetic instead of each waiting on its own.  A caller
 // that walks a row of elements names the next element's state (sn) so that the
 // last Gauss point requests that element's point 0 (begin() then skips it).
-template <typename T, int H, int NGP>
+// PNL: the power-law PseudoPlastic model may occur (its pow() pushes the
+// residual kernel past 256 registers, so that kernel has a variant without it).
+template <typename T, int H, int NGP, bool PNL = true>
 struct PointModel {
     const Phase<T> *P;
     int             kind;
@@ -416,12 +419,55 @@ struct PointModel {
                 for (int i = 0; i < 6; ++i)
                     s[i] = (T) sd[i];
             } break;
-            case MODEL_PSEUDO_NL: pseudo_nl_point<T>(pr, e, s); break;
-            default: pseudo_point<T>(pr, e, s); break;
+            default:
+                if constexpr (PNL)
+                    if (kind == MODEL_PSEUDO_NL) {
+                        pseudo_nl_point<T>(pr, e, s);
+                        break;
+                    }
+                pseudo_point<T>(pr, e, s);
+                break;
             }
         }
     }
 };
+
+// sigma of the linear models only (the linear matvec K d never sees another
+// kind; anything else gives NaN)
+template <typename T, int H>
+__device__ __forceinline__ void linear_point(const Phase<T> &P, const T (&e)[H == 1 ? 3 : 6],
+                                             T (&s)[H == 1 ? 3 : 6])
+{
+    constexpr int n = H == 1 ? 3 : 6;
+    if (P.kind == MODEL_THERMAL_ISO || P.kind == MODEL_ELASTIC_ISO) {
+        if constexpr (H == 1) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                s[i] = P.p[0] * e[i];
+        } else {
+            const T b1 = P.p[0] * (e[0] + e[1] + e[2]), b2 = (T) 2 * P.p[1];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                s[i] = b1 + b2 * e[i];
+#pragma unroll
+            for (int i = 3; i < 6; ++i)
+                s[i] = b2 * e[i];
+        }
+    } else if (P.kind == MODEL_THERMAL_TRI || P.kind == MODEL_ELASTIC_TRI) {
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            T a = 0;
+#pragma unroll
+            for (int j = 0; j < n; ++j)
+                a += P.p[n * i + j] * e[j];
+            s[i] = a;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+            s[i] = (T) NAN;
+    }
+}
 
 // material_point for any strain vector length (finite strain: NS = 9)
 template <typename T, int H, int NS>

@@ -216,6 +216,7 @@ struct fans_gpu_ctx {
     int                         nphases = 0;
     std::vector<int>            phase_kind;
     bool                        all_linear = true, has_j2 = false, all_iso = false;
+    bool                        has_pnl = false;  // a PseudoPlasticNonLinearHardening phase
     bool                        cubic = false;  // hx == hy == hz
     double                      kref[81]{};
     double                      keref[576]{};
@@ -278,7 +279,7 @@ struct fans_gpu_ctx {
     // buffer current at the first captured iteration
     bool                                                use_graphs = true;
     std::vector<std::pair<const void *, hipGraphExec_t>> graphs;
-    std::array<int, 6>                                  graph_sig{};
+    std::array<int, 7>                                  graph_sig{};
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
     std::vector<hipEvent_t>                             ev_pool;
     double                                              fam_ms[KF_COUNT]{};
@@ -392,7 +393,7 @@ void drop_graphs(fans_gpu_ctx *c)
 // read through fixed pointers), e.g. for a batch of RVEs on one context
 void refresh_graphs(fans_gpu_ctx *c)
 {
-    const std::array<int, 6> sig = {c->all_iso, c->cubic, c->voxel_moduli, c->gxq, c->gyq, c->nphases};
+    const std::array<int, 7> sig = {c->all_iso, c->cubic, c->voxel_moduli, c->gxq, c->gyq, c->nphases, c->has_pnl};
     if (sig != c->graph_sig) {
         drop_graphs(c);
         c->graph_sig = sig;
@@ -672,6 +673,9 @@ int launch_assemble(fans_gpu_ctx *c, AsmArgs<T> a)
         else if (c->all_iso)                                                                                    \
             hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_ISO>), dim3(c->nblk_asm), dim3(kThreads), 0,    \
                                c->stream, a);                                                                   \
+        else if (MODE == ASM_RESID && c->has_pnl)                                                               \
+            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, MODE == ASM_RESID ? ASM_GENERAL_PNL : ASM_GENERAL>),   \
+                               dim3(c->nblk_asm), dim3(kThreads), 0, c->stream, a);                             \
         else                                                                                                    \
             hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_GENERAL>), dim3(c->nblk_asm), dim3(kThreads),   \
                                0, c->stream, a);                                                                \
@@ -1710,6 +1714,7 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
     std::vector<const double *> pp(c->nphases, nullptr);
     c->all_linear = true;
     c->has_j2     = false;
+    c->has_pnl    = false;
     int     ofs   = 0;
     int64_t pofs  = 0;
     for (int g = 0; g < ng; ++g) {
@@ -1717,6 +1722,8 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
             c->all_linear = false;
         if (kind[g] == FANS_MODEL_J2_NEW)
             c->has_j2 = true;
+        if (kind[g] == FANS_MODEL_PSEUDO_PLASTIC_NL)
+            c->has_pnl = true;
         for (int i = 0; i < nphase[g]; ++i) {
             const int p = phase_ids[ofs + i];
             if (p < 0 || group_of[p] >= 0)
