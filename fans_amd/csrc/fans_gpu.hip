@@ -1175,20 +1175,20 @@ int solve_cg_fp64(fans_gpu_ctx *c)
             if ((rc = reduce(c, c->partial, (int) c->fft64.nblk_z, 1, RED_DELTA)))
                 return rc;
         }
-        hipLaunchKernelGGL((k_direction<double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, d, c->s, c->sc,
-                           c->Nh);
-        // --- LineSearchSecant ---
+        // --- LineSearchSecant --- (d = s + beta d, rpd = <r, d> and the first
+        // step u += alpha_warm d in one pass)
         const int    MaxIt = c->d.ls_max_iter;
         const double ltol  = c->d.ls_tol;
         double       lerr = 10.0, ap = 0.0, ac = c->alpha_warm;
         int          lit = 0;
-        hipLaunchKernelGGL((k_dot<double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r, d, c->Nh,
-                           c->d.measure, c->partial);
+        {
+            Timed tv(c, KF_VECTOR);
+            hipLaunchKernelGGL((k_ls_start<double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, d, c->s, c->r,
+                               c->u, c->sc, ac, c->Nh, c->partial);
+        }
         if ((rc = reduce(c, c->partial, c->nblk_vec, 1, RED_SUM)) || (rc = read_scalars(c, sc, nullptr)))
             return rc;
         double rpd = sc[SC_RES0];
-        hipLaunchKernelGGL((k_axpby<double, double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->u, d, ac,
-                           1.0, c->Nh);
         if ((rc = mixed_update(c)) || (rc = residual64(c, c->u, c->rnew, d)))
             return rc;
         if ((rc = reduce(c, c->partial + c->nblk_asm, c->nblk_asm, 1, RED_SUM)) ||

@@ -188,6 +188,32 @@ __global__ __launch_bounds__(256) void k_direction(T *__restrict__ d, const T *_
         d[i] = s[i] + beta * d[i];
 }
 
+// start of the secant line search in one pass (solverCG.h:94, :127-129): the
+// new direction d = s + beta d (beta from the device scalars), per block the
+// partial of <r, d> (slot 0; same grid and accumulation order as k_dot) and the
+// first step u += alpha0 d (the expression of k_axpby with b = 1)
+template <typename T>
+__global__ __launch_bounds__(256) void k_ls_start(T *__restrict__ d, const T *__restrict__ s,
+                                                  const T *__restrict__ r, T *__restrict__ u,
+                                                  const double *__restrict__ sc, double alpha0, int64_t n,
+                                                  double *__restrict__ partial)
+{
+    __shared__ double sm[8];
+    const T           beta = (T) sc[SC_BETA];
+    double            a0   = 0.0;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        const T dv = s[i] + beta * d[i];
+        d[i]       = dv;
+        a0 += (double) r[i] * (double) dv;
+        u[i] = (T) (alpha0 * (double) dv + 1.0 * (double) u[i]);
+    }
+    double     v[1]  = {a0};
+    const bool mx[1] = {false};
+    block_reduce<1>(v, mx, sm);
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = v[0];
+}
+
 template <typename T, typename TS>
 __global__ __launch_bounds__(256) void k_convert(T *__restrict__ y, const TS *__restrict__ x, int64_t n)
 {
