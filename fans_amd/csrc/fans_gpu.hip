@@ -275,6 +275,8 @@ struct fans_gpu_ctx {
     // timing
     bool                                                timing = false;
     int                                                 fft_chunks = 1;
+    // the single-operator entry points run the mixed form's FP32 kernels
+    bool                                                op_fp32 = false;
     // captured CG iterations (hipGraph), keyed by precision and the direction
     // buffer current at the first captured iteration
     bool                                                use_graphs = true;
@@ -2360,6 +2362,28 @@ int fans_gpu_residual(fans_gpu_ctx *c, const double *u, double *r, int32_t linea
         return fail(c, FANS_GPU_EINVAL, "linear matvec needs linear materials");
     HIPCHK(c, copy_on(c, c->dB, u, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
     int rc;
+    if (c->op_fp32 && linear_kd) {
+        // the FP32 inner solve's matvec: rnew32 = K s32, s32 = (float) u
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->s32,
+                           c->dB, c->Nh);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = init_scalars(c, 0.0, 0, 1)))
+            return rc;
+        AsmArgs<float> a = asm_args<float>(c);
+        a.phases         = c->dph32;
+        a.in             = c->s32;
+        a.din            = c->s32;  // d = s + 0 * d
+        a.dout           = c->dA32;
+        a.out            = c->rnew32;
+        a.scal           = c->sc;
+        if ((rc = launch_assemble<float, ASM_CGLIN>(c, a)))
+            return rc;
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->rnew,
+                           c->rnew32, c->Nh);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, copy_on(c, r, c->rnew, sizeof(double) * c->Nh, hipMemcpyDeviceToHost));
+        return 0;
+    }
     if (linear_kd) {
         // rnew = K d with d = s + 0 * d: feed s = u and beta = 0
         HIPCHK(c, copy_on(c, c->s, u, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
@@ -2389,6 +2413,21 @@ int fans_gpu_convolve(fans_gpu_ctx *c, const double *r, double *out)
     if (!c->have_mat)
         return fail(c, FANS_GPU_ESTATE, "materials not set");
     HIPCHK(c, copy_on(c, c->dB, r, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
+    if (c->op_fp32) {
+        // the FP32 inner solve's convolution (FP32 FFT passes and Green operator)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r32,
+                           c->dB, c->Nh);
+        HIPCHK(c, hipGetLastError());
+        EpiArgs<float> ep{};
+        ep.out = c->s32;
+        if (int rc = convolve_dev<float>(c, c->fft32, c->r32, c->spec32, c->G32, EPI_STORE, ep, nullptr))
+            return rc;
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->rnew,
+                           c->s32, c->Nh);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, copy_on(c, out, c->rnew, sizeof(double) * c->Nh, hipMemcpyDeviceToHost));
+        return 0;
+    }
     EpiArgs<double> ep{};
     ep.out = c->rnew;
     int rc = convolve_dev<double>(c, c->fft64, c->dB, c->spec64, c->G64, EPI_STORE, ep, nullptr);
@@ -2653,6 +2692,12 @@ int fans_gpu_set_option(fans_gpu_ctx *c, int32_t opt, int64_t value)
             c->partial_len = need;
         }
         drop_graphs(c);
+        return 0;
+    }
+    if (opt == FANS_OPT_OPERATOR_FP32) {
+        if (value && !c->r32)
+            return fail(c, FANS_GPU_EINVAL, "FP32 operators need a context created with FANS_PREC_MIXED");
+        c->op_fp32 = value != 0;
         return 0;
     }
     return fail(c, FANS_GPU_EINVAL, "unknown option %d", opt);

@@ -27,6 +27,7 @@ OPT_TIMING = 1
 OPT_FFT_CHUNKS = 2
 OPT_GRAPHS = 3
 OPT_ASM_PLANES = 4
+OPT_OPERATOR_FP32 = 5
 ERRORS = {-1: "EINVAL", -2: "ENODEV", -3: "EHIP", -4: "ESTATE", -5: "EUNSUP"}
 
 
@@ -266,6 +267,11 @@ class GpuSolver:
     def set_asm_planes(self, p):
         """x planes per assembly workgroup (stencil tile depth)."""
         self._check(self.L.fans_gpu_set_option(self.ptr, OPT_ASM_PLANES, int(p)))
+
+    def set_operator_fp32(self, on=True):
+        """convolve() and residual(linear_kd=True) run the mixed form's FP32
+        kernels (needs precision="mixed")."""
+        self._check(self.L.fans_gpu_set_option(self.ptr, OPT_OPERATOR_FP32, int(bool(on))))
 
     def set_graphs(self, on=True):
         """Replay captured CG iterations (hipGraph; default) or launch eagerly."""

@@ -209,8 +209,13 @@ int fans_gpu_fft_c2r(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t
  * of the convolution on k kz-chunks in turn (k >= 1); FANS_OPT_GRAPHS=0/1
  * launches the linear CG iterations eagerly / replays them from a captured
  * hipGraph (default 1); FANS_OPT_ASM_PLANES=p sets the x planes each assembly
- * workgroup marches through (default 16, or n_x/4 below 64). */
-enum { FANS_OPT_TIMING = 1, FANS_OPT_FFT_CHUNKS = 2, FANS_OPT_GRAPHS = 3, FANS_OPT_ASM_PLANES = 4 };
+ * workgroup marches through (default 16, or n_x/4 below 64);
+ * FANS_OPT_OPERATOR_FP32=1 makes fans_gpu_convolve and the linear matvec of
+ * fans_gpu_residual (linear_kd=1) run the FP32 kernels of the mixed form's
+ * inner solve (input rounded to FP32, output widened; contexts created with
+ * FANS_PREC_MIXED only). */
+enum { FANS_OPT_TIMING = 1, FANS_OPT_FFT_CHUNKS = 2, FANS_OPT_GRAPHS = 3, FANS_OPT_ASM_PLANES = 4,
+       FANS_OPT_OPERATOR_FP32 = 5 };
 int fans_gpu_set_option(fans_gpu_ctx *ctx, int32_t opt, int64_t value);
 /* per kernel family of the last solve: out[2k] = total ms, out[2k+1] = launches */
 int         fans_gpu_kernel_times(fans_gpu_ctx *ctx, double *out, int32_t cap, int32_t *count);

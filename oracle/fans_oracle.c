@@ -21,6 +21,21 @@
  *   J2PlasticityNew        pinned: textbook radial return (Simo & Hughes Box 3.1)
  *   mixed BC + secant LS   pinned: golden macro strains of test_MixedBCs.json
  *   homogenized tangent    pinned: symmetry/SPD, Voigt/Reuss bounds, laminate
+ *   compute_error          pinned: numpy L1 / L2 / L-inf norms of the first
+ *                                  residual; relative-type stop rule with the
+ *                                  iteration-0 constant 100
+ *   triclinic models       pinned: homogeneous RVE Sigma = C g0 with C from the
+ *                                  C_ij / K_ij keys; sparse assembly with full
+ *                                  anisotropic matrices; mean reference stiffness
+ *   PseudoPlastic (linear  pinned: deformation-plasticity closed form, elastic
+ *   and power-law)                 3G and hardening H in series beyond sigma_y;
+ *                                  power law sigma_y (eps_eq/eps_0)^n; golden
+ *                                  strains of test_MixedBCs.json
+ *   finite strain          pinned: dP/dF by differences of the stored energy,
+ *                                  objectivity, golden F / P of
+ *                                  test_MixedBCs_LargeStrain.json
+ *   unpinned: exact CG iteration counts and error histories of the reference
+ *   binary (FFTW / Eigen operation order), only restated
  */
 #include "fans_oracle.h"
 
@@ -573,7 +588,7 @@ static const double *phase_params(const or_solver *S, int phase, int *kind)
     return g->params + (size_t) S->phase_local[phase] * g->np;
 }
 
-/* C matrix of a linear phase (LinearThermal.h:19-31, :88-99; LinearElastic.h:150-160, :234-249) */
+/* C matrix of a linear phase (LinearThermal.h:25, :89-92; LinearElastic.h:34-35, :118-126) */
 static void linear_C(int n, int kind, const double *p, double *C)
 {
     memset(C, 0, sizeof(double) * n * n);
@@ -602,7 +617,7 @@ static void linear_C(int n, int kind, const double *p, double *C)
     }
 }
 
-/* sum_p B_p^T C B_p v_e / n_gp (LinearElastic.h:157-159, matmodel.h:243-245) */
+/* sum_p B_p^T C B_p v_e / n_gp (LinearElastic.h:37-39, :135-137; matmodel.h:243-245) */
 static void element_stiffness(const or_solver *S, const double *C, double *Ke)
 {
     int ne = 8 * S->h, ns = S->nstr;
@@ -639,7 +654,7 @@ static void group_reference_stiffness(const or_solver *S, const or_group *g, dou
         return;
     }
     if (g->kind == OR_MODEL_THERMAL_ISO || g->kind == OR_MODEL_THERMAL_TRI || g->kind == OR_MODEL_ELASTIC_TRI) {
-        /* LinearThermal.h:42-45 (kappa_average), LinearElastic.h:266-274 */
+        /* LinearThermal.h:42-45, :109-112 (kappa_average), LinearElastic.h:146-154 */
         double Ci[81];
         for (int i = 0; i < g->nphase; i++) {
             linear_C(S->nstr, g->kind, g->params + (size_t) i * g->np, Ci);
@@ -649,7 +664,7 @@ static void group_reference_stiffness(const or_solver *S, const or_group *g, dou
         for (int k = 0; k < n * n; k++)
             C[k] /= g->nphase;
     } else if (g->kind == OR_MODEL_ELASTIC_ISO) {
-        /* LinearElastic.h:175-189: (max+min)/2 of lambda and mu */
+        /* LinearElastic.h:55-69: (max+min)/2 of lambda and mu */
         double lmax = -INFINITY, lmin = INFINITY, mmax = -INFINITY, mmin = INFINITY;
         for (int i = 0; i < g->nphase; i++) {
             const double *p   = g->params + (size_t) i * g->np;
@@ -1121,7 +1136,7 @@ void or_material_point(int kind, const double *p, const double *e, const double 
     } else if (kind == OR_MODEL_PSEUDO_NL) {
         pseudo_nl_point(p, e, sigma);
     } else if (kind == OR_MODEL_ELASTIC_ISO) {
-        /* LinearElastic.h:163-173 */
+        /* LinearElastic.h:43-53 */
         double lam = p[0] - (2.0 / 3.0) * p[1];
         double b1 = lam * (e[0] + e[1] + e[2]), b2 = 2 * p[1];
         for (int i = 0; i < 3; i++)

@@ -478,3 +478,28 @@ def test_large_strain_mixed_bc_golden(sphere32_ms):
     g.solve()
     assert np.max(np.abs(g.macro_gradient() - golden_F)) < 1e-10
     g.close()
+
+
+@pytest.mark.parametrize("maker,measure,err_type", [(m, meas, et) for m in (cases.thermal, cases.elastic)
+                                                    for meas in ("L1", "L2") for et in ("absolute", "relative")])
+def test_error_measures_history(maker, measure, err_type):
+    """compute_error (solver.h:414-452) with the L1 and L2 measures: the whole
+    error history of the GPU solve against the oracle's.  The recursive CG
+    residual (solverCG.h:106) carries rounding of the order eps * err_0 from the
+    first iterations on, so entry k agrees to ~1e-13 * err_0, not to 1e-13 of
+    itself once err_k has fallen by orders of magnitude."""
+    tol = 1e-9 if err_type == "relative" else 1e-8
+    p = maker(n=16, seed=3, measure=measure, tol=tol)
+    p.err_type = err_type
+    g0 = np.array([0.02, -0.01, 0.005] if p.howmany == 1 else [0.01, -0.02, 0.005, 0.01, 0.0, -0.005])
+    say(f"error measure {measure} {err_type} h={p.howmany}")
+    o, g, it_o, it_g = _solve_pair(p, g0)
+    ho, hg = o.error_history(), g.error_history()
+    assert abs(it_o - it_g) <= 2
+    n = min(len(ho), len(hg))
+    assert relerr(hg[:3], ho[:3]) < 1e-12
+    assert np.max(np.abs(hg[:n] - ho[:n])) < 1e-12 * ho[0]
+    final = hg[-1] if err_type == "absolute" else hg[-1] / hg[0]
+    assert final <= tol
+    assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-9
+    g.close()

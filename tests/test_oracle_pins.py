@@ -681,3 +681,135 @@ def test_large_strain_mixed_bc_golden(sphere32_ms):
     o.solve()
     assert np.max(np.abs(o.macro_gradient() - golden)) < 1e-10
     assert np.max(np.abs(o.homogenized_stress() - np.array(lc.mbc["stress_path"][0]))) < 1e-9
+
+
+# ----------------------------------------------------------------------------- error measures
+@pytest.mark.parametrize("measure,order", [("L1", 1), ("L2", 2), ("Linfinity", np.inf)])
+@pytest.mark.parametrize("maker", [cases.thermal, cases.elastic])
+def test_error_norms_against_numpy(maker, measure, order):
+    """compute_error (solver.h:414-430): err_all[0] is the L1 / L2 / L-inf norm
+    of the first residual over the unpadded field, checked with numpy's vector
+    norms of the oracle's own residual at u = 0 (a dropped square root, a
+    missing absolute value or a max in place of a sum fails here)."""
+    p = maker(n=6, dims=(6, 5, 4), count=2, rmin=1.0, rmax=2.0, measure=measure, n_it=0)
+    o = oracle.Oracle(p)
+    g0 = np.linspace(0.3, -0.2, p.n_str) * 0.01
+    o.set_gradient(g0)
+    assert o.solve() == 0
+    r0 = o.residual(np.zeros(p.dims + (p.howmany,)))
+    ref = np.linalg.norm(r0.reshape(-1), ord=order)
+    assert abs(o.error_history()[0] - ref) < 1e-13 * ref
+
+
+def test_relative_error_stop_rule():
+    """Relative error (solver.h:433-450): err / err_0 after the first iteration
+    and the constant 100 at iteration 0.  A solve stops at the first iteration
+    whose ratio meets the tolerance; a tolerance above 100 stops before any
+    iteration, one below 100 does not."""
+    p = cases.elastic(n=8, seed=2, count=2, rmin=1.5, rmax=2.5, measure="L2", tol=1e-6, n_it=300)
+    p.err_type = "relative"
+    g0 = np.array([0.01, -0.02, 0.005, 0.01, 0.0, -0.005])
+    o = oracle.Oracle(p)
+    o.set_gradient(g0)
+    it = o.solve()
+    h = o.error_history()
+    ratio = h / h[0]
+    first = int(np.argmax(ratio[1:] <= 1e-6)) + 1
+    assert it == first and len(h) == it + 1
+    for tol, expect_zero in ((150.0, True), (50.0, False)):
+        p.tol = tol
+        o = oracle.Oracle(p)
+        o.set_gradient(g0)
+        assert (o.solve() == 0) == expect_zero
+
+
+# ----------------------------------------------------------------------------- triclinic models
+def _triclinic_props(C, sym):
+    """material_properties keys of LinearElasticTriclinic / LinearThermalTriclinic
+    (LinearElastic.h:84-90, LinearThermal.h:59-62): one key per upper-triangle
+    entry, named by its 1-based (row, column)."""
+    n = C[0].shape[0]
+    return {f"{sym}_{i + 1}{j + 1}": [float(c[i, j]) for c in C] for i in range(n) for j in range(i, n)}
+
+
+@pytest.mark.parametrize("model,h", [("LinearElasticTriclinic", 3), ("LinearThermalTriclinic", 1)])
+def test_triclinic_homogeneous_and_reference(model, h):
+    """A homogeneous triclinic RVE: the solve needs no iteration and returns
+    Sigma = C g0 with C the symmetric matrix whose (i, j) entry is the key
+    C_ij (K_ij); the reference stiffness of a two-phase group is the plain mean
+    of the phase matrices (LinearElastic.h:146-154, LinearThermal.h:101)."""
+    rng = np.random.default_rng(21)
+    n = 6 if h == 3 else 3
+    A = rng.standard_normal((n, n))
+    C0 = A @ A.T + n * np.eye(n)
+    B = rng.standard_normal((n, n))
+    C1 = B @ B.T + 2 * n * np.eye(n)
+    sym = "C" if h == 3 else "K"
+    dims = (6, 4, 8)
+    p = Problem(dims=dims, howmany=h, ms=np.zeros(dims, np.uint16),
+                groups=[MaterialGroup(model, [0, 1], _triclinic_props([C0, C1], sym))], tol=1e-12)
+    o = oracle.Oracle(p)
+    g0 = rng.standard_normal(n) * 0.01
+    o.set_gradient(g0)
+    assert o.solve() == 0
+    assert rel(o.homogenized_stress(), C0 @ g0) < 1e-13
+    assert rel(o.reference_stiffness(), 0.5 * (C0 + C1)) < 1e-15
+    # a transposed key order (column-wise fill) would be caught: C0 is not symmetric under it
+    assert np.max(np.abs(C0 - C0.T)) < 1e-12
+
+
+@pytest.mark.parametrize("model,h", [("LinearElasticTriclinic", 3), ("LinearThermalTriclinic", 1)])
+def test_triclinic_residual_against_sparse_assembly(model, h):
+    """Two anisotropic phases: the oracle's residual and linear matvec against the
+    independent sparse assembly (numerical shape-function derivatives, 3x3x3
+    Gauss rule) with the same full matrices."""
+    rng = np.random.default_rng(22)
+    n = 6 if h == 3 else 3
+    Cs = []
+    for shift in (1.0, 5.0):
+        A = rng.standard_normal((n, n))
+        Cs.append(A @ A.T + shift * np.eye(n))
+    dims = (4, 5, 6)
+    from fans_amd.microstructure import random_spheres
+    ms = random_spheres(dims, 2, 1.0, 2.0, seed=4)
+    p = Problem(dims=dims, howmany=h, ms=ms,
+                groups=[MaterialGroup(model, [0, 1], _triclinic_props(Cs, "C" if h == 3 else "K"))])
+    o = oracle.Oracle(p)
+    le = np.array(p.L) / np.array(dims)
+    K, F = fe_numpy.assemble(ms, {0: Cs[0], 1: Cs[1]}, le, h)
+    u = rng.standard_normal(dims + (h,))
+    g0 = rng.standard_normal(n)
+    o.set_gradient(g0)
+    assert rel(o.residual(u), (K @ u.reshape(-1) + F @ g0).reshape(u.shape)) < 1e-8
+    assert rel(o.residual(u, linear_kd=True), (K @ u.reshape(-1)).reshape(u.shape)) < 1e-8
+
+
+# ----------------------------------------------------------------------------- linear-hardening pseudo-plasticity
+def test_pseudo_linear_hardening_beyond_yield():
+    """PseudoPlasticLinearHardening (PseudoPlastic.h:78-116) as deformation
+    plasticity with linear hardening H: with eps_eq = sqrt(2/3)|dev eps| and
+    sigma_eq = sqrt(3/2)|dev sigma|, the law is elastic (sigma_eq = 3 G eps_eq)
+    up to sigma_y and beyond it follows the series combination of the elastic
+    3 G and the hardening H, sigma_eq = sigma_y + 3GH/(3G+H) (eps_eq - sigma_y/(3G));
+    the pressure stays K tr(eps) and dev sigma is parallel to dev eps."""
+    K, G, sy, H = 62.5, 28.8462, 0.1, 0.5
+    prm = np.array([K, G, sy, H])
+    eq_y = sy / (3 * G)
+    slope = 3 * G * H / (3 * G + H)
+    rng = np.random.default_rng(8)
+    for scale in (0.3, 0.99, 1.01, 2.0, 10.0, 200.0):
+        d = rng.standard_normal(6)
+        d[:3] -= d[:3].mean()
+        d *= scale * eq_y / (np.sqrt(2 / 3) * np.linalg.norm(d))
+        e = d + np.r_[np.full(3, 1e-3 * rng.standard_normal()), np.zeros(3)]
+        s, _, _ = oracle.material_point("PseudoPlasticLinearHardening", prm, e)
+        tr = e[:3].sum()
+        de = e - np.r_[np.full(3, tr / 3), np.zeros(3)]
+        ds = s - np.r_[np.full(3, s[:3].mean()), np.zeros(3)]
+        eps_eq = np.sqrt(2 / 3) * np.linalg.norm(de)
+        sig_eq = np.sqrt(1.5) * np.linalg.norm(ds)
+        expect = 3 * G * eps_eq if eps_eq <= eq_y else sy + slope * (eps_eq - eq_y)
+        assert abs(sig_eq - expect) < 1e-12 * expect, scale
+        assert abs(s[:3].mean() - K * tr) < 1e-13 * max(abs(K * tr), 1.0)
+        assert np.max(np.abs(ds * np.linalg.norm(de) - de * np.linalg.norm(ds))) < \
+            1e-12 * np.linalg.norm(ds) * np.linalg.norm(de)
