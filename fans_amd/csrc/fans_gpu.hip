@@ -1679,6 +1679,31 @@ int rebuild_green(fans_gpu_ctx *c)
     return 0;
 }
 
+// Cholesky factorisation of the lower triangle of the row-major n x n matrix A
+// (n <= 9), the test behind MaterialManager::compute_reference_stiffness's
+// "must be symmetric positive definite" (MaterialManager.h:189-192, an Eigen
+// LLT, which reads the lower triangle only): false on a non-positive or NaN pivot.
+static bool cholesky_lower_ok(const double *A, int n)
+{
+    double Lm[81] = {};
+    for (int j = 0; j < n; ++j) {
+        double d = A[j * n + j];
+        for (int k = 0; k < j; ++k)
+            d -= Lm[j * n + k] * Lm[j * n + k];
+        if (!(d > 0.0))
+            return false;
+        const double ljj = std::sqrt(d);
+        Lm[j * n + j]    = ljj;
+        for (int i = j + 1; i < n; ++i) {
+            double v = A[i * n + j];
+            for (int k = 0; k < j; ++k)
+                v -= Lm[i * n + k] * Lm[j * n + k];
+            Lm[i * n + j] = v / ljj;
+        }
+    }
+    return true;
+}
+
 int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, const int32_t *nphase,
                            const int32_t *phase_ids, const double *params, int64_t n_params, const double *ref)
 {
@@ -1743,6 +1768,8 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
             return fail(c, FANS_GPU_EINVAL, "MaterialManager: Phase %d not assigned", p);
     // reference stiffness (MaterialManager.h:177-205)
     const int ns = c->nstr;
+    if (ref && !cholesky_lower_ok(ref, ns))
+        return fail(c, FANS_GPU_EINVAL, "reference_material must be symmetric positive definite");
     c->user_ref  = ref != nullptr;
     if (ref) {
         std::memcpy(c->kref, ref, sizeof(double) * ns * ns);

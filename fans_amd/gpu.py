@@ -136,6 +136,21 @@ def fft_c2r(X, nz, device=0):
     return out
 
 
+# settings fans_gpu_create fixes for the life of a context (grid, physics,
+# element weights and inverse lengths, solver and error settings)
+_CONTEXT_FIELDS = ("dims", "howmany", "strain_type", "L", "fe_type", "method", "measure", "err_type", "tol",
+                   "n_it", "ls_max_iter", "ls_tol")
+
+
+def context_key(problem: Problem):
+    """The part of a problem a context is created for: problems with equal keys
+    can share one context through GpuSolver.load."""
+    return (tuple(int(v) for v in problem.dims), int(problem.howmany), problem.strain_type,
+            tuple(float(v) for v in problem.L), problem.fe_type, problem.method, problem.measure,
+            problem.err_type, float(problem.tol), int(problem.n_it), int(problem.ls_max_iter),
+            float(problem.ls_tol))
+
+
 class GpuSolver:
     """One load case on the GPU: the reference's (MaterialManager, Solver) pair."""
 
@@ -171,13 +186,16 @@ class GpuSolver:
         if timing:
             self._check(self.L.fans_gpu_set_option(self.ptr, OPT_TIMING, 1))
         self.stats = Stats()
+        self._key = context_key(problem)
         self.load(problem)
 
     def load(self, problem: Problem):
         """Microstructure and material tables of `problem` (same grid, physics and
         solver settings as the context was created with); fresh solver state."""
-        if tuple(problem.dims) != self.dims or problem.howmany != self.h:
-            raise ValueError("load(): grid and howmany must match the context")
+        if context_key(problem) != self._key:
+            diff = [n for n, a, b in zip(_CONTEXT_FIELDS, context_key(problem), self._key) if a != b]
+            raise ValueError("load(): the problem differs from the context in " + ", ".join(diff) +
+                             " (fixed when the context was created)")
         self.p = problem
         ms = np.ascontiguousarray(problem.ms, dtype=np.uint16).reshape(-1)
         self._check(self.L.fans_gpu_set_microstructure(self.ptr, ms.ctypes.data_as(C.POINTER(C.c_uint16)), ms.size))

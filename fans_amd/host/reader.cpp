@@ -282,6 +282,23 @@ void Reader::ReadInputFile(const char *input_fn)
                                             std::to_string(n_str));
             ref_material.insert(ref_material.end(), row.begin(), row.end());
         }
+        // MaterialManager::compute_reference_stiffness (MaterialManager.h:189-192):
+        // Cholesky of the lower triangle, as Eigen's LLT reads it
+        std::vector<double> Lm(n_str * n_str, 0.0);
+        for (int c = 0; c < n_str; ++c) {
+            double d = ref_material[c * n_str + c];
+            for (int k = 0; k < c; ++k)
+                d -= Lm[c * n_str + k] * Lm[c * n_str + k];
+            if (!(d > 0.0))
+                throw std::invalid_argument("reference_material must be symmetric positive definite");
+            Lm[c * n_str + c] = std::sqrt(d);
+            for (int i = c + 1; i < n_str; ++i) {
+                double v = ref_material[i * n_str + c];
+                for (int k = 0; k < c; ++k)
+                    v -= Lm[i * n_str + k] * Lm[c * n_str + k];
+                Lm[i * n_str + c] = v / Lm[c * n_str + c];
+            }
+        }
     }
 }
 

@@ -38,3 +38,31 @@ def test_batch_matches_oracle():
             # both forms stop on the FP64 residual at 1e-8; the homogenized
             # responses then agree far below the tolerance's own scale
             assert np.max(np.abs(r.stress - so)) <= 1e-7 * np.max(np.abs(so)), (fp, r.index)
+
+
+def test_batch_groups_split_on_context_settings():
+    """RVEs that differ in any setting fixed at context creation (here the RVE
+    lengths L, the element type and the tolerance) never share a context."""
+    from dataclasses import replace
+
+    from fans_amd.gpu import context_key
+    p = config5_rves(count=1, seed=3)[0]
+    keys = {context_key(p), context_key(replace(p, L=(2.0, 1.0, 1.0))), context_key(replace(p, fe_type="HEX8R")),
+            context_key(replace(p, tol=1e-6)), context_key(replace(p, ms=p.ms.copy()))}
+    assert len(keys) == 4
+
+
+@pytest.mark.gpu
+def test_load_rejects_other_context_settings():
+    from dataclasses import replace
+
+    from fans_amd.gpu import GpuSolver
+    p = [q for q in config5_rves(count=8, seed=5) if q.dims[0] <= 48][0]
+    print(f"[gpu-case] load() with other settings, {p.dims}", flush=True)
+    g = GpuSolver(p)
+    try:
+        g.load(replace(p, ms=p.ms.copy()))  # same settings: accepted
+        with pytest.raises(ValueError, match="L"):
+            g.load(replace(p, L=(2.0, 1.0, 1.0)))
+    finally:
+        g.close()

@@ -62,6 +62,13 @@ def test_usage_and_version():
     ({"linesearch_parameters": {"max_iter": 0}}, "linesearch_parameters"),
     ({"macroscale_loading": [[[1.0, 0.0]]]}, "Invalid length of loading vector"),
     ({"materials": []}, "'materials' must be non-empty array"),
+    ({"reference_material": [[1.0 if i == j else 0.0 for j in range(6)] for i in range(5)]},
+     "reference_material must be 6x6"),
+    # indefinite (a negative pivot) and singular: both fail the Cholesky test
+    ({"reference_material": [[(-1.0 if i == 3 else 2.0) if i == j else 0.0 for j in range(6)] for i in range(6)]},
+     "reference_material must be symmetric positive definite"),
+    ({"reference_material": [[1.0] * 6 for _ in range(6)]},
+     "reference_material must be symmetric positive definite"),
 ])
 def test_input_errors_exit_10(case_dir, edit, msg):
     _need_cli()
