@@ -40,10 +40,11 @@ def family_bytes_per_voxel(nz, h=3, wb=8, green_quarter=True):
         "fft_y_fwd": 2 * S,
         "fft_x_green": 2 * S + Gp,        # 54.6 B at 256^3 with quarter storage
         "fft_y_inv": 2 * S,
-        "fft_z_c2r": S + 2 * V,      # read spectrum + r, write s
-        # FP64: read s, d, phase, u (owned nodes); write d, rnew, u -- the matvec
-        # pass also applies the previous iteration's u -= alpha d
-        "assemble_lin": 6 * V + 1,
+        # CG inverse z pass: read the spectrum, d_prev and u; write s, the new
+        # direction d = s + beta d_prev and u -= alpha d_prev
+        "fft_z_c2r": S + 5 * V,
+        # matvec: read d and the phase; write rnew = K d
+        "assemble_lin": 2 * V + 1,
         # FP64: residual update fused into the forward z pass: read r, rnew, s;
         # write r and the spectrum
         "cg_update_fft_z_r2c": 4 * V + S,
@@ -66,7 +67,7 @@ def headline_problem(n=256, seed=0, tol=1e-10):
                    measure="Linfinity", err_type="absolute", tol=tol, n_it=2000)
 
 
-def run_gpu(p, precision, steps, warmup, g0, timing_solves=2):
+def run_gpu(p, precision, steps, warmup, g0, timing_solves=2, all_load_cases=False):
     """`steps` timed solves (wall clock, synchronous library calls), then
     `timing_solves` identical solves with every launch bracketed by HIP events
     on the library's stream for the per-kernel durations (kept out of the timed
@@ -78,12 +79,14 @@ def run_gpu(p, precision, steps, warmup, g0, timing_solves=2):
         g.reset_state()
         g.set_gradient(g0)
         g.solve()
-    iters, solve_ms = [], []
+    iters, solve_ms, step_s = [], [], []
     t0 = time.perf_counter()
     for _ in range(steps):
+        ts = time.perf_counter()
         g.reset_state()
         g.set_gradient(g0)
-        g.solve()
+        g.solve()  # synchronous: returns after the device finished
+        step_s.append(time.perf_counter() - ts)
         iters.append(int(g.stats.inner_iterations))
         solve_ms.append(float(g.stats.solve_ms))
     t1 = time.perf_counter()
@@ -99,7 +102,20 @@ def run_gpu(p, precision, steps, warmup, g0, timing_solves=2):
             a, b = fam.get(k, (0.0, 0))
             fam[k] = (a + ms, b + n)
     g.set_timing(False)
-    res = dict(iters=iters, solve_ms=solve_ms, wall_s=t1 - t0, fam=fam, fam_iters=fam_iters,
+    others = []
+    for k in range(len(g0)) if all_load_cases else []:
+        # the six Mandel unit strains, one solve each (outside the timed region)
+        e = np.zeros(len(g0))
+        e[k] = 1.0
+        g.reset_state()
+        g.set_gradient(e)
+        g.solve()
+        others.append((int(g.stats.inner_iterations), round(float(g.stats.solve_ms), 2)))
+    g.reset_state()
+    g.set_gradient(g0)
+    g.solve()
+    res = dict(iters=iters, solve_ms=solve_ms, step_s=step_s, wall_s=t1 - t0, fam=fam, fam_iters=fam_iters,
+               load_cases=others,
                stress=g.homogenized_stress(),
                final_error=float(g.stats.final_error), outer=int(g.stats.outer_iterations),
                converged=bool(g.stats.converged))
@@ -107,22 +123,63 @@ def run_gpu(p, precision, steps, warmup, g0, timing_solves=2):
     return res
 
 
-def cpu_baseline(p, g0, iters=2):
-    """The CPU restatement (oracle, OpenMP over the host cores) on the same
-    256^3 workload, bounded to `iters` CG iterations."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def probe_reference_fans():
+    """Whether reference FANS (or the libraries it needs: FFTW3-MPI, parallel
+    HDF5) exists on this host; the preferred CPU baseline runs it with mpirun."""
+    import ctypes.util
+    import shutil
+    found = {"FANS": shutil.which("FANS"), "mpirun": shutil.which("mpirun") or shutil.which("mpiexec"),
+             "libfftw3_mpi": ctypes.util.find_library("fftw3_mpi"), "libfftw3": ctypes.util.find_library("fftw3"),
+             "libhdf5": ctypes.util.find_library("hdf5")}
+    return {k: (v or None) for k, v in found.items()}
+
+
+def cpu_baseline(p, g0, iters=2, reps=5):
+    """CPU baseline beside the GPU run.  Reference FANS (mpirun over FFTW-MPI)
+    is preferred; where this host lacks it (probe_reference_fans) the CPU
+    restatement (oracle, OpenMP) runs the same workload: `reps` timed solves of
+    `iters` CG iterations each on P = min(host CPUs available to this process,
+    OMP_NUM_THREADS, n_x / 4) threads -- the reference's own rank limit of four
+    x planes per rank -- and the median is reported."""
     import oracle
-    cores = min(16, os.cpu_count() or 1)
+    probe = probe_reference_fans()
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = int(os.environ.get("OMP_NUM_THREADS") or avail)
+    cores = max(1, min(avail, omp, p.dims[0] // 4))
     oracle.set_threads(cores)
     from dataclasses import replace
     q = replace(p, n_it=iters, tol=0.0)
     o = oracle.Oracle(q)
-    o.set_gradient(g0)
-    t0 = time.perf_counter()
-    it = o.solve()
-    dt = time.perf_counter() - t0
     n = int(np.prod(p.dims))
-    return {"value": n * it / dt, "unit": "voxel-iters/s", "cores": cores, "kind": "port",
-            "sample": f"oracle CG at {p.dims[0]}^3 linear elastic, {it} iterations ({dt:.1f} s), same microstructure"}
+    zero = np.zeros(tuple(p.dims) + (p.howmany,))
+    rates, secs = [], []
+    for _ in range(reps):
+        o.set_u(zero)
+        o.set_gradient(g0)
+        t0 = time.perf_counter()
+        it = o.solve()
+        dt = time.perf_counter() - t0
+        rates.append(n * it / dt)
+        secs.append(dt)
+    return {"value": float(np.median(rates)), "unit": "voxel-iters/s", "cores": cores, "kind": "port",
+            "sample": f"CPU restatement (oracle, not reference FANS) of the {p.dims[0]}^3 linear elastic CG, same "
+                      f"microstructure and load, {iters} iterations per solve, median of {reps} solves "
+                      f"({min(secs):.1f}-{max(secs):.1f} s each)",
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "cpus_available": avail, "threads": cores,
+            "binding": os.environ.get("OMP_PROC_BIND", "none (OS scheduling)") + (
+                f", places {os.environ['OMP_PLACES']}" if os.environ.get("OMP_PLACES") else ""),
+            "reference_fans_probe": probe}
 
 
 def traffic_from_profiles(family):
@@ -251,6 +308,156 @@ def run_j2_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def _rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
+
+
+def _fields(g):
+    f = g.element_fields(gp=False)
+    return {"u": g.get_u(), "strain": f["strain"], "stress": f["stress"]}
+
+
+def _field_errs(fa, fb):
+    return {k: _rel(fa[k], fb[k]) for k in fb}
+
+
+def run_accuracy_workload(args):
+    """Per-configuration accuracy of the GPU forms: homogenized stress / flux
+    and the fields (u, element strain, element stress) of the mixed form
+    against the FP64 form, and of the FP64 form against the oracle wherever
+    the oracle finishes in seconds.  One JSON line."""
+    import oracle
+    from dataclasses import replace
+
+    from fans_amd import workloads
+    from fans_amd.batch import config5_rves
+    from fans_amd.gpu import GpuSolver
+    oracle.set_threads(max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS") or 16))))
+    out = {"metric": "per-configuration accuracy, mixed vs FP64 GPU and FP64 GPU vs oracle", "n_gpus": 1,
+           "data": "synthetic", "configs": {}}
+
+    def linear_config(name, p, oracle_cases):
+        print(f"[bench] accuracy: {name}", file=sys.stderr, flush=True)
+        g64, gmx = GpuSolver(p), GpuSolver(p, precision="mixed")
+        rows = []
+        for k, lc in enumerate(p.load_cases):
+            g0 = lc.g0_path[0]
+            row = {"load": [float(v) for v in g0]}
+            res = {}
+            for tag, g in (("fp64", g64), ("mixed", gmx)):
+                g.reset_state()
+                g.set_gradient(g0)
+                g.solve()
+                res[tag] = (g.homogenized_stress().copy(), _fields(g), int(g.stats.inner_iterations))
+            row["iterations"] = {"fp64": res["fp64"][2], "mixed_inner": res["mixed"][2]}
+            row["mixed_vs_fp64"] = {"homogenized": _rel(res["mixed"][0], res["fp64"][0]),
+                                    "fields": _field_errs(res["mixed"][1], res["fp64"][1])}
+            if k in oracle_cases:
+                o = oracle.Oracle(p)
+                o.set_gradient(g0)
+                it_o = o.solve()
+                fo = o.element_fields()
+                fo = {"u": o.get_u(), "strain": fo["strain"], "stress": fo["stress"]}
+                row["iterations"]["oracle"] = int(it_o)
+                row["fp64_vs_oracle"] = {"homogenized": _rel(res["fp64"][0], o.homogenized_stress()),
+                                         "fields": _field_errs(res["fp64"][1], fo)}
+                del o
+            rows.append(row)
+        g64.close()
+        gmx.close()
+        return {"load_cases": rows,
+                "max_mixed_vs_fp64_homogenized": max(r["mixed_vs_fp64"]["homogenized"] for r in rows),
+                "max_fp64_vs_oracle_homogenized": max((r["fp64_vs_oracle"]["homogenized"] for r in rows
+                                                       if "fp64_vs_oracle" in r), default=None)}
+
+    p1 = workloads.config1_problem()
+    out["configs"]["1_thermal_32"] = linear_config("config 1 (thermal 32^3, 3 load cases)", p1, range(3))
+    p2 = workloads.config2_problem()
+    out["configs"]["2_elastic_128"] = linear_config("config 2 (elastic 128^3, 6 load cases)", p2, [0])
+    out["configs"]["2_elastic_128"]["inclusion_volume_fraction"] = workloads.inclusion_fraction(p2.ms)
+
+    # config 3: J2 ramp, mixed vs FP64 per load step (the oracle runs the 256^3
+    # residual kernel in the tests; a full 256^3 J2 step is beyond its seconds)
+    n3 = args.size
+    print(f"[bench] accuracy: config 3 (J2 {n3}^3, {args.j2_steps} load steps)", file=sys.stderr, flush=True)
+    p3 = workloads.config3_problem(n=n3)
+    mbc = workloads.config3_ramp()
+    steps = {}
+    for prec in ("fp64", "mixed"):
+        g = GpuSolver(p3, precision=prec)
+        hist = []
+        for t in range(args.j2_steps):
+            g.enable_mixed_bc(mbc, t)
+            g.solve()
+            hist.append((g.homogenized_stress().copy(), g.macro_gradient().copy(), int(g.stats.iterations)))
+            g.extrapolate_displacement()
+        f = _fields(g)
+        f["plastic_strain"], f["hardening"] = g.get_state()
+        steps[prec] = (hist, f)
+        g.close()
+    h64, hmx = steps["fp64"][0], steps["mixed"][0]
+    out["configs"]["3_j2_256"] = {
+        "grid": n3, "load_steps": args.j2_steps,
+        "iterations": {"fp64": [h[2] for h in h64], "mixed": [h[2] for h in hmx]},
+        "mixed_vs_fp64_homogenized_per_step": [_rel(b[0], a[0]) for a, b in zip(h64, hmx)],
+        "mixed_vs_fp64_macro_strain_per_step": [_rel(b[1], a[1]) for a, b in zip(h64, hmx)],
+        "mixed_vs_fp64_fields_last_step": _field_errs(steps["mixed"][1], steps["fp64"][1])}
+    del steps
+
+    # config 4: SIMP objective and its adjoint gradient, mixed vs FP64
+    print("[bench] accuracy: config 4 (96^3 SIMP objective and gradient)", file=sys.stderr, flush=True)
+    import torch
+
+    from fans_amd.pyfans import voxel_problem
+    rho0 = workloads.smoothed_density(96, corr=8.0, seed=0)
+    params = [torch.tensor(v, dtype=torch.float64) for v in (1.0, 0.3, 1e-3, 0.3)]
+    res4 = {}
+    for prec in ("fp64", "mixed"):
+        sv = GpuSolver(voxel_problem((96, 96, 96), tol=1e-10), precision=prec)
+        rho = torch.tensor(rho0, requires_grad=True)
+        J = workloads.config4_objective(sv, rho, params)
+        J.backward()
+        res4[prec] = (float(J.item()), rho.grad.numpy().copy(), _fields(sv))
+        sv.close()
+    out["configs"]["4_simp_96"] = {
+        "objective_fp64": res4["fp64"][0],
+        "mixed_vs_fp64_objective": abs(res4["mixed"][0] - res4["fp64"][0]) / abs(res4["fp64"][0]),
+        "mixed_vs_fp64_gradient": _rel(res4["mixed"][1], res4["fp64"][1]),
+        "mixed_vs_fp64_fields_adjoint_state": _field_errs(res4["mixed"][2], res4["fp64"][2])}
+    del res4
+
+    # config 5: every RVE of the batch, mixed vs FP64; the first RVE of each
+    # (edge, physics) group against the oracle
+    print("[bench] accuracy: config 5 (384 RVEs)", file=sys.stderr, flush=True)
+    probs = config5_rves()
+    worst, worst_o, seen = 0.0, 0.0, set()
+    for i, q in enumerate(probs):
+        g0 = q.load_cases[0].g0_path[0]
+        sig = {}
+        for prec in ("fp64", "mixed"):
+            qq = replace(q, tol=1e-10) if prec == "fp64" else q
+            g = GpuSolver(qq, precision=prec)
+            g.set_gradient(g0)
+            g.solve()
+            sig[prec] = g.homogenized_stress().copy()
+            g.close()
+        worst = max(worst, _rel(sig["mixed"], sig["fp64"]))
+        key = (q.dims[0], q.howmany)
+        if key not in seen:
+            seen.add(key)
+            o = oracle.Oracle(replace(q, tol=1e-10))
+            o.set_gradient(g0)
+            o.solve()
+            worst_o = max(worst_o, _rel(sig["fp64"], o.homogenized_stress()))
+            del o
+    out["configs"]["5_batch_384"] = {
+        "rves": len(probs), "mixed_tol": probs[0].tol,
+        "max_mixed_vs_fp64_homogenized": worst,
+        "max_fp64_vs_oracle_homogenized_first_rve_per_group": worst_o, "groups_checked": len(seen)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,9 +466,9 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--workload", choices=["headline", "batch", "j2"], default="headline",
-                    help="batch: the 384-RVE uneven-size batch (RVEs/s); j2: the J2 load-step ramp; neither is "
-                         "the headline metric")
+    ap.add_argument("--workload", choices=["headline", "batch", "j2", "accuracy"], default="headline",
+                    help="batch: the 384-RVE uneven-size batch (RVEs/s); j2: the J2 load-step ramp; accuracy: "
+                         "per-configuration errors, mixed vs FP64 and FP64 vs oracle; none is the headline metric")
     ap.add_argument("--j2-steps", type=int, default=20, help="load steps of the j2 workload (of 20)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (reads 1 GiB, writes 1 GiB) to scale HBM counters")
@@ -285,16 +492,22 @@ def main():
     if args.workload == "j2":
         run_j2_workload(args)
         return
+    if args.workload == "accuracy":
+        run_accuracy_workload(args)
+        return
     n = args.size
     p = headline_problem(n)
     N = n ** 3
     g0 = np.array([1.0, 0, 0, 0, 0, 0])
     print(f"[bench] {n}^3 elastic: FP64 form, {args.warmup} warmup + {args.steps} timed solves", file=sys.stderr,
           flush=True)
-    r64 = run_gpu(p, "fp64", args.steps, args.warmup, g0)
+    r64 = run_gpu(p, "fp64", args.steps, args.warmup, g0, all_load_cases=True)
     tot_it = sum(r64["iters"])
-    value = N * tot_it / r64["wall_s"]
-    ms_per_step = 1000.0 * r64["wall_s"] / args.steps
+    # median step (median of the timed solves); the mean is
+    # reported beside it
+    med = int(np.argsort(r64["step_s"])[len(r64["step_s"]) // 2])
+    value = N * r64["iters"][med] / r64["step_s"][med]
+    ms_per_step = 1000.0 * r64["step_s"][med]
     # dominant kernel family by device time
     fam = r64["fam"]
     bpv = family_bytes_per_voxel(n)
@@ -313,7 +526,11 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{n}^3 linear elastic CG (HEX8, Linf abs 1e-10), 150 spheres r~U[8,20] seed 0, "
                                "matrix E=3 nu=0.35 / spheres E=70 nu=0.22, Mandel e11 load",
-                   "grid": n, "cg_iterations": r64["iters"][0], "form": "fp64"},
+                   "grid": n, "cg_iterations": r64["iters"][0], "form": "fp64",
+                   "inclusion_volume_fraction": round(float(np.mean(p.ms != 0)), 4)},
+        "mean_ms_per_step": round(1000.0 * r64["wall_s"] / args.steps, 3),
+        "mandel_unit_load_cases": {"iterations": [v[0] for v in r64["load_cases"]],
+                                   "solve_ms": [v[1] for v in r64["load_cases"]]},
         "roofline": roofline,
         "iteration_hbm_frac": round(iter_frac, 4),
         "kernel_ms_per_iteration": {k: round(v[0] / max(r64["fam_iters"], 1), 4) for k, v in fam.items() if v[1]},
@@ -322,8 +539,8 @@ def main():
         print(f"[bench] {n}^3 elastic: mixed form", file=sys.stderr, flush=True)
         rmx = run_gpu(p, "mixed", args.steps, args.warmup, g0)
         rel = float(np.max(np.abs(rmx["stress"] - r64["stress"])) / np.max(np.abs(r64["stress"])))
-        t_mx = sum(rmx["solve_ms"]) / len(rmx["solve_ms"])
-        t_64 = sum(r64["solve_ms"]) / len(r64["solve_ms"])
+        t_mx = float(np.median(rmx["solve_ms"]))
+        t_64 = float(np.median(r64["solve_ms"]))
         out["mixed"] = {"voxel_iters_per_s": round(N * sum(rmx["iters"]) / rmx["wall_s"], 1),
                         "inner_iterations": rmx["iters"][0], "outer_iterations": rmx["outer"],
                         "time_to_solution_speedup": round(t_64 / t_mx, 3), "homog_stress_rel_err": rel,

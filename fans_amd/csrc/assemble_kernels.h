@@ -5,8 +5,7 @@
 // tile of elements (one element per thread, lanes ordered z-fastest so one
 // wave holds 4 rows of 16) and emits the 15x15 nodes whose four surrounding
 // elements of the current plane lie in the tile.  Per x plane:
-//   1. the 17x17 node plane x+1 is staged in LDS (in CG mode the direction
-//      update d = s + beta d happens here and the owned nodes are written);
+//   1. the 17x17 node plane x+1 is staged in LDS;
 //   2. each thread gathers its 8 corners from LDS and computes its element;
 //   3. corner forces are combined along z with DPP row_shr (in registers),
 //      along y with a 16-lane shuffle (LDS only for the wave-boundary row),
@@ -29,21 +28,15 @@ struct AsmArgs {
     int             nty, ntz;  // number of tiles in y, z
     Geom            geom;
     const uint8_t  *ms;        // phase ids [x][y][z]
+    int             ms16;      // 16-bit phase ids (phase_id)
     const Phase<T> *phases;
     int             nphases;
     J2State         js;
     // ASM_RESID: in = u, out = r; norm/dot partials
-    // ASM_CGLIN: in = s, din = d_cur, dout = d_next (d = s + beta d), out = rnew;
-    //            u (nullable): u -= alpha d_cur at owned nodes -- the solution
-    //            update of the previous iteration (alpha at scal[1]), done here
-    //            because this pass reads d_cur anyway
+    // ASM_CGLIN: in = d, out = rnew = K d; partial <d, rnew>
     const T        *in;
-    const T        *din;
-    T              *dout;
     T              *out;
-    T              *u;
     const T        *dotv;      // ASM_RESID optional: partial <out, dotv>
-    const double   *scal;      // device scalars (beta at scal[0] for CGLIN)
     const T        *g0;        // macroscopic gradient (n_str), device
     double         *partial;   // [2][gridDim] : slot 0 = norm/dot, slot 1 = dot
     int             measure;   // 0 L1, 1 L2, 2 Linf  (ASM_RESID)
@@ -63,14 +56,16 @@ __device__ __forceinline__ T shfl_up16(T v)
 // (H=1), modal element arithmetic; ASM_ISO_CUBIC = the same on cubic voxels;
 // ASM_LARGE = the per-Gauss-point path with finite strain (F and P, n_str 9);
 // ASM_GENERAL_PNL = ASM_GENERAL with power-law PseudoPlastic phases present.
-enum { ASM_GENERAL = 0, ASM_ISO = 1, ASM_ISO_CUBIC = 2, ASM_LARGE = 3, ASM_GENERAL_PNL = 4 };
+// ASM_GENERAL_VISCO = ASM_GENERAL with J2ViscoPlastic phases present (their
+// per-call psi / psi_bar update, see PointModel).
+enum { ASM_GENERAL = 0, ASM_ISO = 1, ASM_ISO_CUBIC = 2, ASM_LARGE = 3, ASM_GENERAL_PNL = 4, ASM_GENERAL_VISCO = 5 };
 
 // The per-Gauss-point FP64 residual lands just above 256 registers (one wave
 // per SIMD) unless asked for two waves, which it reaches without spilling; the
 // finite-strain and power-law residuals and the FP64 per-Gauss-point linear
 // matvec would spill there, so they keep one.
 template <typename T, int H, int NGP, int MODE, int VAR>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_LARGE || VAR == ASM_GENERAL_PNL || (VAR == ASM_GENERAL && MODE == ASM_CGLIN && sizeof(T) == 8) ? 1 : 2))) void k_assemble(AsmArgs<T> a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_LARGE || VAR == ASM_GENERAL_PNL || VAR == ASM_GENERAL_VISCO || (VAR == ASM_GENERAL && MODE == ASM_CGLIN && sizeof(T) == 8) ? 1 : 2))) void k_assemble(AsmArgs<T> a)
 {
     constexpr bool ISO = VAR == ASM_ISO || VAR == ASM_ISO_CUBIC, CUBIC = VAR == ASM_ISO_CUBIC;
     if (a.done && *a.done)
@@ -114,12 +109,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
     // the grid edge when 15 does not divide it) only feed unowned sums
     const bool el_needed = Y0 - 1 + max(ey, 1) < ny && Z0 - 1 + max(ez, 1) < nz;
 
-    T beta = (T) 0, alpha = (T) 0;
-    if constexpr (MODE == ASM_CGLIN) {
-        beta  = (T) a.scal[0];
-        alpha = (T) a.scal[1];
-    }
-
     T      carry[H];
     double acc0 = 0.0, acc1 = 0.0;
 #pragma unroll
@@ -130,87 +119,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
     for (int i = 0; i < NS; ++i)
         g0[i] = (MODE == ASM_RESID) ? a.g0[i] : (T) 0;
     const T w = (T) a.geom.w;
-    // isotropic moduli of every phase, staged once per block
+    // isotropic moduli of the first 256 phases, staged once per block (ids
+    // beyond read the phase table)
     __shared__ T PL[256][2];
     if constexpr (ISO) {
-        for (int p = t; p < a.nphases; p += blockDim.x) {
+        for (int p = t; p < min(a.nphases, 256); p += blockDim.x) {
             PL[p][0] = a.phases[p].p[0];
             PL[p][1] = H == 3 ? a.phases[p].p[1] : (T) 0;
         }
     }
-    // per-Gauss-point residual: model kind of every phase, staged once per block
+    // per-Gauss-point residual: model kind of the first 256 phases, staged once
+    // per block
     constexpr bool PM = !ISO && VAR != ASM_LARGE && MODE == ASM_RESID;
     __shared__ int PK[PM ? 256 : 1];
     if constexpr (PM)
-        for (int p = t; p < a.nphases; p += blockDim.x)
+        for (int p = t; p < min(a.nphases, 256); p += blockDim.x)
             PK[p] = a.phases[p].kind;
+    auto kind_of = [&](int ph) { return ph < 256 ? PK[ph] : a.phases[ph].kind; };
 
-    // node plane x+1: fetch (global) then commit (d update, LDS); the phase id
-    // of the next element plane is loaded one plane ahead
-    T pf_own[MODE == ASM_CGLIN ? 2 : 1][H], pf_ext[MODE == ASM_CGLIN ? 2 : 1][H], pf_u[H];
-    auto owned = [&](int xp) { return xp >= X0 && xp < X1 && own_yz; };
+    // node plane x+1: fetch (global) then commit (LDS); the phase id of the
+    // next element plane is loaded one plane ahead
+    T    pf_own[H], pf_ext[H];
     auto fetch_plane = [&](int xp) {
         const int     xg = xp < 0 ? xp + nx : (xp >= nx ? xp - nx : xp);
         const int64_t pb = xg * plane;
 #pragma unroll
-        for (int c = 0; c < H; ++c) {
-            pf_own[0][c] = a.in[(pb + off_own) * H + c];
-            if constexpr (MODE == ASM_CGLIN)
-                pf_own[1][c] = a.din[(pb + off_own) * H + c];
-        }
-        if constexpr (MODE == ASM_CGLIN)
-            if (a.u && owned(xp))
-#pragma unroll
-                for (int c = 0; c < H; ++c)
-                    pf_u[c] = a.u[(pb + off_own) * H + c];
+        for (int c = 0; c < H; ++c)
+            pf_own[c] = a.in[(pb + off_own) * H + c];
         if (ext >= 0)
 #pragma unroll
-            for (int c = 0; c < H; ++c) {
-                pf_ext[0][c] = a.in[(pb + off_ext) * H + c];
-                if constexpr (MODE == ASM_CGLIN)
-                    pf_ext[1][c] = a.din[(pb + off_ext) * H + c];
-            }
+            for (int c = 0; c < H; ++c)
+                pf_ext[c] = a.in[(pb + off_ext) * H + c];
     };
-    auto commit_plane = [&](int xp, int buf) {
-        const int     xg = xp < 0 ? xp + nx : (xp >= nx ? xp - nx : xp);
-        const bool    wr = (MODE == ASM_CGLIN) && owned(xp);
-        const int64_t n  = xg * plane + off_own;
+    auto commit_plane = [&](int buf) {
 #pragma unroll
-        for (int c = 0; c < H; ++c) {
-            T val = pf_own[0][c];
-            if constexpr (MODE == ASM_CGLIN) {
-                val = val + beta * pf_own[1][c];
-                if (wr) {
-                    a.dout[n * H + c] = val;
-                    if (a.u)
-                        a.u[n * H + c] = pf_u[c] - alpha * pf_own[1][c];
-                }
-            }
-            Un[buf][(ey * NL + ez) * H + c] = val;
-        }
+        for (int c = 0; c < H; ++c)
+            Un[buf][(ey * NL + ez) * H + c] = pf_own[c];
         if (ext >= 0)
 #pragma unroll
-            for (int c = 0; c < H; ++c) {
-                T val = pf_ext[0][c];
-                if constexpr (MODE == ASM_CGLIN)
-                    val = val + beta * pf_ext[1][c];
-                Un[buf][(e_a * NL + e_b) * H + c] = val;
-            }
+            for (int c = 0; c < H; ++c)
+                Un[buf][(e_a * NL + e_b) * H + c] = pf_ext[c];
     };
     auto phase_of = [&](int xe) {
         const int xg = xe < 0 ? xe + nx : xe;
-        return (int) a.ms[xg * plane + off_own];
+        return phase_id(a.ms, a.ms16, xg * plane + off_own);
     };
 
     fetch_plane(X0 - 1);
-    commit_plane(X0 - 1, 0);
+    commit_plane(0);
     fetch_plane(X0);
     int                   ph_next = phase_of(X0 - 1);
-    PointModel<T, H, NGP, VAR == ASM_GENERAL_PNL> pm;
+    PointModel<T, H, NGP, VAR == ASM_GENERAL_PNL || VAR == ASM_GENERAL_VISCO, VAR == ASM_GENERAL_VISCO> pm;
     bool pm_pre = false;  // point 0 of the coming element requested
     for (int i = 0; X0 - 1 + i < X1; ++i) {
         const int xe = X0 - 1 + i;
-        commit_plane(xe + 1, (i + 1) & 1);
+        commit_plane((i + 1) & 1);
         const int ph = ph_next;
         if (xe + 1 < X1)
             ph_next = phase_of(xe + 1);
@@ -224,7 +187,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
             for (int c = 0; c < H; ++c)
                 u[cr][c] = src[c];
         }
-        T dv[H];  // d_next at node plane xe (CG mode)
+        T dv[H];  // d at node plane xe (CG mode)
 #pragma unroll
         for (int c = 0; c < H; ++c)
             dv[c] = u[0][c];
@@ -233,7 +196,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
         T             f[8][H];
         if constexpr (ISO) {
             if constexpr (H == 3) {
-                T lam = PL[ph][0], mu = PL[ph][1];
+                T lam = ph < 256 ? PL[ph][0] : a.phases[ph].p[0], mu = ph < 256 ? PL[ph][1] : a.phases[ph].p[1];
                 if (a.vk) {
                     mu  = a.vg[el];
                     lam = a.vk[el] - (T) (2.0 / 3.0) * mu;
@@ -259,7 +222,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
                 element_forces_iso3_modal<T, NGP, CUBIC>(reinterpret_cast<const T(&)[8][3]>(u), a.geom, lw, mw, T0,
                                             reinterpret_cast<T(&)[8][3]>(f));
             } else {
-                const T kw    = (a.vk ? a.vk[el] : PL[ph][0]) * w;
+                const T kw    = (a.vk ? a.vk[el] : (ph < 256 ? PL[ph][0] : a.phases[ph].p[0])) * w;
                 const T T0[3] = {kw * g0[0], kw * g0[1], kw * g0[2]};
                 element_forces_iso1_modal<T, NGP, CUBIC>(reinterpret_cast<const T(&)[8][1]>(u), a.geom, kw, T0,
                                             reinterpret_cast<T(&)[8][1]>(f));
@@ -281,9 +244,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_
                 // an element feeding no owned node reads no state (elastic stand-in)
                 const double *sn = nullptr;
                 if (xe + 1 < X1 && el_needed)
-                    sn = decltype(pm)::state_of(PK[ph_next], a.js, (int64_t) (xe + 1) * splane + sl_own);
-                pm.begin(P, el_needed ? PK[ph] : (int) MODEL_ELASTIC_ISO, a.js, (int64_t) xg * splane + sl_own, sn,
-                         pm_pre);
+                    sn = decltype(pm)::state_of(kind_of(ph_next), a.js, (int64_t) (xe + 1) * splane + sl_own);
+                // owned elements of this block's x range are evaluated here and
+                // nowhere else in the pass: they carry the per-call state updates
+                pm.begin(P, el_needed ? kind_of(ph) : (int) MODEL_ELASTIC_ISO, a.js, (int64_t) xg * splane + sl_own, sn,
+                         pm_pre, own_yz && xe >= X0);
                 pm_pre = true;
                 auto mat = [&](int gp, const T(&e)[NS], T(&s)[NS]) { pm.eval(gp, e, s); };
                 element_forces<T, H, NGP, NS>(u, a.geom, g0, mat, f);

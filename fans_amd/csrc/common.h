@@ -97,6 +97,14 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], const bool (&is_ma
     }
 }
 
+// phase id of voxel i: the device microstructure holds bytes when every id of
+// the grid is below 256 and 16-bit ids (the reference's unsigned short,
+// reader.h:53) otherwise
+__device__ __forceinline__ int phase_id(const uint8_t *ms, int wide, int64_t i)
+{
+    return wide ? (int) reinterpret_cast<const uint16_t *>(ms)[i] : (int) ms[i];
+}
+
 // accumulate |v| into a norm: measure 0 = L1, 1 = L2 (squared), 2 = Linf
 template <typename T>
 __device__ __forceinline__ void acc_norm(double &acc, T v, int measure)

@@ -25,7 +25,9 @@ enum {
     MODEL_PSEUDO_PLASTIC = 6,
     MODEL_SVK            = 7,  // SaintVenantKirchhoff (finite strain)
     MODEL_NEO_HOOKE      = 8,  // CompressibleNeoHookean (finite strain)
-    MODEL_PSEUDO_NL      = 9   // PseudoPlasticNonLinearHardening
+    MODEL_PSEUDO_NL      = 9,  // PseudoPlasticNonLinearHardening
+    MODEL_J2_VISCO_LIN   = 10, // J2ViscoPlastic_LinearIsotropicHardening
+    MODEL_J2_VISCO_NL    = 11  // J2ViscoPlastic_NonLinearIsotropicHardening
 };
 
 // Strain / stress vector length: thermal gradient (3), small strain Mandel (6)
@@ -42,6 +44,7 @@ constexpr int nstr_of()
 //   J2_NEW       K G sigma_y H_iso      PSEUDO       K G sigma_y h eps_crit E_s
 //   SVK          lambda mu              NEO_HOOKE    lambda mu
 //   PSEUDO_NL    K G sigma_y n eps_crit eps_0
+//   J2_VISCO_*   K G sigma_y K_iso H_kin eta dt (sigma_inf delta)
 template <typename T>
 struct Phase {
     int kind;
@@ -99,6 +102,70 @@ __device__ __forceinline__ void j2_point(const double *p, const double (&e)[6], 
         s[i]  = s[i] - gs * st[i];
     }
     qo = q_in + c23 * dgamma;
+}
+
+// J2Plasticity::get_sigma (J2Plasticity.h:65-108) with the isotropic hardening
+// of J2ViscoPlastic_LinearIsotropicHardening (:166-174) or
+// J2ViscoPlastic_NonLinearIsotropicHardening (:197-221), FP64.  p = K G sigma_y
+// K_iso H_kin eta dt (sigma_inf delta); st = committed plastic strain (6),
+// psi_t, psi_bar_t (6).  Returns sigma, the plastic strain eps_t + gamma n and
+// gamma and n, with which the caller advances the current psi (+ gamma
+// sqrt(2/3)) and psi_bar (- gamma n) once per call, as the reference does at
+// every get_sigma (:102-104).  The nonlinear law's Newton loop is the
+// reference's: slope -den (its second term is (2 / 3) * ... in integer
+// arithmetic, i.e. zero, :217), stopping at the first increment not above
+// 1e-10 or after 10 steps.
+__device__ __forceinline__ void visco_point(int kind, const double *p, const double (&e)[6], const double (&st)[13],
+                                            double (&s)[6], double (&ep)[6], double &gamma, double (&n)[6])
+{
+    const double c23 = 0.81649658092772603273;  // sqrt(2/3)
+    const double K = p[0], G = p[1], sy = p[2], Kh = p[3], Hk = p[4];
+    const double psi_t = st[6];
+    double       ee[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+        ee[i] = e[i] - st[i];
+    const double treps = ee[0] + ee[1] + ee[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        s[i] = (K - 2.0 * G / 3.0) * treps + 2 * G * ee[i];
+#pragma unroll
+    for (int i = 3; i < 6; ++i)
+        s[i] = 2 * G * ee[i];
+    const double mean = (s[0] + s[1] + s[2]) / 3.0;
+    double       dmq[6], nrm2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        dmq[i] = (s[i] - (i < 3 ? mean : 0.0)) - (-(2.0 / 3.0) * Hk * st[7 + i]);
+        nrm2 += dmq[i] * dmq[i];
+    }
+    const double nrm  = sqrt(nrm2);
+    const bool   nlaw = kind == MODEL_J2_VISCO_NL;
+    const double q_tr = nlaw ? -Kh * psi_t - (p[7] - sy) * (1 - exp(-p[8] * psi_t)) : -Kh * psi_t;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+        n[i] = nrm < 1e-12 ? 0.0 : dmq[i] / nrm;
+    const double f   = nrm - c23 * (sy - q_tr);
+    const double den = 2 * G + (2.0 / 3.0) * (Kh + Hk) + p[5] / p[6];
+    gamma            = 0.0;
+    if (!(f < 0)) {
+        if (!nlaw) {
+            gamma = f / den;
+        } else {
+            const double sd = c23 * (p[7] - sy), dl = p[8], e0 = exp(-dl * psi_t);
+            double       inc = 1.0;
+            for (int it = 0; inc > 1e-10 && it < 10; ++it) {
+                const double g = f - gamma * den - sd * (-exp(-dl * (psi_t + c23 * gamma)) + e0);
+                inc            = g / den;  // -g / dg with dg = -den
+                gamma += inc;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        s[i] -= gamma * 2 * G * n[i];
+        ep[i] = st[i] + gamma * n[i];
+    }
 }
 
 // PseudoPlasticLinearHardening::get_sigma (PseudoPlastic.h:92-116)
@@ -241,6 +308,10 @@ __device__ __forceinline__ void large_point(int kind, const T *p, const T (&e)[9
 // strain, 6 q) of Gauss point g of slot s = ((x * ny + y) * ntz + tz) * 16 + lz
 // sits at st[(c * ngp + g) * nslot + s].
 constexpr int kStateTile = 15;
+// J2PlasticityNew: components 0..5 plastic strain, 6 q.  J2ViscoPlastic: 0..5
+// plastic strain, 6 psi, 7..12 psi_bar (committed, all slots of an element),
+// 13 psi and 14..19 psi_bar of the step in progress (canonical slot only).
+constexpr int kJ2Comps = 7, kViscoComps = 20;
 struct J2State {
     double *st;     // committed (_t) state
     int64_t nslot;  // slots per component and Gauss point: nx * ny * ntz * 16
@@ -320,7 +391,11 @@ __device__ __forceinline__ void material_point(const Phase<T> &P, const T (&e)[H
 // last Gauss point requests that element's point 0 (begin() then skips it).
 // PNL: the power-law PseudoPlastic model may occur (its pow() pushes the
 // residual kernel past 256 registers, so that kernel has a variant without it).
-template <typename T, int H, int NGP, bool PNL = true>
+// VISCO: the J2ViscoPlastic models may occur; their state is read at the
+// element's slot without prefetch, and with `acc` every evaluation advances the
+// current psi / psi_bar of the slot (an element's canonical slot, evaluated
+// once per pass) like one get_sigma call of the reference.
+template <typename T, int H, int NGP, bool PNL = true, bool VISCO = false>
 struct PointModel {
     const Phase<T> *P;
     int             kind;
@@ -329,6 +404,8 @@ struct PointModel {
     const double   *sn;     // next J2 element of the row (point 0), else null
     int64_t         nel;    // state stride between Gauss points (slots)
     double          nx[7];  // state of the next Gauss point
+    double         *vs;     // VISCO: the element's state slot (Gauss point 0), else null
+    bool            vacc;   // VISCO: advance the current psi / psi_bar
 
     __device__ __forceinline__ static const double *state_of(int kind, const J2State &js, int64_t slot)
     {
@@ -338,7 +415,7 @@ struct PointModel {
     // (from state_of), or null; prefetched: point 0 of this element was
     // requested by the previous element's last point
     __device__ __forceinline__ void begin(const Phase<T> &ph, int knd, const J2State &js, int64_t slot,
-                                          const double *next, bool prefetched)
+                                          const double *next, bool prefetched, bool accumulate = false)
     {
         P    = &ph;
         kind = knd;
@@ -348,12 +425,16 @@ struct PointModel {
         nel = js.nslot;
         sb  = state_of(kind, js, slot);
         sn  = next;
+        if constexpr (VISCO) {
+            vs   = (kind == MODEL_J2_VISCO_LIN || kind == MODEL_J2_VISCO_NL) ? js.st + slot : nullptr;
+            vacc = accumulate;
+        }
         if (!prefetched)
             fetch(0);
     }
-    __device__ __forceinline__ void init(const Phase<T> &ph, const J2State &js, int64_t slot)
+    __device__ __forceinline__ void init(const Phase<T> &ph, const J2State &js, int64_t slot, bool accumulate = false)
     {
-        begin(ph, ph.kind, js, slot, nullptr, false);
+        begin(ph, ph.kind, js, slot, nullptr, false, accumulate);
     }
     __device__ __forceinline__ static void load(const double *b, int gp, int64_t nel, double (&v)[7])
     {
@@ -363,6 +444,7 @@ struct PointModel {
                 v[c] = b[((int64_t) c * NGP + gp) * nel];
     }
     __device__ __forceinline__ void fetch(int gp) { load(sb, gp, nel, nx); }
+    __device__ __forceinline__ void visco_eval(int gp, const T (&e)[H == 1 ? 3 : 6], T (&s)[H == 1 ? 3 : 6]);
     __device__ __forceinline__ void eval(int gp, const T (&e)[H == 1 ? 3 : 6], T (&s)[H == 1 ? 3 : 6])
     {
         if constexpr (H == 1) {
@@ -420,6 +502,11 @@ struct PointModel {
                     s[i] = (T) sd[i];
             } break;
             default:
+                if constexpr (VISCO && sizeof(T) == 8)
+                    if (vs) {
+                        visco_eval(gp, e, s);
+                        break;
+                    }
                 if constexpr (PNL)
                     if (kind == MODEL_PSEUDO_NL) {
                         pseudo_nl_point<T>(pr, e, s);
@@ -431,6 +518,42 @@ struct PointModel {
         }
     }
 };
+
+// one J2ViscoPlastic get_sigma at Gauss point gp (PointModel with VISCO)
+template <typename T, int H, int NGP, bool PNL, bool VISCO>
+__device__ __forceinline__ void visco_gp(PointModel<T, H, NGP, PNL, VISCO> &m, int gp, const T (&e)[6], T (&s)[6])
+{
+    const int64_t nel = m.nel;
+    double        st[13], ed[6], sd[6], epo[6], gamma, n[6], pd[9];
+#pragma unroll
+    for (int c = 0; c < 13; ++c)
+        st[c] = m.vs[((int64_t) c * NGP + gp) * nel];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+        ed[i] = (double) e[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+        pd[i] = (double) m.P->p[i];
+    visco_point(m.kind, pd, ed, st, sd, epo, gamma, n);
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+        s[i] = (T) sd[i];
+    if (m.vacc && gamma != 0.0) {
+        // psi += gamma sqrt(2/3), psi_bar -= gamma n (J2Plasticity.h:103-104)
+        m.vs[((int64_t) 13 * NGP + gp) * nel] += gamma * 0.81649658092772603273;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+            m.vs[((int64_t) (14 + i) * NGP + gp) * nel] -= gamma * n[i];
+    }
+}
+
+template <typename T, int H, int NGP, bool PNL, bool VISCO>
+__device__ __forceinline__ void PointModel<T, H, NGP, PNL, VISCO>::visco_eval(int gp, const T (&e)[H == 1 ? 3 : 6],
+                                                                           T (&s)[H == 1 ? 3 : 6])
+{
+    if constexpr (H == 3)
+        visco_gp(*this, gp, e, s);
+}
 
 // sigma of the linear models only (the linear matvec K d never sees another
 // kind; anything else gives NaN)
@@ -1167,6 +1290,7 @@ struct ElemArgs {
     int             ntz;           // z tiles of the state layout (J2State)
     Geom            geom;
     const uint8_t  *ms;
+    int             ms16;          // phase ids are 16-bit (phase_id)
     const Phase<T> *phases;
     J2State         js;
     const T        *u;
@@ -1217,12 +1341,15 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
         sacc[k] = 0.0;
     int     x = 0, y = 0, z = 0;
     int64_t slot   = tid;
+    int64_t canon  = tid;  // ELEM_COMMIT: the canonical slot of the slot's element
     bool    active = false;
     if constexpr (MODE == ELEM_COMMIT) {
         bool canonical;
         active = tid < a.js.nslot;
-        if (active)
+        if (active) {
             state_slot_element(tid, a.ny, a.nz, a.ntz, x, y, z, canonical);
+            canon = (((int64_t) x * a.ny + y) * a.ntz + z / kStateTile) * 16 + z % kStateTile + 1;
+        }
     } else {
         active = tid < (int64_t) a.nx * a.ny * a.nz;
         if (active) {
@@ -1244,7 +1371,7 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
             for (int c = 0; c < H; ++c)
                 u[cr][c] = a.u[n * H + c];
         }
-        const Phase<T> &P = a.phases[a.ms[el]];
+        const Phase<T> &P = a.phases[phase_id(a.ms, a.ms16, el)];
         T               Gx[2][2][H], Gy[2][2][H], Gz[2][2][H];
         grad_factors<T, H, NGP>(u, a.geom, Gx, Gy, Gz);
         constexpr bool CAN_BBAR = H == 3 && NGP == 8;
@@ -1316,9 +1443,11 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
             }
             return;
         }
-        PointModel<T, H, NGP> pm;
+        // every evaluation of a homogenized-stress or fields pass is one
+        // get_sigma call of the reference (canonical slot: once per element)
+        PointModel<T, H, NGP, true, H == 3 && NS == 6> pm;
         if constexpr (MODE != ELEM_COMMIT && NS != 9)
-            pm.init(P, a.js, slot);
+            pm.init(P, a.js, slot, true);
         double eavg[NS], savg[NS], flag = 0.0;
         bool   pseudo = false;
         if constexpr (MODE == ELEM_FIELDS && H == 3 && NS == 6)
@@ -1342,6 +1471,31 @@ __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
             T e[NS], s[NS];
             grad_to_strain<T, H, NS>(gr, g0, e);
             if constexpr (MODE == ELEM_COMMIT) {
+                if constexpr (H == 3 && NS == 6 && sizeof(T) == 8)
+                    if (P.kind == MODEL_J2_VISCO_LIN || P.kind == MODEL_J2_VISCO_NL) {
+                        // updateInternalVariables (J2Plasticity.h:56-61): the plastic
+                        // strain of the last evaluation (at this u, g0), psi and
+                        // psi_bar of the step from the element's canonical slot
+                        const int64_t ns = a.js.nslot, cs = canon;
+                        double        st[13], ed[6], sd[6], epo[6], gamma, n[6], pd[9];
+#pragma unroll
+                        for (int c = 0; c < 13; ++c)
+                            st[c] = a.js.st[((int64_t) c * NGP + gp) * ns + slot];
+#pragma unroll
+                        for (int c = 0; c < 6; ++c)
+                            ed[c] = (double) e[c];
+#pragma unroll
+                        for (int c = 0; c < 9; ++c)
+                            pd[c] = (double) P.p[c];
+                        visco_point(P.kind, pd, ed, st, sd, epo, gamma, n);
+#pragma unroll
+                        for (int c = 0; c < 6; ++c)
+                            a.js.st[((int64_t) c * NGP + gp) * ns + slot] = epo[c];
+#pragma unroll
+                        for (int c = 0; c < 7; ++c)
+                            a.js.st[((int64_t) (6 + c) * NGP + gp) * ns + slot] =
+                                a.js.st[((int64_t) (13 + c) * NGP + gp) * ns + cs];
+                    }
                 if (P.kind == MODEL_J2_NEW) {
                     double ed[6], epin[6], sd[6], epo[6], qo, pd[4];
 #pragma unroll

@@ -65,9 +65,12 @@ int nparams(int kind)
     case FANS_MODEL_PSEUDO_PLASTIC_NL: return 5;
     case FANS_MODEL_SVK: return 2;
     case FANS_MODEL_NEO_HOOKE: return 2;
+    case FANS_MODEL_J2_VISCO_LIN: return 7;
+    case FANS_MODEL_J2_VISCO_NL: return 9;
     default: return -1;
     }
 }
+bool model_visco(int kind) { return kind == FANS_MODEL_J2_VISCO_LIN || kind == FANS_MODEL_J2_VISCO_NL; }
 bool model_linear(int kind) { return kind >= 1 && kind <= 4; }
 bool model_large(int kind) { return kind == FANS_MODEL_SVK || kind == FANS_MODEL_NEO_HOOKE; }
 
@@ -146,24 +149,34 @@ struct Fft3D {
 
     hipError_t init(int nx_, int ny_, int nz_, int h_)
     {
-        nx = nx_, ny = ny_, nz = nz_, h = h_;
-        M  = nz / 2;
-        Mc = M + 1;
-        if (!make_len(M, 2, Lz) || !make_len(ny, 1, Ly) || !make_len(nx, 1, Lx))
-            return hipErrorInvalidValue;
+        hipError_t e = plan(nx_, ny_, nz_, h_);
+        if (e != hipSuccess)
+            return e;
+        ready   = false;
         auto up = [](cplx<T> *&dst, const std::vector<cplx<T>> &v) {
             hipError_t e = hipMalloc(&dst, sizeof(cplx<T>) * v.size());
             if (e != hipSuccess)
                 return e;
             return hipMemcpy(dst, v.data(), sizeof(cplx<T>) * v.size(), hipMemcpyHostToDevice);
         };
-        hipError_t e;
         if ((e = up(twz, twiddles<T>(nz))) != hipSuccess)
             return e;
         if ((e = up(twy, twiddles<T>(ny))) != hipSuccess)
             return e;
         if ((e = up(twx, twiddles<T>(nx))) != hipSuccess)
             return e;
+        ready = true;
+        return hipSuccess;
+    }
+    // launch shapes and length factorisations only (host arithmetic, no device
+    // memory): the twiddle tables twz / twy / twx are the caller's to provide
+    hipError_t plan(int nx_, int ny_, int nz_, int h_)
+    {
+        nx = nx_, ny = ny_, nz = nz_, h = h_;
+        M  = nz / 2;
+        Mc = M + 1;
+        if (!make_len(M, 2, Lz) || !make_len(ny, 1, Ly) || !make_len(nx, 1, Lx))
+            return hipErrorInvalidValue;
         // One padded LDS tile per workgroup, n * columns <= FFT_MAX_ELEMS so the
         // in-place stages keep every butterfly input in registers.
         const size_t cb = sizeof(cplx<T>);
@@ -217,6 +230,7 @@ struct fans_gpu_ctx {
     std::vector<int>            phase_kind;
     bool                        all_linear = true, has_j2 = false, all_iso = false;
     bool                        has_pnl = false;  // a PseudoPlasticNonLinearHardening phase
+    bool                        has_visco = false;  // a J2ViscoPlastic phase
     bool                        cubic = false;  // hx == hy == hz
     double                      kref[81]{};
     double                      keref[576]{};
@@ -224,7 +238,10 @@ struct fans_gpu_ctx {
     std::vector<Phase<float>>   ph32;
 
     // device buffers
-    uint8_t       *ms = nullptr;
+    uint8_t       *ms = nullptr;  // phase ids: bytes, or 16-bit ids when ms16
+    bool           ms16 = false;
+    int64_t        ms_bytes = 0;  // allocated bytes of ms
+    int            nph_cap  = 0;  // allocated entries of dph64 / dph32
     Phase<double> *dph64 = nullptr;
     Phase<float>  *dph32 = nullptr;
     double        *u = nullptr, *u_prev = nullptr, *r = nullptr, *s = nullptr, *dA = nullptr, *dB = nullptr,
@@ -255,6 +272,7 @@ struct fans_gpu_ctx {
     // launch geometry
     int     cx = 16, nty = 0, ntz = 0, nblk_asm = 0, nblk_vec = 0, nblk_el = 0;
     int64_t nslot = 0;      // J2 state slots per component and Gauss point
+    int     js_comps = 0;   // components of the allocated J2 state
     int     nblk_slot = 0;  // workgroups of the state commit (one thread per slot)
     double  g0[9]{};
     Geom    geom{};
@@ -482,33 +500,64 @@ void launch_strided(fans_gpu_ctx *c, const FftLen &L, const cplx<T> *tw, int bc,
     }
 }
 
+// Split transposes (fourstep_split) for the x+Green pass: measured at 256^3
+// FP64 they took the pass from 0.29 to 0.50 ms (twice the barriers outweigh the
+// doubled residency), so the complex tile stays
+constexpr bool kXGreenSplit = false;
+
+// x+Green launches return their block count (one partial of the spectral
+// <r, s> per block when `partial` is set)
 template <typename T, int H, int N1, int N2>
-void launch_x_green_reg(fans_gpu_ctx *c, cplx<T> *spec, const cplx<T> *tw, const GreenView<T> &gv, int kz_lo,
-                        int kz_hi, const int *done)
+int launch_x_green_reg(fans_gpu_ctx *c, cplx<T> *spec, const cplx<T> *tw, const GreenView<T> &gv, int kz_lo,
+                       int kz_hi, double *partial, const int *done)
 {
     constexpr int C   = reg_cols<T, N1, N2>();
     constexpr int KB  = H == 3 ? (C == 16 ? 5 : C / 4) : C;
     const int     nbk = (kz_hi - kz_lo + KB - 1) / KB;
-    hipLaunchKernelGGL((k_fft_x_green_reg<T, H, N1, N2, KB, C>), dim3(nbk * c->ny), dim3(C * N2), 0, c->stream,
-                       spec, tw, gv, kz_lo, kz_hi, nbk, done);
+    // the complex transpose tile of the 256-point FP64 pass (70 KB) allows two
+    // workgroups per CU; the split tile (35 KB) four
+    constexpr bool SPLIT = kXGreenSplit && (int64_t) N1 * N2 * (C + 1) * sizeof(cplx<T>) > 48 * 1024;
+    hipLaunchKernelGGL((k_fft_x_green_reg<T, H, N1, N2, KB, C, SPLIT>), dim3(nbk * c->ny), dim3(C * N2), 0,
+                       c->stream, spec, tw, gv, kz_lo, kz_hi, nbk, partial, done);
+    return nbk * c->ny;
 }
 
 template <typename T, int H>
-void launch_x_green(fans_gpu_ctx *c, Fft3D<T> &P, cplx<T> *spec, const T *G, int kz_lo, int kz_hi, const int *done)
+int launch_x_green(fans_gpu_ctx *c, Fft3D<T> &P, cplx<T> *spec, const T *G, int kz_lo, int kz_hi, double *partial,
+                   const int *done)
 {
     const GreenView<T> gv{G, c->nx, c->ny, c->gxq, c->gyq, P.Mc};
     switch (reg_len(P.Lx.n) ? P.Lx.n : 0) {
-    case 256: launch_x_green_reg<T, H, 16, 16>(c, spec, P.twx, gv, kz_lo, kz_hi, done); break;
-    case 128: launch_x_green_reg<T, H, 16, 8>(c, spec, P.twx, gv, kz_lo, kz_hi, done); break;
-    case 64: launch_x_green_reg<T, H, 8, 8>(c, spec, P.twx, gv, kz_lo, kz_hi, done); break;
-    case 32: launch_x_green_reg<T, H, 8, 4>(c, spec, P.twx, gv, kz_lo, kz_hi, done); break;
-    case 16: launch_x_green_reg<T, H, 4, 4>(c, spec, P.twx, gv, kz_lo, kz_hi, done); break;
+    case 256: return launch_x_green_reg<T, H, 16, 16>(c, spec, P.twx, gv, kz_lo, kz_hi, partial, done);
+    case 128: return launch_x_green_reg<T, H, 16, 8>(c, spec, P.twx, gv, kz_lo, kz_hi, partial, done);
+    case 64: return launch_x_green_reg<T, H, 8, 8>(c, spec, P.twx, gv, kz_lo, kz_hi, partial, done);
+    case 32: return launch_x_green_reg<T, H, 8, 4>(c, spec, P.twx, gv, kz_lo, kz_hi, partial, done);
+    case 16: return launch_x_green_reg<T, H, 4, 4>(c, spec, P.twx, gv, kz_lo, kz_hi, partial, done);
     default: {
         dim3 g((kz_hi - kz_lo + P.bk - 1) / P.bk, c->ny);
         hipLaunchKernelGGL((k_fft_x_green<T, H>), g, dim3(kThreads), P.smx, c->stream, spec, P.Lx, P.twx, gv, kz_lo,
-                           kz_hi, P.bk, done);
+                           kz_hi, P.bk, partial, done);
+        return (int) (g.x * g.y);
     }
     }
+}
+
+// blocks of one x+Green launch over all kz (the size of its partial array)
+template <typename T>
+int64_t x_green_blocks(const Fft3D<T> &P, int h, int ny)
+{
+    int kb;
+    switch (reg_len(P.Lx.n) ? P.Lx.n : 0) {
+    case 256: kb = reg_cols<T, 16, 16>(); break;
+    case 128: kb = reg_cols<T, 16, 8>(); break;
+    case 64: kb = reg_cols<T, 8, 8>(); break;
+    case 32: kb = reg_cols<T, 8, 4>(); break;
+    case 16: kb = reg_cols<T, 4, 4>(); break;
+    default: return (int64_t) ((P.Mc + P.bk - 1) / P.bk) * ny;
+    }
+    if (h == 3)
+        kb = kb == 16 ? 5 : kb / 4;
+    return (int64_t) ((P.Mc + kb - 1) / kb) * ny;
 }
 
 int reduce(fans_gpu_ctx *c, const double *partial, int nblk, int nq, int stage);
@@ -588,6 +637,10 @@ int convolve_dev(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const T *G
     // run on kz chunks one after the other; a chunk that fits the Infinity
     // Cache keeps the middle passes out of HBM.
     const int nch = std::max(1, std::min(c->fft_chunks, P.Mc));
+    // EPI_CGD: delta = <r, s> from the spectrum (x+Green partials), then
+    // beta (RED_DELTA) before the inverse z pass forms d = s + beta d_prev
+    double *const ep_partial = ep.partial;
+    int           xg_blocks  = 0;
     for (int q = 0; q < nch; ++q) {
         const int kz_lo = (int) ((int64_t) P.Mc * q / nch), kz_hi = (int) ((int64_t) P.Mc * (q + 1) / nch);
         const int c0 = kz_lo * h, c1 = kz_hi * h;
@@ -597,17 +650,22 @@ int convolve_dev(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const T *G
                                      c->nx, done);
         }
         {
-            Timed t(c, KF_FFT_X_GREEN);
-            if (h == 1)
-                launch_x_green<T, 1>(c, P, spec, Gp, kz_lo, kz_hi, done);
-            else
-                launch_x_green<T, 3>(c, P, spec, Gp, kz_lo, kz_hi, done);
+            Timed   t(c, KF_FFT_X_GREEN);
+            double *xp = epi == EPI_CGD ? ep_partial + xg_blocks : nullptr;
+            xg_blocks += h == 1 ? launch_x_green<T, 1>(c, P, spec, Gp, kz_lo, kz_hi, xp, done)
+                                : launch_x_green<T, 3>(c, P, spec, Gp, kz_lo, kz_hi, xp, done);
         }
         {
             Timed t(c, KF_FFT_Y_INV);
             launch_strided<T, true>(c, P.Ly, P.twy, P.bc, P.smy, spec, (int64_t) Kc, c0, c1, (int64_t) c->ny * Kc,
                                     c->nx, done);
         }
+    }
+    if (epi == EPI_CGD) {
+        HIPCHK(c, hipGetLastError());
+        const int rc = reduce(c, ep_partial, xg_blocks, 1, RED_DELTA);
+        if (rc)
+            return rc;
     }
     {
         Timed t(c, KF_FFT_Z_INV);
@@ -616,6 +674,8 @@ int convolve_dev(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const T *G
                 launch_z_c2r<T, 1, EPI_STORE>(c, P, spec, ep, done);
             else if (epi == EPI_CG)
                 launch_z_c2r<T, 1, EPI_CG>(c, P, spec, ep, done);
+            else if (epi == EPI_CGD)
+                launch_z_c2r<T, 1, EPI_CGD>(c, P, spec, ep, done);
             else
                 launch_z_c2r<T, 1, EPI_FP>(c, P, spec, ep, done);
         } else {
@@ -623,6 +683,8 @@ int convolve_dev(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const T *G
                 launch_z_c2r<T, 3, EPI_STORE>(c, P, spec, ep, done);
             else if (epi == EPI_CG)
                 launch_z_c2r<T, 3, EPI_CG>(c, P, spec, ep, done);
+            else if (epi == EPI_CGD)
+                launch_z_c2r<T, 3, EPI_CGD>(c, P, spec, ep, done);
             else
                 launch_z_c2r<T, 3, EPI_FP>(c, P, spec, ep, done);
         }
@@ -646,6 +708,8 @@ AsmArgs<T> asm_args(fans_gpu_ctx *c)
     a.ntz     = c->ntz;
     a.geom    = c->geom;
     a.ms      = c->ms;
+    a.ms16 = c->ms16;
+    a.ms16    = c->ms16;
     a.nphases = c->nphases;
     a.js      = J2State{c->js, c->nslot};
     a.partial = c->partial;
@@ -675,6 +739,10 @@ int launch_assemble(fans_gpu_ctx *c, AsmArgs<T> a)
         else if (c->all_iso)                                                                                    \
             hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_ISO>), dim3(c->nblk_asm), dim3(kThreads), 0,    \
                                c->stream, a);                                                                   \
+        else if (MODE == ASM_RESID && HH == 3 && c->has_visco)                                                  \
+            hipLaunchKernelGGL(                                                                                 \
+                (k_assemble<T, HH, GG, MODE, MODE == ASM_RESID && HH == 3 ? ASM_GENERAL_VISCO : ASM_GENERAL>),   \
+                dim3(c->nblk_asm), dim3(kThreads), 0, c->stream, a);                                            \
         else if (MODE == ASM_RESID && c->has_pnl)                                                               \
             hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, MODE == ASM_RESID ? ASM_GENERAL_PNL : ASM_GENERAL>),   \
                                dim3(c->nblk_asm), dim3(kThreads), 0, c->stream, a);                             \
@@ -783,6 +851,7 @@ int homog_stress(fans_gpu_ctx *c, double *out)
     a.ntz     = c->ntz;
     a.geom    = c->geom;
     a.ms      = c->ms;
+    a.ms16 = c->ms16;
     a.phases  = c->dph64;
     a.js      = J2State{c->js, c->nslot};
     a.u       = c->u;
@@ -839,6 +908,7 @@ int commit_state(fans_gpu_ctx *c)
     a.ntz    = c->ntz;
     a.geom   = c->geom;
     a.ms     = c->ms;
+    a.ms16 = c->ms16;
     a.phases = c->dph64;
     a.js     = J2State{c->js, c->nslot};
     a.u      = c->u;
@@ -991,15 +1061,14 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext
     int        chunk = 4, rc;
     int        is[IS_COUNT];
     bool       first = true;
-    // When the matvec pass of iteration j applies u -= alpha_(j-1) d_(j-1)
-    // (cg_update_u_in_matvec), the update of iteration k-1 is still pending
-    // when the stop test fires in iteration k.  Iteration j writes its
-    // direction into the buffer that was dnext at entry for even j and into
-    // dcur for odd j.
+    // The inverse z pass of iteration j applies u -= alpha_(j-1) d_(j-1), so
+    // the update of iteration k-1 is still pending when the stop test fires in
+    // iteration k.  Iteration j writes its direction into the buffer that was
+    // dnext at entry for even j and into dcur for odd j.
     T *const d_even = dnext, *const d_odd = dcur;
     auto     finish = [&]() -> int {
         const int k = is[IS_ITER];
-        if (cg_update_u_in_matvec<T>() && k >= 1) {
+        if (k >= 1) {
             T *dl = ((k - 1) % 2 == 0) ? d_even : d_odd;
             hipLaunchKernelGGL((k_sub_scaled<T>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, u, dl,
                                c->sc + SC_ALPHA, c->Nh);
@@ -1011,24 +1080,24 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext
     };
     // one CG iteration (with the update of the previous one unless `first`)
     auto iteration = [&](bool with_pro) -> int {
-        const ProArgs<T> pa{rnew, u, dcur, s, c->sc + SC_ALPHA, c->d.measure, c->partial};
-        EpiArgs<T>       ep{};
+        const ProArgs<T> pa{rnew, s, c->sc + SC_ALPHA, c->d.measure, c->partial};
+        // s = -G r, delta and beta from the spectrum, then d = s + beta d_cur
+        // and u -= alpha d_cur (previous iteration) in the inverse z pass
+        EpiArgs<T> ep{};
         ep.out     = s;
-        ep.r       = r;
         ep.partial = c->partial;
+        ep.dprev   = dcur;
+        ep.dnext   = dnext;
+        ep.usol    = u;
+        ep.scal    = c->sc;  // SC_BETA == 0, SC_ALPHA == 1
         int e;
-        if ((e = convolve_dev<T>(c, P, r, spec, Gp, EPI_CG, ep, done, with_pro ? &pa : nullptr)))
+        if ((e = convolve_dev<T>(c, P, r, spec, Gp, EPI_CGD, ep, done, with_pro ? &pa : nullptr)))
             return e;
-        if ((e = reduce(c, c->partial, (int) P.nblk_z, 1, RED_DELTA)))
-            return e;
+        // rnew = K d, <d, rnew>
         AsmArgs<T> a = asm_args<T>(c);
         a.phases     = ph;
-        a.in         = s;
-        a.din        = dcur;
-        a.dout       = dnext;
+        a.in         = dnext;
         a.out        = rnew;
-        a.u          = cg_update_u_in_matvec<T>() ? u : nullptr;  // u -= alpha d_cur (previous iteration)
-        a.scal       = c->sc;  // SC_BETA == 0, SC_ALPHA == 1
         a.done       = done;
         if ((e = launch_assemble<T, ASM_CGLIN>(c, a)))
             return e;
@@ -1350,6 +1419,7 @@ int alloc_fields(fans_gpu_ctx *c)
     HIPCHK(c, hipMalloc(&c->is, sizeof(int) * IS_COUNT));
     HIPCHK(c, hipMalloc(&c->hist, sizeof(double) * (c->d.n_it + 2)));
     HIPCHK(c, hipMalloc(&c->ms, c->N));
+    c->ms_bytes = c->N;
     if (c->d.precision == FANS_PREC_MIXED) {
         const size_t fb = sizeof(float) * c->Nh;
         HIPCHK(c, hipMalloc(&c->r32, fb));
@@ -1362,7 +1432,8 @@ int alloc_fields(fans_gpu_ctx *c)
     }
     // partial sums: the largest grid of any reducing kernel times its quantities
     int64_t need = 2 * std::max<int64_t>({(int64_t) c->nblk_asm, (int64_t) c->nblk_vec, c->fft64.nblk_z,
-                                          c->fft32.nblk_z});
+                                          c->fft32.nblk_z, x_green_blocks(c->fft64, c->h, c->ny),
+                                          x_green_blocks(c->fft32, c->h, c->ny)});
     need         = std::max<int64_t>(need, (int64_t) c->nstr * c->nblk_el);
     c->partial_len = need;
     HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * need));
@@ -1631,15 +1702,32 @@ int fans_gpu_set_microstructure(fans_gpu_ctx *c, const uint16_t *ms, int64_t cou
 {
     if (!c || !ms || count != c->N)
         return fail(c, FANS_GPU_EINVAL, "microstructure must hold n_x*n_y*n_z entries");
-    std::vector<uint8_t> m8(c->N);
-    c->ms_max = -1;
-    for (int64_t i = 0; i < c->N; ++i) {
-        if (ms[i] > 255)
-            return fail(c, FANS_GPU_EUNSUP, "phase ids above 255 are not supported");
-        m8[i]     = (uint8_t) ms[i];
-        c->ms_max = std::max(c->ms_max, (int) ms[i]);
+    int mx = -1;
+    for (int64_t i = 0; i < c->N; ++i)
+        mx = std::max(mx, (int) ms[i]);
+    // bytes on the device when every id fits (1 B per voxel of phase traffic),
+    // else the 16-bit ids as given
+    const bool    wide  = mx > 255;
+    const int64_t bytes = wide ? 2 * c->N : c->N;
+    if (bytes > c->ms_bytes) {
+        (void) hipFree(c->ms);
+        c->ms       = nullptr;
+        c->ms_bytes = 0;
+        HIPCHK(c, hipMalloc(&c->ms, bytes));
+        c->ms_bytes = bytes;
     }
-    HIPCHK(c, copy_on(c, c->ms, m8.data(), c->N, hipMemcpyHostToDevice));
+    if (wide) {
+        HIPCHK(c, copy_on(c, c->ms, ms, bytes, hipMemcpyHostToDevice));
+    } else {
+        std::vector<uint8_t> m8(c->N);
+        for (int64_t i = 0; i < c->N; ++i)
+            m8[i] = (uint8_t) ms[i];
+        HIPCHK(c, copy_on(c, c->ms, m8.data(), bytes, hipMemcpyHostToDevice));
+    }
+    if (wide != c->ms16)
+        drop_graphs(c);  // the captured launches carry the id width by value
+    c->ms16    = wide;
+    c->ms_max  = mx;
     c->have_ms = true;
     return 0;
 }
@@ -1733,8 +1821,8 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
                     (long long) n_params);
     if (maxp < 0)
         return fail(c, FANS_GPU_EINVAL, "MaterialManager: No phases defined");
-    if (maxp > 255)
-        return fail(c, FANS_GPU_EUNSUP, "phase ids above 255 are not supported");
+    if (maxp > 65535)
+        return fail(c, FANS_GPU_EINVAL, "phase ids are unsigned 16-bit (reader.h:53): %d is too large", maxp);
     c->nphases = maxp + 1;
     c->phase_kind.assign(c->nphases, 0);
     std::vector<int>          group_of(c->nphases, -1), local_of(c->nphases, 0);
@@ -1742,13 +1830,16 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
     c->all_linear = true;
     c->has_j2     = false;
     c->has_pnl    = false;
+    c->has_visco  = false;
     int     ofs   = 0;
     int64_t pofs  = 0;
     for (int g = 0; g < ng; ++g) {
         if (!model_linear(kind[g]))
             c->all_linear = false;
-        if (kind[g] == FANS_MODEL_J2_NEW)
+        if (kind[g] == FANS_MODEL_J2_NEW || model_visco(kind[g]))
             c->has_j2 = true;
+        if (model_visco(kind[g]))
+            c->has_visco = true;
         if (kind[g] == FANS_MODEL_PSEUDO_PLASTIC_NL)
             c->has_pnl = true;
         for (int i = 0; i < nphase[g]; ++i) {
@@ -1814,7 +1905,9 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
                 for (int i = 0; i < 6; ++i)
                     C[i * 6 + i] += 2 * mr;
             } else if (kind[g] == FANS_MODEL_J2_NEW || kind[g] == FANS_MODEL_PSEUDO_PLASTIC ||
-                       kind[g] == FANS_MODEL_PSEUDO_PLASTIC_NL) {
+                       kind[g] == FANS_MODEL_PSEUDO_PLASTIC_NL || model_visco(kind[g])) {
+                // mean K, mean G (J2PlasticityNew.h:111-121, PseudoPlastic.h:43-54,
+                // J2Plasticity.h:110-120)
                 double Kb = 0, Gb = 0;
                 for (int i = 0; i < npg; ++i) {
                     Kb += gp[i * npk];
@@ -1865,6 +1958,11 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
             for (int i = 0; i < 4; ++i)
                 P.p[i] = q[i];
             break;
+        case FANS_MODEL_J2_VISCO_LIN:
+        case FANS_MODEL_J2_VISCO_NL:
+            for (int i = 0; i < nparams(P.kind); ++i)
+                P.p[i] = q[i];
+            break;
         case FANS_MODEL_PSEUDO_PLASTIC:
             for (int i = 0; i < 4; ++i)
                 P.p[i] = q[i];
@@ -1891,17 +1989,32 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
         for (int i = 0; i < 36; ++i)
             c->ph32[p].p[i] = (float) P.p[i];
     }
-    if (!c->dph64)
-        HIPCHK(c, hipMalloc(&c->dph64, sizeof(Phase<double>) * 256));
-    if (!c->dph32)
-        HIPCHK(c, hipMalloc(&c->dph32, sizeof(Phase<float>) * 256));
+    if (c->nphases > c->nph_cap) {
+        (void) hipFree(c->dph64);
+        (void) hipFree(c->dph32);
+        c->dph64 = nullptr;
+        c->dph32 = nullptr;
+        c->nph_cap = 0;
+        const int cap = std::max(256, c->nphases);
+        HIPCHK(c, hipMalloc(&c->dph64, sizeof(Phase<double>) * cap));
+        HIPCHK(c, hipMalloc(&c->dph32, sizeof(Phase<float>) * cap));
+        c->nph_cap = cap;
+        drop_graphs(c);  // captured launches hold the old table addresses
+    }
     HIPCHK(c, copy_on(c, c->dph64, c->ph64.data(), sizeof(Phase<double>) * c->nphases, hipMemcpyHostToDevice));
     HIPCHK(c, copy_on(c, c->dph32, c->ph32.data(), sizeof(Phase<float>) * c->nphases, hipMemcpyHostToDevice));
-    // J2 state (zero, Solver ctor solver.h:139)
-    if (c->has_j2 && !c->js) {
-        const size_t bytes = sizeof(double) * 7 * c->ngp * c->nslot;
+    // J2 state (zero, Solver ctor solver.h:139); the visco models' kinematic
+    // and current variables widen it
+    const int ncomp = c->has_visco ? kViscoComps : kJ2Comps;
+    if (c->has_j2 && (!c->js || c->js_comps < ncomp)) {
+        (void) hipFree(c->js);
+        c->js       = nullptr;
+        c->js_comps = 0;
+        const size_t bytes = sizeof(double) * ncomp * c->ngp * c->nslot;
         HIPCHK(c, hipMalloc(&c->js, bytes));
         HIPCHK(c, fill_on(c, c->js, bytes));
+        c->js_comps = ncomp;
+        drop_graphs(c);
     }
     c->voxel_moduli = false;  // new phase tables replace per-voxel moduli
     if (const int rcg = rebuild_green(c))
@@ -2159,7 +2272,7 @@ int fans_gpu_reset_state(fans_gpu_ctx *c)
     if ((rc = fill<double>(c, c->u, 0.0, c->Nh)) || (rc = fill<double>(c, c->u_prev, 0.0, c->Nh)) ||
         (rc = fill<double>(c, c->r, 0.0, c->Nh)))
         return rc;
-    if (c->js && (rc = fill<double>(c, c->js, 0.0, (int64_t) 7 * c->ngp * c->nslot)))
+    if (c->js && (rc = fill<double>(c, c->js, 0.0, (int64_t) c->js_comps * c->ngp * c->nslot)))
         return rc;
     c->mixed_active = false;
     std::memset(c->gvec, 0, sizeof(c->gvec));
@@ -2193,6 +2306,7 @@ int fans_gpu_postprocess(fans_gpu_ctx *c, double *strain_elem, double *stress_el
     a.ntz         = c->ntz;
     a.geom        = c->geom;
     a.ms          = c->ms;
+    a.ms16 = c->ms16;
     a.phases      = c->dph64;
     a.js          = J2State{c->js, c->nslot};
     a.u           = c->u;
@@ -2227,10 +2341,10 @@ int fans_gpu_postprocess(fans_gpu_ctx *c, double *strain_elem, double *stress_el
 #undef EP
     HIPCHK(c, hipGetLastError());
     std::vector<double>  he(c->N * ns), hs(c->N * ns);
-    std::vector<uint8_t> hm(c->N);
+    std::vector<uint8_t> hm(c->ms16 ? 2 * c->N : c->N);
     HIPCHK(c, hipMemcpyAsync(he.data(), de, sizeof(double) * c->N * ns, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(hs.data(), ds, sizeof(double) * c->N * ns, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(hm.data(), c->ms, c->N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hm.data(), c->ms, hm.size(), hipMemcpyDeviceToHost, c->stream));
     if (strain_gp)
         HIPCHK(c, hipMemcpyAsync(strain_gp, dge, sizeof(double) * c->N * ns * ng, hipMemcpyDeviceToHost, c->stream));
     if (stress_gp)
@@ -2252,7 +2366,7 @@ int fans_gpu_postprocess(fans_gpu_ctx *c, double *strain_elem, double *stress_el
     if (phase_counts)
         std::memset(phase_counts, 0, sizeof(int32_t) * n_slots);
     for (int64_t e = 0; e < c->N; ++e) {
-        const int p = hm[e];
+        const int p = c->ms16 ? (int) reinterpret_cast<const uint16_t *>(hm.data())[e] : (int) hm[e];
         for (int j = 0; j < ns; ++j) {
             sa[j] += hs[e * ns + j];
             ea[j] += he[e * ns + j];
@@ -2296,29 +2410,29 @@ int fans_gpu_get_field(fans_gpu_ctx *c, int32_t which, double *host, int64_t cou
         src = c->r;
     else if (which == FANS_FIELD_U_PREV)
         src = c->u_prev;
-    else if (which == FANS_FIELD_PLASTIC_STRAIN || which == FANS_FIELD_HARDENING) {
+    else if (which == FANS_FIELD_PLASTIC_STRAIN || which == FANS_FIELD_HARDENING ||
+             which == FANS_FIELD_KIN_HARDENING) {
         const int ng = c->ngp;
-        n            = which == FANS_FIELD_PLASTIC_STRAIN ? c->N * ng * 6 : c->N * ng;
+        const int nc = which == FANS_FIELD_HARDENING ? 1 : 6;  // components per Gauss point
+        const int c0 = which == FANS_FIELD_PLASTIC_STRAIN ? 0 : (which == FANS_FIELD_HARDENING ? 6 : 7);
+        n            = c->N * ng * nc;
         if (count != n)
             return fail(c, FANS_GPU_EINVAL, "field size mismatch (expected %lld)", (long long) n);
-        if (!c->js) {
+        if (!c->js || c0 + nc > c->js_comps) {
             std::memset(host, 0, sizeof(double) * n);
             return 0;
         }
-        std::vector<double> st((size_t) 7 * ng * c->nslot);
-        HIPCHK(c, copy_on(c, st.data(), c->js, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
+        std::vector<double> st((size_t) nc * ng * c->nslot);
+        HIPCHK(c, copy_on(c, st.data(), c->js + (size_t) c0 * ng * c->nslot, sizeof(double) * st.size(),
+                          hipMemcpyDeviceToHost));
         // element (x, y, z) from its canonical slot: tile z / kStateTile, position z % kStateTile + 1
         for (int64_t e = 0; e < c->N; ++e) {
             const int     z  = (int) (e % c->nz);
             const int64_t xy = e / c->nz;
             const int64_t sl = (xy * c->ntz + z / kStateTile) * 16 + z % kStateTile + 1;
-            for (int g = 0; g < ng; ++g) {
-                if (which == FANS_FIELD_PLASTIC_STRAIN)
-                    for (int k = 0; k < 6; ++k)
-                        host[(e * ng + g) * 6 + k] = st[((size_t) k * ng + g) * c->nslot + sl];
-                else
-                    host[e * ng + g] = st[((size_t) 6 * ng + g) * c->nslot + sl];
-            }
+            for (int g = 0; g < ng; ++g)
+                for (int k = 0; k < nc; ++k)
+                    host[(e * ng + g) * nc + k] = st[((size_t) k * ng + g) * c->nslot + sl];
         }
         return 0;
     } else if (which == FANS_FIELD_PLASTIC_FLAG) {
@@ -2339,6 +2453,7 @@ int fans_gpu_get_field(fans_gpu_ctx *c, int32_t which, double *host, int64_t cou
         a.ntz          = c->ntz;
         a.geom         = c->geom;
         a.ms           = c->ms;
+        a.ms16 = c->ms16;
         a.phases       = c->dph64;
         a.js           = J2State{c->js, c->nslot};
         a.u            = c->u;
@@ -2399,10 +2514,7 @@ int fans_gpu_residual(fans_gpu_ctx *c, const double *u, double *r, int32_t linea
         AsmArgs<float> a = asm_args<float>(c);
         a.phases         = c->dph32;
         a.in             = c->s32;
-        a.din            = c->s32;  // d = s + 0 * d
-        a.dout           = c->dA32;
         a.out            = c->rnew32;
-        a.scal           = c->sc;
         if ((rc = launch_assemble<float, ASM_CGLIN>(c, a)))
             return rc;
         hipLaunchKernelGGL((k_convert<double, float>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->rnew,
@@ -2412,17 +2524,11 @@ int fans_gpu_residual(fans_gpu_ctx *c, const double *u, double *r, int32_t linea
         return 0;
     }
     if (linear_kd) {
-        // rnew = K d with d = s + 0 * d: feed s = u and beta = 0
-        HIPCHK(c, copy_on(c, c->s, u, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
-        if ((rc = init_scalars(c, 0.0, 0, 1)))
-            return rc;
+        // rnew = K u (the CG matvec with d = u)
         AsmArgs<double> a = asm_args<double>(c);
         a.phases          = c->dph64;
-        a.in              = c->s;
-        a.din             = c->dB;
-        a.dout            = c->dA;
+        a.in              = c->dB;
         a.out             = c->rnew;
-        a.scal            = c->sc;
         rc                = launch_assemble<double, ASM_CGLIN>(c, a);
     } else {
         rc = residual64(c, c->dB, c->rnew, nullptr);
@@ -2581,6 +2687,7 @@ int fans_gpu_sensitivity(fans_gpu_ctx *c, const double *ub, const double *g0b, d
     a.ntz    = c->ntz;
     a.geom   = c->geom;
     a.ms     = c->ms;
+    a.ms16 = c->ms16;
     a.phases = c->dph64;
     a.js     = J2State{c->js, c->nslot};
     a.u      = c->u;
@@ -2683,6 +2790,7 @@ int fans_gpu_fft_c2r(int32_t dev, int32_t nx, int32_t ny, int32_t nz, int32_t h,
         return FANS_GPU_EINVAL;
     return fft_standalone(dev, nx, ny, nz, h, in, out, false);
 }
+
 
 int fans_gpu_set_option(fans_gpu_ctx *c, int32_t opt, int64_t value)
 {

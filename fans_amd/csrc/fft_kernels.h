@@ -123,24 +123,13 @@ __device__ void lds_fft(cplx<T> *buf, const FftLen &L, const cplx<T> *__restrict
 // update of iteration i-1 (solverCG.h:106-117): r -= alpha rnew, and emits per
 // block the partials |r| (norm of the measure) and <r, s_prev>, which the
 // reduction turns into the error and deltamid.  Only the updated r enters the
-// transform.
+// transform.  The matching solution update u -= alpha d runs in the inverse z
+// pass of the next iteration (EPI_CGD), which reads that d anyway.
 enum { PRO_NONE = 0, PRO_CG = 1 };
-
-// Where the matching solution update u -= alpha d of iteration i-1 runs.  FP64:
-// in the matvec pass of iteration i, which reads d anyway (one vector less per
-// iteration: 88.8 -> 87.8 ms per 256^3 solve).  FP32: here, in the forward z
-// pass (moved into the FP32 matvec pass it cost as much as it saved).
-template <typename T>
-constexpr bool cg_update_u_in_matvec()
-{
-    return sizeof(T) == 8;
-}
 
 template <typename T>
 struct ProArgs {
     const T      *rnew;     // K d of the previous iteration
-    T            *u;        // solution (or the inner correction); FP32 only
-    const T      *d;        // previous search direction; FP32 only
     const T      *s;        // previous preconditioned residual
     const double *alpha;    // device scalar
     int           measure;  // 0 L1, 1 L2, 2 Linf
@@ -152,15 +141,11 @@ __device__ __forceinline__ T z_load(T *__restrict__ in, int64_t gi, const ProArg
                                     double &a1)
 {
     if constexpr (PRO == PRO_CG) {
-        // r, rnew, s (and FP32 u, d) are not read again before the spectrum
+        // r, rnew and s are not read again before the spectrum
         // passes that follow: streaming (non-temporal) accesses keep them from
         // displacing the freshly written spectrum in the Infinity Cache
         const T rv = __builtin_nontemporal_load(&in[gi]) - alpha * __builtin_nontemporal_load(&pa.rnew[gi]);
         __builtin_nontemporal_store(rv, &in[gi]);
-        if constexpr (!cg_update_u_in_matvec<T>())
-            __builtin_nontemporal_store(__builtin_nontemporal_load(&pa.u[gi]) -
-                                            alpha * __builtin_nontemporal_load(&pa.d[gi]),
-                                        &pa.u[gi]);
         acc_norm(a0, rv, pa.measure);
         a1 += (double) rv * (double) __builtin_nontemporal_load(&pa.s[gi]);
         return rv;
@@ -236,14 +221,22 @@ __global__ __launch_bounds__(256) void k_fft_z_r2c(T *__restrict__ in, cplx<T> *
 }
 
 // ---- z pass, inverse c2r with fused epilogues -------------------------------
-enum { EPI_STORE = 0, EPI_CG = 1, EPI_FP = 2 };
+// EPI_CGD (linear CG, solverCG.h:88-92 and 104): s = -result, the new search
+// direction d = s + beta d_prev (beta from the spectral <r, s> of the x pass)
+// and the previous iteration's solution update u -= alpha d_prev, which needs
+// d_prev, read here anyway.
+enum { EPI_STORE = 0, EPI_CG = 1, EPI_FP = 2, EPI_CGD = 3 };
 
 template <typename T>
 struct EpiArgs {
-    T            *out;      // EPI_STORE: result; EPI_CG: s = -result
+    T            *out;      // EPI_STORE: result; EPI_CG / EPI_CGD: s = -result
     const T      *r;        // EPI_CG: partial <r, s>
     double       *u;        // EPI_FP: u -= result (FP64 displacement)
     double       *partial;  // EPI_CG: one partial sum per block
+    const T      *dprev;    // EPI_CGD: previous direction (read)
+    T            *dnext;    // EPI_CGD: new direction (written)
+    T            *usol;     // EPI_CGD: solution, u -= alpha d_prev
+    const double *scal;     // EPI_CGD: device scalars (beta at [0], alpha at [1])
 };
 
 // c2r pre-processing of one (line, component): `line` points at kz = 0 of the
@@ -260,10 +253,17 @@ __device__ __forceinline__ cplx<T> c2r_tangle(const cplx<T> *__restrict__ line, 
 }
 
 template <typename T, int EPI>
-__device__ __forceinline__ void z_epilogue(const EpiArgs<T> &ep, int64_t gi, T val, double &acc)
+__device__ __forceinline__ void z_epilogue(const EpiArgs<T> &ep, int64_t gi, T val, double &acc, T beta, T alpha)
 {
     if constexpr (EPI == EPI_STORE) {
         ep.out[gi] = val;
+    } else if constexpr (EPI == EPI_CGD) {
+        // d_next is read by the very next pass (the matvec) and stays cached;
+        // s (read one iteration later), d_prev and u stream past the caches
+        const T sv = -val, dp = __builtin_nontemporal_load(&ep.dprev[gi]);
+        __builtin_nontemporal_store(sv, &ep.out[gi]);
+        ep.dnext[gi] = sv + beta * dp;
+        __builtin_nontemporal_store(__builtin_nontemporal_load(&ep.usol[gi]) - alpha * dp, &ep.usol[gi]);
     } else if constexpr (EPI == EPI_CG) {
         // s is read by the very next pass (the matvec) and stays cached; r is
         // not read again this iteration
@@ -311,10 +311,15 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const cplx<T> *__restrict__ i
     const int     nzh = nz * H;
     const int64_t g0  = line0 * nzh;
     double        acc = 0.0;
+    T             beta = (T) 0, alpha = (T) 0;
+    if constexpr (EPI == EPI_CGD) {
+        beta  = (T) ep.scal[0];
+        alpha = (T) ep.scal[1];
+    }
     for (int e = threadIdx.x; e < nl * nzh; e += blockDim.x) {
         const int     l = e / nzh, rem = e - l * nzh, z = rem / H, c = rem - z * H;
         const cplx<T> v = A[(z >> 1) * ld + l * H + c];
-        z_epilogue<T, EPI>(ep, g0 + e, (z & 1) ? v.y : v.x, acc);
+        z_epilogue<T, EPI>(ep, g0 + e, (z & 1) ? v.y : v.x, acc, beta, alpha);
     }
     z_epi_partials<EPI>(ep, acc, red);
 }
@@ -459,11 +464,16 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cp
     lds_fft_ct<T, true, M, NC, LD, NT>(A, tw);
     const int64_t g0  = line0 * NZH;
     double        acc = 0.0;
+    T             beta = (T) 0, alpha = (T) 0;
+    if constexpr (EPI == EPI_CGD) {
+        beta  = (T) ep.scal[0];
+        alpha = (T) ep.scal[1];
+    }
 #pragma unroll 4
     for (int e = threadIdx.x; e < LPB * NZH; e += NT) {
         const int     l = e / NZH, rem = e - l * NZH, z = rem / H, c = rem - z * H;
         const cplx<T> v = A[(z >> 1) * LD + l * H + c];
-        z_epilogue<T, EPI>(ep, g0 + e, (z & 1) ? v.y : v.x, acc);
+        z_epilogue<T, EPI>(ep, g0 + e, (z & 1) ? v.y : v.x, acc, beta, alpha);
     }
     z_epi_partials<EPI>(ep, acc, red);
 }
@@ -495,17 +505,29 @@ __global__ __launch_bounds__(256) void k_fft_strided(cplx<T> *__restrict__ data,
     }
 }
 
+// ---- spectral <r, s> --------------------------------------------------------
+// With s = -G * r, the unnormalised r2c / c2r pair and G holding 1/N,
+// sum_x r(x) s(x) = sum over the full spectrum of conj(r^) s^ (Parseval); the
+// half spectrum kz in [0, nz/2] counts the frequencies 0 < kz < nz/2 twice
+// (their mirrors -kz are the complex conjugates) and kz = 0, nz/2 once.
+__device__ __forceinline__ double kz_weight(int kz, int Mc) { return (kz == 0 || kz == Mc - 1) ? 1.0 : 2.0; }
+
 // ---- x pass with the Green operator between forward and inverse ------------
 // spectrum [x][y][kz][H]; block (bx, y) owns kz in [kz_lo + bx*bk, ... + bk) and all H.
 // G packed per frequency [x][y][kz][H(H+1)/2] = (00 01 02 11 12 22) or (0).
+// partial (nullable): one partial of <r, s> (s = -G r) per block, block index
+// blockIdx.y * gridDim.x + blockIdx.x.
 template <typename T, int H>
 __global__ __launch_bounds__(256) void k_fft_x_green(cplx<T> *__restrict__ data, FftLen L,
                                                      const cplx<T> *__restrict__ tw, GreenView<T> gv, int kz_lo,
-                                                     int kz_hi, int bk, const int *__restrict__ done)
+                                                     int kz_hi, int bk, double *__restrict__ partial,
+                                                     const int *__restrict__ done)
 {
     if (done && *done)
         return;
     const int ny = gv.ny, Mc = gv.Mc;
+    __shared__ double red[8];
+    double            acc = 0.0;
     extern __shared__ char smem_raw[];
     cplx<T>      *A     = reinterpret_cast<cplx<T> *>(smem_raw);
     const int     y     = blockIdx.y;
@@ -525,14 +547,19 @@ __global__ __launch_bounds__(256) void k_fft_x_green(cplx<T> *__restrict__ data,
         T         sx, sy;
         const T  *g = green_at<T, H>(gv, i, y, kz0 + kk, sx, sy);
         cplx<T>  *v = A + i * ld + kk * H;
+        const double wk = kz_weight(kz0 + kk, Mc);
         if constexpr (H == 1) {
-            v[0] = cscale(v[0], g[0]);
+            const cplx<T> a = v[0];
+            v[0]            = cscale(a, g[0]);
+            acc -= wk * ((double) a.x * v[0].x + (double) a.y * v[0].y);
         } else {
             const cplx<T> a = v[0], b = v[1], c = v[2];
             const T       g00 = g[0], g01 = g[1] * sx * sy, g02 = g[2] * sx, g11 = g[3], g12 = g[4] * sy, g22 = g[5];
             v[0]                = cplx<T>{g00 * a.x + g01 * b.x + g02 * c.x, g00 * a.y + g01 * b.y + g02 * c.y};
             v[1]                = cplx<T>{g01 * a.x + g11 * b.x + g12 * c.x, g01 * a.y + g11 * b.y + g12 * c.y};
             v[2]                = cplx<T>{g02 * a.x + g12 * b.x + g22 * c.x, g02 * a.y + g12 * b.y + g22 * c.y};
+            acc -= wk * ((double) a.x * v[0].x + (double) a.y * v[0].y + (double) b.x * v[1].x +
+                         (double) b.y * v[1].y + (double) c.x * v[2].x + (double) c.y * v[2].y);
         }
     }
     __syncthreads();
@@ -540,6 +567,13 @@ __global__ __launch_bounds__(256) void k_fft_x_green(cplx<T> *__restrict__ data,
     for (int w = threadIdx.x; w < L.n * ncols; w += blockDim.x) {
         const int i     = w / ncols, c = w - i * ncols;
         base[i * sn + c] = A[i * ld + c];
+    }
+    if (partial) {
+        double     v1[1] = {acc};
+        const bool mx[1] = {false};
+        block_reduce<1>(v1, mx, red);
+        if (threadIdx.x == 0)
+            partial[blockIdx.y * gridDim.x + blockIdx.x] = v1[0];
     }
 }
 

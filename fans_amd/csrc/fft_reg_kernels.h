@@ -12,6 +12,7 @@
 // transform, a pointwise operation and an inverse transform chain without a
 // second exchange (used by the x pass around the Green operator).
 #pragma once
+#include <type_traits>
 #include "fft_butterflies.h"
 #include "fft_kernels.h"
 
@@ -97,6 +98,50 @@ __device__ __forceinline__ void fourstep(cplx<T> (&v)[N1], int j, cplx<T> *tile,
             v[q * N2 + k2] = w[k2];
     }
     __syncthreads();
+}
+
+// fourstep() with the transpose done in two halves (real parts, then imaginary
+// parts) through a real-valued tile of half the size: the same data movement
+// with twice the barriers, for kernels whose occupancy the full complex tile
+// limits.  `tile` holds N1 * N2 rows of `ld` reals.
+template <typename T, int N1, int N2, bool INV>
+__device__ __forceinline__ void fourstep_split(cplx<T> (&v)[N1], int j, T *tile, int ld, int col,
+                                               const cplx<T> *__restrict__ tw)
+{
+    dft_reg<T, N1, INV>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) {
+        const cplx<T> w = tw[j * k1];
+        v[k1]           = INV ? cmulc(v[k1], w) : cmul(v[k1], w);
+    }
+    cplx<T> w[N1 / N2][N2];
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1)
+            tile[(k1 * N2 + ((j + k1) & (N2 - 1))) * ld + col] = part ? v[k1].y : v[k1].x;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < N1 / N2; ++q) {
+            const int k1 = j + N2 * q;
+#pragma unroll
+            for (int i2 = 0; i2 < N2; ++i2) {
+                const T x = tile[(k1 * N2 + ((i2 + k1) & (N2 - 1))) * ld + col];
+                if (part)
+                    w[q][i2].y = x;
+                else
+                    w[q][i2].x = x;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < N1 / N2; ++q) {
+        dft_reg<T, N2, INV>(w[q]);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2)
+            v[q * N2 + k2] = w[q][k2];
+    }
 }
 
 // register order of the four-step output -> input order of the next transform
@@ -193,20 +238,32 @@ __device__ __forceinline__ float row_shift(float x)
 // idle).  Packing in threes took the 256^3 pass from 0.313 to 0.293 ms (FP64)
 // and 0.186 to 0.163 ms (FP32).
 // Logical block (bk, y) of the XCD-mapped 1-D grid (nbk * ny blocks) owns kz
-// in [kz_lo + bk*KB, ... + KB).
-template <typename T, int H, int N1, int N2, int KB, int C>
+// in [kz_lo + bk*KB, ... + KB).  partial (nullable): the block's partial of the
+// spectral <r, s>, s = -G r (kz_weight), at partial[blockIdx.x].
+// SPLIT: the transposes go through a real-valued tile of half the size
+// (fourstep_split), which lifts the LDS limit on resident workgroups.
+template <typename T, int H, int N1, int N2, int KB, int C, bool SPLIT>
 __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__ data, const cplx<T> *__restrict__ tw,
                                                           GreenView<T> gv, int kz_lo, int kz_hi, int nbk,
-                                                          const int *__restrict__ done)
+                                                          double *__restrict__ partial, const int *__restrict__ done)
 {
     if (done && *done)
         return;
+    __shared__ double red[(C * N2) / 64];
+    double            acc = 0.0;
     constexpr int  N    = N1 * N2;
     constexpr bool GRP3 = H == 3 && C == 3 * KB + 1;
     constexpr int  HP   = H == 3 ? (GRP3 ? 3 : 4) : 1;
     constexpr int  LD   = C + 1;
     static_assert(GRP3 || C == KB * HP, "column layout");
-    __shared__ cplx<T> tile[N * LD];
+    __shared__ cplx<T> tile_c[SPLIT ? 1 : N * LD];
+    __shared__ T       tile_r[SPLIT ? N * LD : 1];
+    auto               transform = [&](cplx<T>(&v)[N1], auto inv) {
+        if constexpr (SPLIT)
+            fourstep_split<T, N1, N2, decltype(inv)::value>(v, threadIdx.x / C, tile_r, LD, threadIdx.x % C, tw);
+        else
+            fourstep<T, N1, N2, decltype(inv)::value>(v, threadIdx.x / C, tile_c, LD, threadIdx.x % C, tw);
+    };
     const int          ny = gv.ny, Mc = gv.Mc;
     const int          t = threadIdx.x, col = t % C, j = t / C;
     const int          kk = col / HP, c = col - HP * kk;
@@ -224,8 +281,9 @@ __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__
 #pragma unroll
     for (int m = 0; m < N1; ++m)
         v[m] = ok ? base[(int64_t) (j + N2 * m) * sn] : cplx<T>{(T) 0, (T) 0};
-    fourstep<T, N1, N2, false>(v, j, tile, LD, col, tw);
-    const int kzg = kz < kz_hi ? kz : kz_lo;  // keep G reads in range for idle lanes
+    transform(v, std::false_type{});
+    const int    kzg = kz < kz_hi ? kz : kz_lo;  // keep G reads in range for idle lanes
+    const double wk  = kz_weight(kz, Mc);
 #pragma unroll
     for (int q = 0; q < N1 / N2; ++q)
 #pragma unroll
@@ -234,6 +292,7 @@ __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__
             T             sx, sy;
             const T      *g = green_at<T, H>(gv, row, y, kzg, sx, sy);
             cplx<T>      &a = v[q * N2 + k2];
+            const cplx<T> a0 = a;  // r^ of this lane's (frequency, component)
             if constexpr (H == 1) {
                 a = cscale(a, g[0]);
             } else {
@@ -263,15 +322,25 @@ __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__
                 }
                 a = cplx<T>{ga * v0.x + gb * v1.x + gc * v2.x, ga * v0.y + gb * v1.y + gc * v2.y};
             }
+            // idle lanes hold zeros (or, for kz past kz_hi, are masked here)
+            if (ok)
+                acc -= wk * ((double) a0.x * a.x + (double) a0.y * a.y);
         }
     fourstep_out_to_in<T, N1, N2>(v);
-    fourstep<T, N1, N2, true>(v, j, tile, LD, col, tw);
+    transform(v, std::true_type{});
     if (ok)
 #pragma unroll
         for (int q = 0; q < N1 / N2; ++q)
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2)
                 base[(int64_t) (j + N2 * q + N1 * k2) * sn] = v[q * N2 + k2];
+    if (partial) {
+        double     v1[1] = {acc};
+        const bool mx[1] = {false};
+        block_reduce<1>(v1, mx, red);
+        if (threadIdx.x == 0)
+            partial[blockIdx.x] = v1[0];
+    }
 }
 
 }  // namespace fans

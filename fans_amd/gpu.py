@@ -22,7 +22,8 @@ METHOD = {"cg": 0, "fp": 1}
 MEASURE = {"L1": 0, "L2": 1, "Linfinity": 2}
 ERR_TYPE = {"absolute": 0, "relative": 1}
 PRECISION = {"fp64": 0, "mixed": 1}
-FIELD = {"u": 0, "r": 1, "u_prev": 2, "plastic_strain": 3, "hardening": 4, "plastic_flag": 5}
+FIELD = {"u": 0, "r": 1, "u_prev": 2, "plastic_strain": 3, "hardening": 4, "plastic_flag": 5,
+         "kinematic_hardening": 6}
 OPT_TIMING = 1
 OPT_FFT_CHUNKS = 2
 OPT_GRAPHS = 3
@@ -296,7 +297,7 @@ class GpuSolver:
         self._check(self.L.fans_gpu_set_option(self.ptr, OPT_GRAPHS, int(bool(on))))
 
     def get_field(self, name):
-        if name in ("plastic_strain",):
+        if name in ("plastic_strain", "kinematic_hardening"):
             out = np.zeros((self.N, self.ngp, 6))
         elif name == "hardening":
             out = np.zeros((self.N, self.ngp))

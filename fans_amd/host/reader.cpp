@@ -43,6 +43,13 @@ const std::map<std::string, std::vector<std::string>> &model_keys()
                                                 "eps_0"};
         k["SaintVenantKirchhoff"]         = {"bulk_modulus", "shear_modulus"};
         k["CompressibleNeoHookean"]       = {"bulk_modulus", "shear_modulus"};
+        // J2Plasticity.h:12-18 (+ :187-188 for the nonlinear law)
+        k["J2ViscoPlastic_LinearIsotropicHardening"] = {
+            "bulk_modulus", "shear_modulus", "yield_stress", "isotropic_hardening_parameter",
+            "kinematic_hardening_parameter", "viscosity", "time_step"};
+        k["J2ViscoPlastic_NonLinearIsotropicHardening"] = {
+            "bulk_modulus", "shear_modulus", "yield_stress", "isotropic_hardening_parameter",
+            "kinematic_hardening_parameter", "viscosity", "time_step", "saturation_stress", "saturation_exponent"};
         return k;
     }();
     return m;
@@ -247,6 +254,12 @@ void Reader::ReadInputFile(const char *input_fn)
             const json         &v = props[g.keys[k]];
             std::vector<double> vals =
                 v.is_array() ? v.get<std::vector<double>>() : std::vector<double>{v.get<double>()};
+            if (g.keys[k] == "time_step") {
+                // one number for the whole model (J2Plasticity.h:19), repeated per phase
+                if (!v.is_number())
+                    throw std::invalid_argument("Material property 'time_step' must be a number");
+                vals.assign(np, v.get<double>());
+            }
             if (vals.size() != np)
                 throw std::invalid_argument("Inconsistent size for material property: " + g.keys[k]);
             for (size_t p = 0; p < np; ++p)

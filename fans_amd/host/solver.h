@@ -48,6 +48,10 @@ inline int model_kind(const std::string &m)
         return FANS_MODEL_SVK;
     if (m == "CompressibleNeoHookean")
         return FANS_MODEL_NEO_HOOKE;
+    if (m == "J2ViscoPlastic_LinearIsotropicHardening")
+        return FANS_MODEL_J2_VISCO_LIN;
+    if (m == "J2ViscoPlastic_NonLinearIsotropicHardening")
+        return FANS_MODEL_J2_VISCO_NL;
     throw std::invalid_argument(m + " is not a supported matmodel");
 }
 
@@ -70,9 +74,20 @@ class MaterialManager {
     }
     bool has_j2() const
     {
-        for (const auto &g : reader_.materials)
-            if (model_kind(g.matmodel) == FANS_MODEL_J2_NEW)
+        for (const auto &g : reader_.materials) {
+            const int k = model_kind(g.matmodel);
+            if (k == FANS_MODEL_J2_NEW || k == FANS_MODEL_J2_VISCO_LIN || k == FANS_MODEL_J2_VISCO_NL)
                 return true;
+        }
+        return false;
+    }
+    bool has_visco() const
+    {
+        for (const auto &g : reader_.materials) {
+            const int k = model_kind(g.matmodel);
+            if (k == FANS_MODEL_J2_VISCO_LIN || k == FANS_MODEL_J2_VISCO_NL)
+                return true;
+        }
         return false;
     }
     bool has_pseudo_plastic() const
@@ -216,11 +231,21 @@ void Solver<howmany, n_str>::postprocess(Reader &reader, int load_idx, int time_
     std::vector<double>  strain_gp(w_eg ? N * n_gp * n_str : 0), stress_gp(w_sg ? N * n_gp * n_str : 0);
     std::vector<double>  sa(n_str), ea(n_str), psa((size_t) nm * n_str), pea((size_t) nm * n_str);
     std::vector<int32_t> pc(nm);
-    check(ctx_,
-          fans_gpu_postprocess(ctx_, w_e ? strain_elem.data() : nullptr, w_s ? stress_elem.data() : nullptr,
-                               w_eg ? strain_gp.data() : nullptr, w_sg ? stress_gp.data() : nullptr, sa.data(),
-                               ea.data(), psa.data(), pea.data(), pc.data(), nm),
-          "postprocess");
+    // the element pass runs only when a result needs it (solver.h:460-479):
+    // otherwise the averages stay zero, as in the reference, and no material
+    // evaluation happens (which the J2ViscoPlastic state would count)
+    bool need_compute = w_s || w_e || w_sg || w_eg || reader.wants("stress_average") || reader.wants("strain_average");
+    for (int m = 0; m < nm && !need_compute; ++m) {
+        char nmn[64];
+        std::snprintf(nmn, sizeof(nmn), "phase_stress_average_phase%d", m);
+        need_compute = reader.wants(nmn);
+    }
+    if (need_compute)
+        check(ctx_,
+              fans_gpu_postprocess(ctx_, w_e ? strain_elem.data() : nullptr, w_s ? stress_elem.data() : nullptr,
+                                   w_eg ? strain_gp.data() : nullptr, w_sg ? stress_gp.data() : nullptr, sa.data(),
+                                   ea.data(), psa.data(), pea.data(), pc.data(), nm),
+              "postprocess");
     std::printf("# Effective Stress .. (");
     for (int i = 0; i < n_str; ++i)
         std::printf("%+.12f ", sa[i]);
@@ -320,6 +345,25 @@ void Solver<howmany, n_str>::postprocess(Reader &reader, int load_idx, int time_
                 reader.writeSlab("plastic_strain_gp", load_idx, time_idx, ep.data(), {n_gp, 6});
             if (w_qg)
                 reader.writeSlab("isotropic_hardening_variable_gp", load_idx, time_idx, q.data(), {n_gp});
+        }
+        // J2Plasticity::postprocess (J2Plasticity.h:246-318): the kinematic
+        // variable psi_bar as well
+        const bool w_k = reader.wants("kinematic_hardening_variable"),
+                   w_kg = reader.wants("kinematic_hardening_variable_gp");
+        if (mat_.has_visco() && (w_k || w_kg)) {
+            std::vector<double> kb(N * n_gp * 6);
+            check(ctx_, fans_gpu_get_field(ctx_, FANS_FIELD_KIN_HARDENING, kb.data(), (int64_t) kb.size()),
+                  "get_field kinematic_hardening");
+            if (w_k) {
+                std::vector<double> km(N * 6, 0.0);
+                for (size_t e = 0; e < N; ++e)
+                    for (int g = 0; g < n_gp; ++g)
+                        for (int c = 0; c < 6; ++c)
+                            km[e * 6 + c] += kb[(e * n_gp + g) * 6 + c] / n_gp;
+                reader.writeSlab("kinematic_hardening_variable", load_idx, time_idx, km.data(), {6});
+            }
+            if (w_kg)
+                reader.writeSlab("kinematic_hardening_variable_gp", load_idx, time_idx, kb.data(), {n_gp, 6});
         }
     }
 

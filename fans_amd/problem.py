@@ -28,7 +28,17 @@ MODEL_PARAMS = {
                                         "eps_0"),
     "SaintVenantKirchhoff": ("bulk_modulus", "shear_modulus"),
     "CompressibleNeoHookean": ("bulk_modulus", "shear_modulus"),
+    "J2ViscoPlastic_LinearIsotropicHardening": ("bulk_modulus", "shear_modulus", "yield_stress",
+                                                "isotropic_hardening_parameter", "kinematic_hardening_parameter",
+                                                "viscosity", "time_step"),
+    "J2ViscoPlastic_NonLinearIsotropicHardening": ("bulk_modulus", "shear_modulus", "yield_stress",
+                                                   "isotropic_hardening_parameter", "kinematic_hardening_parameter",
+                                                   "viscosity", "time_step", "saturation_stress",
+                                                   "saturation_exponent"),
 }
+# keys the reference reads as one number for the whole model (J2Plasticity.h:19);
+# every phase of the group carries a copy
+SCALAR_PARAMS = {"time_step"}
 MODEL_KIND = {
     "LinearThermalIsotropic": 1,
     "LinearThermalTriclinic": 2,
@@ -39,6 +49,8 @@ MODEL_KIND = {
     "SaintVenantKirchhoff": 7,
     "CompressibleNeoHookean": 8,
     "PseudoPlasticNonLinearHardening": 9,
+    "J2ViscoPlastic_LinearIsotropicHardening": 10,
+    "J2ViscoPlastic_NonLinearIsotropicHardening": 11,
 }
 # createMatmodel<3,9> (include/setup.h:62-73): finite strain, F and P row major
 LARGE_STRAIN_MODELS = {"SaintVenantKirchhoff", "CompressibleNeoHookean"}
@@ -63,6 +75,10 @@ class MaterialGroup:
             if k not in self.properties:
                 raise ValueError(f"Missing material property '{k}' for {self.matmodel}")
             v = np.atleast_1d(np.asarray(self.properties[k], dtype=np.float64))
+            if k in SCALAR_PARAMS:
+                if np.ndim(self.properties[k]) != 0:
+                    raise ValueError(f"Material property '{k}' must be a number")
+                v = np.full(len(self.phases), float(v[0]))
             if v.size != len(self.phases):
                 raise ValueError(f"Inconsistent size for material property: {k}")
             cols.append(v)

@@ -32,9 +32,15 @@ from .problem import MaterialGroup, Problem
 
 
 class MicroSimulation:
-    """pyfans/micro.cpp:28-93 on the GPU path (mechanical, small strain)."""
+    """pyfans/micro.cpp:28-93 on the GPU path (mechanical, small strain).
 
-    def __init__(self, sim_id, input_file, precision="fp64", pert_param=1e-6):
+    MicroSimulation(sim_id) reads "input.json" from the working directory like
+    the reference's constructor (micro.hpp:22, default argument; the pybind11
+    module binds the int-only form, micro.cpp:93).  input_file, precision and
+    pert_param are keyword extensions; pert_param is the reference's scalar
+    strain perturbation of the tangent (micro.hpp:31)."""
+
+    def __init__(self, sim_id: int, input_file="input.json", precision="fp64", pert_param=1e-6):
         self.sim_id = sim_id
         self.problem = Problem.from_json(input_file)
         if self.problem.howmany != 3:

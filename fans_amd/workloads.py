@@ -18,7 +18,10 @@ matmodel.h:284-304), the order get_homogenized_tangent uses
                         steps, zero lateral stress), CG L-inf 1e-8 per step
     headline_problem()  256^3 elastic: config 3's sphere law with config 2's
                         materials, Mandel e_11, CG L-inf 1e-10
-(config 4 lives in pyfans.config4_problem, config 5 in batch.config5_rves.)
+    smoothed_density() / config4_objective()
+                        config 4: SIMP density on a smoothed Gaussian
+                        field, objective C_1111 + C_2323 (pyfans autograd)
+(config 5 lives in batch.config5_rves.)
 """
 from __future__ import annotations
 
@@ -92,3 +95,30 @@ def config3_ramp(n_steps=20, emax=0.02):
     return dict(strain_indices=[0, 3, 4, 5], stress_indices=[1, 2],
                 strain_path=np.stack([strain, 0 * strain, 0 * strain, 0 * strain], 1),
                 stress_path=np.zeros((n_steps, 2)), n_steps=n_steps)
+
+
+def smoothed_density(n, corr=8.0, seed=0):
+    """Configuration 4's density: a seeded Gaussian random field smoothed with
+    a periodic Gaussian filter of correlation length `corr` voxels, through a
+    sigmoid, rho in (0, 1)."""
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, n, n))
+    k = np.fft.fftfreq(n)
+    kx, ky, kz = np.meshgrid(k, k, k, indexing="ij")
+    filt = np.exp(-0.5 * (2 * np.pi * corr) ** 2 * (kx ** 2 + ky ** 2 + kz ** 2))
+    f = np.real(np.fft.ifftn(np.fft.fftn(x) * filt))
+    f /= f.std()
+    return 1.0 / (1.0 + np.exp(-3.0 * f))
+
+
+def config4_objective(solver, rho, params):
+    """C_1111 + C_2323 of the homogenized tangent (Mandel entry (5,5) / 2) from
+    two load cases, SIMP moduli of `rho` with params = (E_solid, nu_solid,
+    E_void, nu_void); differentiable through pyfans.homogenized_stress."""
+    import torch
+
+    from .pyfans import homogenized_stress, simp_moduli
+    K, G = simp_moduli(rho, *params)
+    e1 = torch.tensor([1.0, 0, 0, 0, 0, 0], dtype=torch.float64)
+    e6 = torch.tensor([0, 0, 0, 0, 0, 1.0], dtype=torch.float64)
+    return homogenized_stress(solver, K, G, e1)[0] + 0.5 * homogenized_stress(solver, K, G, e6)[5]

@@ -67,9 +67,17 @@ enum {
                                       (finite strain, setup.h:62-73)                       */
     FANS_MODEL_NEO_HOOKE      = 8, /* CompressibleNeoHookean [bulk_modulus shear_modulus]
                                       (finite strain)                                      */
-    FANS_MODEL_PSEUDO_PLASTIC_NL = 9 /* PseudoPlasticNonLinearHardening
+    FANS_MODEL_PSEUDO_PLASTIC_NL = 9, /* PseudoPlasticNonLinearHardening
                                       [bulk_modulus shear_modulus yield_stress
                                        hardening_exponent eps_0]                          */
+    FANS_MODEL_J2_VISCO_LIN   = 10, /* J2ViscoPlastic_LinearIsotropicHardening
+                                      [bulk_modulus shear_modulus yield_stress
+                                       isotropic_hardening_parameter
+                                       kinematic_hardening_parameter viscosity time_step]
+                                      (J2Plasticity.h:162-178; time_step repeated per phase) */
+    FANS_MODEL_J2_VISCO_NL    = 11  /* J2ViscoPlastic_NonLinearIsotropicHardening
+                                      [the seven above, saturation_stress saturation_exponent]
+                                      (J2Plasticity.h:182-243)                             */
 };
 /* fields for fans_gpu_get_field / fans_gpu_set_field */
 enum {
@@ -78,8 +86,11 @@ enum {
     FANS_FIELD_U_PREV         = 2, /* v_u_prev, double [N][h]                          */
     FANS_FIELD_PLASTIC_STRAIN = 3, /* committed plastic strain, double [N][n_gp][6]    */
     FANS_FIELD_HARDENING      = 4, /* committed isotropic hardening q, double [N][n_gp] */
-    FANS_FIELD_PLASTIC_FLAG   = 5  /* PseudoPlastic::postprocess plastic_flag at the current
+    FANS_FIELD_PLASTIC_FLAG   = 5, /* PseudoPlastic::postprocess plastic_flag at the current
                                       u (PseudoPlastic.h:56-64), double [N]; get only   */
+    FANS_FIELD_KIN_HARDENING  = 6  /* committed kinematic variable psi_bar of the
+                                      J2ViscoPlastic models (J2Plasticity.h:150),
+                                      double [N][n_gp][6]; get only                     */
 };
 
 typedef struct {

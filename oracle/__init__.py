@@ -81,6 +81,8 @@ def lib():
         L.or_fft_r2c.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
         L.or_fft_c2r.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
         L.or_material_point.argtypes = [C.c_int, dp, dp, dp, C.c_double, dp, dp, dp]
+        L.or_visco_point.argtypes = [C.c_int, dp, dp, dp, C.c_double, dp, dp, dp, dp, dp]
+        L.or_get_kinematic_state.argtypes = [vp, dp, C.c_int]
         L.or_model_nparams.argtypes = [C.c_int]
         _LIB = L
     return _LIB
@@ -125,6 +127,20 @@ def material_point(kind, params, eps, ep_t=None, q_t=0.0):
     q = np.zeros(1)
     lib().or_material_point(kind, _dp(params), _dp(eps), _dp(ep_t), float(q_t), _dp(sig), _dp(ep), _dp(q))
     return sig, ep, float(q[0])
+
+
+def visco_point(kind, params, eps, ep_t=None, psi_t=0.0, pb_t=None, psi=0.0, pb=None):
+    """One J2Plasticity::get_sigma call: (sigma, ep, psi, psi_bar) with psi and
+    psi_bar accumulated from the given current values."""
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    eps = np.ascontiguousarray(eps, dtype=np.float64)
+    ep_t = np.zeros(6) if ep_t is None else np.ascontiguousarray(ep_t, dtype=np.float64)
+    pb_t = np.zeros(6) if pb_t is None else np.ascontiguousarray(pb_t, dtype=np.float64)
+    pb = np.zeros(6) if pb is None else np.array(pb, dtype=np.float64)
+    sig, ep, ps = np.zeros(6), np.zeros(6), np.array([float(psi)])
+    lib().or_visco_point(kind, _dp(params), _dp(eps), _dp(ep_t), float(psi_t), _dp(pb_t), _dp(sig), _dp(ep),
+                         _dp(ps), _dp(pb))
+    return sig, ep, float(ps[0]), pb
 
 
 class Oracle:
@@ -241,6 +257,12 @@ class Oracle:
         q = np.zeros((self.N, self.ngp))
         self.L.or_get_state(self.ptr, _dp(ep), _dp(q), int(committed))
         return ep, q
+
+    def get_kinematic_state(self, committed=True):
+        """J2ViscoPlastic psi_bar per element and Gauss point, [N][n_gp][6]."""
+        pb = np.zeros((self.N, self.ngp, 6))
+        self.L.or_get_kinematic_state(self.ptr, _dp(pb), int(committed))
+        return pb
 
     def element_fields(self, n_slots=None):
         ns, ng = self.nstr, self.ngp

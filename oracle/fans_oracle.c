@@ -344,7 +344,8 @@ struct or_solver {
     double   *u, *u_prev, *r, *s, *d, *rnew, *rghost;
     double    g0[MAXSTR];
     double   *ep, *ep_t, *q, *q_t; /* per element per Gauss point state */
-    int       has_state;
+    double   *pb, *pb_t;           /* J2ViscoPlastic kinematic variable psi_bar (6 per point) */
+    int       has_state, has_visco;
     double    alpha_warm;
     double   *err_all;
     int       iter;
@@ -371,6 +372,8 @@ int or_model_nparams(int kind)
     case OR_MODEL_PSEUDO_NL: return 5;
     case OR_MODEL_SVK: return 2;
     case OR_MODEL_NEO_HOOKE: return 2;
+    case OR_MODEL_J2_VISCO_LIN: return 7;
+    case OR_MODEL_J2_VISCO_NL: return 9;
     default: return -1;
     }
 }
@@ -563,6 +566,8 @@ void or_destroy(or_solver *S)
     free(S->ep_t);
     free(S->q);
     free(S->q_t);
+    free(S->pb);
+    free(S->pb_t);
     free(S->FE);
     free(S->PF);
     for (int g = 0; g < S->ngroups; g++)
@@ -833,16 +838,24 @@ int or_set_materials(or_solver *S, int n_groups, const int32_t *kind, const int3
         for (int j = 0; j < ne; j++)
             S->keref[((i % S->h) * 8 + i / S->h) * ne + (j % S->h) * 8 + j / S->h] = tmp[i * ne + j];
     /* internal variables (Solver ctor solver.h:139) */
-    S->has_state = 0;
-    for (int g = 0; g < n_groups; g++)
+    S->has_state = S->has_visco = 0;
+    for (int g = 0; g < n_groups; g++) {
         if (S->groups[g].kind == OR_MODEL_J2_NEW)
             S->has_state = 1;
+        if (S->groups[g].kind == OR_MODEL_J2_VISCO_LIN || S->groups[g].kind == OR_MODEL_J2_VISCO_NL)
+            S->has_state = S->has_visco = 1;
+    }
     if (S->has_state) {
+        /* initializeInternalVariables (J2PlasticityNew.h:29-35, J2Plasticity.h:46-54): zeros */
         size_t ng = (size_t) S->N * S->ngp;
         S->ep     = (double *) calloc(ng * 6, sizeof(double));
         S->ep_t   = (double *) calloc(ng * 6, sizeof(double));
         S->q      = (double *) calloc(ng, sizeof(double));
         S->q_t    = (double *) calloc(ng, sizeof(double));
+        if (S->has_visco) {
+            S->pb   = (double *) calloc(ng * 6, sizeof(double));
+            S->pb_t = (double *) calloc(ng * 6, sizeof(double));
+        }
     }
     build_green(S);
     return 0;
@@ -906,6 +919,79 @@ static void j2new_point(const double *p, const double *e, const double *ep_in, d
     if (s_t > 1e-12)
         for (int i = 0; i < 6; i++)
             s[i] = s[i] - dgamma * 2.0 * G * st[i] / s_t;
+}
+
+/* J2Plasticity::get_sigma (J2Plasticity.h:65-108) with the hardening laws of
+ * J2ViscoPlastic_LinearIsotropicHardening (:162-178) and
+ * J2ViscoPlastic_NonLinearIsotropicHardening (:182-243).  p = [K G sigma_y
+ * K_iso H_kin eta dt (sigma_inf delta)].  The trial state reads the committed
+ * plastic strain ep_t, psi_t and psi_bar_t; ep is overwritten, while psi and
+ * psi_bar (the current variables) accumulate gamma at every call, as the
+ * reference does (:102-104).  The Newton iteration of the nonlinear law uses
+ * the reference's derivative, whose (2 / 3) is integer division, i.e. 0 (:217). */
+static void visco_point(int kind, const double *p, const double *e, const double *ep_t, double psi_t,
+                        const double *pb_t, double *s, double *ep, double *psi, double *pb)
+{
+    const double c23 = sqrt(2.0 / 3.0);
+    const double K = p[0], G = p[1], sy = p[2], Kh = p[3], Hk = p[4], eta = p[5], dt = p[6];
+    double       ee[6], dev[6], dmq[6], n[6];
+    for (int i = 0; i < 6; i++)
+        ee[i] = e[i] - ep_t[i];
+    const double treps = ee[0] + ee[1] + ee[2];
+    for (int i = 0; i < 3; i++)
+        s[i] = (K - 2.0 * G / 3.0) * treps;
+    for (int i = 0; i < 3; i++)
+        s[i] += 2 * G * ee[i];
+    for (int i = 3; i < 6; i++)
+        s[i] = 2 * G * ee[i];
+    const double mean = (s[0] + s[1] + s[2]) / 3.0;
+    for (int i = 0; i < 6; i++)
+        dev[i] = s[i] - (i < 3 ? mean : 0.0);
+    double q_tr;
+    if (kind == OR_MODEL_J2_VISCO_LIN)
+        q_tr = -Kh * psi_t;
+    else
+        q_tr = -Kh * psi_t - (p[7] - sy) * (1 - exp(-p[8] * psi_t));
+    double nrm2 = 0;
+    for (int i = 0; i < 6; i++) {
+        dmq[i] = dev[i] - (-(2.0 / 3.0) * Hk * pb_t[i]);
+        nrm2 += dmq[i] * dmq[i];
+    }
+    const double nrm = sqrt(nrm2);
+    for (int i = 0; i < 6; i++)
+        n[i] = nrm < 1e-12 ? 0.0 : dmq[i] / nrm;
+    const double f   = nrm - c23 * (sy - q_tr);
+    const double den = 2 * G + (2.0 / 3.0) * (Kh + Hk) + eta / dt;
+    double       gamma = 0.0;
+    if (f >= 0) {
+        if (kind == OR_MODEL_J2_VISCO_LIN) {
+            gamma = f / den;
+        } else {
+            const double sd = c23 * (p[7] - sy), dl = p[8];
+            double       inc = 1;
+            int          it  = 0;
+            while (inc > 1e-10 && it < 10) {
+                double g  = f - gamma * den - sd * (-exp(-dl * (psi_t + c23 * gamma)) + exp(-dl * psi_t));
+                double dg = -den - (2 / 3) * (p[7] - sy) * dl * exp(-dl * (psi_t + c23 * gamma));
+                inc       = -g / dg;
+                gamma += inc;
+                it++;
+            }
+        }
+    }
+    for (int i = 0; i < 6; i++) {
+        s[i] -= gamma * 2 * G * n[i];
+        ep[i] = ep_t[i] + gamma * n[i];
+    }
+    *psi += gamma * c23;
+    for (int i = 0; i < 6; i++)
+        pb[i] -= gamma * n[i];
+}
+
+void or_visco_point(int kind, const double *params, const double *eps, const double *ep_t, double psi_t,
+                    const double *pb_t, double *sigma, double *ep, double *psi, double *pb)
+{
+    visco_point(kind, params, eps, ep_t, psi_t, pb_t, sigma, ep, psi, pb);
 }
 
 /* PseudoPlasticLinearHardening::get_sigma (PseudoPlastic.h:92-116) */
@@ -1179,6 +1265,10 @@ static void element_strain_stress(or_solver *S, const double *ue, int phase, ptr
         if (kind == OR_MODEL_J2_NEW) {
             size_t gi = (size_t) el * S->ngp + p;
             j2new_point(pp, e, S->ep_t + gi * 6, S->q_t[gi], s, S->ep + gi * 6, S->q + gi);
+        } else if (kind == OR_MODEL_J2_VISCO_LIN || kind == OR_MODEL_J2_VISCO_NL) {
+            size_t gi = (size_t) el * S->ngp + p;
+            visco_point(kind, pp, e, S->ep_t + gi * 6, S->q_t[gi], S->pb_t + gi * 6, s, S->ep + gi * 6, S->q + gi,
+                        S->pb + gi * 6);
         } else {
             or_material_point(kind, pp, e, NULL, 0.0, s, NULL, NULL);
         }
@@ -1553,10 +1643,12 @@ int or_solve(or_solver *S)
         solve_fp(S);
     else
         solve_cg(S);
-    if (S->has_state) { /* J2PlasticityNew::updateInternalVariables (:37-41) */
+    if (S->has_state) { /* updateInternalVariables (J2PlasticityNew.h:37-41, J2Plasticity.h:56-61) */
         size_t ng = (size_t) S->N * S->ngp;
         memcpy(S->ep_t, S->ep, sizeof(double) * ng * 6);
         memcpy(S->q_t, S->q, sizeof(double) * ng);
+        if (S->has_visco)
+            memcpy(S->pb_t, S->pb, sizeof(double) * ng * 6);
     }
     return S->iter;
 }
@@ -1614,6 +1706,15 @@ void or_extrapolate_displacement(or_solver *S)
 void or_get_u(or_solver *S, double *out) { memcpy(out, S->u, sizeof(double) * (size_t) S->N * S->h); }
 void or_set_u(or_solver *S, const double *in) { memcpy(S->u, in, sizeof(double) * (size_t) S->N * S->h); }
 void or_get_r(or_solver *S, double *out) { memcpy(out, S->r, sizeof(double) * (size_t) S->N * S->h); }
+
+void or_get_kinematic_state(or_solver *S, double *pb, int committed)
+{
+    size_t ng = (size_t) S->N * S->ngp;
+    if (!S->has_visco)
+        memset(pb, 0, sizeof(double) * ng * 6);
+    else
+        memcpy(pb, committed ? S->pb_t : S->pb, sizeof(double) * ng * 6);
+}
 
 void or_get_state(or_solver *S, double *pe, double *q, int committed)
 {

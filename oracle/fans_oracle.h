@@ -55,8 +55,12 @@ enum {
     OR_MODEL_PSEUDO_PLASTIC  = 6, /* [bulk shear yield_stress hardening]             */
     OR_MODEL_SVK             = 7, /* SaintVenantKirchhoff [bulk shear] (large strain) */
     OR_MODEL_NEO_HOOKE       = 8, /* CompressibleNeoHookean [bulk shear] (large strain) */
-    OR_MODEL_PSEUDO_NL       = 9  /* PseudoPlasticNonLinearHardening
+    OR_MODEL_PSEUDO_NL       = 9, /* PseudoPlasticNonLinearHardening
                                      [bulk shear yield_stress hardening_exponent eps_0] */
+    OR_MODEL_J2_VISCO_LIN    = 10, /* J2ViscoPlastic_LinearIsotropicHardening [bulk shear
+                                      yield_stress iso_hardening kin_hardening viscosity time_step] */
+    OR_MODEL_J2_VISCO_NL     = 11  /* J2ViscoPlastic_NonLinearIsotropicHardening [the 7 above,
+                                      saturation_stress saturation_exponent] */
 };
 
 typedef struct {
@@ -107,6 +111,8 @@ void        or_get_u(or_solver *S, double *out);
 void        or_set_u(or_solver *S, const double *in);
 void        or_get_r(or_solver *S, double *out);
 void        or_get_state(or_solver *S, double *plastic_strain, double *q, int committed);
+/* J2ViscoPlastic kinematic variable psi_bar, [N][n_gp][6] (zeros for other models) */
+void        or_get_kinematic_state(or_solver *S, double *psi_bar, int committed);
 
 /* PseudoPlastic::postprocess (PseudoPlastic.h:56-64): per element the mean of
  * the Gauss-point plastic flags (0 for elements of other models) */
@@ -132,6 +138,9 @@ int         or_n_gp(or_solver *S);
 /* standalone transforms, layout [x][y][z][h] <-> [x][y][kz][h] */
 void        or_fft_r2c(int nx, int ny, int nz, int h, const double *in, double *out_cplx);
 void        or_fft_c2r(int nx, int ny, int nz, int h, const double *in_cplx, double *out);
+/* one J2Plasticity::get_sigma call (J2Plasticity.h:65-108): psi / psi_bar accumulate */
+void        or_visco_point(int kind, const double *params, const double *eps, const double *ep_t, double psi_t,
+                           const double *pb_t, double *sigma, double *ep, double *psi, double *pb);
 /* single Gauss point stress update for closed-form checks */
 void        or_material_point(int kind, const double *params, const double *eps, const double *ep_t, double q_t,
                               double *sigma, double *ep, double *q);

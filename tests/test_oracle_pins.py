@@ -813,3 +813,130 @@ def test_pseudo_linear_hardening_beyond_yield():
         assert abs(s[:3].mean() - K * tr) < 1e-13 * max(abs(K * tr), 1.0)
         assert np.max(np.abs(ds * np.linalg.norm(de) - de * np.linalg.norm(ds))) < \
             1e-12 * np.linalg.norm(ds) * np.linalg.norm(de)
+
+
+# ----------------------------------------------------------------------------- J2ViscoPlastic
+VISCO = dict(K=62.5, G=28.8462, sy=0.1, Kh=0.0, Hk=0.0, eta=1.0, dt=0.01, sinf=0.15, delta=1000.0)  # test_J2Plasticity.json
+
+
+def _visco_params(nonlinear, **kw):
+    v = dict(VISCO, **kw)
+    p = [v["K"], v["G"], v["sy"], v["Kh"], v["Hk"], v["eta"], v["dt"]]
+    return np.array(p + ([v["sinf"], v["delta"]] if nonlinear else []))
+
+
+def test_visco_linear_closed_form():
+    """Linear law, one call from a virgin state: the trial deviator n = dev(2G e)
+    / |.|, gamma = f / (2G + 2/3 (K_iso + H_kin) + eta / dt) with f = |dev| -
+    sqrt(2/3) sigma_y, so the returned deviatoric norm exceeds the yield radius
+    by exactly gamma (eta / dt + 2/3 (K_iso + H_kin)); flow is deviatoric, the
+    pressure is K tr(eps), psi = sqrt(2/3) gamma and psi_bar = -gamma n."""
+    rng = np.random.default_rng(12)
+    for Kh, Hk in ((0.0, 0.0), (1.5, 0.7)):
+        prm = _visco_params(False, Kh=Kh, Hk=Hk)
+        K, G, sy, eta, dt = prm[0], prm[1], prm[2], prm[5], prm[6]
+        for _ in range(20):
+            e = rng.standard_normal(6) * 0.004
+            s, ep, psi, pb = oracle.visco_point(10, prm, e)
+            dev_e = e.copy()
+            dev_e[:3] -= e[:3].mean()
+            nt = np.linalg.norm(2 * G * dev_e)
+            f = nt - np.sqrt(2 / 3) * sy
+            gamma = max(0.0, f / (2 * G + 2 / 3 * (Kh + Hk) + eta / dt))
+            dev = s.copy()
+            dev[:3] -= s[:3].mean()
+            assert abs(np.linalg.norm(dev) - (nt - 2 * G * gamma)) < 1e-12 * nt
+            if gamma > 0:
+                assert abs(np.linalg.norm(dev) - np.sqrt(2 / 3) * sy - gamma * (eta / dt + 2 / 3 * (Kh + Hk))) < 1e-12
+            assert abs(psi - np.sqrt(2 / 3) * gamma) < 1e-15
+            assert np.max(np.abs(pb + ep)) < 1e-15  # psi_bar = -gamma n, ep = gamma n from a virgin state
+            assert abs(ep[:3].sum()) < 1e-15
+            assert abs(s[:3].mean() - K * e[:3].sum()) < 1e-12
+
+
+def test_visco_nonlinear_iteration():
+    """Nonlinear (saturation) law, gamma of one call from a virgin state against
+    the root of g(x) = f - x den - sqrt(2/3) (s_inf - s_y) (1 - e^(-d sqrt(2/3) x))
+    found independently by bisection.  The reference's Newton loop uses the slope
+    -den alone (its (2 / 3) * ... term is integer division, J2Plasticity.h:217)
+    and stops at the first increment that is not above 1e-10 -- signed, not
+    absolute -- so from gamma = 0 it overshoots the root once and stops:
+    gamma = x1 + g(x1) / den with x1 = f / den, within O(r^2) of the root for
+    r = sqrt(2/3)^2 (s_inf - s_y) d / den.  With the reference's exponent
+    (1000, r = 0.2) it stays visibly off the root; with a small one (5) it is
+    within r^2."""
+    from scipy.optimize import brentq
+    e = np.array([0.004, -0.002, -0.002, 0.0008, 0.0, 0.0])  # d sqrt(2/3) gamma ~ 1 at d = 1000
+    for dl, close in ((5.0, True), (1000.0, False)):
+        prm = _visco_params(True, delta=dl)
+        K, G, sy, Kh, Hk, eta, dt, sinf, _ = prm
+        den = 2 * G + 2 / 3 * (Kh + Hk) + eta / dt
+        c = np.sqrt(2 / 3)
+        dev_e = e.copy()
+        dev_e[:3] -= e[:3].mean()
+        f = np.linalg.norm(2 * G * dev_e) - c * sy
+        g = lambda x: f - x * den - c * (sinf - sy) * (1.0 - np.exp(-dl * c * x))
+        root = brentq(g, 0.0, f / den, xtol=1e-18)
+        _, _, psi, _ = oracle.visco_point(11, prm, e)
+        gamma = psi / c
+        r = c * c * (sinf - sy) * dl / den
+        assert gamma < f / den and abs(gamma - root) <= 2 * r * r * root
+        if close:
+            assert abs(gamma - root) < 1e-5 * root
+        else:
+            assert abs(gamma - root) > 1e-4 * root  # a converged Newton solve would sit on the root
+
+
+def test_visco_accumulates_per_call():
+    """psi and psi_bar gain gamma at every get_sigma call within a step
+    (J2Plasticity.h:103-104) while the plastic strain is overwritten: two calls
+    at the same strain double the increments and leave ep unchanged."""
+    prm = _visco_params(False, Kh=0.5, Hk=0.2)
+    e = np.array([0.01, -0.004, -0.004, 0.003, 0.001, 0.0])
+    s1, ep1, psi1, pb1 = oracle.visco_point(10, prm, e)
+    s2, ep2, psi2, pb2 = oracle.visco_point(10, prm, e, psi=psi1, pb=pb1)
+    assert psi1 > 0
+    assert np.array_equal(s1, s2) and np.array_equal(ep1, ep2)
+    assert abs(psi2 - 2 * psi1) < 1e-16 and np.max(np.abs(pb2 - 2 * pb1)) < 1e-16
+
+
+def test_visco_homogeneous_rve_steps():
+    """Homogeneous RVE under a strain path: the residual vanishes at the start,
+    so each solve makes exactly one get_sigma call per Gauss point (the initial
+    residual) before committing, and the homogenized stress after it one more;
+    the state then follows a single-point integration with those calls: the
+    current psi / psi_bar carry every call's increment and become the committed
+    ones at the next solve's commit."""
+    dims = (4, 4, 4)
+    for kind, name in ((10, "J2ViscoPlastic_LinearIsotropicHardening"),
+                       (11, "J2ViscoPlastic_NonLinearIsotropicHardening")):
+        prm = _visco_params(kind == 11, Kh=0.4, Hk=0.3)
+        keys = ("bulk_modulus", "shear_modulus", "yield_stress", "isotropic_hardening_parameter",
+                "kinematic_hardening_parameter", "viscosity", "time_step", "saturation_stress",
+                "saturation_exponent")
+        props = {k: ([v] if k != "time_step" else v) for k, v in zip(keys, prm)}
+        p = Problem(dims=dims, howmany=3, ms=np.zeros(dims, np.uint16), groups=[MaterialGroup(name, [0], props)],
+                    tol=1e-12, n_it=100)
+        o = oracle.Oracle(p)
+        ep_t, psi_t, pb_t = np.zeros(6), 0.0, np.zeros(6)  # committed
+        psi_c, pb_c = 0.0, np.zeros(6)                       # current
+        yielded = 0
+        for eps in np.linspace(0.001, 0.01, 8):
+            g0 = np.array([eps, -0.3 * eps, -0.3 * eps, 0.2 * eps, 0, 0])
+            o.set_gradient(g0)
+            assert o.solve() == 0
+            # the solve's call, then updateInternalVariables
+            _, ep, psi_c, pb_c = oracle.visco_point(kind, prm, g0, ep_t, psi_t, pb_t, psi_c, pb_c)
+            ep_t, psi_t, pb_t = ep, psi_c, pb_c.copy()
+            epo, qo = o.get_state(committed=True)
+            assert np.max(np.abs(epo - ep_t)) < 1e-15 and np.max(np.abs(qo - psi_t)) < 1e-15
+            assert np.max(np.abs(o.get_kinematic_state(committed=True) - pb_t)) < 1e-15
+            # the homogenized stress: one more call per point after the commit
+            so = o.homogenized_stress()
+            s2, _, psi_c, pb_c = oracle.visco_point(kind, prm, g0, ep_t, psi_t, pb_t, psi_c, pb_c)
+            assert np.max(np.abs(so - s2)) < 1e-13 * np.max(np.abs(s2))
+            yielded += psi_t > 0
+        assert yielded >= 4
+        eq, qc = o.get_state(committed=False)
+        assert np.max(np.abs(qc - psi_c)) < 1e-15
+        assert np.max(np.abs(o.get_kinematic_state(committed=False) - pb_c)) < 1e-15

@@ -15,6 +15,7 @@ import pytest
 
 import oracle
 from fans_amd.problem import MaterialGroup, Problem
+from fans_amd.workloads import config4_objective, smoothed_density
 
 R2 = np.sqrt(0.5)
 
@@ -88,19 +89,6 @@ def test_sensitivity_formula_matches_fd(h, fe):
         assert np.max(np.abs(wrong - sens[:, 1])) > 1e-3 * np.max(np.abs(sens[:, 1]))
 
 
-def smoothed_density(n, corr=8.0, seed=0):
-    """Seeded Gaussian random field smoothed with a periodic Gaussian filter of
-    correlation length `corr` voxels, through a sigmoid: rho in (0, 1)."""
-    rng = np.random.default_rng(seed)
-    x = rng.standard_normal((n, n, n))
-    k = np.fft.fftfreq(n)
-    kx, ky, kz = np.meshgrid(k, k, k, indexing="ij")
-    filt = np.exp(-0.5 * (2 * np.pi * corr) ** 2 * (kx ** 2 + ky ** 2 + kz ** 2))
-    f = np.real(np.fft.ifftn(np.fft.fftn(x) * filt))
-    f /= f.std()
-    return 1.0 / (1.0 + np.exp(-3.0 * f))
-
-
 # ---------------------------------------------------------------------------- GPU
 @pytest.mark.gpu
 @pytest.mark.parametrize("h,fe", [(1, "HEX8"), (3, "HEX8"), (3, "BBAR")])
@@ -132,16 +120,6 @@ def test_voxel_moduli_forward_and_sensitivity(h, fe):
     ref = formula_sensitivity(p, o.element_fields(), ob.element_fields())
     assert np.max(np.abs(sens - ref)) <= 1e-8 * np.max(np.abs(ref))
     s.close()
-
-
-def config4_objective(solver, rho, params):
-    import torch
-    from fans_amd.pyfans import homogenized_stress, simp_moduli
-    K, G = simp_moduli(rho, *params)
-    e1 = torch.tensor([1.0, 0, 0, 0, 0, 0], dtype=torch.float64)
-    e6 = torch.tensor([0, 0, 0, 0, 0, 1.0], dtype=torch.float64)
-    # C_1111 and C_2323 (Mandel entry (5,5) / 2) of the homogenized tangent
-    return homogenized_stress(solver, K, G, e1)[0] + 0.5 * homogenized_stress(solver, K, G, e6)[5]
 
 
 @pytest.mark.gpu
@@ -202,3 +180,57 @@ def test_autograd_mixed_matches_fp64():
         grads.append(rho.grad.numpy().copy())
         s.close()
     assert np.max(np.abs(grads[1] - grads[0])) <= 1e-6 * np.max(np.abs(grads[0]))
+
+
+def _micro_input(tmp_path, ms):
+    """test_LinearElastic.json of the reference on sphere32, written as
+    input.json next to its microstructure file."""
+    import json
+    import os
+
+    from fans_amd import hdf5io
+    from tests.conftest import GOLDEN
+    dset = "/sphere/32x32x32/ms"
+    hdf5io.write_microstructure(str(tmp_path / "sphere32.h5"), dset, ms)
+    j = json.load(open(os.path.join(GOLDEN, "test_LinearElastic.json")))
+    j["microstructure"]["filepath"] = str(tmp_path / "sphere32.h5")
+    j["microstructure"]["datasetname"] = dset
+    json.dump(j, open(tmp_path / "input.json", "w"))
+    return j
+
+
+def test_micro_simulation_constructor_shape():
+    """MicroSimulation(sim_id) with the reference's default input file name."""
+    import inspect
+
+    from fans_amd.pyfans import MicroSimulation
+    sig = inspect.signature(MicroSimulation.__init__)
+    assert list(sig.parameters)[:3] == ["self", "sim_id", "input_file"]
+    assert sig.parameters["input_file"].default == "input.json"
+
+
+@pytest.mark.gpu
+def test_micro_simulation_default_input(tmp_path, monkeypatch, sphere32_ms):
+    """MicroSimulation(0) reads ./input.json; solve() returns the homogenized
+    stress and the upper triangle of the tangent in the reference's cmat1..7
+    layout (micro.cpp:66-79), against the oracle."""
+    from fans_amd.pyfans import MicroSimulation
+    _micro_input(tmp_path, sphere32_ms)
+    monkeypatch.chdir(tmp_path)
+    print("[gpu-case] MicroSimulation(0) on ./input.json (sphere32 elastic)", flush=True)
+    sim = MicroSimulation(0)
+    g0 = np.array([1e-3, -2e-4, 5e-4, 1e-4, 0.0, -3e-4])
+    out = sim.solve({"strains1to3": g0[:3], "strains4to6": g0[3:]}, 0.0)
+    o = oracle.Oracle(sim.problem)
+    o.set_gradient(g0)
+    o.solve()
+    so = o.homogenized_stress()
+    Co = o.homogenized_tangent(1e-6)
+    s = np.array(out["stresses1to3"] + out["stresses4to6"])
+    assert np.max(np.abs(s - so)) <= 1e-10 * np.max(np.abs(so))
+    cm = np.concatenate([out[f"cmat{k}"] for k in range(1, 8)])
+    ref = np.array([Co[0, 0], Co[0, 1], Co[0, 2], Co[0, 3], Co[0, 4], Co[0, 5], Co[1, 1], Co[1, 2], Co[1, 3],
+                    Co[1, 4], Co[1, 5], Co[2, 2], Co[2, 3], Co[2, 4], Co[2, 5], Co[3, 3], Co[3, 4], Co[3, 5],
+                    Co[4, 4], Co[4, 5], Co[5, 5]])
+    assert np.max(np.abs(cm - ref)) <= 1e-6 * np.max(np.abs(ref))
+    sim.solver.close()
