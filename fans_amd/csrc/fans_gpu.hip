@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -307,6 +308,9 @@ struct fans_gpu_ctx {
 };
 
 namespace {
+
+// message of the last failed context-free call of this thread
+thread_local char g_noalloc_err[256] = "";
 
 int fail(fans_gpu_ctx *c, int code, const char *fmt, ...)
 {
@@ -1612,7 +1616,13 @@ extern "C" {
 
 int fans_gpu_abi_version(void) { return FANS_GPU_ABI_VERSION; }
 
-const char *fans_gpu_last_error(const fans_gpu_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *fans_gpu_last_error(const fans_gpu_ctx *ctx)
+{
+    if (ctx)
+        return ctx->err.c_str();
+    // context-free entry points (fans_gpu_fft_r2c_noalloc) leave their message here
+    return g_noalloc_err[0] ? g_noalloc_err : "null context";
+}
 
 int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *d)
 {
@@ -2791,6 +2801,135 @@ int fans_gpu_fft_c2r(int32_t dev, int32_t nx, int32_t ny, int32_t nz, int32_t h,
     return fft_standalone(dev, nx, ny, nz, h, in, out, false);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// fans_gpu_fft_r2c_noalloc: fans_gpu_fft_r2c on caller-owned device buffers and
+// the caller's stream, for callers that cannot allocate.  The only memory
+// beyond input and output is the three twiddle tables, kept in `scratch`, a
+// file-scope device array (each device holds its own copy for the life of the
+// process).  Host code never passes `scratch` itself to a kernel: its device
+// address comes from hipGetSymbolAddress, once per device with that device
+// current, and is kept in g_scratch_dev.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kScratchCplx = 16384;  // complex FP64 twiddles: 256 KiB per device
+constexpr int kScratchAlign = 32;    // table offsets in complex entries (512 B)
+constexpr int kMaxDevices   = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[2 * kScratchCplx];
+
+namespace {
+std::mutex g_scratch_mu;
+double    *g_scratch_dev[kMaxDevices] = {};
+
+int noalloc_fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_noalloc_err, sizeof(g_noalloc_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+// device address of `scratch` on the current device `dev`
+int scratch_address(int dev, double **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void      *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return noalloc_fail(FANS_GPU_EHIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                                hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+int round_up(int v, int a) { return (v + a - 1) / a * a; }
+
+// w[k] = exp(-2 pi i k / n) for the three tables at their offsets
+__global__ void k_twiddles3(cplx<double> *w, int nz, int oy, int ny, int ox, int nx)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int       n, k, o;
+    if (t < nz)
+        n = nz, k = t, o = 0;
+    else if (t < nz + ny)
+        n = ny, k = t - nz, o = oy;
+    else if (t < nz + ny + nx)
+        n = nx, k = t - nz - ny, o = ox;
+    else
+        return;
+    double sn, cs;
+    sincospi(-2.0 * (double) k / (double) n, &sn, &cs);
+    w[o + k] = mk(cs, sn);
+}
+}  // namespace
+
+extern "C" int fans_gpu_fft_r2c_noalloc(int32_t dev, int32_t nx, int32_t ny, int32_t nz, int32_t h,
+                                        const double *in, double *out_cplx, void *stream)
+{
+    g_noalloc_err[0] = '\0';
+    if (!in || !out_cplx || (h != 1 && h != 3) || nx < 1 || ny < 1 || nz < 2 || nz % 2)
+        return noalloc_fail(FANS_GPU_EINVAL, "fft_r2c_noalloc: bad arguments (null buffer, h not 1 or 3, or "
+                                             "n_z not even)");
+    const int oy = round_up(nz, kScratchAlign), ox = oy + round_up(ny, kScratchAlign);
+    if ((int64_t) ox + nx > kScratchCplx)
+        return noalloc_fail(FANS_GPU_EINVAL,
+                            "fft_r2c_noalloc: twiddle tables for %d x %d x %d need %lld complex entries, the "
+                            "scratch array holds %d (n_x + n_y + n_z up to about %d)",
+                            nx, ny, nz, (long long) ox + nx, kScratchCplx, kScratchCplx - 2 * kScratchAlign);
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0)
+        return noalloc_fail(FANS_GPU_ENODEV, "no HIP device");
+    if (dev < 0 || dev >= nd || dev >= kMaxDevices)
+        return noalloc_fail(FANS_GPU_EINVAL, "fft_r2c_noalloc: device %d out of range", dev);
+    // the host-side fields the launch helpers read: grid, stream, plan
+    fans_gpu_ctx c;
+    c.nx = nx, c.ny = ny, c.nz = nz, c.h = h;
+    c.stream    = static_cast<hipStream_t>(stream);
+    auto     &P = c.fft64;
+    if (P.plan(nx, ny, nz, h) != hipSuccess)
+        return noalloc_fail(FANS_GPU_EINVAL, "fft_r2c_noalloc: unsupported grid %d x %d x %d (lengths must be "
+                                             "products of 2, 3 and 5 within the kernels' limits)",
+                            nx, ny, nz);
+    int prev = -1;
+    if (hipGetDevice(&prev) != hipSuccess)
+        return noalloc_fail(FANS_GPU_EHIP, "hipGetDevice failed");
+    if (prev != dev && hipSetDevice(dev) != hipSuccess)
+        return noalloc_fail(FANS_GPU_EHIP, "hipSetDevice(%d) failed", dev);
+    int     rc = 0;
+    double *w  = nullptr;
+    if (!(rc = scratch_address(dev, &w))) {
+        auto *tw = reinterpret_cast<cplx<double> *>(w);
+        P.twz = tw, P.twy = tw + oy, P.twx = tw + ox;
+        const int nt = nz + ny + nx;
+        hipLaunchKernelGGL(k_twiddles3, dim3((nt + 255) / 256), dim3(256), 0, c.stream, tw, nz, oy, ny, ox, nx);
+        auto     *spec = reinterpret_cast<cplx<double> *>(out_cplx);
+        const int Kc   = P.Mc * h;
+        const ProArgs<double> pa{};
+        if (h == 1)
+            launch_z_r2c<double, 1, PRO_NONE>(&c, P, const_cast<double *>(in), spec, pa, nullptr);
+        else
+            launch_z_r2c<double, 3, PRO_NONE>(&c, P, const_cast<double *>(in), spec, pa, nullptr);
+        launch_strided<double, false>(&c, P.Ly, P.twy, P.bc, P.smy, spec, (int64_t) Kc, 0, Kc, (int64_t) ny * Kc,
+                                      nx, nullptr);
+        launch_strided<double, false>(&c, P.Lx, P.twx, P.bcx, P.smx_plain, spec, (int64_t) ny * Kc, 0, ny * Kc,
+                                      (int64_t) 0, 1, nullptr);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            rc = noalloc_fail(FANS_GPU_EHIP, "fft_r2c_noalloc: launch failed: %s", hipGetErrorString(e));
+    }
+    if (prev != dev)
+        (void) hipSetDevice(prev);
+    return rc;
+}
+
+extern "C" {
 
 int fans_gpu_set_option(fans_gpu_ctx *c, int32_t opt, int64_t value)
 {

@@ -62,6 +62,7 @@ EXPORTS = [
     "fans_gpu_homogenized_stress", "fans_gpu_homogenized_tangent", "fans_gpu_extrapolate_displacement",
     "fans_gpu_postprocess", "fans_gpu_get_field", "fans_gpu_set_field", "fans_gpu_residual", "fans_gpu_convolve",
     "fans_gpu_get_green", "fans_gpu_get_reference_stiffness", "fans_gpu_fft_r2c", "fans_gpu_fft_c2r",
+    "fans_gpu_fft_r2c_noalloc",
     "fans_gpu_set_option", "fans_gpu_kernel_times", "fans_gpu_kernel_family_name", "fans_gpu_reset_state",
     "fans_gpu_set_voxel_moduli", "fans_gpu_sensitivity",
 ]

@@ -213,6 +213,16 @@ int fans_gpu_get_reference_stiffness(fans_gpu_ctx *ctx, double *out, int32_t n_s
  * the FFTW plans of solver.h:221-222); spectrum [nx][ny][nz/2+1][h] interleaved */
 int fans_gpu_fft_r2c(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t h, const double *in, double *out_cplx);
 int fans_gpu_fft_c2r(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t h, const double *in_cplx, double *out);
+/* fans_gpu_fft_r2c without allocation -- no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * serving loops).  `in` and `out_cplx` are device buffers of the layouts above,
+ * `stream` a hipStream_t (NULL: the null stream); the call only enqueues work
+ * on it.  The twiddle tables live in a file-scope device array of 16384 complex
+ * doubles per device, so n_x + n_y + n_z must stay below about 16300
+ * (FANS_GPU_EINVAL otherwise, message via fans_gpu_last_error(NULL)).  Calls
+ * that overlap on one device (other streams or threads) race on that array. */
+int fans_gpu_fft_r2c_noalloc(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t h, const double *in,
+                             double *out_cplx, void *stream);
 
 /* options (no reference counterpart): FANS_OPT_TIMING=1 brackets every launch
  * with HIP events on the context's stream (replaces the clock() counters of
