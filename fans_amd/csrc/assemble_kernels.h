@@ -19,7 +19,7 @@ namespace fans {
 
 enum { ASM_RESID = 0, ASM_CGLIN = 1 };
 #ifndef FANS_ASM_MATVEC_WAVES
-#define FANS_ASM_MATVEC_WAVES 4
+#define FANS_ASM_MATVEC_WAVES 3
 #endif
 constexpr int kAsmMatvecWaves = FANS_ASM_MATVEC_WAVES;
 constexpr int TILE = 16;  // element tile edge (y and z)
@@ -68,7 +68,8 @@ enum { ASM_GENERAL = 0, ASM_ISO = 1, ASM_ISO_CUBIC = 2, ASM_LARGE = 3, ASM_GENER
 // per SIMD) unless asked for two waves, which it reaches without spilling; the
 // finite-strain, power-law and visco-plastic residuals and the FP64
 // per-Gauss-point linear matvec would spill there, so they keep one.  The FP64
-// modal matvec (the CG hot loop) fits 128 registers: four waves per SIMD.
+// modal matvec takes 132 registers (three waves per SIMD); squeezed into 128
+// for four it was 2 % slower at 256^3.
 template <typename T, int MODE, int VAR>
 constexpr int kAsmWaves()
 {

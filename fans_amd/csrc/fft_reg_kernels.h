@@ -243,6 +243,15 @@ __device__ __forceinline__ float row_shift(float x)
     return __int_as_float(row_shift_b32<S>(__float_as_int(x)));
 }
 
+#ifndef FANS_XG_LDS_GREEN
+#define FANS_XG_LDS_GREEN 0
+#endif
+constexpr bool kXGreenLdsGreen = FANS_XG_LDS_GREEN;
+#ifndef FANS_XG_KEEP_INPUT
+#define FANS_XG_KEEP_INPUT 1
+#endif
+constexpr bool kXGreenKeepInput = FANS_XG_KEEP_INPUT;
+
 // ---- x pass: forward FFT, Green multiply, inverse FFT ---------------------
 // spectrum [x][y][kz][H]; the C columns of a block are (kz, c).  For H = 3
 // they are packed in threes when C = 3 KB + 1 (C = 16: five frequencies per row
@@ -289,6 +298,11 @@ __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__
 #pragma unroll
     for (int m = 0; m < N1; ++m)
         v[m] = ok ? base[(int64_t) (j + N2 * m) * sn] : cplx<T>{(T) 0, (T) 0};
+    cplx<T> rkeep[kXGreenKeepInput ? N1 : 1];
+    if constexpr (kXGreenKeepInput)
+#pragma unroll
+        for (int m = 0; m < N1; ++m)
+            rkeep[m] = v[m];
     if constexpr (SPLIT)
         fourstep_split<T, N1, N2, false>(v, j, tile_r, LD, col, tw);
     else
@@ -307,7 +321,7 @@ __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__
     const int     kyq = fy ? ny - y : y;
     T *const      Gs  = SPLIT ? tile_r : reinterpret_cast<T *>(tile_c);
     static_assert((SPLIT ? 1 : 2) * N * LD >= N * KB * P, "the Green block fits the tile");
-    {
+    if constexpr (kXGreenLdsGreen) {
         const int      per = nk * P, tot = gv.gxq * per;
         const T *const src = gv.G + ((int64_t) kyq * Mc + kz0) * P;
         const int64_t  rs  = (int64_t) gv.gyq * Mc * P;  // stride between stored kx rows
@@ -316,7 +330,8 @@ __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__
             Gs[r * (KB * P) + m] = src[r * rs + m];
         }
     }
-    __syncthreads();
+    if constexpr (kXGreenLdsGreen)
+        __syncthreads();
     const int kkg = kk < nk ? kk : nk - 1;  // idle lanes read a valid slot
 #pragma unroll
     for (int q = 0; q < N1 / N2; ++q)
@@ -325,7 +340,9 @@ __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__
             const int  row = j + N2 * q + N1 * k2;  // x frequency
             const bool fx  = row >= gv.gxq;
             const T    sx  = fx ? (T) -1 : (T) 1;
-            const T   *g   = Gs + (fx ? gv.nx - row : row) * (KB * P) + kkg * P;
+            const T   *g   = kXGreenLdsGreen
+                                 ? Gs + (fx ? gv.nx - row : row) * (KB * P) + kkg * P
+                                 : gv.G + (((int64_t) (fx ? gv.nx - row : row) * gv.gyq + kyq) * Mc + kz0 + kkg) * P;
             cplx<T>   &a   = v[q * N2 + k2];
             if constexpr (H == 1) {
                 a = cscale(a, g[0]);
@@ -357,7 +374,8 @@ __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__
                 a = cplx<T>{ga * v0.x + gb * v1.x + gc * v2.x, ga * v0.y + gb * v1.y + gc * v2.y};
             }
         }
-    __syncthreads();  // the tile is the inverse transform's again
+    if constexpr (kXGreenLdsGreen)
+        __syncthreads();  // the tile is the inverse transform's again
     fourstep_out_to_in<T, N1, N2>(v);
     if constexpr (SPLIT)
         fourstep_split<T, N1, N2, true>(v, j, tile_r, LD, col, tw);
@@ -374,7 +392,9 @@ __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__
             for (int q = 0; q < N1 / N2; ++q)
 #pragma unroll
                 for (int k2 = 0; k2 < N2; ++k2) {
-                    const cplx<T> rin = base[(int64_t) (j + N2 * q + N1 * k2) * sn];
+                    // input element x = j + N2 q + N1 k2 is v[m] with m = q + (N1/N2) k2 at entry
+                    const cplx<T> rin = kXGreenKeepInput ? rkeep[q + (N1 / N2) * k2]
+                                                         : base[(int64_t) (j + N2 * q + N1 * k2) * sn];
                     const cplx<T> &o  = v[q * N2 + k2];
                     acc -= wk * ((double) rin.x * o.x + (double) rin.y * o.y);
                 }
