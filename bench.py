@@ -466,13 +466,25 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-load-cases", action="store_true",
+                    help="skip the six Mandel unit load cases reported beside the timed solves (profiling runs)")
     ap.add_argument("--workload", choices=["headline", "batch", "j2", "accuracy"], default="headline",
                     help="batch: the 384-RVE uneven-size batch (RVEs/s); j2: the J2 load-step ramp; accuracy: "
                          "per-configuration errors, mixed vs FP64 and FP64 vs oracle; none is the headline metric")
     ap.add_argument("--j2-steps", type=int, default=20, help="load steps of the j2 workload (of 20)")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="time the library's own streaming copy (16 B per lane, 2 GiB moved per copy) and print "
+                         "its TB/s, the attainable bandwidth of a pure streaming pass")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (reads 1 GiB, writes 1 GiB) to scale HBM counters")
     args = ap.parse_args()
+    if args.stream_ceiling:
+        from fans_amd.gpu import stream_bandwidth
+        for nb in (1 << 28, 1 << 30, 1 << 32):
+            ms, moved = stream_bandwidth(nb, reps=20)
+            print(json.dumps({"bytes_per_buffer": nb, "ms_per_copy": round(ms, 4),
+                              "TB_per_s": round(moved / (ms * 1e-3) / 1e12, 3)}), flush=True)
+        return
     if args.calibrate_copy:
         import torch
         a = torch.ones(128 * 1024 * 1024, dtype=torch.float64, device="cuda")
@@ -501,7 +513,7 @@ def main():
     g0 = np.array([1.0, 0, 0, 0, 0, 0])
     print(f"[bench] {n}^3 elastic: FP64 form, {args.warmup} warmup + {args.steps} timed solves", file=sys.stderr,
           flush=True)
-    r64 = run_gpu(p, "fp64", args.steps, args.warmup, g0, all_load_cases=True)
+    r64 = run_gpu(p, "fp64", args.steps, args.warmup, g0, all_load_cases=not args.no_load_cases)
     tot_it = sum(r64["iters"])
     # median step (median of the timed solves); the mean is
     # reported beside it

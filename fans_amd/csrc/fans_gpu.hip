@@ -2931,6 +2931,47 @@ extern "C" int fans_gpu_fft_r2c_noalloc(int32_t dev, int32_t nx, int32_t ny, int
 
 extern "C" {
 
+int fans_gpu_stream_bandwidth(int32_t dev, int64_t bytes, int32_t reps, double *ms_per_copy)
+{
+    if (bytes < (1 << 20) || reps < 1 || !ms_per_copy)
+        return FANS_GPU_EINVAL;
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0)
+        return FANS_GPU_ENODEV;
+    if (dev < 0 || dev >= nd || hipSetDevice(dev) != hipSuccess)
+        return FANS_GPU_EINVAL;
+    const int64_t n = bytes / 16;
+    double2      *a = nullptr, *b = nullptr;
+    hipEvent_t    e0 = nullptr, e1 = nullptr;
+    int           rc = FANS_GPU_EHIP;
+    if (hipMalloc(&a, n * 16) == hipSuccess && hipMalloc(&b, n * 16) == hipSuccess &&
+        hipMemset(a, 0, n * 16) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+        hipEventCreate(&e1) == hipSuccess) {
+        // 8 blocks per CU-sized share of work: 2048 blocks, contiguous chunks
+        const int     nb  = 2048;
+        const int64_t per = (n + nb - 1) / nb;
+        hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, nullptr, b, a, n, per);  // warm-up
+        (void) hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, nullptr, (r & 1) ? a : b, (r & 1) ? b : a, n,
+                               per);
+        (void) hipEventRecord(e1, nullptr);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+            hipGetLastError() == hipSuccess) {
+            *ms_per_copy = ms / reps;
+            rc           = 0;
+        }
+    }
+    if (e0)
+        (void) hipEventDestroy(e0);
+    if (e1)
+        (void) hipEventDestroy(e1);
+    (void) hipFree(a);
+    (void) hipFree(b);
+    return rc;
+}
+
 int fans_gpu_set_option(fans_gpu_ctx *c, int32_t opt, int64_t value)
 {
     if (!c)

@@ -170,6 +170,17 @@ __global__ __launch_bounds__(256) void k_sub_scaled(T *__restrict__ y, const T *
         y[i] -= s * x[i];
 }
 
+// Streaming copy, 16 bytes per lane per access, each block owning a
+// contiguous chunk (the bandwidth reference of the passes above: the same
+// access width and no arithmetic)
+__global__ __launch_bounds__(256) void k_stream_copy(double2 *__restrict__ dst, const double2 *__restrict__ src,
+                                                     int64_t n, int64_t per_block)
+{
+    const int64_t b0 = (int64_t) blockIdx.x * per_block, b1 = min(b0 + per_block, n);
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x)
+        dst[i] = src[i];
+}
+
 // y = a * x + b * y  (a, b from host)
 template <typename T, typename TX>
 __global__ __launch_bounds__(256) void k_axpby(T *__restrict__ y, const TX *__restrict__ x, double a, double b, int64_t n)

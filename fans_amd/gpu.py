@@ -62,7 +62,7 @@ EXPORTS = [
     "fans_gpu_homogenized_stress", "fans_gpu_homogenized_tangent", "fans_gpu_extrapolate_displacement",
     "fans_gpu_postprocess", "fans_gpu_get_field", "fans_gpu_set_field", "fans_gpu_residual", "fans_gpu_convolve",
     "fans_gpu_get_green", "fans_gpu_get_reference_stiffness", "fans_gpu_fft_r2c", "fans_gpu_fft_c2r",
-    "fans_gpu_fft_r2c_noalloc",
+    "fans_gpu_fft_r2c_noalloc", "fans_gpu_stream_bandwidth",
     "fans_gpu_set_option", "fans_gpu_kernel_times", "fans_gpu_kernel_family_name", "fans_gpu_reset_state",
     "fans_gpu_set_voxel_moduli", "fans_gpu_sensitivity",
 ]
@@ -116,6 +116,17 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def stream_bandwidth(nbytes=1 << 30, reps=20, device=0):
+    """(ms per copy, bytes moved per copy) of the library's streaming copy."""
+    L = lib()
+    L.fans_gpu_stream_bandwidth.argtypes = [C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
+    ms = C.c_double()
+    rc = L.fans_gpu_stream_bandwidth(device, int(nbytes), int(reps), C.byref(ms))
+    if rc:
+        raise FansGpuError(rc, "fans_gpu_stream_bandwidth failed")
+    return ms.value, 2 * (int(nbytes) // 16) * 16
 
 
 def fft_r2c(x, device=0):

@@ -224,6 +224,13 @@ int fans_gpu_fft_c2r(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t
 int fans_gpu_fft_r2c_noalloc(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t h, const double *in,
                              double *out_cplx, void *stream);
 
+/* measurement helper (no reference counterpart): average milliseconds of one
+ * device-to-device copy of `bytes` (>= 1 MiB) by the library's own streaming
+ * kernel (16 bytes per lane, contiguous chunks per workgroup), over `reps`
+ * copies after one warm-up -- the attainable-bandwidth reference for the
+ * streaming passes (2 * bytes moved per copy) */
+int fans_gpu_stream_bandwidth(int32_t device, int64_t bytes, int32_t reps, double *ms_per_copy);
+
 /* options (no reference counterpart): FANS_OPT_TIMING=1 brackets every launch
  * with HIP events on the context's stream (replaces the clock() counters of
  * solver.h:286-297); FANS_OPT_FFT_CHUNKS=k runs the y / x+Green / y^-1 passes

@@ -54,7 +54,9 @@ typedef double complex cplx;
 #define MAXE 24 /* 8 * howmany */
 
 /* ------------------------------------------------------------------------ */
-/* FFT: recursive mixed-radix decimation in time, any length                 */
+/* FFT: iterative Stockham autosort, radix 4 / 2 / 3 / 5 (any other prime by */
+/* a direct DFT); real transforms through a complex transform of half the    */
+/* length (the unnormalised r2c / c2r of FFTW, which solver.h:221-222 uses)   */
 /* ------------------------------------------------------------------------ */
 typedef struct {
     int   n;
@@ -82,128 +84,201 @@ static void plan_init(fft_plan *P, int n)
 
 static void plan_free(fft_plan *P) { free(P->w); }
 
-static inline cplx tw(const fft_plan *P, long e, int sign)
+/* in-place DFT of length p (sign -1 forward, +1 inverse) */
+static void dft_small(cplx *v, int p, int sign, const fft_plan *P)
 {
-    e %= P->n;
-    return sign < 0 ? P->w[e] : conj(P->w[e]);
-}
-
-static void fft_rec(const cplx *in, ptrdiff_t is, cplx *out, int n, const fft_plan *P, long wstep, const int *f, int sign)
-{
-    if (n == 1) {
-        out[0] = in[0];
-        return;
-    }
-    int p = f[0], m = n / p;
-    for (int q = 0; q < p; q++)
-        fft_rec(in + q * is, is * p, out + q * m, m, P, wstep * p, f + 1, sign);
-    cplx  stackbuf[16];
-    cplx *tmp = p <= 16 ? stackbuf : (cplx *) malloc(sizeof(cplx) * (size_t) p);
-    for (int k = 0; k < m; k++) {
-        for (int q = 0; q < p; q++)
-            tmp[q] = out[q * m + k] * tw(P, (long) q * k * wstep, sign);
+    const double s3 = 0.86602540378443864676, c5a = 0.30901699437494742410, c5b = -0.80901699437494742410,
+                 s5a = 0.95105651629515357212, s5b = 0.58778525229247312917;
+    const cplx   mi = sign < 0 ? -I : I; /* exp(sign * i pi / 2) */
+    if (p == 2) {
+        cplx a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    } else if (p == 4) {
+        cplx s02 = v[0] + v[2], d02 = v[0] - v[2], s13 = v[1] + v[3], d13 = v[1] - v[3];
+        v[0] = s02 + s13;
+        v[1] = d02 + mi * d13;
+        v[2] = s02 - s13;
+        v[3] = d02 - mi * d13;
+    } else if (p == 3) {
+        cplx t1 = v[1] + v[2], t2 = v[0] - 0.5 * t1, t3 = mi * s3 * (v[1] - v[2]);
+        v[0] = v[0] + t1;
+        v[1] = t2 + t3;
+        v[2] = t2 - t3;
+    } else if (p == 5) {
+        cplx a1 = v[1] + v[4], a2 = v[2] + v[3], b1 = v[1] - v[4], b2 = v[2] - v[3];
+        cplx r1 = v[0] + c5a * a1 + c5b * a2, r2 = v[0] + c5b * a1 + c5a * a2;
+        cplx i1 = mi * (s5a * b1 + s5b * b2), i2 = mi * (s5b * b1 - s5a * b2);
+        v[0] = v[0] + a1 + a2;
+        v[1] = r1 + i1;
+        v[4] = r1 - i1;
+        v[2] = r2 + i2;
+        v[3] = r2 - i2;
+    } else {
+        cplx  stackbuf[16];
+        cplx *tmp = p <= 16 ? stackbuf : (cplx *) malloc(sizeof(cplx) * (size_t) p);
+        long  st  = P->n / p;
         for (int t = 0; t < p; t++) {
             cplx acc = 0;
-            for (int q = 0; q < p; q++)
-                acc += tmp[q] * tw(P, (long) q * t * m * wstep, sign);
-            out[k + t * m] = acc;
+            for (int q = 0; q < p; q++) {
+                cplx w = P->w[((long) t * q % p) * st];
+                acc += v[q] * (sign < 0 ? w : conj(w));
+            }
+            tmp[t] = acc;
         }
+        memcpy(v, tmp, sizeof(cplx) * (size_t) p);
+        if (tmp != stackbuf)
+            free(tmp);
     }
-    if (tmp != stackbuf)
-        free(tmp);
 }
 
-/* transform every line of one axis of a [nx][ny][nzc][h] complex array */
+/* one line of length P->n in a[] (work: scratch of the same length) */
+static void fft_line(const fft_plan *P, cplx *a, cplx *work, int sign)
+{
+    const int n = P->n;
+    cplx     *src = a, *dst = work;
+    int       Ns  = 1;
+    for (int s = 0; s < P->nf; s++) {
+        const int  p  = P->f[s], nb = n / p;
+        const long ts = n / (Ns * p);
+        cplx       v[64];
+        cplx      *vv = p <= 64 ? v : (cplx *) malloc(sizeof(cplx) * (size_t) p);
+        for (int j = 0; j < nb; j++) {
+            const int k = j % Ns;
+            for (int t = 0; t < p; t++) {
+                cplx x = src[j + t * nb];
+                if (t && k) {
+                    cplx w = P->w[(long) t * k * ts];
+                    x *= sign < 0 ? w : conj(w);
+                }
+                vv[t] = x;
+            }
+            dft_small(vv, p, sign, P);
+            const int o = (j / Ns) * Ns * p + k;
+            for (int t = 0; t < p; t++)
+                dst[o + t * Ns] = vv[t];
+        }
+        if (vv != v)
+            free(vv);
+        Ns *= p;
+        cplx *tmp = src;
+        src       = dst;
+        dst       = tmp;
+    }
+    if (src != a)
+        memcpy(a, src, sizeof(cplx) * (size_t) n);
+}
+
+/* complex transform of every line of one axis of a [nx][ny][nzc][h] array,
+ * axis 0 (x) or 1 (y); lines of consecutive columns go through a contiguous
+ * buffer in groups of CB */
+#define CB 8
 static void fft_axis(cplx *F, int nx, int ny, int nzc, int h, int axis, int sign)
 {
-    int       n = axis == 0 ? nx : (axis == 1 ? ny : nzc);
-    ptrdiff_t stride, nlines;
-    if (axis == 2) {
-        stride = h;
-        nlines = (ptrdiff_t) nx * ny * h;
-    } else if (axis == 1) {
-        stride = (ptrdiff_t) nzc * h;
-        nlines = (ptrdiff_t) nx * nzc * h;
-    } else {
-        stride = (ptrdiff_t) ny * nzc * h;
-        nlines = (ptrdiff_t) ny * nzc * h;
-    }
+    const int       n      = axis == 0 ? nx : ny;
+    const ptrdiff_t stride = axis == 0 ? (ptrdiff_t) ny * nzc * h : (ptrdiff_t) nzc * h;
+    const ptrdiff_t ncol   = (ptrdiff_t) nzc * h;                       /* contiguous columns per row */
+    const ptrdiff_t nouter = axis == 0 ? ny : nx;                       /* rows of columns */
+    const ptrdiff_t ostep  = axis == 0 ? (ptrdiff_t) nzc * h : (ptrdiff_t) ny * nzc * h;
+    const ptrdiff_t ngrp   = (ncol + CB - 1) / CB;
     fft_plan P;
     plan_init(&P, n);
 #pragma omp parallel
     {
-        cplx *a = (cplx *) malloc(sizeof(cplx) * (size_t) n);
-        cplx *b = (cplx *) malloc(sizeof(cplx) * (size_t) n);
+        cplx *buf  = (cplx *) malloc(sizeof(cplx) * (size_t) n * CB);
+        cplx *work = (cplx *) malloc(sizeof(cplx) * (size_t) n);
 #pragma omp for schedule(static)
-        for (ptrdiff_t L = 0; L < nlines; L++) {
-            ptrdiff_t off;
-            if (axis == 2) {
-                ptrdiff_t xy = L / h, c = L % h;
-                off          = xy * nzc * h + c;
-            } else if (axis == 1) {
-                ptrdiff_t x = L / ((ptrdiff_t) nzc * h), zc = L % ((ptrdiff_t) nzc * h);
-                off         = x * (ptrdiff_t) ny * nzc * h + zc;
-            } else {
-                off = L;
-            }
+        for (ptrdiff_t L = 0; L < nouter * ngrp; L++) {
+            const ptrdiff_t o = L / ngrp, c0 = (L % ngrp) * CB;
+            const int       nc = (int) (ncol - c0 < CB ? ncol - c0 : CB);
+            cplx           *base = F + o * ostep + c0;
             for (int i = 0; i < n; i++)
-                a[i] = F[off + i * stride];
-            fft_rec(a, 1, b, n, &P, 1, P.f, sign);
+                for (int c = 0; c < nc; c++)
+                    buf[(size_t) c * n + i] = base[i * stride + c];
+            for (int c = 0; c < nc; c++)
+                fft_line(&P, buf + (size_t) c * n, work, sign);
             for (int i = 0; i < n; i++)
-                F[off + i * stride] = b[i];
+                for (int c = 0; c < nc; c++)
+                    base[i * stride + c] = buf[(size_t) c * n + i];
         }
-        free(a);
-        free(b);
+        free(buf);
+        free(work);
     }
     plan_free(&P);
 }
+#undef CB
 
-/* unnormalised 3-D r2c like fftw_mpi_plan_many_dft_r2c (solver.h:221) */
+/* unnormalised 3-D r2c like fftw_mpi_plan_many_dft_r2c (solver.h:221): z by a
+ * half-length complex transform, X[k] = (Z[k] + conj Z[M-k]) / 2 + W^k (Z[k] -
+ * conj Z[M-k]) / (2i), then y and x */
 void or_fft_r2c(int nx, int ny, int nz, int h, const double *in, double *out_c)
 {
-    ptrdiff_t N  = (ptrdiff_t) nx * ny * nz;
-    cplx     *F  = (cplx *) malloc(sizeof(cplx) * (size_t) (N * h));
-    cplx     *o  = (cplx *) out_c;
-    int       nc = nz / 2 + 1;
-#pragma omp parallel for schedule(static)
-    for (ptrdiff_t i = 0; i < N * h; i++)
-        F[i] = in[i];
-    fft_axis(F, nx, ny, nz, h, 2, -1);
-    fft_axis(F, nx, ny, nz, h, 1, -1);
-    fft_axis(F, nx, ny, nz, h, 0, -1);
-#pragma omp parallel for schedule(static)
-    for (ptrdiff_t xy = 0; xy < (ptrdiff_t) nx * ny; xy++)
-        for (int kz = 0; kz < nc; kz++)
-            for (int c = 0; c < h; c++)
-                o[(xy * nc + kz) * h + c] = F[(xy * nz + kz) * h + c];
-    free(F);
+    const int M = nz / 2, nc = M + 1;
+    cplx     *o = (cplx *) out_c;
+    fft_plan  Pz, Pw;
+    plan_init(&Pz, M);
+    plan_init(&Pw, nz);
+#pragma omp parallel
+    {
+        cplx *z = (cplx *) malloc(sizeof(cplx) * (size_t) M), *work = (cplx *) malloc(sizeof(cplx) * (size_t) M);
+#pragma omp for schedule(static)
+        for (ptrdiff_t L = 0; L < (ptrdiff_t) nx * ny * h; L++) {
+            const ptrdiff_t xy = L / h, c = L % h;
+            const double   *x  = in + xy * nz * h + c;
+            for (int m = 0; m < M; m++)
+                z[m] = x[(2 * m) * h] + I * x[(2 * m + 1) * h];
+            fft_line(&Pz, z, work, -1);
+            for (int k = 0; k <= M; k++) {
+                const cplx zk = z[k == M ? 0 : k], zc = conj(z[k == 0 ? 0 : M - k]);
+                o[(xy * nc + k) * h + c] = 0.5 * (zk + zc) + Pw.w[k] * (zk - zc) / (2.0 * I);
+            }
+        }
+        free(z);
+        free(work);
+    }
+    plan_free(&Pz);
+    plan_free(&Pw);
+    fft_axis(o, nx, ny, nc, h, 1, -1);
+    fft_axis(o, nx, ny, nc, h, 0, -1);
 }
 
-/* unnormalised 3-D c2r like fftw_mpi_plan_many_dft_c2r (solver.h:222) */
+/* unnormalised 3-D c2r like fftw_mpi_plan_many_dft_c2r (solver.h:222): x, y,
+ * then z through Z[k] = (X[k] + conj X[M-k]) + i W^-k (X[k] - conj X[M-k]) and a
+ * half-length inverse transform (output = n_x n_y n_z x irfftn) */
 void or_fft_c2r(int nx, int ny, int nz, int h, const double *in_c, double *out)
 {
-    ptrdiff_t   N  = (ptrdiff_t) nx * ny * nz;
-    cplx       *F  = (cplx *) malloc(sizeof(cplx) * (size_t) (N * h));
-    const cplx *X  = (const cplx *) in_c;
-    int         nc = nz / 2 + 1;
-#pragma omp parallel for schedule(static)
-    for (ptrdiff_t xy = 0; xy < (ptrdiff_t) nx * ny; xy++) {
-        int x = (int) (xy / ny), y = (int) (xy % ny);
-        int xm = (nx - x) % nx, ym = (ny - y) % ny;
-        for (int kz = 0; kz < nz; kz++)
-            for (int c = 0; c < h; c++) {
-                if (kz < nc)
-                    F[(xy * nz + kz) * h + c] = X[(xy * nc + kz) * h + c];
-                else
-                    F[(xy * nz + kz) * h + c] = conj(X[(((ptrdiff_t) xm * ny + ym) * nc + (nz - kz)) * h + c]);
+    const int       M = nz / 2, nc = M + 1;
+    const ptrdiff_t NC = (ptrdiff_t) nx * ny * nc * h;
+    cplx           *F  = (cplx *) malloc(sizeof(cplx) * (size_t) NC);
+    memcpy(F, in_c, sizeof(cplx) * (size_t) NC);
+    fft_axis(F, nx, ny, nc, h, 0, +1);
+    fft_axis(F, nx, ny, nc, h, 1, +1);
+    fft_plan Pz, Pw;
+    plan_init(&Pz, M);
+    plan_init(&Pw, nz);
+#pragma omp parallel
+    {
+        cplx *z = (cplx *) malloc(sizeof(cplx) * (size_t) M), *work = (cplx *) malloc(sizeof(cplx) * (size_t) M);
+#pragma omp for schedule(static)
+        for (ptrdiff_t L = 0; L < (ptrdiff_t) nx * ny * h; L++) {
+            const ptrdiff_t xy = L / h, c = L % h;
+            const cplx     *X  = F + xy * nc * h + c;
+            for (int k = 0; k < M; k++) {
+                const cplx xk = X[k * h], xm = conj(X[(M - k) * h]);
+                z[k]          = (xk + xm) + I * conj(Pw.w[k]) * (xk - xm);
             }
+            fft_line(&Pz, z, work, +1);
+            double *x = out + xy * nz * h + c;
+            for (int m = 0; m < M; m++) {
+                x[(2 * m) * h]     = creal(z[m]);
+                x[(2 * m + 1) * h] = cimag(z[m]);
+            }
+        }
+        free(z);
+        free(work);
     }
-    fft_axis(F, nx, ny, nz, h, 0, +1);
-    fft_axis(F, nx, ny, nz, h, 1, +1);
-    fft_axis(F, nx, ny, nz, h, 2, +1);
-#pragma omp parallel for schedule(static)
-    for (ptrdiff_t i = 0; i < N * h; i++)
-        out[i] = creal(F[i]);
+    plan_free(&Pz);
+    plan_free(&Pw);
     free(F);
 }
 
