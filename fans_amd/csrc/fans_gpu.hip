@@ -261,6 +261,7 @@ struct fans_gpu_ctx {
     float        *g0f = nullptr;
     double       *partial = nullptr;
     int64_t       partial_len = 0;
+    double       *partial_e   = nullptr;  // [nblk_z] of the linear CG's inverse z pass
     double       *sc = nullptr;
     int          *is = nullptr;
     double       *hist = nullptr;
@@ -564,7 +565,8 @@ int64_t x_green_blocks(const Fft3D<T> &P, int h, int ny)
     return (int64_t) ((P.Mc + kb - 1) / kb) * ny;
 }
 
-int reduce(fans_gpu_ctx *c, const double *partial, int nblk, int nq, int stage);
+int reduce(fans_gpu_ctx *c, const double *partial, int nblk, int nq, int stage, const double *partial_b = nullptr,
+           int nblk_b = 0);
 
 // z passes: compile-time kernels where M = nz/2 is a power of two in 16..128
 // and the line count divides evenly, else the generic ones
@@ -608,8 +610,8 @@ void launch_z_c2r(fans_gpu_ctx *c, Fft3D<T> &P, const cplx<T> *spec, const EpiAr
 
 // ---------------------------------------------------------------------------
 // FFT convolution: out = G * r  with the requested epilogue.  With `pro` the
-// forward z pass first applies the previous CG update to r (and u) and the
-// RED_UPDATE reduction follows it.
+// forward z pass first applies the previous CG residual update to r and the
+// RED_ERR_NEXT reduction (error, stop test) follows it.
 // ---------------------------------------------------------------------------
 template <typename T>
 int convolve_dev(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const T *Gp, int epi, EpiArgs<T> ep,
@@ -633,7 +635,7 @@ int convolve_dev(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const T *G
     }
     HIPCHK(c, hipGetLastError());
     if (pro) {
-        const int rc = reduce(c, pro->partial, (int) P.nblk_z, 2, RED_UPDATE);
+        const int rc = reduce(c, pro->partial, (int) P.nblk_z, 1, RED_ERR_NEXT);
         if (rc)
             return rc;
     }
@@ -643,7 +645,7 @@ int convolve_dev(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const T *G
     const int nch = std::max(1, std::min(c->fft_chunks, P.Mc));
     // EPI_CGD: delta = <r, s> from the spectrum (x+Green partials), then
     // beta (RED_DELTA) before the inverse z pass forms d = s + beta d_prev
-    double *const ep_partial = ep.partial;
+    double *const xg_partial = c->partial;
     int           xg_blocks  = 0;
     for (int q = 0; q < nch; ++q) {
         const int kz_lo = (int) ((int64_t) P.Mc * q / nch), kz_hi = (int) ((int64_t) P.Mc * (q + 1) / nch);
@@ -655,7 +657,7 @@ int convolve_dev(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const T *G
         }
         {
             Timed   t(c, KF_FFT_X_GREEN);
-            double *xp = epi == EPI_CGD ? ep_partial + xg_blocks : nullptr;
+            double *xp = epi == EPI_CGD ? xg_partial + xg_blocks : nullptr;
             xg_blocks += h == 1 ? launch_x_green<T, 1>(c, P, spec, Gp, kz_lo, kz_hi, xp, done)
                                 : launch_x_green<T, 3>(c, P, spec, Gp, kz_lo, kz_hi, xp, done);
         }
@@ -667,7 +669,7 @@ int convolve_dev(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const T *G
     }
     if (epi == EPI_CGD) {
         HIPCHK(c, hipGetLastError());
-        const int rc = reduce(c, ep_partial, xg_blocks, 1, RED_DELTA);
+        const int rc = reduce(c, xg_partial, xg_blocks, 1, RED_DELTA);
         if (rc)
             return rc;
     }
@@ -712,7 +714,6 @@ AsmArgs<T> asm_args(fans_gpu_ctx *c)
     a.ntz     = c->ntz;
     a.geom    = c->geom;
     a.ms      = c->ms;
-    a.ms16 = c->ms16;
     a.ms16    = c->ms16;
     a.nphases = c->nphases;
     a.js      = J2State{c->js, c->nslot};
@@ -794,11 +795,12 @@ int residual64(fans_gpu_ctx *c, const double *uin, double *rout, const double *d
     return launch_assemble<double, ASM_RESID>(c, a);
 }
 
-int reduce(fans_gpu_ctx *c, const double *partial, int nblk, int nq, int stage)
+int reduce(fans_gpu_ctx *c, const double *partial, int nblk, int nq, int stage, const double *partial_b,
+           int nblk_b)
 {
     Timed t(c, KF_REDUCE);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kReduceThreads), 0, c->stream, partial, nblk, nq, stage, c->sc, c->is,
-                       c->hist);
+                       c->hist, partial_b, nblk_b);
     HIPCHK(c, hipGetLastError());
     return 0;
 }
@@ -1048,17 +1050,24 @@ void host_sym_pinv(int n, const double *A, double *P)
 // solver loops
 // ---------------------------------------------------------------------------
 
-// Linear CG fast path (solverCG.h:84-114, islinear && !mixed); every scalar
+// Linear CG fast path (solverCG.h:84-117, islinear && !mixed); every scalar
 // lives on the device, the host only polls the done flag once per chunk.
-// Linear CG (solverCG.h:84-117) with device-side scalars.  Iteration i runs
-//   z r2c [+ the update of iteration i-1: r -= alpha K d, u -= alpha d]
-//   -> RED_UPDATE (err, deltamid; sets done) -> y, x+Green, y^-1
-//   -> z c2r (s = -G r, <r, s>) -> RED_DELTA (beta)
-//   -> assembly (d = s + beta d, K d, <d, K d>) -> RED_ALPHA
-// so the update costs no pass of its own.  After RED_UPDATE sets `done` every
-// later kernel returns at entry; u and r then hold the final iterate.
+// Iteration i runs
+//   z r2c [+ the residual update of iteration i-1: r -= alpha K d]
+//   -> RED_ERR_NEXT (err; sets done) -> y, x+Green (delta = <r, s> by
+//   Parseval), y^-1 -> RED_DELTA (beta from delta and deltamid)
+//   -> z c2r (d = s + beta d_prev with s = -G r, s itself not stored;
+//      <d, K d_prev> from the previous matvec's output; the solution update of
+//      iteration i-1, u -= alpha d_prev)
+//   -> matvec (K d, <d, K d>) -> RED_ALPHA (alpha, and deltamid of iteration
+//      i+1 = <r - alpha K d, s> = delta - alpha (<d, K d> - beta <d, K d_prev>),
+//      the reference's <r, s> of solverCG.h:87 in exact arithmetic, K being
+//      symmetric)
+// so no update costs a pass of its own.  After RED_ERR_NEXT sets `done` every
+// later kernel returns at entry; u and r then hold the final iterate but for
+// the pending u -= alpha d of iteration i-1, which finish() applies.
 template <typename T>
-int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext, T *rnew, T *u, cplx<T> *spec,
+int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *&dcur, T *&dnext, T *rnew, T *u, cplx<T> *spec,
                    const T *Gp, const Phase<T> *ph, int *iters_out)
 {
     const int *done  = c->is + IS_DONE;
@@ -1084,13 +1093,15 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext
     };
     // one CG iteration (with the update of the previous one unless `first`)
     auto iteration = [&](bool with_pro) -> int {
-        const ProArgs<T> pa{rnew, s, c->sc + SC_ALPHA, c->d.measure, c->partial};
-        // s = -G r, delta and beta from the spectrum, then d = s + beta d_cur
-        // and u -= alpha d_cur (previous iteration) in the inverse z pass
+        const ProArgs<T> pa{rnew, c->sc + SC_ALPHA, c->d.measure, c->partial};
+        // delta and beta from the spectrum, then d = s + beta d_cur in the
+        // inverse z pass
         EpiArgs<T> ep{};
-        ep.out     = s;
-        ep.partial = c->partial;
+        // <d, K d_prev> (read after the matvec); the first iteration has
+        // d_prev = 0, and rnew holds no K d_prev yet
+        ep.partial = c->partial_e;
         ep.dprev   = dcur;
+        ep.qprev   = with_pro ? rnew : nullptr;
         ep.dnext   = dnext;
         ep.usol    = u;
         ep.scal    = c->sc;  // SC_BETA == 0, SC_ALPHA == 1
@@ -1105,7 +1116,9 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext
         a.done       = done;
         if ((e = launch_assemble<T, ASM_CGLIN>(c, a)))
             return e;
-        if ((e = reduce(c, c->partial, c->nblk_asm, 1, RED_ALPHA)))
+        // quantity 1 of the first iteration: the matvec's zero slot 1
+        if ((e = reduce(c, c->partial, c->nblk_asm, 2, RED_ALPHA, with_pro ? c->partial_e : nullptr,
+                        (int) P.nblk_z)))
             return e;
         std::swap(dcur, dnext);
         return 0;
@@ -1160,7 +1173,7 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *s, T *&dcur, T *&dnext
             if (c->timing) {
                 if ((rc = read_scalars(c, nullptr, is)))
                     return rc;
-                // done was set by this iteration's RED_UPDATE (the fused update
+                // done was set by this iteration's RED_ERR_NEXT (the fused update
                 // and that reduction did work) or before it (nothing did)
                 if (is[IS_DONE])
                     drop_timing(c, had_pro ? mark + 2 : mark);
@@ -1200,7 +1213,7 @@ int solve_cg_fp64(fans_gpu_ctx *c)
     if (linear) {
         double *dcur = c->dA, *dnext = c->dB;
         int     it   = 0;
-        if ((rc = cg_linear_loop<double>(c, c->fft64, c->r, c->s, dcur, dnext, c->rnew, c->u, c->spec64,
+        if ((rc = cg_linear_loop<double>(c, c->fft64, c->r, dcur, dnext, c->rnew, c->u, c->spec64,
                                                  c->G64, c->dph64, &it)))
             return rc;
         c->iter = it;
@@ -1375,7 +1388,7 @@ int solve_cg_mixed(fans_gpu_ctx *c)
             return rc;
         float *dcur = c->dA32, *dnext = c->dB32;
         int    it   = 0;
-        if ((rc = cg_linear_loop<float>(c, c->fft32, c->r32, c->s32, dcur, dnext, c->rnew32, c->x32, c->spec32,
+        if ((rc = cg_linear_loop<float>(c, c->fft32, c->r32, dcur, dnext, c->rnew32, c->x32, c->spec32,
                                                c->G32, c->dph32, &it)))
             return rc;
         inner += it;
@@ -1441,6 +1454,9 @@ int alloc_fields(fans_gpu_ctx *c)
     need         = std::max<int64_t>(need, (int64_t) c->nstr * c->nblk_el);
     c->partial_len = need;
     HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * need));
+    // per-block partials of the linear CG's inverse z pass, kept apart because
+    // the matvec that follows writes `partial`
+    HIPCHK(c, hipMalloc(&c->partial_e, sizeof(double) * std::max<int64_t>(c->fft64.nblk_z, c->fft32.nblk_z)));
     return 0;
 }
 
@@ -1449,7 +1465,7 @@ void free_all(fans_gpu_ctx *c)
     for (void *p : {(void *) c->ms, (void *) c->dph64, (void *) c->dph32, (void *) c->u, (void *) c->u_prev,
                     (void *) c->r, (void *) c->s, (void *) c->dA, (void *) c->dB, (void *) c->rnew,
                     (void *) c->spec64, (void *) c->G64, (void *) c->keref_d, (void *) c->G32, (void *) c->js,
-                    (void *) c->g0d, (void *) c->g0f, (void *) c->partial, (void *) c->sc, (void *) c->is,
+                    (void *) c->g0d, (void *) c->g0f, (void *) c->partial, (void *) c->partial_e, (void *) c->sc, (void *) c->is,
                     (void *) c->hist, (void *) c->r32, (void *) c->s32, (void *) c->dA32, (void *) c->dB32,
                     (void *) c->rnew32, (void *) c->x32, (void *) c->spec32, (void *) c->vk64, (void *) c->vg64,
                     (void *) c->vk32, (void *) c->vg32, (void *) c->ub_d, (void *) c->sens_d})
@@ -2487,7 +2503,8 @@ int fans_gpu_get_field(fans_gpu_ctx *c, int32_t which, double *host, int64_t cou
         return fail(c, FANS_GPU_EINVAL, "unknown field %d", which);
     if (count != n)
         return fail(c, FANS_GPU_EINVAL, "field size mismatch (expected %lld)", (long long) n);
-    HIPCHK(c, copy_on(c, host, src, sizeof(double) * n, hipMemcpyDeviceToHost));
+    // the nodal fields may go to host or device memory
+    HIPCHK(c, copy_on(c, host, src, sizeof(double) * n, hipMemcpyDefault));
     return 0;
 }
 
@@ -2500,7 +2517,7 @@ int fans_gpu_set_field(fans_gpu_ctx *c, int32_t which, const double *host, int64
         return fail(c, FANS_GPU_EINVAL, "field %d is not writable", which);
     if (count != c->Nh)
         return fail(c, FANS_GPU_EINVAL, "field size mismatch");
-    HIPCHK(c, copy_on(c, dst, host, sizeof(double) * count, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c, dst, host, sizeof(double) * count, hipMemcpyDefault));
     return 0;
 }
 
@@ -2622,9 +2639,6 @@ int fans_gpu_set_voxel_moduli(fans_gpu_ctx *c, const double *k, const double *g,
                     (long long) count);
     if (!c->all_iso)
         return fail(c, FANS_GPU_EINVAL, "per-voxel moduli need LinearElasticIsotropic / LinearThermalIsotropic phases");
-    for (int64_t i = 0; i < count; ++i)
-        if (!(k[i] > 0.0) || (c->h == 3 && !(g[i] > 0.0)))
-            return fail(c, FANS_GPU_EINVAL, "voxel moduli must be positive (voxel %lld)", (long long) i);
     const size_t b64 = sizeof(double) * c->N, b32 = sizeof(float) * c->N;
     if (!c->vk64) {
         HIPCHK(c, hipMalloc(&c->vk64, b64));
@@ -2632,8 +2646,26 @@ int fans_gpu_set_voxel_moduli(fans_gpu_ctx *c, const double *k, const double *g,
         HIPCHK(c, hipMalloc(&c->vk32, b32));
         HIPCHK(c, hipMalloc(&c->vg32, b32));
     }
-    HIPCHK(c, copy_on(c, c->vk64, k, b64, hipMemcpyHostToDevice));
-    HIPCHK(c, copy_on(c, c->vg64, c->h == 3 ? g : k, b64, hipMemcpyHostToDevice));
+    // k and g may be host or device arrays (the runtime tells them apart)
+    HIPCHK(c, copy_on(c, c->vk64, k, b64, hipMemcpyDefault));
+    HIPCHK(c, copy_on(c, c->vg64, c->h == 3 ? g : k, b64, hipMemcpyDefault));
+    // input check and reference-stiffness statistics on the device
+    const int nb = (int) std::min<int64_t>(1024, (c->N + kThreads - 1) / kThreads);
+    if (c->partial_len < 6 * (int64_t) nb)
+        return fail(c, FANS_GPU_EINVAL, "grid too small for the moduli statistics");
+    hipLaunchKernelGGL(k_moduli_stats, dim3(nb), dim3(kThreads), 0, c->stream, c->vk64, c->vg64, c->N, c->partial);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> ps((size_t) 6 * nb);
+    HIPCHK(c, copy_on(c, ps.data(), c->partial, sizeof(double) * ps.size(), hipMemcpyDeviceToHost));
+    double st[6] = {0.0, -INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int b = 0; b < nb; ++b) {
+        st[0] += ps[b];
+        for (int q = 1; q < 5; ++q)
+            st[q] = std::max(st[q], ps[(size_t) q * nb + b]);
+        st[5] += ps[(size_t) 5 * nb + b];
+    }
+    if (st[5] > 0.0)
+        return fail(c, FANS_GPU_EINVAL, "voxel moduli must be positive (%lld voxels are not)", (long long) st[5]);
     hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_vec(c->N)), dim3(kThreads), 0, c->stream, c->vk32,
                        c->vk64, c->N);
     hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_vec(c->N)), dim3(kThreads), 0, c->stream, c->vg32,
@@ -2646,19 +2678,11 @@ int fans_gpu_set_voxel_moduli(fans_gpu_ctx *c, const double *k, const double *g,
     if (!c->user_ref) {
         std::memset(c->kref, 0, sizeof(c->kref));
         if (c->h == 1) {
-            double m = 0.0;
-            for (int64_t i = 0; i < count; ++i)
-                m += k[i];
-            m /= (double) count;
+            const double m = st[0] / (double) count;
             for (int i = 0; i < 3; ++i)
                 c->kref[i * 3 + i] = m;
         } else {
-            double lmax = -INFINITY, lmin = INFINITY, mmax = -INFINITY, mmin = INFINITY;
-            for (int64_t i = 0; i < count; ++i) {
-                const double lam = k[i] - (2.0 / 3.0) * g[i];
-                lmax = std::max(lmax, lam), lmin = std::min(lmin, lam);
-                mmax = std::max(mmax, g[i]), mmin = std::min(mmin, g[i]);
-            }
+            const double lmin = -st[1], lmax = st[2], mmin = -st[3], mmax = st[4];
             const double lr = (lmax + lmin) / 2, mr = (mmax + mmin) / 2;
             for (int i = 0; i < 3; ++i)
                 for (int j = 0; j < 3; ++j)
@@ -2688,7 +2712,8 @@ int fans_gpu_sensitivity(fans_gpu_ctx *c, const double *ub, const double *g0b, d
         HIPCHK(c, hipMalloc(&c->ub_d, sizeof(double) * (c->Nh + 16)));
         HIPCHK(c, hipMalloc(&c->sens_d, sizeof(double) * c->N * 2));
     }
-    HIPCHK(c, copy_on(c, c->ub_d, ub, sizeof(double) * c->Nh, hipMemcpyHostToDevice));
+    // ub and sens may be host or device arrays; g0b is host memory
+    HIPCHK(c, copy_on(c, c->ub_d, ub, sizeof(double) * c->Nh, hipMemcpyDefault));
     HIPCHK(c, copy_on(c, c->ub_d + c->Nh, g0b, sizeof(double) * c->nstr, hipMemcpyHostToDevice));
     ElemArgs<double> a{};
     a.nx     = c->nx;
@@ -2721,7 +2746,7 @@ int fans_gpu_sensitivity(fans_gpu_ctx *c, const double *ub, const double *g0b, d
     }
 #undef EP
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, copy_on(c, sens, c->sens_d, sizeof(double) * count, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_on(c, sens, c->sens_d, sizeof(double) * count, hipMemcpyDefault));
     return 0;
 }
 

@@ -121,33 +121,31 @@ __device__ void lds_fft(cplx<T> *buf, const FftLen &L, const cplx<T> *__restrict
 // ---- CG update fused into the forward z pass --------------------------------
 // PRO_CG: the forward z pass of CG iteration i first finishes the residual
 // update of iteration i-1 (solverCG.h:106-117): r -= alpha rnew, and emits per
-// block the partials |r| (norm of the measure) and <r, s_prev>, which the
-// reduction turns into the error and deltamid.  Only the updated r enters the
-// transform.  The matching solution update u -= alpha d runs in the inverse z
-// pass of the next iteration (EPI_CGD), which reads that d anyway.
+// block the partial |r| (norm of the measure), which the reduction turns into
+// the error.  Only the updated r enters the transform.  deltamid = <r, s_prev>
+// needs no s here: the CG scalars form it from <d, K d> (matvec) and
+// <d, K d_prev> (inverse z pass).  The matching solution update u -= alpha d
+// runs in the inverse z pass of the next iteration, which reads that d anyway.
 enum { PRO_NONE = 0, PRO_CG = 1 };
 
 template <typename T>
 struct ProArgs {
     const T      *rnew;     // K d of the previous iteration
-    const T      *s;        // previous preconditioned residual
     const double *alpha;    // device scalar
     int           measure;  // 0 L1, 1 L2, 2 Linf
-    double       *partial;  // [2][gridDim.x]
+    double       *partial;  // [gridDim.x]
 };
 
 template <typename T, int PRO>
-__device__ __forceinline__ T z_load(T *__restrict__ in, int64_t gi, const ProArgs<T> &pa, T alpha, double &a0,
-                                    double &a1)
+__device__ __forceinline__ T z_load(T *__restrict__ in, int64_t gi, const ProArgs<T> &pa, T alpha, double &a0)
 {
     if constexpr (PRO == PRO_CG) {
-        // r, rnew and s are not read again before the spectrum
-        // passes that follow: streaming (non-temporal) accesses keep them from
-        // displacing the freshly written spectrum in the Infinity Cache
+        // r and rnew are not read again before the spectrum passes that
+        // follow: streaming (non-temporal) accesses keep them from displacing
+        // the freshly written spectrum in the Infinity Cache
         const T rv = __builtin_nontemporal_load(&in[gi]) - alpha * __builtin_nontemporal_load(&pa.rnew[gi]);
         __builtin_nontemporal_store(rv, &in[gi]);
         acc_norm(a0, rv, pa.measure);
-        a1 += (double) rv * (double) __builtin_nontemporal_load(&pa.s[gi]);
         return rv;
     } else {
         return in[gi];
@@ -155,16 +153,14 @@ __device__ __forceinline__ T z_load(T *__restrict__ in, int64_t gi, const ProArg
 }
 
 template <int PRO, typename T>
-__device__ __forceinline__ void z_pro_partials(const ProArgs<T> &pa, double a0, double a1, double *red)
+__device__ __forceinline__ void z_pro_partials(const ProArgs<T> &pa, double a0, double *red)
 {
     if constexpr (PRO == PRO_CG) {
-        double     v[2]  = {a0, a1};
-        const bool mx[2] = {pa.measure == 2, false};
-        block_reduce<2>(v, mx, red);
-        if (threadIdx.x == 0) {
-            pa.partial[blockIdx.x]             = v[0];
-            pa.partial[gridDim.x + blockIdx.x] = v[1];
-        }
+        double     v[1]  = {a0};
+        const bool mx[1] = {pa.measure == 2};
+        block_reduce<1>(v, mx, red);
+        if (threadIdx.x == 0)
+            pa.partial[blockIdx.x] = v[0];
     }
 }
 
@@ -201,13 +197,13 @@ __global__ __launch_bounds__(256) void k_fft_z_r2c(T *__restrict__ in, cplx<T> *
     const int     nzh   = nz * H;
     const int64_t g0    = line0 * nzh;
     T             alpha = (T) 0;
-    double        a0 = 0.0, a1 = 0.0;
+    double        a0    = 0.0;
     if constexpr (PRO == PRO_CG)
         alpha = (T) *pa.alpha;
     for (int e = threadIdx.x; e < nl * nzh; e += blockDim.x) {
         const int l = e / nzh, rem = e - l * nzh, z = rem / H, c = rem - z * H;
         T        *d = reinterpret_cast<T *>(&A[(z >> 1) * ld + l * H + c]);
-        d[z & 1]    = z_load<T, PRO>(in, g0 + e, pa, alpha, a0, a1);
+        d[z & 1]    = z_load<T, PRO>(in, g0 + e, pa, alpha, a0);
     }
     __syncthreads();
     lds_fft<T, false>(A, LM, tw, ncols, ld);
@@ -217,25 +213,30 @@ __global__ __launch_bounds__(256) void k_fft_z_r2c(T *__restrict__ in, cplx<T> *
         const int l = e / (Mc * H), rem = e - l * (Mc * H), k = rem / H, c = rem - k * H;
         dst[e]      = r2c_untangle(A, ld, l * H + c, k, M, tw);
     }
-    z_pro_partials<PRO>(pa, a0, a1, red);
+    z_pro_partials<PRO>(pa, a0, red);
 }
 
 // ---- z pass, inverse c2r with fused epilogues -------------------------------
-// EPI_CGD (linear CG, solverCG.h:88-92 and 104): s = -result, the new search
-// direction d = s + beta d_prev (beta from the spectral <r, s> of the x pass)
-// and the previous iteration's solution update u -= alpha d_prev, which needs
-// d_prev, read here anyway.
+// EPI_CGD (linear CG, solverCG.h:88-92): the new search direction
+// d = s + beta d_prev with s = -result (beta from the spectral <r, s> of the x
+// pass); s itself is not stored.  With qprev = K d_prev (the previous matvec's
+// output) it emits per block the partial <d, K d_prev>, from which the CG
+// scalars form the next deltamid.  With usol set it also applies the previous
+// iteration's solution update u -= alpha d_prev (d_prev is read here anyway;
+// moved into the matvec, the extra accesses cost that pass more than they
+// saved here).
 enum { EPI_STORE = 0, EPI_CG = 1, EPI_FP = 2, EPI_CGD = 3 };
 
 template <typename T>
 struct EpiArgs {
-    T            *out;      // EPI_STORE: result; EPI_CG / EPI_CGD: s = -result
+    T            *out;      // EPI_STORE: result; EPI_CG: s = -result
     const T      *r;        // EPI_CG: partial <r, s>
     double       *u;        // EPI_FP: u -= result (FP64 displacement)
     double       *partial;  // EPI_CG: one partial sum per block
     const T      *dprev;    // EPI_CGD: previous direction (read)
+    const T      *qprev;    // EPI_CGD: K d_prev (nullable), partial <d, K d_prev>
     T            *dnext;    // EPI_CGD: new direction (written)
-    T            *usol;     // EPI_CGD: solution, u -= alpha d_prev
+    T            *usol;     // EPI_CGD: solution, u -= alpha d_prev (nullable)
     const double *scal;     // EPI_CGD: device scalars (beta at [0], alpha at [1])
 };
 
@@ -259,11 +260,14 @@ __device__ __forceinline__ void z_epilogue(const EpiArgs<T> &ep, int64_t gi, T v
         ep.out[gi] = val;
     } else if constexpr (EPI == EPI_CGD) {
         // d_next is read by the very next pass (the matvec) and stays cached;
-        // s (read one iteration later), d_prev and u stream past the caches
-        const T sv = -val, dp = __builtin_nontemporal_load(&ep.dprev[gi]);
-        __builtin_nontemporal_store(sv, &ep.out[gi]);
-        ep.dnext[gi] = sv + beta * dp;
-        __builtin_nontemporal_store(__builtin_nontemporal_load(&ep.usol[gi]) - alpha * dp, &ep.usol[gi]);
+        // d_prev and u stream past the caches
+        const T dp = __builtin_nontemporal_load(&ep.dprev[gi]);
+        const T dn = beta * dp - val;
+        ep.dnext[gi] = dn;
+        if (ep.qprev)
+            acc += (double) dn * (double) __builtin_nontemporal_load(&ep.qprev[gi]);
+        if (ep.usol)
+            __builtin_nontemporal_store(__builtin_nontemporal_load(&ep.usol[gi]) - alpha * dp, &ep.usol[gi]);
     } else if constexpr (EPI == EPI_CG) {
         // s is read by the very next pass (the matvec) and stays cached; r is
         // not read again this iteration
@@ -278,7 +282,7 @@ __device__ __forceinline__ void z_epilogue(const EpiArgs<T> &ep, int64_t gi, T v
 template <int EPI, typename T>
 __device__ __forceinline__ void z_epi_partials(const EpiArgs<T> &ep, double acc, double *red)
 {
-    if constexpr (EPI == EPI_CG) {
+    if (EPI == EPI_CG || (EPI == EPI_CGD && ep.qprev)) {
         double     v[1]  = {acc};
         const bool mx[1] = {false};
         block_reduce<1>(v, mx, red);
@@ -418,18 +422,18 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_r2c_ct(T *__res
     constexpr int NT = zct_threads<H, M>(), LPB = zct_lpb<H, M>();
     constexpr int NZH = 2 * M * H, NC = LPB * H, LD = NC + 1, MC = M + 1;
     __shared__ cplx<T> A[M * LD];
-    __shared__ double  red[2 * (NT / 64 + 1)];
+    __shared__ double  red[NT / 64 + 1];
     const int64_t      line0 = (int64_t) blockIdx.x * LPB;
     const int64_t      g0    = line0 * NZH;
     T                  alpha = (T) 0;
-    double             a0 = 0.0, a1 = 0.0;
+    double             a0    = 0.0;
     if constexpr (PRO == PRO_CG)
         alpha = (T) *pa.alpha;
 #pragma unroll 4
     for (int e = threadIdx.x; e < LPB * NZH; e += NT) {
         const int l = e / NZH, rem = e - l * NZH, z = rem / H, c = rem - z * H;
         T        *d = reinterpret_cast<T *>(&A[(z >> 1) * LD + l * H + c]);
-        d[z & 1]    = z_load<T, PRO>(in, g0 + e, pa, alpha, a0, a1);
+        d[z & 1]    = z_load<T, PRO>(in, g0 + e, pa, alpha, a0);
     }
     __syncthreads();
     lds_fft_ct<T, false, M, NC, LD, NT>(A, tw);
@@ -439,7 +443,7 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_r2c_ct(T *__res
         const int l = e / (MC * H), rem = e - l * (MC * H), k = rem / H, c = rem - k * H;
         dst[e]      = r2c_untangle(A, LD, l * H + c, k, M, tw);
     }
-    z_pro_partials<PRO>(pa, a0, a1, red);
+    z_pro_partials<PRO>(pa, a0, red);
 }
 
 template <typename T, int H, int M, int EPI>

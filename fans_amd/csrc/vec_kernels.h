@@ -30,8 +30,7 @@ enum { IS_DONE = 0, IS_ITER = 1, IS_NIT = 2, IS_ERRTYPE = 3, IS_MEASURE = 4, IS_
 enum {
     RED_ERR_INIT = 0,  // q0 = norm partials      -> err0, iter = 0, delta = 1, deltamid = 0
     RED_DELTA    = 1,  // q0 = <r,s> partials     -> delta, beta
-    RED_ALPHA    = 2,  // q0 = <d,rnew> partials  -> alpha
-    RED_UPDATE   = 3,  // q0 = norm, q1 = <r,s>   -> err, deltamid, iter++
+    RED_ALPHA    = 2,  // q0 = <d,rnew>, q1 = <d,rnew_prev> (nq 2) -> alpha (and deltamid)
     RED_ERR_NEXT = 4,  // q0 = norm partials      -> err, iter++
     RED_SUM      = 5   // q0..q(nq-1) sums        -> SC_RES0 + q
 };
@@ -55,8 +54,11 @@ __device__ __forceinline__ void record_error(double *sc, int *is, double *hist, 
 // four independent accumulators (fixed order) before the block reduction
 constexpr int kReduceThreads = 1024;
 
+// partial_b (nullable): quantity 1 comes from partial_b[0, nblk_b) instead of
+// partial[nblk, 2 nblk) (two producing kernels of different grids)
 __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double *__restrict__ partial, int nblk, int nq,
-                                                           int stage, double *sc, int *is, double *hist)
+                                                           int stage, double *sc, int *is, double *hist,
+                                                           const double *__restrict__ partial_b, int nblk_b)
 {
     __shared__ double sm[2 * (kReduceThreads / 64)];
     const int measure = is[IS_MEASURE];
@@ -67,19 +69,21 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double *__restr
     for (int q = 0; q < nq; q += 2) {
         const int nqq = (nq - q) < 2 ? 1 : 2;
         for (int k = 0; k < 2; ++k) {
-            const bool norm_slot = (k + q == 0) && (stage == RED_ERR_INIT || stage == RED_UPDATE || stage == RED_ERR_NEXT);
+            const bool norm_slot = (k + q == 0) && (stage == RED_ERR_INIT || stage == RED_ERR_NEXT);
             mx[k]                = norm_slot && measure == 2;
             double a[4] = {0.0, 0.0, 0.0, 0.0};
             if (k < nqq) {
-                const double *src = partial + (int64_t) (q + k) * nblk;
+                const bool    alt = partial_b && q + k == 1;
+                const double *src = alt ? partial_b : partial + (int64_t) (q + k) * nblk;
+                const int     nb  = alt ? nblk_b : nblk;
                 int           b   = threadIdx.x;
-                for (; b + 3 * kReduceThreads < nblk; b += 4 * kReduceThreads)
+                for (; b + 3 * kReduceThreads < nb; b += 4 * kReduceThreads)
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const double p = src[b + j * kReduceThreads];
                         a[j]           = mx[k] ? fmax(a[j], p) : a[j] + p;
                     }
-                for (; b < nblk; b += kReduceThreads)
+                for (; b < nb; b += kReduceThreads)
                     a[0] = mx[k] ? fmax(a[0], src[b]) : a[0] + src[b];
             }
             v[k] = mx[k] ? fmax(fmax(a[0], a[1]), fmax(a[2], a[3])) : (a[0] + a[1]) + (a[2] + a[3]);
@@ -102,10 +106,11 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double *__restr
             } else if (stage == RED_ALPHA) {
                 sc[SC_DN]    = v[0];
                 sc[SC_ALPHA] = sc[SC_DELTA] / v[0];
-            } else if (stage == RED_UPDATE) {
-                sc[SC_DELTAMID] = v[1];
-                is[IS_ITER] += 1;
-                record_error(sc, is, hist, finish_norm(v[0], measure));
+                // deltamid of the next iteration, <r - alpha K d, s> with
+                // s = d - beta d_prev (solverCG.h:87, 91 and 104):
+                // delta - alpha (<d, K d> - beta <K d_prev, d>), K symmetric
+                if (nq == 2)
+                    sc[SC_DELTAMID] = sc[SC_DELTA] - sc[SC_ALPHA] * (v[0] - sc[SC_BETA] * v[1]);
             } else if (stage == RED_ERR_NEXT) {
                 is[IS_ITER] += 1;
                 record_error(sc, is, hist, finish_norm(v[0], measure));
@@ -113,6 +118,31 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double *__restr
         }
         __syncthreads();
     }
+}
+
+// Statistics of per-voxel isotropic moduli for the reference stiffness and the
+// input check, one partial per block and quantity ([6][gridDim.x]):
+// sum k, -min lambda, max lambda, -min g, max g (lambda = k - 2/3 g) and the
+// count of voxels with k <= 0 or g <= 0 (g = k for conduction, H = 1)
+__global__ __launch_bounds__(256) void k_moduli_stats(const double *__restrict__ k, const double *__restrict__ g,
+                                                      int64_t n, double *__restrict__ partial)
+{
+    __shared__ double sm[6 * 4];
+    double            v[6] = {0.0, -INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        const double ki = k[i], gi = g[i], lam = ki - (2.0 / 3.0) * gi;
+        v[0] += ki;
+        v[1] = fmax(v[1], -lam);
+        v[2] = fmax(v[2], lam);
+        v[3] = fmax(v[3], -gi);
+        v[4] = fmax(v[4], gi);
+        v[5] += (ki > 0.0 && gi > 0.0) ? 0.0 : 1.0;
+    }
+    const bool mx[6] = {false, true, true, true, true, false};
+    block_reduce<6>(v, mx, sm);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; ++q)
+            partial[(int64_t) q * gridDim.x + blockIdx.x] = v[q];
 }
 
 // partial <a, b> (slot 0) and optionally the norm of a (slot 1)

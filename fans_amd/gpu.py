@@ -118,6 +118,39 @@ def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
 
 
+def _on_device(x):
+    """True for a torch tensor in GPU memory (the library then reads or writes it
+    in place; torch must have been imported before the library was loaded so
+    that both share one HIP runtime)."""
+    return bool(getattr(x, "is_cuda", False))
+
+
+def _hip_runtimes():
+    """Files of the HIP runtimes mapped into this process (Linux)."""
+    try:
+        with open("/proc/self/maps") as f:
+            return {ln.split()[-1] for ln in f if "libamdhip64" in ln}
+    except OSError:
+        return set()
+
+
+def _dev_dp(t, n):
+    """Device pointer of a contiguous float64 torch tensor of n values, after
+    torch's queued work on it has finished (the library uses its own stream).
+    Refuses when torch and the library run on two different HIP runtimes (the
+    library was loaded before torch): the pointer would mean nothing to it."""
+    import torch
+
+    lib()
+    if len(_hip_runtimes()) > 1:
+        raise RuntimeError("torch and libfans_gpu.so use different HIP runtimes; import torch before the "
+                           "library is first used to pass GPU tensors")
+    if t.dtype != torch.float64 or not t.is_contiguous() or t.numel() != n:
+        raise ValueError(f"device array: need {n} contiguous float64 values")
+    torch.cuda.synchronize(t.device)
+    return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_double))
+
+
 def stream_bandwidth(nbytes=1 << 30, reps=20, device=0):
     """(ms per copy, bytes moved per copy) of the library's streaming copy."""
     L = lib()
@@ -320,7 +353,12 @@ class GpuSolver:
         self._check(self.L.fans_gpu_get_field(self.ptr, FIELD[name], _dp(out), out.size))
         return out
 
-    def get_u(self):
+    def get_u(self, out=None):
+        """u as a numpy array, or written into `out` (a float64 torch tensor in GPU
+        memory with N * h values: a device-to-device copy)."""
+        if out is not None and _on_device(out):
+            self._check(self.L.fans_gpu_get_field(self.ptr, FIELD["u"], _dev_dp(out, self.N * self.h), out.numel()))
+            return out
         return self.get_field("u")
 
     def get_r(self):
@@ -374,8 +412,15 @@ class GpuSolver:
 
     def set_voxel_moduli(self, k, g=None):
         """Per-voxel isotropic moduli [x][y][z]: bulk k and shear g (mechanics) or
-        conductivity k (thermal); rebuilds the reference stiffness and Green operator."""
+        conductivity k (thermal); rebuilds the reference stiffness and Green operator.
+        k, g: numpy arrays, or float64 torch tensors in GPU memory (copied on the
+        device, nothing crosses to the host)."""
         N = int(np.prod(self.p.dims))
+        if _on_device(k):
+            kp = _dev_dp(k.reshape(-1), N)
+            gp = None if g is None or self.p.howmany == 1 else _dev_dp(g.reshape(-1), N)
+            self._check(self.L.fans_gpu_set_voxel_moduli(self.ptr, kp, gp, N))
+            return
         kk = np.ascontiguousarray(np.asarray(k, dtype=np.float64).reshape(-1))
         gg = None if g is None else np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(-1))
         if kk.size != N or (gg is not None and gg.size != N):
@@ -384,9 +429,20 @@ class GpuSolver:
 
     def sensitivity(self, ub, g0b):
         """Per-element Gauss integrals of eps_a^T dC/dp eps_b (p = K, G; thermal:
-        k) between the current solution and (ub, g0b); [N, 2] or [N, 1]."""
+        k) between the current solution and (ub, g0b); [N, 2] or [N, 1].  With ub a
+        float64 torch tensor in GPU memory the result is one too (no host copy)."""
         N = int(np.prod(self.p.dims))
         nq = 1 if self.p.howmany == 1 else 2
+        if _on_device(ub):
+            import torch
+
+            gb = np.ascontiguousarray(np.asarray(g0b, dtype=np.float64).reshape(-1))
+            if gb.size != self.p.n_str:
+                raise ValueError("sensitivity: gradient size mismatch")
+            out = torch.empty((N, nq), dtype=torch.float64, device=ub.device)
+            self._check(self.L.fans_gpu_sensitivity(self.ptr, _dev_dp(ub.reshape(-1), N * self.p.howmany), _dp(gb),
+                                                    _dev_dp(out, N * nq), out.numel()))
+            return out
         ubb = np.ascontiguousarray(np.asarray(ub, dtype=np.float64).reshape(-1))
         gb = np.ascontiguousarray(np.asarray(g0b, dtype=np.float64).reshape(-1))
         if ubb.size != N * self.p.howmany or gb.size != self.p.n_str:

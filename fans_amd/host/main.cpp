@@ -23,7 +23,8 @@ static void runSolver(Reader &reader, int precision)
     for (size_t li = 0; li < reader.load_cases.size(); ++li) {
         const LoadCase                 &lc = reader.load_cases[li];
         MaterialManager<howmany, n_str> matmanager(reader);
-        Solver<howmany, n_str>          solver(reader, matmanager, precision);
+        auto                            solver_ptr = createSolver<howmany, n_str>(reader, matmanager, precision);
+        Solver<howmany, n_str>         &solver     = *solver_ptr;
         std::printf("\n# Load case %zu/%zu: %d time steps\n", li + 1, reader.load_cases.size(), lc.n_steps);
         for (int t = 0; t < lc.n_steps; ++t) {
             std::printf("#   Time step %d/%d (load case %zu/%zu)\n", t + 1, lc.n_steps, li + 1,

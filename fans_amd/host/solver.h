@@ -2,10 +2,17 @@
 //
 // They keep the reference's names and call order so that runSolver
 // (src/main.cpp:10-41) reads the same:
-//   MaterialManager<howmany, n_str>   include/MaterialManager.h (set_gradient, phases)
-//   Solver<howmany, n_str>            include/solver.h:11-103 (solve, postprocess,
-//                                     enableMixedBC, extrapolateDisplacement,
+//   Matmodel<howmany, n_str>          include/matmodel.h:10-75 (n_mat, n_gp, FE_type,
+//                                     setGradient, macroscale_loading; the
+//                                     constitutive update itself runs on the GPU)
+//   MaterialManager<howmany, n_str>   include/MaterialManager.h (models, all_linear,
+//                                     set_gradient, phases)
+//   Solver<howmany, n_str>            include/solver.h:11-103 (solve, internalSolve,
+//                                     postprocess, enableMixedBC,
+//                                     extrapolateDisplacement,
 //                                     get_homogenized_stress / _tangent)
+//   SolverCG / SolverFP, createSolver include/solverCG.h:6-34, solverFP.h:6-23,
+//                                     setup.h:75-85
 // One Solver owns one GPU context (one load case, like main.cpp:15-16).  Errors
 // of the GPU library are raised as std::runtime_error with its message, like the
 // reference's exceptions.
@@ -15,6 +22,8 @@
 #include "reader.h"
 
 #include <cstdio>
+#include <initializer_list>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -55,50 +64,99 @@ inline int model_kind(const std::string &m)
     throw std::invalid_argument(m + " is not a supported matmodel");
 }
 
+inline bool model_is_linear(int kind)
+{
+    return kind == FANS_MODEL_THERMAL_ISO || kind == FANS_MODEL_THERMAL_TRI || kind == FANS_MODEL_ELASTIC_ISO ||
+           kind == FANS_MODEL_ELASTIC_TRI;
+}
+
 template <int howmany, int n_str>
 class Solver;
+
+// Matmodel (include/matmodel.h:10-75): one entry of the input's "materials"
+// array.  The host object keeps what the reference's model exposes at the
+// solver boundary -- its phases (n_mat), Gauss points (n_gp from FE_type),
+// macroscale_loading and linearity -- while get_sigma, the element residual
+// and the internal variables live on the device (the per-phase tables built
+// from `params`, in the order of the model's material_properties keys).
+template <int howmany, int n_str>
+class Matmodel {
+  public:
+    static constexpr int num_str = n_str;
+    Matmodel(const Reader &reader, const MaterialGroup &g)
+        : name(g.matmodel), kind(model_kind(g.matmodel)), n_mat((int) g.phases.size()),
+          n_gp(reader.FE_type == "HEX8R" ? 1 : 8), FE_type(reader.FE_type), phases(g.phases), params(g.params)
+    {
+        // the per-(howmany, n_str) name checks of createMatmodel (setup.h:21-73)
+        const bool thermal = kind == FANS_MODEL_THERMAL_ISO || kind == FANS_MODEL_THERMAL_TRI;
+        const bool large   = kind == FANS_MODEL_SVK || kind == FANS_MODEL_NEO_HOOKE;
+        if (howmany == 1 && !thermal)
+            throw std::invalid_argument(name + " is not a valid matmodel for thermal problem");
+        if (n_str == 6 && (thermal || large))
+            throw std::invalid_argument(name + " is not a valid small strain material model");
+        if (n_str == 9 && !large)
+            throw std::invalid_argument(name + " is not a valid large strain material model");
+    }
+    virtual ~Matmodel() = default;
+    void setGradient(std::vector<double> g0) { macroscale_loading = std::move(g0); }
+    bool is_linear() const { return model_is_linear(kind); }
+
+    std::string         name;
+    int                 kind;
+    int                 n_mat;  // number of phases of this model
+    int                 n_gp;
+    std::string         FE_type;
+    std::vector<int>    phases;
+    std::vector<double> params;  // [n_mat][keys]
+    std::vector<double> macroscale_loading;
+};
+
+// createMatmodel / createMaterialManager (include/setup.h:21-73, 87-91): the
+// model names of setup.h; an unknown name throws invalid_argument
+template <int howmany, int n_str>
+std::unique_ptr<Matmodel<howmany, n_str>> createMatmodel(const Reader &reader, const MaterialGroup &g)
+{
+    return std::make_unique<Matmodel<howmany, n_str>>(reader, g);
+}
 
 // MaterialManager (include/MaterialManager.h): material groups -> per-phase
 // tables, reference stiffness and Green operator (built on the device)
 template <int howmany, int n_str>
 class MaterialManager {
   public:
-    explicit MaterialManager(const Reader &reader) : reader_(reader) {}
+    explicit MaterialManager(const Reader &reader) : reader_(reader)
+    {
+        for (const auto &g : reader.materials) {
+            models.push_back(createMatmodel<howmany, n_str>(reader, g));
+            all_linear = all_linear && models.back()->is_linear();
+        }
+    }
+    // MaterialManager::set_gradient: every model gets the macroscale loading
     void set_gradient(const std::vector<double> &g0)
     {
         if ((int) g0.size() != n_str)
             throw std::invalid_argument("set_gradient: expected " + std::to_string(n_str) + " components");
         g0_ = g0;
+        for (auto &m : models)
+            m->setGradient(g0);
         if (solver_)
             check(solver_, fans_gpu_set_gradient(solver_, g0.data(), n_str), "set_gradient");
     }
-    bool has_j2() const
+
+    std::vector<std::unique_ptr<Matmodel<howmany, n_str>>> models;
+    bool                                                   all_linear = true;
+
+    bool has_kind(std::initializer_list<int> kinds) const
     {
-        for (const auto &g : reader_.materials) {
-            const int k = model_kind(g.matmodel);
-            if (k == FANS_MODEL_J2_NEW || k == FANS_MODEL_J2_VISCO_LIN || k == FANS_MODEL_J2_VISCO_NL)
-                return true;
-        }
+        for (const auto &m : models)
+            for (int k : kinds)
+                if (m->kind == k)
+                    return true;
         return false;
     }
-    bool has_visco() const
-    {
-        for (const auto &g : reader_.materials) {
-            const int k = model_kind(g.matmodel);
-            if (k == FANS_MODEL_J2_VISCO_LIN || k == FANS_MODEL_J2_VISCO_NL)
-                return true;
-        }
-        return false;
-    }
-    bool has_pseudo_plastic() const
-    {
-        for (const auto &g : reader_.materials) {
-            const int k = model_kind(g.matmodel);
-            if (k == FANS_MODEL_PSEUDO_PLASTIC || k == FANS_MODEL_PSEUDO_PLASTIC_NL)
-                return true;
-        }
-        return false;
-    }
+    bool has_j2() const { return has_kind({FANS_MODEL_J2_NEW, FANS_MODEL_J2_VISCO_LIN, FANS_MODEL_J2_VISCO_NL}); }
+    bool has_visco() const { return has_kind({FANS_MODEL_J2_VISCO_LIN, FANS_MODEL_J2_VISCO_NL}); }
+    bool has_pseudo_plastic() const { return has_kind({FANS_MODEL_PSEUDO_PLASTIC, FANS_MODEL_PSEUDO_PLASTIC_NL}); }
 
   private:
     friend class Solver<howmany, n_str>;
@@ -107,12 +165,12 @@ class MaterialManager {
         solver_ = ctx;
         std::vector<int32_t> kind, nph, pids;
         std::vector<double>  params;
-        for (const auto &g : reader_.materials) {
-            kind.push_back(model_kind(g.matmodel));
-            nph.push_back((int32_t) g.phases.size());
-            for (int p : g.phases)
+        for (const auto &m : models) {
+            kind.push_back(m->kind);
+            nph.push_back((int32_t) m->n_mat);
+            for (int p : m->phases)
                 pids.push_back(p);
-            params.insert(params.end(), g.params.begin(), g.params.end());
+            params.insert(params.end(), m->params.begin(), m->params.end());
         }
         check(ctx,
               fans_gpu_set_materials(ctx, (int32_t) kind.size(), kind.data(), nph.data(), pids.data(), params.data(),
@@ -130,7 +188,9 @@ class MaterialManager {
 template <int howmany, int n_str>
 class Solver {
   public:
-    Solver(Reader &reader, MaterialManager<howmany, n_str> &mat, int precision = FANS_PREC_FP64)
+    // method: FANS_METHOD_CG or FANS_METHOD_FP (SolverCG / SolverFP below)
+    Solver(Reader &reader, MaterialManager<howmany, n_str> &mat, int precision = FANS_PREC_FP64,
+           int method = -1)
         : reader_(reader), mat_(mat)
     {
         fans_gpu_desc d{};
@@ -139,7 +199,7 @@ class Solver {
         d.large_strain = n_str == 9;
         d.fe_type     = reader.FE_type == "HEX8R" ? FANS_FE_HEX8R : (reader.FE_type == "BBAR" ? FANS_FE_BBAR
                                                                                                 : FANS_FE_HEX8);
-        d.method      = reader.method == "fp" ? FANS_METHOD_FP : FANS_METHOD_CG;
+        d.method      = method >= 0 ? method : (reader.method == "fp" ? FANS_METHOD_FP : FANS_METHOD_CG);
         d.measure     = reader.err_measure == "L1" ? FANS_MEASURE_L1
                         : reader.err_measure == "L2" ? FANS_MEASURE_L2
                                                      : FANS_MEASURE_LINF;
@@ -164,17 +224,24 @@ class Solver {
               "set_microstructure");
         mat_.attach(ctx_);
     }
-    ~Solver() { fans_gpu_destroy(ctx_); }
+    virtual ~Solver() { fans_gpu_destroy(ctx_); }
     Solver(const Solver &)            = delete;
     Solver &operator=(const Solver &) = delete;
 
-    // Solver::solve (solver.h:282-300)
+    // Solver::solve (solver.h:282-300): internalSolve, then the internal
+    // variables are committed -- on the device, inside fans_gpu_solve
     void solve()
+    {
+        internalSolve();
+        std::printf("# Complete FANS - Number of iterations: %d (%.3f ms on the GPU)\n", iter, stats_.solve_ms);
+    }
+    // the iteration of the context's method (SolverCG / SolverFP)
+    virtual void internalSolve()
     {
         check(ctx_, fans_gpu_solve(ctx_, &stats_), "solve");
         iter = stats_.iterations;
-        std::printf("# Complete FANS - Number of iterations: %d (%.3f ms on the GPU)\n", iter, stats_.solve_ms);
     }
+    bool isMixedBCActive() const { return mixed_; }
     void enableMixedBC(const MixedBC &m, size_t step)
     {
         check(ctx_,
@@ -182,8 +249,13 @@ class Solver {
                                        m.idx_F.data(), m.n_steps, m.strain_path.data(), m.stress_path.data(),
                                        (int32_t) step),
               "enableMixedBC");
+        mixed_ = true;
     }
-    void disableMixedBC() { check(ctx_, fans_gpu_disable_mixed_bc(ctx_), "disableMixedBC"); }
+    void disableMixedBC()
+    {
+        check(ctx_, fans_gpu_disable_mixed_bc(ctx_), "disableMixedBC");
+        mixed_ = false;
+    }
     void extrapolateDisplacement() { check(ctx_, fans_gpu_extrapolate_displacement(ctx_), "extrapolateDisplacement"); }
     std::vector<double> get_homogenized_stress()
     {
@@ -212,12 +284,46 @@ class Solver {
 
     int iter = 0;
 
-  private:
+  protected:
     Reader                          &reader_;
     MaterialManager<howmany, n_str> &mat_;
     fans_gpu_ctx                    *ctx_ = nullptr;
     fans_gpu_stats                   stats_{};
+    bool                             mixed_ = false;
 };
+
+// SolverCG (solverCG.h:6-34): Green-preconditioned CG, linear fast path and
+// secant line search for nonlinear models
+template <int howmany, int n_str>
+class SolverCG : public Solver<howmany, n_str> {
+  public:
+    SolverCG(Reader &reader, MaterialManager<howmany, n_str> &mat, int precision = FANS_PREC_FP64)
+        : Solver<howmany, n_str>(reader, mat, precision, FANS_METHOD_CG)
+    {
+    }
+};
+
+// SolverFP (solverFP.h:6-23): the fixed-point (Moulinec-Suquet) iteration
+template <int howmany, int n_str>
+class SolverFP : public Solver<howmany, n_str> {
+  public:
+    SolverFP(Reader &reader, MaterialManager<howmany, n_str> &mat, int precision = FANS_PREC_FP64)
+        : Solver<howmany, n_str>(reader, mat, precision, FANS_METHOD_FP)
+    {
+    }
+};
+
+// createSolver (setup.h:75-85)
+template <int howmany, int n_str>
+std::unique_ptr<Solver<howmany, n_str>> createSolver(Reader &reader, MaterialManager<howmany, n_str> &mat,
+                                                     int precision = FANS_PREC_FP64)
+{
+    if (reader.method == "fp")
+        return std::make_unique<SolverFP<howmany, n_str>>(reader, mat, precision);
+    if (reader.method == "cg")
+        return std::make_unique<SolverCG<howmany, n_str>>(reader, mat, precision);
+    throw std::invalid_argument(reader.method + " is not a valid method");
+}
 
 template <int howmany, int n_str>
 void Solver<howmany, n_str>::postprocess(Reader &reader, int load_idx, int time_idx)

@@ -21,6 +21,12 @@ moduli and the fluctuation solves the discrete equilibrium, so these are the
 exact derivatives of the discrete homogenized stress (up to the solver
 tolerance).  No reference counterpart: the reference's per-phase parameters
 with uint16 phase ids cannot express per-voxel moduli.
+
+With K and G in GPU memory (torch imported before the library, so that both
+share one HIP runtime) the moduli, the forward solution kept for the backward
+pass and the per-voxel gradients stay on the device: only the n_str-vectors
+g0, Sigma_bar and the upstream gradient cross to the host.  A density may be
+stored in FP32; simp_moduli evaluates the moduli in FP64.
 """
 from __future__ import annotations
 
@@ -80,8 +86,14 @@ class HomogenizedStress(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, K, G, g0, solver):
-        k = K.detach().to("cpu", torch.float64).numpy()
-        g = None if solver.p.howmany == 1 else G.detach().to("cpu", torch.float64).numpy()
+        mech = solver.p.howmany != 1
+        ctx.on_device = K.is_cuda
+        if ctx.on_device:
+            k = K.detach().to(torch.float64).contiguous()
+            g = G.detach().to(torch.float64).contiguous() if mech else None
+        else:
+            k = K.detach().to("cpu", torch.float64).numpy()
+            g = G.detach().to("cpu", torch.float64).numpy() if mech else None
         gv = g0.detach().to("cpu", torch.float64).numpy()
         solver.set_voxel_moduli(k, g)
         solver.reset_state()
@@ -89,7 +101,10 @@ class HomogenizedStress(torch.autograd.Function):
         solver.solve()
         sig = solver.homogenized_stress()
         ctx.solver = solver
-        ctx.u = solver.get_u().copy()
+        if ctx.on_device:
+            ctx.u = solver.get_u(out=torch.empty(solver.N * solver.h, dtype=torch.float64, device=K.device))
+        else:
+            ctx.u = solver.get_u().copy()
         ctx.g0 = gv.copy()
         ctx.k, ctx.g = k, g
         ctx.dims = K.shape
@@ -107,12 +122,16 @@ class HomogenizedStress(torch.autograd.Function):
         dg0 = s.homogenized_stress()  # Sigma_bar(v) = C_bar v = C_bar^T v
         sens = s.sensitivity(ctx.u, ctx.g0)
         V = float(np.prod(s.p.L))
-        dev = grad_sigma.device
-        dK = torch.as_tensor(sens[:, 0] / V, dtype=torch.float64, device=dev).reshape(ctx.dims)
-        dG = None
-        if sens.shape[1] > 1:
-            dG = torch.as_tensor(sens[:, 1] / V, dtype=torch.float64, device=dev).reshape(ctx.dims)
-        return dK, dG, torch.as_tensor(dg0, dtype=torch.float64, device=dev), None
+        if ctx.on_device:
+            dK = (sens[:, 0] / V).reshape(ctx.dims)
+            dG = (sens[:, 1] / V).reshape(ctx.dims) if sens.shape[1] > 1 else None
+        else:
+            dev = grad_sigma.device
+            dK = torch.as_tensor(sens[:, 0] / V, dtype=torch.float64, device=dev).reshape(ctx.dims)
+            dG = None
+            if sens.shape[1] > 1:
+                dG = torch.as_tensor(sens[:, 1] / V, dtype=torch.float64, device=dev).reshape(ctx.dims)
+        return dK, dG, torch.as_tensor(dg0, dtype=torch.float64, device=grad_sigma.device), None
 
 
 def homogenized_stress(solver: GpuSolver, K, G, g0):
@@ -124,7 +143,10 @@ def homogenized_stress(solver: GpuSolver, K, G, g0):
 
 def simp_moduli(rho, E_solid, nu_solid, E_void, nu_void, p=3.0):
     """SIMP interpolation E(rho) = E_void + rho^p (E_solid - E_void), likewise
-    nu, then bulk and shear moduli; all torch, differentiable."""
+    nu, then bulk and shear moduli; all torch, differentiable.  A density stored
+    in FP32 is widened first: the moduli and their gradients are FP64."""
+    if rho.dtype != torch.float64:
+        rho = rho.to(torch.float64)
     wgt = rho ** p
     E = E_void + wgt * (E_solid - E_void)
     nu = nu_void + wgt * (nu_solid - nu_void)

@@ -79,7 +79,11 @@ enum {
                                       [the seven above, saturation_stress saturation_exponent]
                                       (J2Plasticity.h:182-243)                             */
 };
-/* fields for fans_gpu_get_field / fans_gpu_set_field */
+/* fields for fans_gpu_get_field / fans_gpu_set_field.  The nodal fields
+ * (U, R, U_PREV) may be read into and written from host or device memory; a
+ * device pointer must come from the HIP runtime the library is loaded with
+ * (with PyTorch in the process: import torch first, so that both share its
+ * runtime).  The Gauss-point and element fields go to host memory. */
 enum {
     FANS_FIELD_U              = 0, /* displacement fluctuation v_u, double [N][h]      */
     FANS_FIELD_R              = 1, /* residual v_r, double [N][h]                      */
@@ -154,12 +158,15 @@ int fans_gpu_get_gradient(fans_gpu_ctx *ctx, double *g0, int32_t n_str);
  * moduli (SIMP densities, pyfans differentiation); the reference stiffness
  * follows the rules of LinearElastic.h:55-69 / LinearThermal.h:19-45 as if
  * every voxel were a phase (unless set_materials got ref_material), and the
- * Green operator is rebuilt.  Requires isotropic linear phases. */
+ * Green operator is rebuilt.  Requires isotropic linear phases.  k and g may be
+ * host or device arrays (as for the nodal fields); the check that every value
+ * is positive and the reference-stiffness statistics run on the device. */
 int fans_gpu_set_voxel_moduli(fans_gpu_ctx *ctx, const double *k, const double *g, int64_t count);
 
 /* Sensitivity of the homogenized response to per-voxel moduli.  For the
  * context's current solution (u, g0) and a second solution ub for the macro
- * gradient g0b (host arrays, [N][h] and n_str), writes per element the Gauss
+ * gradient g0b ([N][h], host or device, and n_str, host), writes per element
+ * (sens: host or device) the Gauss
  * integrals sens[e] = (int_e eps_a^T dC/dK eps_b, int_e eps_a^T dC/dG eps_b)
  * (thermal: int_e eps_a . eps_b, one value), eps_a = B u + g0, eps_b = B ub + g0b.
  * When both solve their load cases, d(g0b . Sigma_bar(g0)) / dK_e = sens[e][0] / V

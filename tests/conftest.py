@@ -4,6 +4,15 @@ import sys
 import numpy as np
 import pytest
 
+# One HIP runtime per test process: PyTorch bundles its own libamdhip64, and the
+# library binds to whichever copy is loaded first.  Importing torch before any
+# test loads the library makes both use torch's, so the device-resident autograd
+# tests can hand torch GPU tensors to the library.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
