@@ -268,6 +268,10 @@ struct fans_gpu_ctx {
     // FP32 inner solve (mixed form)
     float        *r32 = nullptr, *s32 = nullptr, *dA32 = nullptr, *dB32 = nullptr, *rnew32 = nullptr, *x32 = nullptr;
     cplx<float>  *spec32 = nullptr;
+    // linear CG direction rings: slots 0 and 1 are dA / dB (dA32 / dB32), the
+    // others are allocated by the first linear CG solve
+    double *ring64[kDirRing]{};
+    float  *ring32[kDirRing]{};
     Fft3D<double> fft64;
     Fft3D<float>  fft32;
 
@@ -1052,58 +1056,71 @@ void host_sym_pinv(int n, const double *A, double *P)
 
 // Linear CG fast path (solverCG.h:84-117, islinear && !mixed); every scalar
 // lives on the device, the host only polls the done flag once per chunk.
-// Iteration i runs
-//   z r2c [+ the residual update of iteration i-1: r -= alpha K d]
+// Iteration j runs
+//   z r2c [+ the residual update of iteration j-1: r -= alpha K d]
 //   -> RED_ERR_NEXT (err; sets done) -> y, x+Green (delta = <r, s> by
 //   Parseval), y^-1 -> RED_DELTA (beta from delta and deltamid)
-//   -> z c2r (d = s + beta d_prev with s = -G r, s itself not stored;
-//      <d, K d_prev> from the previous matvec's output; the solution update of
-//      iteration i-1, u -= alpha d_prev)
-//   -> matvec (K d, <d, K d>) -> RED_ALPHA (alpha, and deltamid of iteration
-//      i+1 = <r - alpha K d, s> = delta - alpha (<d, K d> - beta <d, K d_prev>),
-//      the reference's <r, s> of solverCG.h:87 in exact arithmetic, K being
-//      symmetric)
-// so no update costs a pass of its own.  After RED_ERR_NEXT sets `done` every
-// later kernel returns at entry; u and r then hold the final iterate but for
-// the pending u -= alpha d of iteration i-1, which finish() applies.
+//   -> z c2r (d_j = s + beta d_(j-1) with s = -G r, s itself not stored;
+//      <d_j, K d_(j-1)> from the previous matvec's output)
+//   -> matvec (K d_j, <d_j, K d_j>) -> RED_ALPHA (alpha_j, and deltamid of
+//      iteration j+1 = <r - alpha K d, s> = delta - alpha (<d, K d> - beta
+//      <d, K d_prev>), the reference's <r, s> of solverCG.h:87 in exact
+//      arithmetic, K being symmetric)
+// The directions go round a ring of kDirRing buffers (d_j in slot j % kDirRing)
+// and the solution update u -= alpha_j d_j (solverCG.h:105) is applied for
+// kDirRing iterations at a time, after the iteration that fills the last slot:
+// one pass reading kDirRing directions instead of a read-modify-write of u per
+// iteration.  After RED_ERR_NEXT sets `done` every later kernel returns at
+// entry; r then holds the final residual and finish() applies the directions
+// left in the ring.  ring[kDirRing - 1] must hold zeros on entry (d_(-1) = 0).
 template <typename T>
-int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *&dcur, T *&dnext, T *rnew, T *u, cplx<T> *spec,
-                   const T *Gp, const Phase<T> *ph, int *iters_out)
+int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *const *ring, T *rnew, T *u, cplx<T> *spec, const T *Gp,
+                   const Phase<T> *ph, int *iters_out)
 {
     const int *done  = c->is + IS_DONE;
     int        chunk = 4, rc;
     int        is[IS_COUNT];
-    bool       first = true;
-    // The inverse z pass of iteration j applies u -= alpha_(j-1) d_(j-1), so
-    // the update of iteration k-1 is still pending when the stop test fires in
-    // iteration k.  Iteration j writes its direction into the buffer that was
-    // dnext at entry for even j and into dcur for odd j.
-    T *const d_even = dnext, *const d_odd = dcur;
-    auto     finish = [&]() -> int {
+    // u -= alpha_m d_m for the ring slots 0 .. count-1 (iterations filled in order)
+    auto ring_update = [&](int count, const int *dflag) -> int {
+        Timed      t(c, KF_CG_UPDATE);
+        DirRing<T> dr{};
+        for (int m = 0; m < kDirRing; ++m)
+            dr.d[m] = ring[m];
+        if (count == kDirRing)
+            hipLaunchKernelGGL((k_ring_update<T, true>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, u, dr,
+                               c->sc + SC_ARING, count, c->Nh, dflag);
+        else
+            hipLaunchKernelGGL((k_ring_update<T, false>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, u, dr,
+                               c->sc + SC_ARING, count, c->Nh, dflag);
+        HIPCHK(c, hipGetLastError());
+        return 0;
+    };
+    // iterations 0 .. k-1 have run; those after the last full ring are pending
+    auto finish = [&]() -> int {
         const int k = is[IS_ITER];
-        if (k >= 1) {
-            T *dl = ((k - 1) % 2 == 0) ? d_even : d_odd;
-            hipLaunchKernelGGL((k_sub_scaled<T>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, u, dl,
-                               c->sc + SC_ALPHA, c->Nh);
-            HIPCHK(c, hipGetLastError());
+        if (k % kDirRing) {
+            const int e = ring_update(k % kDirRing, nullptr);
+            if (e)
+                return e;
         }
         if (iters_out)
             *iters_out = k;
         return 0;
     };
-    // one CG iteration (with the update of the previous one unless `first`)
-    auto iteration = [&](bool with_pro) -> int {
+    // CG iteration j (with the residual update of iteration j-1 unless j == 0)
+    auto iteration = [&](int j) -> int {
+        const bool       with_pro = j > 0;
+        T *const         dprev    = ring[(j + kDirRing - 1) % kDirRing];
+        T *const         dnext    = ring[j % kDirRing];
         const ProArgs<T> pa{rnew, c->sc + SC_ALPHA, c->d.measure, c->partial};
-        // delta and beta from the spectrum, then d = s + beta d_cur in the
-        // inverse z pass
+        // delta and beta from the spectrum, then d = s + beta d_prev in the
+        // inverse z pass, with <d, K d_prev> (read after the matvec); the first
+        // iteration has d_prev = 0, and rnew holds no K d_prev yet
         EpiArgs<T> ep{};
-        // <d, K d_prev> (read after the matvec); the first iteration has
-        // d_prev = 0, and rnew holds no K d_prev yet
         ep.partial = c->partial_e;
-        ep.dprev   = dcur;
+        ep.dprev   = dprev;
         ep.qprev   = with_pro ? rnew : nullptr;
         ep.dnext   = dnext;
-        ep.usol    = u;
         ep.scal    = c->sc;  // SC_BETA == 0, SC_ALPHA == 1
         int e;
         if ((e = convolve_dev<T>(c, P, r, spec, Gp, EPI_CGD, ep, done, with_pro ? &pa : nullptr)))
@@ -1120,36 +1137,35 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *&dcur, T *&dnext, T *r
         if ((e = reduce(c, c->partial, c->nblk_asm, 2, RED_ALPHA, with_pro ? c->partial_e : nullptr,
                         (int) P.nblk_z)))
             return e;
-        std::swap(dcur, dnext);
+        if (j % kDirRing == kDirRing - 1 && (e = ring_update(kDirRing, done)))
+            return e;
         return 0;
     };
-    // After the first iteration every iteration has the same launches; a pair
-    // of iterations returns dcur/dnext to their roles, so kGraphIters (even)
-    // iterations are captured once into a hipGraph and replayed (one launch
-    // per replay instead of ~10 per iteration).  Timing mode stays eager.
-    constexpr int kGraphIters = 8;
+    // After iteration 0 the launches of iteration j depend on j % kDirRing
+    // only, so kDirRing iterations are captured once into a hipGraph and
+    // replayed (one launch per replay instead of ~10 per iteration).  Timing
+    // mode stays eager.
+    static_assert(kDirRing == 8, "one captured graph holds one turn of the direction ring");
     if (c->use_graphs && !c->timing) {
-        if ((rc = iteration(false)))
+        if ((rc = iteration(0)))
             return rc;
-        first = false;
         hipGraphExec_t exec = nullptr;
         for (auto &g : c->graphs)
-            if (g.first == (const void *) dcur)
+            if (g.first == (const void *) ring[0])
                 exec = g.second;
         if (!exec) {
             hipGraph_t graph = nullptr;
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int crc = 0;
-            T  *d0  = dcur;
-            for (int k = 0; k < kGraphIters && !crc; ++k)
-                crc = iteration(true);
+            for (int j = 1; j <= kDirRing && !crc; ++j)
+                crc = iteration(j);
             const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
             if (crc)
                 return crc;
             HIPCHK(c, ec);
             HIPCHK(c, hipGraphInstantiateWithFlags(&exec, graph, 0));
             (void) hipGraphDestroy(graph);
-            c->graphs.push_back({(const void *) d0, exec});
+            c->graphs.push_back({(const void *) ring[0], exec});
         }
         for (;;) {
             HIPCHK(c, hipGraphLaunch(exec, c->stream));
@@ -1160,23 +1176,21 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *&dcur, T *&dnext, T *r
         }
         return finish();
     }
-    for (;;) {
+    for (int j = 0;;) {
         // timing mode checks `done` after every iteration so that launches that
         // returned at entry are not counted in the per-kernel durations
         const int n = c->timing ? 1 : chunk;
-        for (int k = 0; k < n; ++k) {
-            const size_t mark    = c->ev_used.size();
-            const bool   had_pro = !first;
-            if ((rc = iteration(had_pro)))
+        for (int k = 0; k < n; ++k, ++j) {
+            const size_t mark = c->ev_used.size();
+            if ((rc = iteration(j)))
                 return rc;
-            first = false;
             if (c->timing) {
                 if ((rc = read_scalars(c, nullptr, is)))
                     return rc;
                 // done was set by this iteration's RED_ERR_NEXT (the fused update
                 // and that reduction did work) or before it (nothing did)
                 if (is[IS_DONE])
-                    drop_timing(c, had_pro ? mark + 2 : mark);
+                    drop_timing(c, j > 0 ? mark + 2 : mark);
             }
         }
         if (!c->timing && (rc = read_scalars(c, nullptr, is)))
@@ -1197,24 +1211,39 @@ int fetch_history(fans_gpu_ctx *c, int iters)
     return 0;
 }
 
+// the direction ring of the linear CG in precision T (slots 0, 1 = dA, dB)
+template <typename T>
+int ensure_ring(fans_gpu_ctx *c, T *(&ring)[kDirRing], T *dA, T *dB)
+{
+    ring[0] = dA;
+    ring[1] = dB;
+    for (int m = 2; m < kDirRing; ++m)
+        if (!ring[m])
+            HIPCHK(c, hipMalloc(&ring[m], sizeof(T) * c->Nh));
+    return 0;
+}
+
 int solve_cg_fp64(fans_gpu_ctx *c)
 {
     int rc;
     if ((rc = init_scalars(c, c->tol, c->tol_type, c->d.n_it)))
         return rc;
     c->alpha_warm = 0.1;
-    if ((rc = fill<double>(c, c->s, 0.0, c->Nh)) || (rc = fill<double>(c, c->dA, 0.0, c->Nh)))
+    const bool linear = c->all_linear && !c->mixed_active;
+    if (linear && (rc = ensure_ring<double>(c, c->ring64, c->dA, c->dB)))
+        return rc;
+    // d_(-1) = 0: dA for the nonlinear loop, the ring's last slot for the linear one
+    if ((rc = fill<double>(c, c->s, 0.0, c->Nh)) ||
+        (rc = fill<double>(c, linear ? c->ring64[kDirRing - 1] : c->dA, 0.0, c->Nh)))
         return rc;
     if ((rc = residual64(c, c->u, c->r, nullptr)))
         return rc;
     if ((rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_INIT)))
         return rc;
-    const bool linear = c->all_linear && !c->mixed_active;
     if (linear) {
-        double *dcur = c->dA, *dnext = c->dB;
-        int     it   = 0;
-        if ((rc = cg_linear_loop<double>(c, c->fft64, c->r, dcur, dnext, c->rnew, c->u, c->spec64,
-                                                 c->G64, c->dph64, &it)))
+        int it = 0;
+        if ((rc = cg_linear_loop<double>(c, c->fft64, c->r, c->ring64, c->rnew, c->u, c->spec64, c->G64, c->dph64,
+                                         &it)))
             return rc;
         c->iter = it;
         return fetch_history(c, it);
@@ -1381,15 +1410,17 @@ int solve_cg_mixed(fans_gpu_ctx *c)
         const double err_out = sc[SC_ERR];
         if ((rc = init_scalars(c, std::max(eta * err_out, 0.25 * c->tol), FANS_ERR_ABSOLUTE, c->d.n_it - inner)))
             return rc;
-        hipLaunchKernelGGL(k_inner_start, dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r32, c->x32, c->dA32,
-                           c->r, c->Nh, c->d.measure, c->partial);
+        if ((rc = ensure_ring<float>(c, c->ring32, c->dA32, c->dB32)))
+            return rc;
+        // x = 0 and d_(-1) = 0 in the ring's last slot
+        hipLaunchKernelGGL(k_inner_start, dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r32, c->x32,
+                           c->ring32[kDirRing - 1], c->r, c->Nh, c->d.measure, c->partial);
         HIPCHK(c, hipGetLastError());
         if ((rc = reduce(c, c->partial, c->nblk_vec, 1, RED_ERR_INIT)))
             return rc;
-        float *dcur = c->dA32, *dnext = c->dB32;
-        int    it   = 0;
-        if ((rc = cg_linear_loop<float>(c, c->fft32, c->r32, dcur, dnext, c->rnew32, c->x32, c->spec32,
-                                               c->G32, c->dph32, &it)))
+        int it = 0;
+        if ((rc = cg_linear_loop<float>(c, c->fft32, c->r32, c->ring32, c->rnew32, c->x32, c->spec32, c->G32,
+                                        c->dph32, &it)))
             return rc;
         inner += it;
         outer += 1;
@@ -1471,6 +1502,12 @@ void free_all(fans_gpu_ctx *c)
                     (void *) c->vk32, (void *) c->vg32, (void *) c->ub_d, (void *) c->sens_d})
         if (p)
             (void) hipFree(p);
+    for (int m = 2; m < kDirRing; ++m) {
+        if (c->ring64[m])
+            (void) hipFree(c->ring64[m]);
+        if (c->ring32[m])
+            (void) hipFree(c->ring32[m]);
+    }
     drop_graphs(c);
     c->fft64.release();
     c->fft32.release();

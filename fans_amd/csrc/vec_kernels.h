@@ -22,8 +22,12 @@ enum {
     SC_DN       = 7,
     SC_TOL      = 8,
     SC_RES0     = 16,  // generic reduction results [16, 32)
-    SC_COUNT    = 32
+    SC_ARING    = 32,  // linear CG: alpha of iteration j at SC_ARING + j % kDirRing
+    SC_COUNT    = 40
 };
+// linear CG: search directions kept per iteration before the solution update
+// u -= sum alpha_j d_j is applied in one pass
+constexpr int kDirRing = 8;
 // device integer slots
 enum { IS_DONE = 0, IS_ITER = 1, IS_NIT = 2, IS_ERRTYPE = 3, IS_MEASURE = 4, IS_COUNT = 8 };
 
@@ -106,6 +110,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double *__restr
             } else if (stage == RED_ALPHA) {
                 sc[SC_DN]    = v[0];
                 sc[SC_ALPHA] = sc[SC_DELTA] / v[0];
+                sc[SC_ARING + is[IS_ITER] % kDirRing] = sc[SC_ALPHA];
                 // deltamid of the next iteration, <r - alpha K d, s> with
                 // s = d - beta d_prev (solverCG.h:87, 91 and 104):
                 // delta - alpha (<d, K d> - beta <K d_prev, d>), K symmetric
@@ -189,15 +194,40 @@ __global__ __launch_bounds__(256) void k_inner_start(float *__restrict__ r32, fl
         partial[blockIdx.x] = v[0];
 }
 
-// y -= a x with the scalar a on the device (the CG solution update left
-// pending when the iteration stops)
+// Linear CG solution update over the direction ring: u -= alpha_j d_j for the
+// ring slots 0 .. count-1 in order (iteration order, each step rounded as the
+// reference's per-iteration update v_u_real -= alpha * d_real,
+// solverCG.h:105), alpha_j from the device scalars
 template <typename T>
-__global__ __launch_bounds__(256) void k_sub_scaled(T *__restrict__ y, const T *__restrict__ x,
-                                                    const double *__restrict__ a, int64_t n)
+struct DirRing {
+    const T *d[kDirRing];
+};
+
+// FULL: all kDirRing slots (count ignored; every load unconditional, so all of
+// them are in flight together)
+template <typename T, bool FULL>
+__global__ __launch_bounds__(256) void k_ring_update(T *__restrict__ u, DirRing<T> ring,
+                                                    const double *__restrict__ alphas, int count, int64_t n,
+                                                    const int *__restrict__ done)
 {
-    const T s = (T) *a;
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
-        y[i] -= s * x[i];
+    if (done && *done)
+        return;
+    T a[kDirRing];
+#pragma unroll
+    for (int m = 0; m < kDirRing; ++m)
+        a[m] = (FULL || m < count) ? (T) alphas[m] : (T) 0;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        T dv[kDirRing];
+#pragma unroll
+        for (int m = 0; m < kDirRing; ++m)
+            dv[m] = (FULL || m < count) ? __builtin_nontemporal_load(&ring.d[m][i]) : (T) 0;
+        T v = __builtin_nontemporal_load(&u[i]);
+#pragma unroll
+        for (int m = 0; m < kDirRing; ++m)
+            if (FULL || m < count)
+                v -= a[m] * dv[m];
+        __builtin_nontemporal_store(v, &u[i]);
+    }
 }
 
 // Streaming copy, 16 bytes per lane per access, each block owning a
