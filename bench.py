@@ -40,10 +40,12 @@ def family_bytes_per_voxel(nz, h=3, wb=8, green_quarter=True):
         "fft_y_fwd": 2 * S,
         "fft_x_green": 2 * S + Gp,        # 54.6 B at 256^3 with quarter storage
         "fft_y_inv": 2 * S,
-        # CG inverse z pass: read the spectrum, d_prev, K d_prev (for
-        # <d, K d_prev>) and u; write the new direction d = s + beta d_prev
-        # and u -= alpha d_prev
-        "fft_z_c2r": S + 5 * V,
+        # CG inverse z pass: read the spectrum, d_prev and K d_prev (for
+        # <d, K d_prev>); write the new direction d = s + beta d_prev
+        "fft_z_c2r": S + 3 * V,
+        # CG solution update once per eight iterations: read eight directions
+        # and u, write u
+        "cg_update": 10 * V,
         # matvec: read d and the phase; write rnew = K d
         "assemble_lin": 2 * V + 1,
         # residual update fused into the forward z pass: read r and rnew;

@@ -184,7 +184,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAsmWaves<T
     fetch_plane(X0 - 1);
     commit_plane(0);
     fetch_plane(X0);
-
     int                   ph_next = phase_of(X0 - 1);
     PointModel<T, H, NGP, VAR == ASM_GENERAL_PNL || VAR == ASM_GENERAL_VISCO, VAR == ASM_GENERAL_VISCO> pm;
     bool pm_pre = false;  // point 0 of the coming element requested

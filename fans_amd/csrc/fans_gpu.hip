@@ -1121,7 +1121,7 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *const *ring, T *rnew, 
         ep.dprev   = dprev;
         ep.qprev   = with_pro ? rnew : nullptr;
         ep.dnext   = dnext;
-        ep.scal    = c->sc;  // SC_BETA == 0, SC_ALPHA == 1
+        ep.scal    = c->sc;  // SC_BETA == 0
         int e;
         if ((e = convolve_dev<T>(c, P, r, spec, Gp, EPI_CGD, ep, done, with_pro ? &pa : nullptr)))
             return e;

@@ -221,10 +221,7 @@ __global__ __launch_bounds__(256) void k_fft_z_r2c(T *__restrict__ in, cplx<T> *
 // d = s + beta d_prev with s = -result (beta from the spectral <r, s> of the x
 // pass); s itself is not stored.  With qprev = K d_prev (the previous matvec's
 // output) it emits per block the partial <d, K d_prev>, from which the CG
-// scalars form the next deltamid.  With usol set it also applies the previous
-// iteration's solution update u -= alpha d_prev (d_prev is read here anyway;
-// moved into the matvec, the extra accesses cost that pass more than they
-// saved here).
+// scalars form the next deltamid.
 enum { EPI_STORE = 0, EPI_CG = 1, EPI_FP = 2, EPI_CGD = 3 };
 
 template <typename T>
@@ -232,12 +229,11 @@ struct EpiArgs {
     T            *out;      // EPI_STORE: result; EPI_CG: s = -result
     const T      *r;        // EPI_CG: partial <r, s>
     double       *u;        // EPI_FP: u -= result (FP64 displacement)
-    double       *partial;  // EPI_CG: one partial sum per block
+    double       *partial;  // EPI_CG / EPI_CGD: one partial sum per block
     const T      *dprev;    // EPI_CGD: previous direction (read)
     const T      *qprev;    // EPI_CGD: K d_prev (nullable), partial <d, K d_prev>
     T            *dnext;    // EPI_CGD: new direction (written)
-    T            *usol;     // EPI_CGD: solution, u -= alpha d_prev (nullable)
-    const double *scal;     // EPI_CGD: device scalars (beta at [0], alpha at [1])
+    const double *scal;     // EPI_CGD: device scalars (beta at [0])
 };
 
 // c2r pre-processing of one (line, component): `line` points at kz = 0 of the
@@ -254,7 +250,7 @@ __device__ __forceinline__ cplx<T> c2r_tangle(const cplx<T> *__restrict__ line, 
 }
 
 template <typename T, int EPI>
-__device__ __forceinline__ void z_epilogue(const EpiArgs<T> &ep, int64_t gi, T val, double &acc, T beta, T alpha)
+__device__ __forceinline__ void z_epilogue(const EpiArgs<T> &ep, int64_t gi, T val, double &acc, T beta)
 {
     if constexpr (EPI == EPI_STORE) {
         ep.out[gi] = val;
@@ -266,8 +262,6 @@ __device__ __forceinline__ void z_epilogue(const EpiArgs<T> &ep, int64_t gi, T v
         ep.dnext[gi] = dn;
         if (ep.qprev)
             acc += (double) dn * (double) __builtin_nontemporal_load(&ep.qprev[gi]);
-        if (ep.usol)
-            __builtin_nontemporal_store(__builtin_nontemporal_load(&ep.usol[gi]) - alpha * dp, &ep.usol[gi]);
     } else if constexpr (EPI == EPI_CG) {
         // s is read by the very next pass (the matvec) and stays cached; r is
         // not read again this iteration
@@ -315,20 +309,23 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const cplx<T> *__restrict__ i
     const int     nzh = nz * H;
     const int64_t g0  = line0 * nzh;
     double        acc = 0.0;
-    T             beta = (T) 0, alpha = (T) 0;
-    if constexpr (EPI == EPI_CGD) {
-        beta  = (T) ep.scal[0];
-        alpha = (T) ep.scal[1];
-    }
+    T             beta = (T) 0;
+    if constexpr (EPI == EPI_CGD)
+        beta = (T) ep.scal[0];
     for (int e = threadIdx.x; e < nl * nzh; e += blockDim.x) {
         const int     l = e / nzh, rem = e - l * nzh, z = rem / H, c = rem - z * H;
         const cplx<T> v = A[(z >> 1) * ld + l * H + c];
-        z_epilogue<T, EPI>(ep, g0 + e, (z & 1) ? v.y : v.x, acc, beta, alpha);
+        z_epilogue<T, EPI>(ep, g0 + e, (z & 1) ? v.y : v.x, acc, beta);
     }
     z_epi_partials<EPI>(ep, acc, red);
 }
 
 // ---- z passes with compile-time shape ---------------------------------------
+#ifndef FANS_C2R_PREFETCH
+#define FANS_C2R_PREFETCH 1
+#endif
+constexpr bool kC2rPrefetch = FANS_C2R_PREFETCH != 0;
+
 // For M = nz/2 in {16, 32, 64, 128} the line count per block, the thread count
 // and every index are compile-time constants (no runtime divisions), and the
 // thread count equals the butterfly count of the first radix stage.
@@ -459,6 +456,23 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cp
     __shared__ double  red[NT / 64 + 1];
     const int64_t      line0 = (int64_t) blockIdx.x * LPB;
     const cplx<T>     *src   = in + line0 * MC * H;
+    const int64_t      g0    = line0 * NZH;
+    // EPI_CGD, FP64: the per-thread values of d_prev and K d_prev, requested
+    // before the transform so that their loads overlap it (0.354 -> 0.306 ms
+    // at 256^3; in FP32 the registers it takes cost more occupancy than it
+    // gains, 0.208 -> 0.220 ms)
+    constexpr int PER = (LPB * NZH) / NT;
+    constexpr bool PRE = EPI == EPI_CGD && kC2rPrefetch && sizeof(T) == 8 && (LPB * NZH) % NT == 0;
+    T              dpv[PRE ? PER : 1], qpv[PRE ? PER : 1];
+    if constexpr (PRE) {
+        const T *__restrict__ dp = ep.dprev;
+        const T *__restrict__ qp = ep.qprev;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            dpv[i] = __builtin_nontemporal_load(&dp[g0 + threadIdx.x + i * NT]);
+            qpv[i] = qp ? __builtin_nontemporal_load(&qp[g0 + threadIdx.x + i * NT]) : (T) 0;
+        }
+    }
 #pragma unroll 4
     for (int e = threadIdx.x; e < LPB * M * H; e += NT) {
         const int l = e / (M * H), rem = e - l * (M * H), k = rem / H, c = rem - k * H;
@@ -466,18 +480,28 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cp
     }
     __syncthreads();
     lds_fft_ct<T, true, M, NC, LD, NT>(A, tw);
-    const int64_t g0  = line0 * NZH;
-    double        acc = 0.0;
-    T             beta = (T) 0, alpha = (T) 0;
-    if constexpr (EPI == EPI_CGD) {
-        beta  = (T) ep.scal[0];
-        alpha = (T) ep.scal[1];
-    }
+    double acc  = 0.0;
+    T      beta = (T) 0;
+    if constexpr (EPI == EPI_CGD)
+        beta = (T) ep.scal[0];
+    if constexpr (PRE) {
+        T *__restrict__ dn = ep.dnext;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int     e = threadIdx.x + i * NT;
+            const int     l = e / NZH, rem = e - l * NZH, z = rem / H, c = rem - z * H;
+            const cplx<T> v = A[(z >> 1) * LD + l * H + c];
+            const T       d = beta * dpv[i] - ((z & 1) ? v.y : v.x);
+            dn[g0 + e]      = d;
+            acc += (double) d * (double) qpv[i];
+        }
+    } else {
 #pragma unroll 4
-    for (int e = threadIdx.x; e < LPB * NZH; e += NT) {
-        const int     l = e / NZH, rem = e - l * NZH, z = rem / H, c = rem - z * H;
-        const cplx<T> v = A[(z >> 1) * LD + l * H + c];
-        z_epilogue<T, EPI>(ep, g0 + e, (z & 1) ? v.y : v.x, acc, beta, alpha);
+        for (int e = threadIdx.x; e < LPB * NZH; e += NT) {
+            const int     l = e / NZH, rem = e - l * NZH, z = rem / H, c = rem - z * H;
+            const cplx<T> v = A[(z >> 1) * LD + l * H + c];
+            z_epilogue<T, EPI>(ep, g0 + e, (z & 1) ? v.y : v.x, acc, beta);
+        }
     }
     z_epi_partials<EPI>(ep, acc, red);
 }
