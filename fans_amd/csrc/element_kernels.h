@@ -1262,15 +1262,17 @@ __device__ __forceinline__ void element_forces_iso1_modal(const T (&u)[8][1], co
 }
 
 // lane l <- lane l-1 inside each row of 16 lanes (DPP row_shr:1); row starts get 0
+// (bound_ctrl set: the first lane of each row reads 0 from the DPP source
+// itself, so no zeroed destination has to be prepared)
 __device__ __forceinline__ double dpp_row_shr1(double v)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x111, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x111, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x111, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x111, 0xf, 0xf, true);
     return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ float dpp_row_shr1(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x111, 0xf, 0xf, true));
 }
 
 // ---------------------------------------------------------------------------

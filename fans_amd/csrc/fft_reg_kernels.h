@@ -225,12 +225,12 @@ __device__ __forceinline__ float quad_bcast(float x)
 }
 
 // lane l <- lane l + S inside each row of 16 lanes (S > 0: row_shl, S < 0:
-// row_shr); lanes whose source leaves the row get 0
+// row_shr); lanes whose source leaves the row get 0 (bound_ctrl)
 template <int S>
 __device__ __forceinline__ int row_shift_b32(int x)
 {
     constexpr int ctrl = S > 0 ? 0x100 + S : 0x110 - S;
-    return __builtin_amdgcn_update_dpp(0, x, ctrl, 0xf, 0xf, false);
+    return __builtin_amdgcn_mov_dpp(x, ctrl, 0xf, 0xf, true);
 }
 template <int S>
 __device__ __forceinline__ double row_shift(double x)
