@@ -76,8 +76,8 @@ constexpr int kAsmWaves()
     if (VAR == ASM_LARGE || VAR == ASM_GENERAL_PNL || VAR == ASM_GENERAL_VISCO ||
         (VAR == ASM_GENERAL && MODE == ASM_CGLIN && sizeof(T) == 8))
         return 1;
-    if ((VAR == ASM_ISO || VAR == ASM_ISO_CUBIC) && MODE == ASM_CGLIN && sizeof(T) == 8)
-        return kAsmMatvecWaves;
+    if ((VAR == ASM_ISO || VAR == ASM_ISO_CUBIC) && MODE == ASM_CGLIN)
+        return sizeof(T) == 8 ? kAsmMatvecWaves : 6;
     return 2;
 }
 template <typename T, int H, int NGP, int MODE, int VAR>
@@ -154,18 +154,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAsmWaves<T
     auto kind_of = [&](int ph) { return ph < 256 ? PK[ph] : a.phases[ph].kind; };
 
     // node plane x+1: fetch (global) then commit (LDS); the phase id of the
-    // next element plane is loaded one plane ahead
+    // next element plane is loaded one plane ahead.  Addresses are a
+    // per-thread base (loop invariant) plus a per-plane offset that is the
+    // same for the whole workgroup (scalar arithmetic).
+    const int64_t planeH = plane * H;
+    const T *const in_own = a.in + off_own * H, *const in_ext = a.in + off_ext * H;
     T    pf_own[H], pf_ext[H];
     auto fetch_plane = [&](int xp) {
         const int     xg = xp < 0 ? xp + nx : (xp >= nx ? xp - nx : xp);
-        const int64_t pb = xg * plane;
+        const int64_t po = xg * planeH;
 #pragma unroll
         for (int c = 0; c < H; ++c)
-            pf_own[c] = a.in[(pb + off_own) * H + c];
+            pf_own[c] = in_own[po + c];
         if (ext >= 0)
 #pragma unroll
             for (int c = 0; c < H; ++c)
-                pf_ext[c] = a.in[(pb + off_ext) * H + c];
+                pf_ext[c] = in_ext[po + c];
     };
     auto commit_plane = [&](int buf) {
 #pragma unroll
@@ -176,9 +180,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAsmWaves<T
             for (int c = 0; c < H; ++c)
                 Un[buf][(e_a * NL + e_b) * H + c] = pf_ext[c];
     };
+    const uint8_t *const ms_own = a.ms + (a.ms16 ? 2 : 1) * off_own;
     auto phase_of = [&](int xe) {
         const int xg = xe < 0 ? xe + nx : xe;
-        return phase_id(a.ms, a.ms16, xg * plane + off_own);
+        return phase_id(ms_own, a.ms16, xg * plane);
     };
 
     fetch_plane(X0 - 1);
@@ -302,11 +307,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAsmWaves<T
                 for (int c = 0; c < H; ++c)
                     up[ax][c] = Yb[wave - 1][(ez * 2 + ax) * H + c];
         if (i >= 1 && own_yz) {
-            const int64_t n = xe * plane + off_own;
+            const int64_t po = xe * planeH;
 #pragma unroll
             for (int c = 0; c < H; ++c) {
-                const T res      = carry[c] + P2[0][0][c] + up[0][c];
-                a.out[n * H + c] = res;
+                const T res            = carry[c] + P2[0][0][c] + up[0][c];
+                a.out[off_own * H + po + c] = res;
                 if constexpr (MODE == ASM_CGLIN) {
                     acc0 += (double) dv[c] * (double) res;
                 } else {
@@ -318,7 +323,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAsmWaves<T
                     else
                         acc0 += ar;
                     if (a.dotv)
-                        acc1 += (double) res * (double) a.dotv[n * H + c];
+                        acc1 += (double) res * (double) a.dotv[off_own * H + po + c];
                 }
             }
         }
