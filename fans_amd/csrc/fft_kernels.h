@@ -324,7 +324,11 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const cplx<T> *__restrict__ i
 #ifndef FANS_C2R_PREFETCH
 #define FANS_C2R_PREFETCH 1
 #endif
-constexpr bool kC2rPrefetch = FANS_C2R_PREFETCH != 0;
+#ifndef FANS_C2R_PREFETCH32
+#define FANS_C2R_PREFETCH32 1
+#endif
+constexpr bool kC2rPrefetch   = FANS_C2R_PREFETCH != 0;
+constexpr int  kC2rPrefetch32 = FANS_C2R_PREFETCH32;
 
 // For M = nz/2 in {16, 32, 64, 128} the line count per block, the thread count
 // and every index are compile-time constants (no runtime divisions), and the
@@ -457,20 +461,24 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cp
     const int64_t      line0 = (int64_t) blockIdx.x * LPB;
     const cplx<T>     *src   = in + line0 * MC * H;
     const int64_t      g0    = line0 * NZH;
-    // EPI_CGD, FP64: the per-thread values of d_prev and K d_prev, requested
-    // before the transform so that their loads overlap it (0.354 -> 0.306 ms
-    // at 256^3; in FP32 the registers it takes cost more occupancy than it
-    // gains, 0.208 -> 0.220 ms)
+    // EPI_CGD: the per-thread values of d_prev and (FP64) K d_prev, requested
+    // before the transform so that their loads overlap it (FP64 0.354 ->
+    // 0.306 ms at 256^3; FP32 0.200 -> 0.192 ms with d_prev alone, 0.208 ->
+    // 0.220 ms with both, whose registers cost more occupancy than they gain)
     constexpr int PER = (LPB * NZH) / NT;
-    constexpr bool PRE = EPI == EPI_CGD && kC2rPrefetch && sizeof(T) == 8 && (LPB * NZH) % NT == 0;
-    T              dpv[PRE ? PER : 1], qpv[PRE ? PER : 1];
+    // FP64: both arrays ahead; FP32: kC2rPrefetch32 = 1 d_prev only, 2 both
+    constexpr bool PRE  = EPI == EPI_CGD && kC2rPrefetch && (sizeof(T) == 8 || kC2rPrefetch32 > 0) &&
+                         (LPB * NZH) % NT == 0;
+    constexpr bool PREQ = PRE && (sizeof(T) == 8 || kC2rPrefetch32 > 1);
+    T              dpv[PRE ? PER : 1], qpv[PREQ ? PER : 1];
     if constexpr (PRE) {
         const T *__restrict__ dp = ep.dprev;
         const T *__restrict__ qp = ep.qprev;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             dpv[i] = __builtin_nontemporal_load(&dp[g0 + threadIdx.x + i * NT]);
-            qpv[i] = qp ? __builtin_nontemporal_load(&qp[g0 + threadIdx.x + i * NT]) : (T) 0;
+            if constexpr (PREQ)
+                qpv[i] = qp ? __builtin_nontemporal_load(&qp[g0 + threadIdx.x + i * NT]) : (T) 0;
         }
     }
 #pragma unroll 4
@@ -493,7 +501,10 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cp
             const cplx<T> v = A[(z >> 1) * LD + l * H + c];
             const T       d = beta * dpv[i] - ((z & 1) ? v.y : v.x);
             dn[g0 + e]      = d;
-            acc += (double) d * (double) qpv[i];
+            if constexpr (PREQ)
+                acc += (double) d * (double) qpv[i];
+            else if (ep.qprev)
+                acc += (double) d * (double) __builtin_nontemporal_load(&ep.qprev[g0 + e]);
         }
     } else {
 #pragma unroll 4
