@@ -1483,6 +1483,7 @@ int alloc_fields(fans_gpu_ctx *c)
                                           c->fft32.nblk_z, x_green_blocks(c->fft64, c->h, c->ny),
                                           x_green_blocks(c->fft32, c->h, c->ny)});
     need         = std::max<int64_t>(need, (int64_t) c->nstr * c->nblk_el);
+    need         = std::max<int64_t>(need, 6 * 1024);  // k_moduli_stats: 6 quantities, <= 1024 blocks
     c->partial_len = need;
     HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * need));
     // per-block partials of the linear CG's inverse z pass, kept apart because
@@ -2687,9 +2688,7 @@ int fans_gpu_set_voxel_moduli(fans_gpu_ctx *c, const double *k, const double *g,
     HIPCHK(c, copy_on(c, c->vk64, k, b64, hipMemcpyDefault));
     HIPCHK(c, copy_on(c, c->vg64, c->h == 3 ? g : k, b64, hipMemcpyDefault));
     // input check and reference-stiffness statistics on the device
-    const int nb = (int) std::min<int64_t>(1024, (c->N + kThreads - 1) / kThreads);
-    if (c->partial_len < 6 * (int64_t) nb)
-        return fail(c, FANS_GPU_EINVAL, "grid too small for the moduli statistics");
+    const int nb = (int) std::min<int64_t>(1024, (c->N + kThreads - 1) / kThreads);  // partial_len >= 6 * 1024
     hipLaunchKernelGGL(k_moduli_stats, dim3(nb), dim3(kThreads), 0, c->stream, c->vk64, c->vg64, c->N, c->partial);
     HIPCHK(c, hipGetLastError());
     std::vector<double> ps((size_t) 6 * nb);

@@ -80,3 +80,21 @@ def test_mixed_bc_validation():
                             "strain": [[0, 0, 0]], "stress": [[0, 0, 0, 0]]}, 6)
     with pytest.raises(ValueError, match="either"):
         mixed_bc_from_json({"strain_indices": [0], "stress_indices": [1], "strain": [[0]], "stress": [[0]]}, 6)
+
+
+def test_odd_nz_is_refused(sphere32_ms):
+    """n_z must be even (the r2c half-length transform); the reference only
+    warns, this build refuses before any context exists."""
+    ms = np.ascontiguousarray(sphere32_ms[:, :, :31])
+    with pytest.raises(ValueError, match="n_z must be even"):
+        Problem.from_json(os.path.join(GOLDEN, "test_LinearThermal.json"), ms=ms)
+
+
+@pytest.mark.gpu
+def test_grid_factors_are_refused(sphere32_ms):
+    """Grid edges with a prime factor other than 2, 3 and 5 are refused by
+    fans_gpu_create (FANS_GPU_EUNSUP) with the limit in the message."""
+    ms = np.ascontiguousarray(sphere32_ms[:28, :, :])  # 28 = 4 * 7
+    p = Problem.from_json(os.path.join(GOLDEN, "test_LinearThermal.json"), ms=ms)
+    with pytest.raises(gpu.FansGpuError, match="products of 2, 3 and 5"):
+        gpu.GpuSolver(p)
