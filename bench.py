@@ -203,6 +203,22 @@ def run_batch_workload(args):
     print("[bench] batch: setup + autotune, then the timed solves", file=sys.stderr, flush=True)
     rep = solve_batch(problems, precision="mixed")
     its = [r.iterations for r in rep.results]
+    scaling = None
+    if args.batch_workers:
+        # the same batch with fewer concurrent groups: wall time against the
+        # sum of the library's per-solve times (its share of the wall time
+        # shows what the concurrency leaves idle or contends for)
+        scaling = []
+        for w in [int(v) for v in args.batch_workers.split(",")]:
+            print(f"[bench] batch with {w} concurrent groups", file=sys.stderr, flush=True)
+            rw = solve_batch(problems, precision="mixed", workers=w)
+            by = {}
+            for r in rw.results:
+                by.setdefault(f"{r.dims[0]}^3 h={r.howmany}", []).append(r.solve_ms)
+            sm = sum(r.solve_ms for r in rw.results)
+            scaling.append({"workers": w, "wall_ms": round(1000 * rw.wall_s, 1), "rves_per_s": round(rw.rves_per_s, 1),
+                            "sum_solve_ms": round(sm, 1),
+                            "solve_ms_per_rve": {k: round(float(np.mean(v)), 3) for k, v in sorted(by.items())}})
     out = {"metric": "RVEs/s (384 uneven RVEs, mixed form, tol 1e-8)", "value": round(rep.rves_per_s, 2),
            "unit": "RVEs/s", "n_gpus": 1, "steps": 1, "warmup": 0,
            "ms_per_step": round(1000 * rep.wall_s, 1), "higher_is_better": True, "scaling": "weak",
@@ -214,6 +230,8 @@ def run_batch_workload(args):
            "iterations": {"min": int(min(its)), "mean": round(float(np.mean(its)), 1), "max": int(max(its))},
            "converged": int(sum(r.converged for r in rep.results)),
            "autotuned_asm_planes": {f"{k[0][0]}^3 h={k[1]}": v for k, v in rep.tuned.items()}}
+    if scaling:
+        out["worker_scaling"] = scaling
     print(json.dumps(out), flush=True)
 
 
@@ -475,6 +493,9 @@ def main():
                     help="batch: the 384-RVE uneven-size batch (RVEs/s); j2: the J2 load-step ramp; accuracy: "
                          "per-configuration errors, mixed vs FP64 and FP64 vs oracle; none is the headline metric")
     ap.add_argument("--j2-steps", type=int, default=20, help="load steps of the j2 workload (of 20)")
+    ap.add_argument("--batch-workers", default="",
+                    help="batch workload: also run it with these numbers of concurrent groups (e.g. 1,2,4) and "
+                         "report wall time against the summed per-solve times")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="time the library's own streaming copy (16 B per lane, 2 GiB moved per copy) and print "
                          "its TB/s, the attainable bandwidth of a pure streaming pass")
