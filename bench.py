@@ -230,6 +230,12 @@ def run_batch_workload(args):
            "iterations": {"min": int(min(its)), "mean": round(float(np.mean(its)), 1), "max": int(max(its))},
            "converged": int(sum(r.converged for r in rep.results)),
            "autotuned_asm_planes": {f"{k[0][0]}^3 h={k[1]}": v for k, v in rep.tuned.items()}}
+    out["units"] = sorted(rep.units, key=lambda u: u["group"])
+    if args.batch_single:
+        # the same batch with one single-RVE context per group (RVEs in turn)
+        print("[bench] batch with single-RVE contexts", file=sys.stderr, flush=True)
+        rs = solve_batch(problems, precision="mixed", batched=False)
+        out["single_rve_contexts"] = {"rves_per_s": round(rs.rves_per_s, 2), "wall_ms": round(1000 * rs.wall_s, 1)}
     if scaling:
         out["worker_scaling"] = scaling
     print(json.dumps(out), flush=True)
@@ -493,6 +499,8 @@ def main():
                     help="batch: the 384-RVE uneven-size batch (RVEs/s); j2: the J2 load-step ramp; accuracy: "
                          "per-configuration errors, mixed vs FP64 and FP64 vs oracle; none is the headline metric")
     ap.add_argument("--j2-steps", type=int, default=20, help="load steps of the j2 workload (of 20)")
+    ap.add_argument("--batch-single", action="store_true",
+                    help="batch workload: also run it with one single-RVE context per group (RVEs in turn)")
     ap.add_argument("--batch-workers", default="",
                     help="batch workload: also run it with these numbers of concurrent groups (e.g. 1,2,4) and "
                          "report wall time against the summed per-solve times")

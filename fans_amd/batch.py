@@ -1,13 +1,17 @@
 """Batched solves of many independent RVEs of uneven sizes.
 
 Each RVE is one reference run (main.cpp:10-41 for one load case): its own
-microstructure, phases and Green operator.  RVEs are grouped by (grid,
-physics); every group keeps one GPU context (buffers, FFT twiddles, captured
-CG iterations) and loads its RVEs in turn, so per-RVE setup is the material
-tables and the Green operator only.  Groups run concurrently, one host thread
-and one HIP stream each (the C ABI calls release the interpreter lock).  Before
-the timed batch, a small autotuner times the stencil tile depth (x planes per
-assembly workgroup) on each group's first RVE and keeps the fastest.
+microstructure, phases and Green operator.  RVEs are grouped by their context
+settings (grid, physics, element, solver settings).  Each group, in chunks of
+at most `max_slots` RVEs, is one batched GPU context (fans_gpu_create_batch):
+every RVE has its own buffers in it, and one solve runs all of them at once,
+every kernel launch covering each RVE still iterating (the captured CG
+iterations replay for the whole group; finished RVEs leave the launch grid).
+Groups run concurrently, one host thread and one HIP stream each (the C ABI
+calls release the interpreter lock).  Before the timed batch, a small
+autotuner times the stencil tile depth (x planes per assembly workgroup) on
+each group's first RVE and keeps the fastest.  With batched=False each group
+is one single-RVE context that loads and solves its RVEs in turn.
 
 The synthetic batch of the differentiation-free throughput benchmark (384 RVEs,
 edges {32, 48, 64, 80, 96}, half thermal, half elastic, 2-3 phases with random
@@ -22,7 +26,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .gpu import GpuSolver, context_key
+from .gpu import GpuBatch, GpuSolver, context_key
 from .microstructure import random_spheres
 from .problem import LoadCase, MaterialGroup, Problem, isotropic_KG
 
@@ -45,6 +49,9 @@ class BatchReport:
     results: list = field(default_factory=list)
     wall_s: float = 0.0
     tuned: dict = field(default_factory=dict)
+    # per context: host wall ms of loading its RVEs, of the solve (and the
+    # library's device time of it) and of reading the results
+    units: list = field(default_factory=list)
 
     @property
     def rves_per_s(self):
@@ -59,8 +66,8 @@ class BatchReport:
         return self.voxel_iters / self.wall_s if self.wall_s > 0 else 0.0
 
 
-def config5_rves(count=384, seed=0, tol=1e-8):
-    """Deterministic synthetic batch: edge uniform from EDGES, physics
+def config5_rves(count=384, seed=0, tol=1e-8, edges=EDGES):
+    """Deterministic synthetic batch: edge uniform from `edges`, physics
     alternating (even thermal, odd elastic), 2-3 phases, moduli E in [1, 100],
     nu in [0.2, 0.4], conductivity in [1, 50], sphere radii U[n/16, n/8] with
     the count set for a target volume fraction U[0.1, 0.4] (non-overlapping
@@ -68,7 +75,7 @@ def config5_rves(count=384, seed=0, tol=1e-8):
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
-        n = int(rng.choice(EDGES))
+        n = int(rng.choice(edges))
         thermal = i % 2 == 0
         nph = int(rng.integers(2, 4))
         vf = float(rng.uniform(0.1, 0.4))
@@ -113,33 +120,63 @@ def _autotune(solver: GpuSolver, g0, candidates=(4, 8, 16), repeats=1):
     return best
 
 
-def solve_batch(problems, precision="mixed", workers=None, autotune=True):
+def solve_batch(problems, precision="mixed", workers=None, autotune=True, batched=True, max_slots=64):
     """Solve every problem's first load-case step; returns a BatchReport.
-    workers: concurrent groups (default: all groups, at most 16 -- on the
-    384-RVE batch one stream per group gave 1390 RVEs/s against 1190 with 4)."""
-    # one context per distinct creation setting (grid, physics, L, element type,
-    # solver and error settings): GpuSolver.load only swaps microstructure and
-    # material tables
+    workers: concurrent groups (default: all groups, at most 16).  batched:
+    one batched context per group chunk of at most max_slots RVEs (else one
+    single-RVE context per group, RVEs in turn)."""
     groups = {}
     for i, p in enumerate(problems):
         groups.setdefault(context_key(p), []).append(i)
+    units = []
+    for key, idx in groups.items():
+        step = max_slots if batched else len(idx)
+        units += [(key, tuple(idx[k:k + step])) for k in range(0, len(idx), step)]
     report = BatchReport()
 
     # setup, outside the timed region like the reference's plan creation:
-    # one context per group, autotuned on the group's first RVE
+    # the contexts, and the tile depth autotuned on each group's first RVE
     solvers = {}
 
-    def setup(key):
-        first = problems[groups[key][0]]
-        s = GpuSolver(first, precision=precision)
+    def setup(unit):
+        key, idx = unit
+        first = problems[idx[0]]
+        if autotune and key not in report.tuned:
+            t = GpuSolver(first, precision=precision)
+            try:
+                report.tuned[key] = _autotune(t, first.load_cases[0].g0_path[0])
+            finally:
+                t.close()
+        s = GpuBatch([problems[i] for i in idx], precision=precision) if batched else \
+            GpuSolver(first, precision=precision)
         if autotune:
-            report.tuned[key] = _autotune(s, first.load_cases[0].g0_path[0])
-        solvers[key] = s
+            s.set_asm_planes(report.tuned[key])
+        solvers[unit] = s
 
-    def run_group(key):
-        s = solvers[key]
+    def run_unit(unit):
+        s, (_, idx) = solvers[unit], unit
         res = []
-        for i in groups[key]:
+        if batched:
+            t0 = time.perf_counter()
+            for b, i in enumerate(idx):
+                s.load_slot(b, problems[i], problems[i].load_cases[0].g0_path[0])
+            t1 = time.perf_counter()
+            s.solve()
+            t2 = time.perf_counter()
+            its, conv, _ = s.results()
+            stress = s.stresses()
+            t3 = time.perf_counter()
+            p0 = problems[idx[0]]
+            report.units.append({"group": f"{p0.dims[0]}^3 h={p0.howmany}", "rves": len(idx),
+                                 "load_ms": round(1e3 * (t1 - t0), 2), "solve_ms": round(1e3 * (t2 - t1), 2),
+                                 "device_solve_ms": round(float(s.stats.solve_ms), 2),
+                                 "results_ms": round(1e3 * (t3 - t2), 2), "max_iterations": int(max(its))})
+            for b, i in enumerate(idx):
+                p = problems[i]
+                res.append(RVEResult(i, tuple(p.dims), p.howmany, int(its[b]), stress[b].copy(),
+                                     float(s.stats.solve_ms), bool(conv[b])))
+            return res
+        for i in idx:
             p = problems[i]
             s.load(p)
             s.set_gradient(p.load_cases[0].g0_path[0])
@@ -148,15 +185,15 @@ def solve_batch(problems, precision="mixed", workers=None, autotune=True):
                                  s.homogenized_stress().copy(), float(s.stats.solve_ms), bool(s.stats.converged)))
         return res
 
-    # largest groups first so the tail is short
-    order = sorted(groups, key=lambda k: -np.prod(k[0]) * len(groups[k]))
+    # largest units first so the tail is short
+    order = sorted(units, key=lambda u: -np.prod(u[0][0]) * len(u[1]))
     try:
-        for key in order:
-            setup(key)
+        for unit in order:
+            setup(unit)
         t0 = time.perf_counter()
         nw = workers if workers else min(len(order), 16)
         with ThreadPoolExecutor(max_workers=max(1, nw)) as ex:
-            for res in ex.map(run_group, order):
+            for res in ex.map(run_unit, order):
                 report.results.extend(res)
         report.wall_s = time.perf_counter() - t0
     finally:

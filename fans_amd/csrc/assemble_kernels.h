@@ -44,7 +44,7 @@ struct AsmArgs {
     const T        *g0;        // macroscopic gradient (n_str), device
     double         *partial;   // [2][gridDim] : slot 0 = norm/dot, slot 1 = dot
     int             measure;   // 0 L1, 1 L2, 2 Linf  (ASM_RESID)
-    const int      *done;
+    Slots           sl;        // RVE slots (done flags: finished slots return at entry)
     const T        *vk;        // per-voxel isotropic moduli (nullable, ISO only):
     const T        *vg;        // bulk K and shear G, or conductivity k in vk
 };
@@ -84,8 +84,20 @@ template <typename T, int H, int NGP, int MODE, int VAR>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAsmWaves<T, MODE, VAR>()))) void k_assemble(AsmArgs<T> a)
 {
     constexpr bool ISO = VAR == ASM_ISO || VAR == ASM_ISO_CUBIC, CUBIC = VAR == ASM_ISO_CUBIC;
-    if (a.done && *a.done)
+    const int sb = slot_of(a.sl);
+    if (sb < 0)
         return;
+    if (a.sl.map) {  // batched launch: this slot's buffers
+        a.in      = a.in + sb * a.sl.field;
+        a.out     = a.out + sb * a.sl.field;
+        a.dotv    = slot_ptr(a.dotv, sb * a.sl.field);
+        a.g0      = slot_ptr(a.g0, (int64_t) sb * 9);
+        a.ms      = a.ms + sb * a.sl.ms;
+        a.phases  = a.phases + sb * a.sl.ph;
+        a.partial = slot_ptr(a.partial, sb * a.sl.part);
+        a.vk      = slot_ptr(a.vk, sb * (a.sl.field / H));
+        a.vg      = slot_ptr(a.vg, sb * (a.sl.field / H));
+    }
     constexpr int NL = TILE + 1;  // node tile edge
     constexpr int NS = nstr_of<H, VAR == ASM_LARGE>();
     __shared__ T      Un[2][NL * NL * H];

@@ -118,6 +118,46 @@ __device__ __forceinline__ void acc_norm(double &acc, T v, int measure)
         acc += a;
 }
 
+// device integer scalars of one RVE slot (the double scalars are laid out in
+// vec_kernels.h, kScSlot per slot)
+enum { IS_DONE = 0, IS_ITER = 1, IS_NIT = 2, IS_ERRTYPE = 3, IS_MEASURE = 4, IS_COUNT = 8 };
+constexpr int kScSlot = 40;
+
+// RVE slots of a launch.  A context holds one RVE or a batch of same-shape
+// RVEs whose per-RVE buffers lie at a fixed stride from slot 0's.  A batched
+// launch enumerates its slots in grid z through `map` (slot = map[blockIdx.z],
+// entries < 0: none) and the kernels move every buffer to the slot's offset;
+// without a map the launch has one z block and its buffers are the RVE's own
+// (slot 0, no offsets).  With `is`, a slot whose done flag is set returns at
+// entry.
+struct Slots {
+    const int *map   = nullptr;
+    const int *is    = nullptr;  // IS_COUNT ints per slot
+    int64_t    field = 0;        // elements of a field array (u, r, d, K d ...)
+    int64_t    spec  = 0;        // complex entries of a spectrum
+    int64_t    green = 0;        // entries of a Green operator
+    int64_t    part  = 0;        // doubles of a partial-sum buffer
+    int64_t    ms    = 0;        // bytes of a phase-id array
+    int64_t    ph    = 0;        // entries of a phase table
+    int64_t    hist  = 0;        // doubles of an error history
+};
+
+// slot of this block, or -1 if it has none or (check_done) is finished
+__device__ __forceinline__ int slot_of(const Slots &s, bool check_done = true)
+{
+    const int b = s.map ? s.map[blockIdx.z] : (int) blockIdx.z;
+    if (b < 0 || (check_done && s.is && s.is[b * IS_COUNT + IS_DONE]))
+        return -1;
+    return b;
+}
+
+// p + off unless p is null (optional buffers stay null)
+template <typename P>
+__device__ __forceinline__ P *slot_ptr(P *p, int64_t off)
+{
+    return p ? p + off : p;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2).  Map the hardware block id so that each XCD gets a contiguous range
 // of logical blocks: neighbouring blocks that share cache lines (halo nodes,

@@ -1308,6 +1308,7 @@ struct ElemArgs {
     const T        *ub;            // SENS: second displacement field and gradient
     const T        *g0b;
     double         *sens;          // SENS: [N][H == 1 ? 1 : 2]
+    Slots           sl;            // RVE slots (u, ms, phases, g0, partial at the slot's offset)
 };
 
 // isotropic stress with per-voxel moduli: K, G (mechanics, Mandel) or k (thermal)
@@ -1336,6 +1337,18 @@ template <typename T, int H, int NGP, int MODE, int NS = (H == 1 ? 3 : 6)>
 __global__ __launch_bounds__(256) void k_element_pass(ElemArgs<T> a)
 {
     __shared__ double red[NS * 4];
+    {
+        const int sb = slot_of(a.sl, false);
+        if (sb < 0)
+            return;
+        if (a.sl.map) {  // batched launch: this slot's buffers
+            a.u       = slot_ptr(a.u, sb * a.sl.field);
+            a.g0      = slot_ptr(a.g0, (int64_t) sb * 9);
+            a.ms      = a.ms + sb * a.sl.ms;
+            a.phases  = a.phases + sb * a.sl.ph;
+            a.partial = slot_ptr(a.partial, sb * a.sl.part);
+        }
+    }
     const int64_t     tid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     double            sacc[NS];
 #pragma unroll

@@ -213,6 +213,48 @@ struct Fft3D {
     }
 };
 
+// RVE slots of one batched context at most
+constexpr int kMaxSlots = 4096;
+
+// per-RVE device buffers of a context and the stride kind of each (Slots);
+// g0 holds 9 entries per slot, the device scalars SC_COUNT / IS_COUNT
+#define FANS_SLOT_BUFFERS(X)                                                                                   \
+    X(double, u, field) X(double, u_prev, field) X(double, r, field) X(double, s, field) X(double, dA, field)   \
+    X(double, dB, field) X(double, rnew, field) X(cplx<double>, spec64, spec) X(double, G64, green)            \
+    X(float, G32, green) X(uint8_t, ms, ms) X(Phase<double>, dph64, ph) X(Phase<float>, dph32, ph)            \
+    X(double, g0d, g0) X(float, g0f, g0) X(double, partial, part) X(double, partial_e, part) X(double, sc, sc) \
+    X(int, is, is) X(double, hist, hist) X(float, r32, field) X(float, s32, field) X(float, dA32, field)       \
+    X(float, dB32, field) X(float, rnew32, field) X(float, x32, field) X(cplx<float>, spec32, spec)
+enum SlotKind { SK_field, SK_spec, SK_green, SK_ms, SK_ph, SK_g0, SK_part, SK_sc, SK_is, SK_hist };
+
+// slot 0's address of every per-RVE buffer of a batched context
+struct SlotBase {
+#define X(T, name, kind) T *name = nullptr;
+    FANS_SLOT_BUFFERS(X)
+#undef X
+    double *ring64[kDirRing]{};
+    float  *ring32[kDirRing]{};
+};
+
+// host state of one RVE of a batched context while another slot is selected
+#define FANS_SLOT_HOST(X)                                                                                      \
+    X(have_ms) X(have_mat) X(ms16) X(ms_max) X(nphases) X(phase_kind) X(all_linear) X(has_j2) X(all_iso)        \
+    X(has_pnl) X(has_visco) X(user_ref) X(voxel_moduli) X(ph64) X(ph32) X(gxq) X(gyq) X(iter) X(inner_iters)    \
+    X(outer_iters) X(err_hist) X(solve_ms) X(kref) X(keref) X(g0) X(gvec) X(gvec_prev)
+struct SlotHost {
+    bool                       have_ms = false, have_mat = false, ms16 = false;
+    int                        ms_max = -1, nphases = 0;
+    std::vector<int>           phase_kind;
+    bool                       all_linear = true, has_j2 = false, all_iso = false, has_pnl = false;
+    bool                       has_visco = false, user_ref = false, voxel_moduli = false;
+    std::vector<Phase<double>> ph64;
+    std::vector<Phase<float>>  ph32;
+    int                        gxq = 0, gyq = 0, iter = 0, inner_iters = 0, outer_iters = 0;
+    std::vector<double>        err_hist;
+    double                     solve_ms = 0;
+    double                     kref[81]{}, keref[576]{}, g0[9]{}, gvec[9]{}, gvec_prev[9]{};
+};
+
 }  // namespace
 
 struct fans_gpu_ctx {
@@ -275,6 +317,22 @@ struct fans_gpu_ctx {
     Fft3D<double> fft64;
     Fft3D<float>  fft32;
 
+    // RVE slots (fans_gpu_create_batch): nb same-shape RVEs whose per-RVE
+    // device buffers lie nb times at the strides of `strides`.  The buffer
+    // fields above address the selected slot `cur`, `base` keeps slot 0's
+    // addresses and `parked` the host state of the slots not selected.  A
+    // batched solve points the fields at slot 0 and launches gz slots per
+    // kernel (grid z) through the device map `slot_map`.
+    int                   nb = 1, cur = 0, gz = 1;
+    bool                  use_map  = false;
+    int                  *slot_map = nullptr;
+    int                   map_cap  = 0;
+    std::vector<int>      map_h, map_slots;  // uploaded map and the slots it holds
+    Slots                 strides{};
+    SlotBase              base{};
+    std::vector<SlotHost> parked;
+    std::array<int, 7>    batch_sig{};
+
     // launch geometry
     int     cx = 16, nty = 0, ntz = 0, nblk_asm = 0, nblk_vec = 0, nblk_el = 0;
     int64_t nslot = 0;      // J2 state slots per component and Gauss point
@@ -304,7 +362,8 @@ struct fans_gpu_ctx {
     // captured CG iterations (hipGraph), keyed by precision and the direction
     // buffer current at the first captured iteration
     bool                                                use_graphs = true;
-    std::vector<std::pair<const void *, hipGraphExec_t>> graphs;
+    // key: the direction buffer and the slots per launch (gz)
+    std::vector<std::pair<std::pair<const void *, int>, hipGraphExec_t>> graphs;
     std::array<int, 7>                                  graph_sig{};
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
     std::vector<hipEvent_t>                             ev_pool;
@@ -457,6 +516,90 @@ void drop_timing(fans_gpu_ctx *c, size_t from)
 int grid_vec(int64_t n) { return (int) std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, 2048)); }
 
 // ---------------------------------------------------------------------------
+// RVE slots
+// ---------------------------------------------------------------------------
+int64_t slot_stride(const fans_gpu_ctx *c, SlotKind k)
+{
+    switch (k) {
+    case SK_field: return c->strides.field;
+    case SK_spec: return c->strides.spec;
+    case SK_green: return c->strides.green;
+    case SK_ms: return c->strides.ms;
+    case SK_ph: return c->strides.ph;
+    case SK_g0: return 9;
+    case SK_part: return c->strides.part;
+    case SK_sc: return SC_COUNT;
+    case SK_is: return IS_COUNT;
+    case SK_hist: return c->strides.hist;
+    }
+    return 0;
+}
+
+// point the buffer fields at slot b (batched contexts)
+void set_views(fans_gpu_ctx *c, int b)
+{
+#define X(T, name, kind) c->name = c->base.name ? c->base.name + b * slot_stride(c, SK_##kind) : nullptr;
+    FANS_SLOT_BUFFERS(X)
+#undef X
+    for (int m = 2; m < kDirRing; ++m) {
+        c->ring64[m] = c->base.ring64[m] ? c->base.ring64[m] + b * c->strides.field : nullptr;
+        c->ring32[m] = c->base.ring32[m] ? c->base.ring32[m] + b * c->strides.field : nullptr;
+    }
+    c->cur = b;
+}
+
+void swap_host(fans_gpu_ctx *c, SlotHost &h)
+{
+#define X(f) std::swap(c->f, h.f);
+    FANS_SLOT_HOST(X)
+#undef X
+}
+
+// make slot b the one that the single-RVE entry points address
+void select_slot(fans_gpu_ctx *c, int b)
+{
+    if (b == c->cur)
+        return;
+    swap_host(c, c->parked[c->cur]);
+    swap_host(c, c->parked[b]);
+    set_views(c, b);
+}
+
+// slots of a launch: the selected slot alone (its buffers are the fields), or
+// during a batched solve the slots of the device map; `gated`: finished slots
+// return at entry
+Slots launch_slots(const fans_gpu_ctx *c, bool gated)
+{
+    Slots s = c->strides;
+    s.map   = c->use_map ? c->slot_map : nullptr;
+    s.is    = gated ? c->is : nullptr;
+    return s;
+}
+
+// launch the given slots from now on: grid z is the next power of two (graphs
+// are captured per grid), the map entries past the slots are -1
+int set_map(fans_gpu_ctx *c, const std::vector<int> &slots)
+{
+    int gz = 1;
+    while (gz < (int) slots.size())
+        gz *= 2;
+    c->map_h.assign(gz, -1);
+    std::copy(slots.begin(), slots.end(), c->map_h.begin());
+    c->map_slots = slots;
+    HIPCHK(c, copy_on(c, c->slot_map, c->map_h.data(), sizeof(int) * gz, hipMemcpyHostToDevice));
+    c->gz      = gz;
+    c->use_map = true;
+    return 0;
+}
+
+void clear_map(fans_gpu_ctx *c)
+{
+    c->gz      = 1;
+    c->use_map = false;
+    c->map_slots.clear();
+}
+
+// ---------------------------------------------------------------------------
 // power-of-two lengths 16..256 use the register four-step kernels
 // ---------------------------------------------------------------------------
 bool reg_len(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256; }
@@ -489,8 +632,8 @@ void launch_strided_reg(fans_gpu_ctx *c, cplx<T> *spec, const cplx<T> *tw, int64
 {
     constexpr int C   = reg_cols<T, N1, N2, strided_max_threads<T>()>();
     const int     nbx = (c1 - c0 + C - 1) / C;
-    hipLaunchKernelGGL((k_fft_strided_reg<T, N1, N2, C, INV>), dim3(nbx * nouter), dim3(C * N2), 0, c->stream, spec,
-                       tw, sn, c0, c1, ostride, nbx, done);
+    hipLaunchKernelGGL((k_fft_strided_reg<T, N1, N2, C, INV>), dim3(nbx * nouter, 1, c->gz), dim3(C * N2), 0,
+                       c->stream, spec, tw, sn, c0, c1, ostride, nbx, launch_slots(c, done != nullptr));
 }
 
 template <typename T, bool INV>
@@ -504,8 +647,8 @@ void launch_strided(fans_gpu_ctx *c, const FftLen &L, const cplx<T> *tw, int bc,
     case 32: launch_strided_reg<T, INV, 8, 4>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;
     case 16: launch_strided_reg<T, INV, 4, 4>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;
     default:
-        hipLaunchKernelGGL((k_fft_strided<T, INV>), dim3((c1 - c0 + bc - 1) / bc, nouter), dim3(kThreads), smem,
-                           c->stream, spec, L, tw, sn, c0, c1, ostride, bc, done);
+        hipLaunchKernelGGL((k_fft_strided<T, INV>), dim3((c1 - c0 + bc - 1) / bc, nouter, c->gz), dim3(kThreads),
+                           smem, c->stream, spec, L, tw, sn, c0, c1, ostride, bc, launch_slots(c, done != nullptr));
     }
 }
 
@@ -526,8 +669,8 @@ int launch_x_green_reg(fans_gpu_ctx *c, cplx<T> *spec, const cplx<T> *tw, const 
     // the complex transpose tile of the 256-point FP64 pass (70 KB) allows two
     // workgroups per CU; the split tile (35 KB) four
     constexpr bool SPLIT = kXGreenSplit && (int64_t) N1 * N2 * (C + 1) * sizeof(cplx<T>) > 48 * 1024;
-    hipLaunchKernelGGL((k_fft_x_green_reg<T, H, N1, N2, KB, C, SPLIT>), dim3(nbk * c->ny), dim3(C * N2), 0,
-                       c->stream, spec, tw, gv, kz_lo, kz_hi, nbk, partial, done);
+    hipLaunchKernelGGL((k_fft_x_green_reg<T, H, N1, N2, KB, C, SPLIT>), dim3(nbk * c->ny, 1, c->gz), dim3(C * N2),
+                       0, c->stream, spec, tw, gv, kz_lo, kz_hi, nbk, partial, launch_slots(c, done != nullptr));
     return nbk * c->ny;
 }
 
@@ -543,9 +686,9 @@ int launch_x_green(fans_gpu_ctx *c, Fft3D<T> &P, cplx<T> *spec, const T *G, int 
     case 32: return launch_x_green_reg<T, H, 8, 4>(c, spec, P.twx, gv, kz_lo, kz_hi, partial, done);
     case 16: return launch_x_green_reg<T, H, 4, 4>(c, spec, P.twx, gv, kz_lo, kz_hi, partial, done);
     default: {
-        dim3 g((kz_hi - kz_lo + P.bk - 1) / P.bk, c->ny);
+        dim3 g((kz_hi - kz_lo + P.bk - 1) / P.bk, c->ny, c->gz);
         hipLaunchKernelGGL((k_fft_x_green<T, H>), g, dim3(kThreads), P.smx, c->stream, spec, P.Lx, P.twx, gv, kz_lo,
-                           kz_hi, P.bk, partial, done);
+                           kz_hi, P.bk, partial, launch_slots(c, done != nullptr));
         return (int) (g.x * g.y);
     }
     }
@@ -578,17 +721,18 @@ template <typename T, int H, int PRO>
 void launch_z_r2c(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const ProArgs<T> &pa, const int *done)
 {
     const int64_t nlines = (int64_t) c->nx * c->ny;
+    const Slots   sl     = launch_slots(c, done != nullptr);
 #define ZF(MM)                                                                                                  \
-    hipLaunchKernelGGL((k_fft_z_r2c_ct<T, H, MM, PRO>), dim3((unsigned) (nlines / zct_lpb<H, MM>())),           \
-                       dim3(zct_threads<H, MM>()), 0, c->stream, rin, spec, P.twz, pa, done)
+    hipLaunchKernelGGL((k_fft_z_r2c_ct<T, H, MM, PRO>), dim3((unsigned) (nlines / zct_lpb<H, MM>()), 1, c->gz),  \
+                       dim3(zct_threads<H, MM>()), 0, c->stream, rin, spec, P.twz, pa, sl)
     switch (P.zct ? P.M : 0) {
     case 128: ZF(128); break;
     case 64: ZF(64); break;
     case 32: ZF(32); break;
     case 16: ZF(16); break;
     default:
-        hipLaunchKernelGGL((k_fft_z_r2c<T, H, PRO>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, rin, spec,
-                           P.Lz, P.twz, c->nz, nlines, P.lpb, pa, done);
+        hipLaunchKernelGGL((k_fft_z_r2c<T, H, PRO>), dim3(P.nblk_z, 1, c->gz), dim3(kThreads), P.smz, c->stream, rin,
+                           spec, P.Lz, P.twz, c->nz, nlines, P.lpb, pa, sl);
     }
 #undef ZF
 }
@@ -597,17 +741,18 @@ template <typename T, int H, int EPI>
 void launch_z_c2r(fans_gpu_ctx *c, Fft3D<T> &P, const cplx<T> *spec, const EpiArgs<T> &ep, const int *done)
 {
     const int64_t nlines = (int64_t) c->nx * c->ny;
+    const Slots   sl     = launch_slots(c, done != nullptr);
 #define ZI(MM)                                                                                                  \
-    hipLaunchKernelGGL((k_fft_z_c2r_ct<T, H, MM, EPI>), dim3((unsigned) (nlines / zct_lpb<H, MM>())),           \
-                       dim3(zct_threads<H, MM>()), 0, c->stream, spec, P.twz, ep, done)
+    hipLaunchKernelGGL((k_fft_z_c2r_ct<T, H, MM, EPI>), dim3((unsigned) (nlines / zct_lpb<H, MM>()), 1, c->gz),  \
+                       dim3(zct_threads<H, MM>()), 0, c->stream, spec, P.twz, ep, sl)
     switch (P.zct ? P.M : 0) {
     case 128: ZI(128); break;
     case 64: ZI(64); break;
     case 32: ZI(32); break;
     case 16: ZI(16); break;
     default:
-        hipLaunchKernelGGL((k_fft_z_c2r<T, H, EPI>), dim3(P.nblk_z), dim3(kThreads), P.smz, c->stream, spec, P.Lz,
-                           P.twz, c->nz, nlines, P.lpb, ep, done);
+        hipLaunchKernelGGL((k_fft_z_c2r<T, H, EPI>), dim3(P.nblk_z, 1, c->gz), dim3(kThreads), P.smz, c->stream, spec,
+                           P.Lz, P.twz, c->nz, nlines, P.lpb, ep, sl);
     }
 #undef ZI
 }
@@ -723,7 +868,7 @@ AsmArgs<T> asm_args(fans_gpu_ctx *c)
     a.js      = J2State{c->js, c->nslot};
     a.partial = c->partial;
     a.measure = c->d.measure;
-    a.done    = nullptr;
+    a.sl      = launch_slots(c, false);
     if (c->voxel_moduli) {
         if constexpr (sizeof(T) == sizeof(double)) {
             a.vk = reinterpret_cast<const T *>(c->vk64);
@@ -743,30 +888,30 @@ int launch_assemble(fans_gpu_ctx *c, AsmArgs<T> a)
 #define ASM(HH, GG)                                                                                             \
     do {                                                                                                        \
         if (c->all_iso && c->cubic)                                                                             \
-            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_ISO_CUBIC>), dim3(c->nblk_asm), dim3(kThreads), \
+            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_ISO_CUBIC>), dim3(c->nblk_asm, 1, c->gz), dim3(kThreads), \
                                0, c->stream, a);                                                                \
         else if (c->all_iso)                                                                                    \
-            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_ISO>), dim3(c->nblk_asm), dim3(kThreads), 0,    \
+            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_ISO>), dim3(c->nblk_asm, 1, c->gz), dim3(kThreads), 0,    \
                                c->stream, a);                                                                   \
         else if (MODE == ASM_RESID && HH == 3 && c->has_visco)                                                  \
             hipLaunchKernelGGL(                                                                                 \
                 (k_assemble<T, HH, GG, MODE, MODE == ASM_RESID && HH == 3 ? ASM_GENERAL_VISCO : ASM_GENERAL>),   \
-                dim3(c->nblk_asm), dim3(kThreads), 0, c->stream, a);                                            \
+                dim3(c->nblk_asm, 1, c->gz), dim3(kThreads), 0, c->stream, a);                                            \
         else if (MODE == ASM_RESID && c->has_pnl)                                                               \
             hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, MODE == ASM_RESID ? ASM_GENERAL_PNL : ASM_GENERAL>),   \
-                               dim3(c->nblk_asm), dim3(kThreads), 0, c->stream, a);                             \
+                               dim3(c->nblk_asm, 1, c->gz), dim3(kThreads), 0, c->stream, a);                             \
         else                                                                                                    \
-            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_GENERAL>), dim3(c->nblk_asm), dim3(kThreads),   \
+            hipLaunchKernelGGL((k_assemble<T, HH, GG, MODE, ASM_GENERAL>), dim3(c->nblk_asm, 1, c->gz), dim3(kThreads),   \
                                0, c->stream, a);                                                                \
     } while (0)
     if (c->nstr == 9) {
         // finite strain: only the residual (the models are nonlinear)
         if constexpr (MODE == ASM_RESID) {
             if (c->ngp == 8)
-                hipLaunchKernelGGL((k_assemble<T, 3, 8, MODE, ASM_LARGE>), dim3(c->nblk_asm), dim3(kThreads), 0,
+                hipLaunchKernelGGL((k_assemble<T, 3, 8, MODE, ASM_LARGE>), dim3(c->nblk_asm, 1, c->gz), dim3(kThreads), 0,
                                    c->stream, a);
             else
-                hipLaunchKernelGGL((k_assemble<T, 3, 1, MODE, ASM_LARGE>), dim3(c->nblk_asm), dim3(kThreads), 0,
+                hipLaunchKernelGGL((k_assemble<T, 3, 1, MODE, ASM_LARGE>), dim3(c->nblk_asm, 1, c->gz), dim3(kThreads), 0,
                                    c->stream, a);
         } else {
             return fail(c, FANS_GPU_EINVAL, "linear matvec needs linear materials");
@@ -803,8 +948,8 @@ int reduce(fans_gpu_ctx *c, const double *partial, int nblk, int nq, int stage, 
            int nblk_b)
 {
     Timed t(c, KF_REDUCE);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kReduceThreads), 0, c->stream, partial, nblk, nq, stage, c->sc, c->is,
-                       c->hist, partial_b, nblk_b);
+    hipLaunchKernelGGL(k_reduce, dim3(1, 1, c->gz), dim3(kReduceThreads), 0, c->stream, partial, nblk, nq, stage, c->sc,
+                       c->is, c->hist, partial_b, nblk_b, launch_slots(c, false));
     HIPCHK(c, hipGetLastError());
     return 0;
 }
@@ -869,18 +1014,19 @@ int homog_stress(fans_gpu_ctx *c, double *out)
     a.vk      = c->voxel_moduli ? c->vk64 : nullptr;
     a.vg      = c->voxel_moduli ? c->vg64 : nullptr;
     a.partial = c->partial;
+    a.sl      = launch_slots(c, false);
     {
         Timed t(c, KF_ELEMENT);
 #define EP(HH, GG)                                                                                              \
-    hipLaunchKernelGGL((k_element_pass<double, HH, GG, ELEM_HOMOG>), dim3(c->nblk_el), dim3(kThreads), 0, \
+    hipLaunchKernelGGL((k_element_pass<double, HH, GG, ELEM_HOMOG>), dim3(c->nblk_el, 1, c->gz), dim3(kThreads), 0, \
                        c->stream, a)
         if (c->nstr == 9) {
             if (c->ngp == 8)
-                hipLaunchKernelGGL((k_element_pass<double, 3, 8, ELEM_HOMOG, 9>), dim3(c->nblk_el), dim3(kThreads),
-                                   0, c->stream, a);
+                hipLaunchKernelGGL((k_element_pass<double, 3, 8, ELEM_HOMOG, 9>), dim3(c->nblk_el, 1, c->gz),
+                                   dim3(kThreads), 0, c->stream, a);
             else
-                hipLaunchKernelGGL((k_element_pass<double, 3, 1, ELEM_HOMOG, 9>), dim3(c->nblk_el), dim3(kThreads),
-                                   0, c->stream, a);
+                hipLaunchKernelGGL((k_element_pass<double, 3, 1, ELEM_HOMOG, 9>), dim3(c->nblk_el, 1, c->gz),
+                                   dim3(kThreads), 0, c->stream, a);
         } else if (c->h == 1) {
             if (c->ngp == 8)
                 EP(1, 8);
@@ -896,7 +1042,7 @@ int homog_stress(fans_gpu_ctx *c, double *out)
     }
     HIPCHK(c, hipGetLastError());
     int rc = reduce(c, c->partial, c->nblk_el, c->nstr, RED_SUM);
-    if (rc)
+    if (rc || !out)
         return rc;
     double sc[SC_COUNT];
     if ((rc = read_scalars(c, sc, nullptr)))
@@ -1073,33 +1219,80 @@ void host_sym_pinv(int n, const double *A, double *P)
 // iteration.  After RED_ERR_NEXT sets `done` every later kernel returns at
 // entry; r then holds the final residual and finish() applies the directions
 // left in the ring.  ring[kDirRing - 1] must hold zeros on entry (d_(-1) = 0).
+// integer scalars of every slot of a batched context (fields at slot 0)
+int read_is_all(fans_gpu_ctx *c, std::vector<int> &is)
+{
+    is.resize((size_t) c->nb * IS_COUNT);
+    HIPCHK(c, hipMemcpyAsync(is.data(), c->is, sizeof(int) * is.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Linear CG iterations (solverCG.h:84-117) until the stop test holds: of the
+// selected RVE, or batched (c->use_map) of every slot of the map.  A batched
+// loop runs its slots in lockstep, each stopping on its own test (its
+// launches return at entry from then on); the finished slots leave the map
+// whenever the remaining ones fit a grid half as deep.
 template <typename T>
 int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *const *ring, T *rnew, T *u, cplx<T> *spec, const T *Gp,
                    const Phase<T> *ph, int *iters_out)
 {
-    const int *done  = c->is + IS_DONE;
-    int        chunk = 4, rc;
-    int        is[IS_COUNT];
-    // u -= alpha_m d_m for the ring slots 0 .. count-1 (iterations filled in order)
-    auto ring_update = [&](int count, const int *dflag) -> int {
+    const int             *done  = c->is + IS_DONE;
+    const bool             batch = c->use_map;
+    const std::vector<int> slots0 = c->map_slots;
+    int                    chunk = 4, rc;
+    int                    is[IS_COUNT];
+    std::vector<int>       isall;
+    bool                   stop = false;
+    // u -= alpha_m d_m for the ring slots 0 .. count-1 (iterations filled in
+    // order); count < 0: each RVE slot's pending count; gated: finished RVE
+    // slots skip it
+    auto ring_update = [&](int count, bool gated) -> int {
         Timed      t(c, KF_CG_UPDATE);
         DirRing<T> dr{};
         for (int m = 0; m < kDirRing; ++m)
             dr.d[m] = ring[m];
+        const Slots sl = launch_slots(c, true);
         if (count == kDirRing)
-            hipLaunchKernelGGL((k_ring_update<T, true>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, u, dr,
-                               c->sc + SC_ARING, count, c->Nh, dflag);
+            hipLaunchKernelGGL((k_ring_update<T, true>), dim3(c->nblk_vec, 1, c->gz), dim3(kThreads), 0, c->stream, u,
+                               dr, c->sc + SC_ARING, count, c->Nh, sl, gated ? 1 : 0);
         else
-            hipLaunchKernelGGL((k_ring_update<T, false>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, u, dr,
-                               c->sc + SC_ARING, count, c->Nh, dflag);
+            hipLaunchKernelGGL((k_ring_update<T, false>), dim3(c->nblk_vec, 1, c->gz), dim3(kThreads), 0, c->stream, u,
+                               dr, c->sc + SC_ARING, count, c->Nh, sl, gated ? 1 : 0);
         HIPCHK(c, hipGetLastError());
         return 0;
     };
+    // the stop test after a run of iterations; batched, the finished slots
+    // leave the map once the others fit a shallower grid
+    auto poll = [&]() -> int {
+        if (!batch) {
+            if ((rc = read_scalars(c, nullptr, is)))
+                return rc;
+            stop = is[IS_DONE];
+            return 0;
+        }
+        if ((rc = read_is_all(c, isall)))
+            return rc;
+        std::vector<int> act;
+        for (int b : c->map_slots)
+            if (!isall[(size_t) b * IS_COUNT + IS_DONE])
+                act.push_back(b);
+        stop   = act.empty();
+        int gz = 1;
+        while (gz < (int) act.size())
+            gz *= 2;
+        return (!stop && gz < c->gz) ? set_map(c, act) : 0;
+    };
     // iterations 0 .. k-1 have run; those after the last full ring are pending
     auto finish = [&]() -> int {
+        if (batch) {
+            if (c->map_slots != slots0 && (rc = set_map(c, slots0)))
+                return rc;
+            return ring_update(-1, false);
+        }
         const int k = is[IS_ITER];
         if (k % kDirRing) {
-            const int e = ring_update(k % kDirRing, nullptr);
+            const int e = ring_update(k % kDirRing, false);
             if (e)
                 return e;
         }
@@ -1130,48 +1323,49 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *const *ring, T *rnew, 
         a.phases     = ph;
         a.in         = dnext;
         a.out        = rnew;
-        a.done       = done;
+        a.sl         = launch_slots(c, true);
         if ((e = launch_assemble<T, ASM_CGLIN>(c, a)))
             return e;
         // quantity 1 of the first iteration: the matvec's zero slot 1
         if ((e = reduce(c, c->partial, c->nblk_asm, 2, RED_ALPHA, with_pro ? c->partial_e : nullptr,
                         (int) P.nblk_z)))
             return e;
-        if (j % kDirRing == kDirRing - 1 && (e = ring_update(kDirRing, done)))
+        if (j % kDirRing == kDirRing - 1 && (e = ring_update(kDirRing, true)))
             return e;
         return 0;
     };
     // After iteration 0 the launches of iteration j depend on j % kDirRing
-    // only, so kDirRing iterations are captured once into a hipGraph and
-    // replayed (one launch per replay instead of ~10 per iteration).  Timing
-    // mode stays eager.
+    // only, so kDirRing iterations are captured once into a hipGraph (per
+    // direction buffer and grid depth) and replayed (one launch per replay
+    // instead of ~10 per iteration).  Timing mode stays eager.
     static_assert(kDirRing == 8, "one captured graph holds one turn of the direction ring");
     if (c->use_graphs && !c->timing) {
         if ((rc = iteration(0)))
             return rc;
-        hipGraphExec_t exec = nullptr;
-        for (auto &g : c->graphs)
-            if (g.first == (const void *) ring[0])
-                exec = g.second;
-        if (!exec) {
-            hipGraph_t graph = nullptr;
-            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int crc = 0;
-            for (int j = 1; j <= kDirRing && !crc; ++j)
-                crc = iteration(j);
-            const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
-            if (crc)
-                return crc;
-            HIPCHK(c, ec);
-            HIPCHK(c, hipGraphInstantiateWithFlags(&exec, graph, 0));
-            (void) hipGraphDestroy(graph);
-            c->graphs.push_back({(const void *) ring[0], exec});
-        }
         for (;;) {
+            const std::pair<const void *, int> key{(const void *) ring[0], c->gz};
+            hipGraphExec_t                     exec = nullptr;
+            for (auto &g : c->graphs)
+                if (g.first == key)
+                    exec = g.second;
+            if (!exec) {
+                hipGraph_t graph = nullptr;
+                HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                int crc = 0;
+                for (int j = 1; j <= kDirRing && !crc; ++j)
+                    crc = iteration(j);
+                const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+                if (crc)
+                    return crc;
+                HIPCHK(c, ec);
+                HIPCHK(c, hipGraphInstantiateWithFlags(&exec, graph, 0));
+                (void) hipGraphDestroy(graph);
+                c->graphs.push_back({key, exec});
+            }
             HIPCHK(c, hipGraphLaunch(exec, c->stream));
-            if ((rc = read_scalars(c, nullptr, is)))
+            if ((rc = poll()))
                 return rc;
-            if (is[IS_DONE])
+            if (stop)
                 break;
         }
         return finish();
@@ -1193,9 +1387,9 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *const *ring, T *rnew, 
                     drop_timing(c, j > 0 ? mark + 2 : mark);
             }
         }
-        if (!c->timing && (rc = read_scalars(c, nullptr, is)))
+        if ((rc = poll()))
             return rc;
-        if (is[IS_DONE])
+        if (stop)
             break;
         chunk = std::min(chunk * 2, 16);
     }
@@ -1320,8 +1514,8 @@ int solve_cg_fp64(fans_gpu_ctx *c)
             if (an <= 0.0)
                 an = 0.5 * (ap + ac);
             lerr = std::fabs(an - ac);
-            hipLaunchKernelGGL((k_axpby<double, double>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->u, d,
-                               an - ac, 1.0, c->Nh);
+            hipLaunchKernelGGL((k_axpby<double, double>), dim3(c->nblk_vec, 1, c->gz), dim3(kThreads), 0, c->stream,
+                               c->u, d, an - ac, 1.0, c->Nh, launch_slots(c, false));
             ap  = ac;
             rpd = r1pd;
             ac  = an;
@@ -1368,7 +1562,7 @@ int solve_fp_fp64(fans_gpu_ctx *c)
             a.in              = c->u;
             a.out             = c->r;
             a.g0              = c->g0d;
-            a.done            = done;
+            a.sl              = launch_slots(c, true);
             if ((rc = launch_assemble<double, ASM_RESID>(c, a)) ||
                 (rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_NEXT)))
                 return rc;
@@ -1413,8 +1607,8 @@ int solve_cg_mixed(fans_gpu_ctx *c)
         if ((rc = ensure_ring<float>(c, c->ring32, c->dA32, c->dB32)))
             return rc;
         // x = 0 and d_(-1) = 0 in the ring's last slot
-        hipLaunchKernelGGL(k_inner_start, dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->r32, c->x32,
-                           c->ring32[kDirRing - 1], c->r, c->Nh, c->d.measure, c->partial);
+        hipLaunchKernelGGL(k_inner_start, dim3(c->nblk_vec, 1, c->gz), dim3(kThreads), 0, c->stream, c->r32, c->x32,
+                           c->ring32[kDirRing - 1], c->r, c->Nh, c->d.measure, c->partial, launch_slots(c, false));
         HIPCHK(c, hipGetLastError());
         if ((rc = reduce(c, c->partial, c->nblk_vec, 1, RED_ERR_INIT)))
             return rc;
@@ -1425,8 +1619,8 @@ int solve_cg_mixed(fans_gpu_ctx *c)
         inner += it;
         outer += 1;
         // u += x: the inner CG drove r(x) = R(u) + K x to zero (x -= alpha d)
-        hipLaunchKernelGGL((k_axpby<double, float>), dim3(c->nblk_vec), dim3(kThreads), 0, c->stream, c->u, c->x32,
-                           1.0, 1.0, c->Nh);
+        hipLaunchKernelGGL((k_axpby<double, float>), dim3(c->nblk_vec, 1, c->gz), dim3(kThreads), 0, c->stream, c->u,
+                           c->x32, 1.0, 1.0, c->Nh, launch_slots(c, false));
         // true residual
         if ((rc = residual64(c, c->u, c->r, nullptr)))
             return rc;
@@ -1447,9 +1641,274 @@ int solve_cg_mixed(fans_gpu_ctx *c)
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Batched solve: every RVE slot of a batched context, linear materials, CG in
+// the FP64 or the mixed form.  Each slot takes the steps, and gets the
+// results, of a solve of that RVE alone (same kernels, launch geometry and
+// summation order; it stops on its own test), while each launch covers every
+// slot still running.
+// ---------------------------------------------------------------------------
+struct BatchScalars {
+    std::vector<double> sc;
+    std::vector<int>    is;
+};
+
+int read_all_scalars(fans_gpu_ctx *c, BatchScalars &v)
+{
+    v.sc.resize((size_t) c->nb * SC_COUNT);
+    v.is.resize((size_t) c->nb * IS_COUNT);
+    HIPCHK(c, hipMemcpyAsync(v.sc.data(), c->sc, sizeof(double) * v.sc.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(v.is.data(), c->is, sizeof(int) * v.is.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int write_all_scalars(fans_gpu_ctx *c, const BatchScalars &v)
+{
+    HIPCHK(c, hipMemcpyAsync(c->sc, v.sc.data(), sizeof(double) * v.sc.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->is, v.is.data(), sizeof(int) * v.is.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// slot b's scalars as init_scalars sets a single context's
+void init_slot_scalars(const fans_gpu_ctx *c, BatchScalars &v, int b, double tol, int err_type, int n_it)
+{
+    double *sc = &v.sc[(size_t) b * SC_COUNT];
+    int    *is = &v.is[(size_t) b * IS_COUNT];
+    std::fill(sc, sc + SC_COUNT, 0.0);
+    std::fill(is, is + IS_COUNT, 0);
+    sc[SC_TOL]     = tol;
+    sc[SC_DELTA]   = 1.0;
+    is[IS_NIT]     = n_it;
+    is[IS_ERRTYPE] = err_type;
+    is[IS_MEASURE] = c->d.measure;
+}
+
+// solve_cg_fp64's linear path for every slot
+int solve_batch_cg64(fans_gpu_ctx *c, std::vector<SlotHost *> &H)
+{
+    const int        nb = c->nb;
+    std::vector<int> all(nb);
+    for (int b = 0; b < nb; ++b)
+        all[b] = b;
+    BatchScalars v;
+    v.sc.assign((size_t) nb * SC_COUNT, 0.0);
+    v.is.assign((size_t) nb * IS_COUNT, 0);
+    for (int b = 0; b < nb; ++b)
+        init_slot_scalars(c, v, b, c->tol, c->tol_type, c->d.n_it);
+    int rc;
+    if ((rc = write_all_scalars(c, v)) || (rc = ensure_ring<double>(c, c->ring64, c->dA, c->dB)))
+        return rc;
+    // d_(-1) = 0 in the ring's last slot, s = 0 (every slot at once)
+    if ((rc = fill<double>(c, c->s, 0.0, c->Nh * nb)) ||
+        (rc = fill<double>(c, c->ring64[kDirRing - 1], 0.0, c->Nh * nb)))
+        return rc;
+    if ((rc = set_map(c, all)) || (rc = residual64(c, c->u, c->r, nullptr)) ||
+        (rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_INIT)))
+        return rc;
+    if ((rc = cg_linear_loop<double>(c, c->fft64, c->r, c->ring64, c->rnew, c->u, c->spec64, c->G64, c->dph64,
+                                     nullptr)))
+        return rc;
+    if ((rc = read_all_scalars(c, v)))
+        return rc;
+    std::vector<double> hist((size_t) nb * c->strides.hist);
+    HIPCHK(c, hipMemcpyAsync(hist.data(), c->hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int b = 0; b < nb; ++b) {
+        const int     it = v.is[(size_t) b * IS_COUNT + IS_ITER];
+        const double *hb = &hist[(size_t) b * c->strides.hist];
+        H[b]->iter        = it;
+        H[b]->inner_iters = H[b]->outer_iters = 0;
+        H[b]->err_hist.assign(hb, hb + it + 1);
+    }
+    return 0;
+}
+
+// solve_cg_mixed for every slot: the slots still above the tolerance run the
+// next inner FP32 solve together
+int solve_batch_mixed(fans_gpu_ctx *c, std::vector<SlotHost *> &H)
+{
+    const int        nb = c->nb;
+    std::vector<int> all(nb);
+    for (int b = 0; b < nb; ++b)
+        all[b] = b;
+    BatchScalars v;
+    v.sc.assign((size_t) nb * SC_COUNT, 0.0);
+    v.is.assign((size_t) nb * IS_COUNT, 0);
+    for (int b = 0; b < nb; ++b)
+        init_slot_scalars(c, v, b, c->tol, c->tol_type, c->d.n_it);
+    int rc;
+    if ((rc = write_all_scalars(c, v)) || (rc = ensure_ring<float>(c, c->ring32, c->dA32, c->dB32)))
+        return rc;
+    if ((rc = set_map(c, all)) || (rc = residual64(c, c->u, c->r, nullptr)) ||
+        (rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_INIT)) || (rc = read_all_scalars(c, v)))
+        return rc;
+    std::vector<double>              err0(nb);
+    std::vector<std::vector<double>> ohist(nb);
+    std::vector<int>                 inner(nb, 0), outer(nb, 0), it(nb, 0);
+    std::vector<char>                fin(nb);
+    for (int b = 0; b < nb; ++b) {
+        err0[b]  = v.sc[(size_t) b * SC_COUNT + SC_ERR];
+        ohist[b] = {err0[b]};
+        fin[b]   = (char) v.is[(size_t) b * IS_COUNT + IS_DONE];
+    }
+    const double eta = 2e-5;
+    for (;;) {
+        std::vector<int> act;
+        for (int b = 0; b < nb; ++b)
+            if (!fin[b] && inner[b] < c->d.n_it)
+                act.push_back(b);
+        if (act.empty())
+            break;
+        // inner FP32 CG on K x = r, x = 0, per slot to eta times its outer error
+        for (int b : act) {
+            const double err_out = v.sc[(size_t) b * SC_COUNT + SC_ERR];
+            init_slot_scalars(c, v, b, std::max(eta * err_out, 0.25 * c->tol), FANS_ERR_ABSOLUTE, c->d.n_it - inner[b]);
+        }
+        if ((rc = write_all_scalars(c, v)) || (rc = set_map(c, act)))
+            return rc;
+        hipLaunchKernelGGL(k_inner_start, dim3(c->nblk_vec, 1, c->gz), dim3(kThreads), 0, c->stream, c->r32, c->x32,
+                           c->ring32[kDirRing - 1], c->r, c->Nh, c->d.measure, c->partial, launch_slots(c, false));
+        HIPCHK(c, hipGetLastError());
+        if ((rc = reduce(c, c->partial, c->nblk_vec, 1, RED_ERR_INIT)))
+            return rc;
+        if ((rc = cg_linear_loop<float>(c, c->fft32, c->r32, c->ring32, c->rnew32, c->x32, c->spec32, c->G32,
+                                        c->dph32, nullptr)))
+            return rc;
+        if ((rc = read_all_scalars(c, v)))
+            return rc;
+        for (int b : act)
+            it[b] = v.is[(size_t) b * IS_COUNT + IS_ITER];
+        // u += x and the true residual of the slots that ran
+        hipLaunchKernelGGL((k_axpby<double, float>), dim3(c->nblk_vec, 1, c->gz), dim3(kThreads), 0, c->stream, c->u,
+                           c->x32, 1.0, 1.0, c->Nh, launch_slots(c, false));
+        HIPCHK(c, hipGetLastError());
+        if ((rc = residual64(c, c->u, c->r, nullptr)))
+            return rc;
+        for (int b : act)
+            init_slot_scalars(c, v, b, c->tol, c->tol_type, 1 << 30);
+        if ((rc = write_all_scalars(c, v)) || (rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_INIT)) ||
+            (rc = read_all_scalars(c, v)))
+            return rc;
+        for (int b : act) {
+            const double err = v.sc[(size_t) b * SC_COUNT + SC_ERR];
+            inner[b] += it[b];
+            outer[b] += 1;
+            ohist[b].push_back(err);
+            const double e = c->tol_type == FANS_ERR_ABSOLUTE ? err : err / err0[b];
+            fin[b]         = !(e > c->tol) || it[b] == 0;  // it == 0: FP32 floor reached without progress
+        }
+    }
+    for (int b = 0; b < nb; ++b) {
+        H[b]->iter        = inner[b];
+        H[b]->inner_iters = inner[b];
+        H[b]->outer_iters = outer[b];
+        H[b]->err_hist    = ohist[b];
+    }
+    return 0;
+}
+
+// Every slot's host state in `parked` and the fields at slot 0 for a batched
+// operation; the slots must be loaded alike (linear materials, same element
+// variant and Green storage).  batch_end restores the selected slot.
+int batch_begin(fans_gpu_ctx *c, std::vector<SlotHost *> &H, int &sel)
+{
+    sel = c->cur;
+    swap_host(c, c->parked[sel]);
+    set_views(c, 0);
+    H.resize(c->nb);
+    for (int b = 0; b < c->nb; ++b)
+        H[b] = &c->parked[b];
+    int nph = 0;
+    for (int b = 0; b < c->nb; ++b) {
+        const SlotHost &h = *H[b], &h0 = *H[0];
+        if (!h.have_ms || !h.have_mat)
+            return fail(c, FANS_GPU_ESTATE, "slot %d: set_microstructure and set_materials must precede this call", b);
+        if (h.ms_max >= h.nphases)
+            return fail(c, FANS_GPU_EINVAL, "slot %d: MaterialManager: Phase %d not assigned", b, h.nphases);
+        if (!h.all_linear || h.voxel_moduli)
+            return fail(c, FANS_GPU_EINVAL, "slot %d: a batched context solves linear phase tables only", b);
+        if (h.all_iso != h0.all_iso || h.ms16 != h0.ms16 || h.gxq != h0.gxq || h.gyq != h0.gyq)
+            return fail(c, FANS_GPU_EINVAL,
+                        "slot %d: the slots of a batch need the same kind of phase table, id width and Green storage",
+                        b);
+        nph = std::max(nph, h.nphases);
+    }
+    // the launch arguments of the batch (captured graphs hold them by value)
+    const SlotHost &h0 = *H[0];
+    c->nphases      = nph;
+    c->all_iso      = h0.all_iso;
+    c->ms16         = h0.ms16;
+    c->gxq          = h0.gxq;
+    c->gyq          = h0.gyq;
+    c->all_linear   = true;
+    c->has_j2       = false;
+    c->has_pnl      = false;
+    c->has_visco    = false;
+    c->voxel_moduli = false;
+    const std::array<int, 7> sig = {c->all_iso, c->cubic, c->ms16, c->gxq, c->gyq, c->nphases, 1};
+    if (sig != c->batch_sig) {
+        drop_graphs(c);
+        c->batch_sig = sig;
+    }
+    return 0;
+}
+
+void batch_end(fans_gpu_ctx *c, int sel)
+{
+    clear_map(c);
+    swap_host(c, c->parked[sel]);
+    set_views(c, sel);
+}
+
+int solve_batch(fans_gpu_ctx *c)
+{
+    if (c->d.method != FANS_METHOD_CG)
+        return fail(c, FANS_GPU_EINVAL, "a batched context solves with the CG method");
+    if (c->timing)
+        return fail(c, FANS_GPU_EINVAL, "per-kernel timing is not available on a batched context");
+    std::vector<SlotHost *> H;
+    int                     sel = 0;
+    int                     rc  = batch_begin(c, H, sel);
+    if (!rc)
+        rc = c->d.precision == FANS_PREC_MIXED ? solve_batch_mixed(c, H) : solve_batch_cg64(c, H);
+    batch_end(c, sel);
+    return rc;
+}
+
+// iteration counts, final error and stop test of one RVE's last solve
+struct SolveResult {
+    int    iterations = 0, inner = 0, outer = 0;
+    double final_error = 0.0;
+    bool   converged   = false;
+};
+template <typename S>
+SolveResult solve_result(const fans_gpu_ctx *c, const S &h)
+{
+    SolveResult r;
+    r.iterations    = h.iter;
+    r.inner         = h.inner_iters ? h.inner_iters : h.iter;
+    r.outer         = h.outer_iters;
+    r.final_error   = h.err_hist.empty() ? 0.0 : h.err_hist.back();
+    const double e0 = h.err_hist.empty() ? 1.0 : h.err_hist[0];
+    const double ev = c->tol_type == FANS_ERR_ABSOLUTE ? r.final_error : r.final_error / e0;
+    r.converged     = ev <= c->tol;
+    return r;
+}
+SolveResult slot_result(const fans_gpu_ctx *c, int b)
+{
+    return b == c->cur ? solve_result(c, *c) : solve_result(c, c->parked[b]);
+}
+
+// every per-RVE buffer nb times (batched contexts: slot b at b * stride);
+// a batched context also allocates now what a single one allocates on first
+// use (Green operators, phase tables, the CG direction ring), so that no slot
+// buffer moves later
 int alloc_fields(fans_gpu_ctx *c)
 {
-    const size_t vb = sizeof(double) * c->Nh;
+    const int64_t nb = c->nb;
+    const size_t  vb = sizeof(double) * c->Nh * nb;
     HIPCHK(c, hipMalloc(&c->u, vb));
     HIPCHK(c, hipMalloc(&c->u_prev, vb));
     HIPCHK(c, hipMalloc(&c->r, vb));
@@ -1460,40 +1919,86 @@ int alloc_fields(fans_gpu_ctx *c)
     HIPCHK(c, hipMemsetAsync(c->u, 0, vb, c->stream));
     HIPCHK(c, hipMemsetAsync(c->u_prev, 0, vb, c->stream));
     HIPCHK(c, hipMemsetAsync(c->r, 0, vb, c->stream));
-    HIPCHK(c, hipMalloc(&c->spec64, sizeof(cplx<double>) * c->Nf * c->h));
-    HIPCHK(c, hipMalloc(&c->g0d, sizeof(double) * 9));
-    HIPCHK(c, hipMalloc(&c->g0f, sizeof(float) * 9));
-    HIPCHK(c, hipMalloc(&c->sc, sizeof(double) * SC_COUNT));
-    HIPCHK(c, hipMalloc(&c->is, sizeof(int) * IS_COUNT));
-    HIPCHK(c, hipMalloc(&c->hist, sizeof(double) * (c->d.n_it + 2)));
-    HIPCHK(c, hipMalloc(&c->ms, c->N));
-    c->ms_bytes = c->N;
-    if (c->d.precision == FANS_PREC_MIXED) {
-        const size_t fb = sizeof(float) * c->Nh;
+    HIPCHK(c, hipMalloc(&c->spec64, sizeof(cplx<double>) * c->Nf * c->h * nb));
+    HIPCHK(c, hipMalloc(&c->g0d, sizeof(double) * 9 * nb));
+    HIPCHK(c, hipMalloc(&c->g0f, sizeof(float) * 9 * nb));
+    HIPCHK(c, hipMalloc(&c->sc, sizeof(double) * SC_COUNT * nb));
+    HIPCHK(c, hipMalloc(&c->is, sizeof(int) * IS_COUNT * nb));
+    HIPCHK(c, hipMemsetAsync(c->is, 0, sizeof(int) * IS_COUNT * nb, c->stream));
+    HIPCHK(c, hipMalloc(&c->hist, sizeof(double) * (c->d.n_it + 2) * nb));
+    // phase ids: bytes, room for 16-bit ids per slot of a batch
+    c->ms_bytes = nb > 1 ? 2 * c->N : c->N;
+    HIPCHK(c, hipMalloc(&c->ms, c->ms_bytes * nb));
+    const bool mixed = c->d.precision == FANS_PREC_MIXED;
+    if (mixed) {
+        const size_t fb = sizeof(float) * c->Nh * nb;
         HIPCHK(c, hipMalloc(&c->r32, fb));
         HIPCHK(c, hipMalloc(&c->s32, fb));
         HIPCHK(c, hipMalloc(&c->dA32, fb));
         HIPCHK(c, hipMalloc(&c->dB32, fb));
         HIPCHK(c, hipMalloc(&c->rnew32, fb));
         HIPCHK(c, hipMalloc(&c->x32, fb));
-        HIPCHK(c, hipMalloc(&c->spec32, sizeof(cplx<float>) * c->Nf * c->h));
+        HIPCHK(c, hipMalloc(&c->spec32, sizeof(cplx<float>) * c->Nf * c->h * nb));
     }
-    // partial sums: the largest grid of any reducing kernel times its quantities
-    int64_t need = 2 * std::max<int64_t>({(int64_t) c->nblk_asm, (int64_t) c->nblk_vec, c->fft64.nblk_z,
-                                          c->fft32.nblk_z, x_green_blocks(c->fft64, c->h, c->ny),
-                                          x_green_blocks(c->fft32, c->h, c->ny)});
+    // partial sums: the largest grid of any reducing kernel times its
+    // quantities (batched: for any assembly tile depth, so that the per-slot
+    // stride never changes)
+    const int64_t asm_blocks = nb > 1 ? (int64_t) c->nx * c->nty * c->ntz : c->nblk_asm;
+    int64_t       need = 2 * std::max<int64_t>({asm_blocks, (int64_t) c->nblk_vec, c->fft64.nblk_z, c->fft32.nblk_z,
+                                                x_green_blocks(c->fft64, c->h, c->ny),
+                                                x_green_blocks(c->fft32, c->h, c->ny)});
     need         = std::max<int64_t>(need, (int64_t) c->nstr * c->nblk_el);
     need         = std::max<int64_t>(need, 6 * 1024);  // k_moduli_stats: 6 quantities, <= 1024 blocks
     c->partial_len = need;
-    HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * need));
+    HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * need * nb));
     // per-block partials of the linear CG's inverse z pass, kept apart because
-    // the matvec that follows writes `partial`
-    HIPCHK(c, hipMalloc(&c->partial_e, sizeof(double) * std::max<int64_t>(c->fft64.nblk_z, c->fft32.nblk_z)));
+    // the matvec that follows writes `partial` (batched: at the same stride)
+    HIPCHK(c, hipMalloc(&c->partial_e, sizeof(double) *
+                                           (nb > 1 ? need * nb : std::max<int64_t>(c->fft64.nblk_z, c->fft32.nblk_z))));
+    const int npk      = c->h * (c->h + 1) / 2;
+    c->strides.field   = c->Nh;
+    c->strides.spec    = c->Nf * c->h;
+    c->strides.green   = c->Nf * npk;
+    c->strides.part    = need;
+    c->strides.ms      = c->ms_bytes;
+    c->strides.hist    = c->d.n_it + 2;
+    if (nb > 1) {
+        c->nph_cap       = 256;
+        c->strides.ph    = c->nph_cap;
+        HIPCHK(c, hipMalloc(&c->dph64, sizeof(Phase<double>) * c->nph_cap * nb));
+        HIPCHK(c, hipMalloc(&c->dph32, sizeof(Phase<float>) * c->nph_cap * nb));
+        HIPCHK(c, hipMalloc(&c->G64, sizeof(double) * c->Nf * npk * nb));
+        if (mixed)
+            HIPCHK(c, hipMalloc(&c->G32, sizeof(float) * c->Nf * npk * nb));
+        for (int m = 2; m < kDirRing; ++m) {
+            if (mixed)
+                HIPCHK(c, hipMalloc(&c->ring32[m], sizeof(float) * c->Nh * nb));
+            else
+                HIPCHK(c, hipMalloc(&c->ring64[m], sizeof(double) * c->Nh * nb));
+        }
+        c->map_cap = 1;
+        while (c->map_cap < c->nb)
+            c->map_cap *= 2;
+        HIPCHK(c, hipMalloc(&c->slot_map, sizeof(int) * c->map_cap));
+#define X(T, name, kind) c->base.name = c->name;
+        FANS_SLOT_BUFFERS(X)
+#undef X
+        for (int m = 2; m < kDirRing; ++m) {
+            c->base.ring64[m] = c->ring64[m];
+            c->base.ring32[m] = c->ring32[m];
+        }
+        c->parked.assign(c->nb, SlotHost{});
+        set_views(c, 0);
+    }
     return 0;
 }
 
 void free_all(fans_gpu_ctx *c)
 {
+    if (c->nb > 1 && c->base.u)
+        set_views(c, 0);  // the fields hold the allocations again
+    if (c->slot_map)
+        (void) hipFree(c->slot_map);
     for (void *p : {(void *) c->ms, (void *) c->dph64, (void *) c->dph32, (void *) c->u, (void *) c->u_prev,
                     (void *) c->r, (void *) c->s, (void *) c->dA, (void *) c->dB, (void *) c->rnew,
                     (void *) c->spec64, (void *) c->G64, (void *) c->keref_d, (void *) c->G32, (void *) c->js,
@@ -1678,7 +2183,7 @@ const char *fans_gpu_last_error(const fans_gpu_ctx *ctx)
     return g_noalloc_err[0] ? g_noalloc_err : "null context";
 }
 
-int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *d)
+static int create_ctx(fans_gpu_ctx **out, const fans_gpu_desc *d, int slots)
 {
     if (!out || !d)
         return FANS_GPU_EINVAL;
@@ -1700,6 +2205,7 @@ int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *d)
         return FANS_GPU_EINVAL;
     fans_gpu_ctx *c = new fans_gpu_ctx();
     c->d            = *d;
+    c->nb           = slots;
     if (hipSetDevice(d->device) != hipSuccess) {
         delete c;
         return FANS_GPU_EHIP;
@@ -1747,6 +2253,69 @@ int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *d)
         return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
+}
+
+int fans_gpu_create(fans_gpu_ctx **out, const fans_gpu_desc *d) { return create_ctx(out, d, 1); }
+
+int fans_gpu_create_batch(fans_gpu_ctx **out, const fans_gpu_desc *d, int32_t slots)
+{
+    if (out)
+        *out = nullptr;
+    if (slots < 1 || slots > kMaxSlots || (d && d->method != FANS_METHOD_CG))
+        return FANS_GPU_EINVAL;
+    return create_ctx(out, d, slots);
+}
+
+int fans_gpu_select_slot(fans_gpu_ctx *c, int32_t slot)
+{
+    if (!c)
+        return FANS_GPU_EINVAL;
+    if (slot < 0 || slot >= c->nb)
+        return fail(c, FANS_GPU_EINVAL, "slot %d outside [0, %d)", slot, c->nb);
+    select_slot(c, slot);
+    return 0;
+}
+
+int fans_gpu_batch_results(fans_gpu_ctx *c, int32_t *iterations, int32_t *converged, double *final_error,
+                           int32_t count)
+{
+    if (!c)
+        return FANS_GPU_EINVAL;
+    if (count != c->nb)
+        return fail(c, FANS_GPU_EINVAL, "count must be the context's slot count %d", c->nb);
+    for (int b = 0; b < c->nb; ++b) {
+        const SolveResult r = slot_result(c, b);
+        if (iterations)
+            iterations[b] = r.inner;
+        if (converged)
+            converged[b] = r.converged;
+        if (final_error)
+            final_error[b] = r.final_error;
+    }
+    return 0;
+}
+
+int fans_gpu_batch_homogenized_stress(fans_gpu_ctx *c, double *out, int32_t n_str, int32_t count)
+{
+    if (!c || !out || n_str != c->nstr)
+        return FANS_GPU_EINVAL;
+    if (count != c->nb)
+        return fail(c, FANS_GPU_EINVAL, "count must be the context's slot count %d", c->nb);
+    if (c->nb == 1)
+        return fans_gpu_homogenized_stress(c, out, n_str);
+    std::vector<SlotHost *> H;
+    int                     sel = 0;
+    int                     rc  = batch_begin(c, H, sel);
+    std::vector<int>        all(c->nb);
+    for (int b = 0; b < c->nb; ++b)
+        all[b] = b;
+    BatchScalars v;
+    if (!rc && !(rc = set_map(c, all)) && !(rc = homog_stress(c, nullptr)) && !(rc = read_all_scalars(c, v)))
+        for (int b = 0; b < c->nb; ++b)
+            for (int k = 0; k < c->nstr; ++k)
+                out[(size_t) b * c->nstr + k] = v.sc[(size_t) b * SC_COUNT + SC_RES0 + k] / (double) c->N;
+    batch_end(c, sel);
+    return rc;
 }
 
 int fans_gpu_destroy(fans_gpu_ctx *c)
@@ -2053,6 +2622,12 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
         for (int i = 0; i < 36; ++i)
             c->ph32[p].p[i] = (float) P.p[i];
     }
+    // a batched context keeps its fixed per-slot tables and no J2 state
+    if (c->nb > 1 && (c->nphases > c->nph_cap || !c->all_linear)) {
+        c->have_mat = false;
+        return fail(c, FANS_GPU_EINVAL, "a batched context takes at most %d phases, all with linear models",
+                    c->nph_cap);
+    }
     if (c->nphases > c->nph_cap) {
         (void) hipFree(c->dph64);
         (void) hipFree(c->dph32);
@@ -2089,7 +2664,8 @@ int fans_gpu_set_materials(fans_gpu_ctx *c, int32_t ng, const int32_t *kind, con
         if (c->phase_kind[p] != (c->h == 1 ? FANS_MODEL_THERMAL_ISO : FANS_MODEL_ELASTIC_ISO))
             c->all_iso = false;
     c->have_mat = true;
-    refresh_graphs(c);
+    if (c->nb == 1)  // batched solves check their graphs themselves
+        refresh_graphs(c);
     return 0;
 }
 
@@ -2118,6 +2694,8 @@ int fans_gpu_get_gradient(fans_gpu_ctx *c, double *g0, int32_t n)
 int fans_gpu_enable_mixed_bc(fans_gpu_ctx *c, int32_t nE, const int32_t *iE, int32_t nF, const int32_t *iF,
                              int32_t nsteps, const double *FE, const double *PF, int32_t step)
 {
+    if (c && c->nb > 1)
+        return fail(c, FANS_GPU_EINVAL, "mixed boundary conditions are not available on a batched context");
     if (!c)
         return FANS_GPU_EINVAL;
     if (!c->have_mat)
@@ -2214,8 +2792,9 @@ int fans_gpu_solve(fans_gpu_ctx *c, fans_gpu_stats *st)
 {
     if (!c)
         return FANS_GPU_EINVAL;
-    if (int rc = check_ready(c))
-        return rc;
+    if (c->nb == 1)
+        if (int rc = check_ready(c))
+            return rc;
     for (int k = 0; k < KF_COUNT; ++k) {
         c->fam_ms[k] = 0;
         c->fam_n[k]  = 0;
@@ -2226,7 +2805,9 @@ int fans_gpu_solve(fans_gpu_ctx *c, fans_gpu_stats *st)
     HIPCHK(c, hipEventCreate(&e1));
     HIPCHK(c, hipEventRecord(e0, c->stream));
     int rc;
-    if (c->d.precision == FANS_PREC_MIXED && c->d.method == FANS_METHOD_CG)
+    if (c->nb > 1)
+        rc = solve_batch(c);
+    else if (c->d.precision == FANS_PREC_MIXED && c->d.method == FANS_METHOD_CG)
         rc = solve_cg_mixed(c);
     else if (c->d.method == FANS_METHOD_FP)
         rc = solve_fp_fp64(c);
@@ -2241,6 +2822,26 @@ int fans_gpu_solve(fans_gpu_ctx *c, fans_gpu_stats *st)
     (void) hipEventDestroy(e0);
     (void) hipEventDestroy(e1);
     c->solve_ms = ms;
+    if (c->nb > 1) {
+        // the batch's time for every slot; the statistics: the largest
+        // iteration counts and error, converged if every slot is
+        for (auto &h : c->parked)
+            h.solve_ms = ms;
+        if (st) {
+            *st           = fans_gpu_stats{};
+            st->converged = 1;
+            for (int b = 0; b < c->nb; ++b) {
+                const SolveResult r  = slot_result(c, b);
+                st->iterations       = std::max(st->iterations, r.iterations);
+                st->inner_iterations = std::max(st->inner_iterations, r.inner);
+                st->outer_iterations = std::max(st->outer_iterations, r.outer);
+                st->final_error      = std::max(st->final_error, r.final_error);
+                st->converged        = st->converged && r.converged;
+            }
+            st->solve_ms = ms;
+        }
+        return 0;
+    }
     if (c->timing)
         harvest_timing(c);
     // MaterialManager::update_internal_variables (solver.h:299)
@@ -2248,14 +2849,13 @@ int fans_gpu_solve(fans_gpu_ctx *c, fans_gpu_stats *st)
         return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (st) {
-        st->iterations       = c->iter;
-        st->inner_iterations = c->inner_iters ? c->inner_iters : c->iter;
-        st->outer_iterations = c->outer_iters;
+        const SolveResult r  = solve_result(c, *c);
+        st->iterations       = r.iterations;
+        st->inner_iterations = r.inner;
+        st->outer_iterations = r.outer;
         st->solve_ms         = c->solve_ms;
-        st->final_error      = c->err_hist.empty() ? 0.0 : c->err_hist.back();
-        const double e0v     = c->err_hist.empty() ? 1.0 : c->err_hist[0];
-        const double ev      = c->tol_type == FANS_ERR_ABSOLUTE ? st->final_error : st->final_error / e0v;
-        st->converged        = ev <= c->tol;
+        st->final_error      = r.final_error;
+        st->converged        = r.converged;
     }
     return 0;
 }
@@ -2283,6 +2883,8 @@ int fans_gpu_homogenized_stress(fans_gpu_ctx *c, double *out, int32_t n)
 
 int fans_gpu_homogenized_tangent(fans_gpu_ctx *c, double pert, double *out, int32_t n)
 {
+    if (c && c->nb > 1)
+        return fail(c, FANS_GPU_EINVAL, "the homogenized tangent is not available on a batched context");
     if (!c || !out || n != c->nstr)
         return FANS_GPU_EINVAL;
     if (int rc0 = check_ready(c))
@@ -2668,6 +3270,8 @@ int fans_gpu_get_green(fans_gpu_ctx *c, double *out, int64_t count)
 
 int fans_gpu_set_voxel_moduli(fans_gpu_ctx *c, const double *k, const double *g, int64_t count)
 {
+    if (c && c->nb > 1)
+        return fail(c, FANS_GPU_EINVAL, "per-voxel moduli are not available on a batched context");
     if (!c || !k || (c->h == 3 && !g))
         return FANS_GPU_EINVAL;
     if (!c->have_mat)
@@ -2735,6 +3339,8 @@ int fans_gpu_set_voxel_moduli(fans_gpu_ctx *c, const double *k, const double *g,
 
 int fans_gpu_sensitivity(fans_gpu_ctx *c, const double *ub, const double *g0b, double *sens, int64_t count)
 {
+    if (c && c->nb > 1)
+        return fail(c, FANS_GPU_EINVAL, "the sensitivity is not available on a batched context");
     if (!c || !ub || !g0b || !sens)
         return FANS_GPU_EINVAL;
     if (int rc = check_ready(c))

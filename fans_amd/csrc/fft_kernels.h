@@ -183,10 +183,16 @@ __device__ __forceinline__ cplx<T> r2c_untangle(const cplx<T> *A, int ld, int co
 template <typename T, int H, int PRO>
 __global__ __launch_bounds__(256) void k_fft_z_r2c(T *__restrict__ in, cplx<T> *__restrict__ out, FftLen LM,
                                                    const cplx<T> *__restrict__ tw, int nz, int64_t nlines, int lpb,
-                                                   ProArgs<T> pa, const int *__restrict__ done)
+                                                   ProArgs<T> pa, Slots sl)
 {
-    if (done && *done)
+    const int sb = slot_of(sl);
+    if (sb < 0)
         return;
+    if (sl.map) {  // batched launch: this slot's buffers
+        in += sb * sl.field;
+        out += sb * sl.spec;
+        slot_pro(pa, sb, sl);
+    }
     extern __shared__ char smem_raw[];
     __shared__ double red[8];
     cplx<T>      *A     = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -235,6 +241,27 @@ struct EpiArgs {
     T            *dnext;    // EPI_CGD: new direction (written)
     const double *scal;     // EPI_CGD: device scalars (beta at [0])
 };
+
+// the argument blocks at RVE slot sb
+template <typename T>
+__device__ __forceinline__ void slot_pro(ProArgs<T> &pa, int sb, const Slots &sl)
+{
+    pa.rnew    = slot_ptr(pa.rnew, sb * sl.field);
+    pa.alpha   = slot_ptr(pa.alpha, (int64_t) sb * kScSlot);
+    pa.partial = slot_ptr(pa.partial, sb * sl.part);
+}
+template <typename T>
+__device__ __forceinline__ void slot_epi(EpiArgs<T> &ep, int sb, const Slots &sl)
+{
+    ep.out     = slot_ptr(ep.out, sb * sl.field);
+    ep.r       = slot_ptr(ep.r, sb * sl.field);
+    ep.u       = slot_ptr(ep.u, sb * sl.field);
+    ep.partial = slot_ptr(ep.partial, sb * sl.part);
+    ep.dprev   = slot_ptr(ep.dprev, sb * sl.field);
+    ep.qprev   = slot_ptr(ep.qprev, sb * sl.field);
+    ep.dnext   = slot_ptr(ep.dnext, sb * sl.field);
+    ep.scal    = slot_ptr(ep.scal, (int64_t) sb * kScSlot);
+}
 
 // c2r pre-processing of one (line, component): `line` points at kz = 0 of the
 // component (stride H); the half-length transform input is
@@ -288,10 +315,15 @@ __device__ __forceinline__ void z_epi_partials(const EpiArgs<T> &ep, double acc,
 template <typename T, int H, int EPI>
 __global__ __launch_bounds__(256) void k_fft_z_c2r(const cplx<T> *__restrict__ in, FftLen LM,
                                                    const cplx<T> *__restrict__ tw, int nz, int64_t nlines, int lpb,
-                                                   EpiArgs<T> ep, const int *__restrict__ done)
+                                                   EpiArgs<T> ep, Slots sl)
 {
-    if (done && *done)
+    const int sb = slot_of(sl);
+    if (sb < 0)
         return;
+    if (sl.map) {  // batched launch: this slot's buffers
+        in += sb * sl.spec;
+        slot_epi(ep, sb, sl);
+    }
     extern __shared__ char smem_raw[];
     __shared__ double red[8];
     cplx<T>          *A     = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -416,10 +448,16 @@ __device__ __forceinline__ void lds_fft_ct(cplx<T> *buf, const cplx<T> *__restri
 template <typename T, int H, int M, int PRO>
 __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_r2c_ct(T *__restrict__ in, cplx<T> *__restrict__ out,
                                                                       const cplx<T> *__restrict__ tw, ProArgs<T> pa,
-                                                                      const int *__restrict__ done)
+                                                                      Slots sl)
 {
-    if (done && *done)
+    const int sb = slot_of(sl);
+    if (sb < 0)
         return;
+    if (sl.map) {  // batched launch: this slot's buffers
+        in += sb * sl.field;
+        out += sb * sl.spec;
+        slot_pro(pa, sb, sl);
+    }
     constexpr int NT = zct_threads<H, M>(), LPB = zct_lpb<H, M>();
     constexpr int NZH = 2 * M * H, NC = LPB * H, LD = NC + 1, MC = M + 1;
     __shared__ cplx<T> A[M * LD];
@@ -450,10 +488,15 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_r2c_ct(T *__res
 template <typename T, int H, int M, int EPI>
 __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cplx<T> *__restrict__ in,
                                                                       const cplx<T> *__restrict__ tw, EpiArgs<T> ep,
-                                                                      const int *__restrict__ done)
+                                                                      Slots sl)
 {
-    if (done && *done)
+    const int sb = slot_of(sl);
+    if (sb < 0)
         return;
+    if (sl.map) {  // batched launch: this slot's buffers
+        in += sb * sl.spec;
+        slot_epi(ep, sb, sl);
+    }
     constexpr int NT = zct_threads<H, M>(), LPB = zct_lpb<H, M>();
     constexpr int NZH = 2 * M * H, NC = LPB * H, LD = NC + 1, MC = M + 1;
     __shared__ cplx<T> A[M * LD];
@@ -523,10 +566,14 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cp
 template <typename T, bool INV>
 __global__ __launch_bounds__(256) void k_fft_strided(cplx<T> *__restrict__ data, FftLen L,
                                                      const cplx<T> *__restrict__ tw, int64_t sn, int c0, int c1,
-                                                     int64_t ostride, int bc, const int *__restrict__ done)
+                                                     int64_t ostride, int bc, Slots sl)
 {
-    if (done && *done)
+    const int sb = slot_of(sl);
+    if (sb < 0)
         return;
+    if (sl.map) {  // batched launch: this slot's buffers
+        data += sb * sl.spec;
+    }
     extern __shared__ char smem_raw[];
     cplx<T>  *A     = reinterpret_cast<cplx<T> *>(smem_raw);
     const int b0    = c0 + blockIdx.x * bc;
@@ -559,11 +606,16 @@ __device__ __forceinline__ double kz_weight(int kz, int Mc) { return (kz == 0 ||
 template <typename T, int H>
 __global__ __launch_bounds__(256) void k_fft_x_green(cplx<T> *__restrict__ data, FftLen L,
                                                      const cplx<T> *__restrict__ tw, GreenView<T> gv, int kz_lo,
-                                                     int kz_hi, int bk, double *__restrict__ partial,
-                                                     const int *__restrict__ done)
+                                                     int kz_hi, int bk, double *__restrict__ partial, Slots sl)
 {
-    if (done && *done)
+    const int sb = slot_of(sl);
+    if (sb < 0)
         return;
+    if (sl.map) {  // batched launch: this slot's buffers
+        data += sb * sl.spec;
+        gv.G += sb * sl.green;
+        partial = slot_ptr(partial, sb * sl.part);
+    }
     const int ny = gv.ny, Mc = gv.Mc;
     __shared__ double red[8];
     double            acc = 0.0;

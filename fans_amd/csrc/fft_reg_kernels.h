@@ -182,11 +182,14 @@ __device__ __forceinline__ void fourstep_out_to_in(cplx<T> (&v)[N1])
 template <typename T, int N1, int N2, int C, bool INV>
 __global__ __launch_bounds__(C * N2) void k_fft_strided_reg(cplx<T> *__restrict__ data,
                                                             const cplx<T> *__restrict__ tw, int64_t sn, int c0,
-                                                            int c1, int64_t ostride, int nbx,
-                                                            const int *__restrict__ done)
+                                                            int c1, int64_t ostride, int nbx, Slots sl)
 {
-    if (done && *done)
+    const int sb = slot_of(sl);
+    if (sb < 0)
         return;
+    if (sl.map) {  // batched launch: this slot's buffers
+        data += sb * sl.spec;
+    }
     constexpr int N  = N1 * N2;
     constexpr int LD = C + 1;
     __shared__ cplx<T> tile[N * LD];
@@ -268,10 +271,16 @@ constexpr bool kXGreenKeepInput = FANS_XG_KEEP_INPUT;
 template <typename T, int H, int N1, int N2, int KB, int C, bool SPLIT>
 __global__ __launch_bounds__(C *N2) void k_fft_x_green_reg(cplx<T> *__restrict__ data, const cplx<T> *__restrict__ tw,
                                                           GreenView<T> gv, int kz_lo, int kz_hi, int nbk,
-                                                          double *__restrict__ partial, const int *__restrict__ done)
+                                                          double *__restrict__ partial, Slots sl)
 {
-    if (done && *done)
+    const int sb = slot_of(sl);
+    if (sb < 0)
         return;
+    if (sl.map) {  // batched launch: this slot's buffers
+        data += sb * sl.spec;
+        gv.G += sb * sl.green;
+        partial = slot_ptr(partial, sb * sl.part);
+    }
     __shared__ double red[(C * N2) / 64];
     double            acc = 0.0;
     constexpr int  N    = N1 * N2;

@@ -23,13 +23,12 @@ enum {
     SC_TOL      = 8,
     SC_RES0     = 16,  // generic reduction results [16, 32)
     SC_ARING    = 32,  // linear CG: alpha of iteration j at SC_ARING + j % kDirRing
-    SC_COUNT    = 40
+    SC_COUNT    = kScSlot
 };
 // linear CG: search directions kept per iteration before the solution update
 // u -= sum alpha_j d_j is applied in one pass
 constexpr int kDirRing = 8;
-// device integer slots
-enum { IS_DONE = 0, IS_ITER = 1, IS_NIT = 2, IS_ERRTYPE = 3, IS_MEASURE = 4, IS_COUNT = 8 };
+// device integer slots: IS_* (common.h)
 
 enum {
     RED_ERR_INIT = 0,  // q0 = norm partials      -> err0, iter = 0, delta = 1, deltamid = 0
@@ -60,11 +59,23 @@ constexpr int kReduceThreads = 1024;
 
 // partial_b (nullable): quantity 1 comes from partial_b[0, nblk_b) instead of
 // partial[nblk, 2 nblk) (two producing kernels of different grids)
+// One block per slot of the launch (grid z); every buffer at its slot's offset.
 __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double *__restrict__ partial, int nblk, int nq,
                                                            int stage, double *sc, int *is, double *hist,
-                                                           const double *__restrict__ partial_b, int nblk_b)
+                                                           const double *__restrict__ partial_b, int nblk_b,
+                                                           Slots sl)
 {
     __shared__ double sm[2 * (kReduceThreads / 64)];
+    const int         sb = slot_of(sl, false);
+    if (sb < 0)
+        return;
+    if (sl.map) {  // batched launch: this slot's buffers
+        partial   = partial + sb * sl.part;
+        partial_b = slot_ptr(partial_b, sb * sl.part);
+        sc += (int64_t) sb * SC_COUNT;
+        is += (int64_t) sb * IS_COUNT;
+        hist += sb * sl.hist;
+    }
     const int measure = is[IS_MEASURE];
     if (stage != RED_ERR_INIT && stage != RED_SUM && is[IS_DONE])
         return;
@@ -176,9 +187,19 @@ __global__ __launch_bounds__(256) void k_dot(const T *__restrict__ a, const T *_
 // partial of r32 (slot 0); same grid and accumulation order as k_dot's norm
 __global__ __launch_bounds__(256) void k_inner_start(float *__restrict__ r32, float *__restrict__ x,
                                                      float *__restrict__ d, const double *__restrict__ r, int64_t n,
-                                                     int measure, double *__restrict__ partial)
+                                                     int measure, double *__restrict__ partial, Slots sl)
 {
     __shared__ double sm[8];
+    const int         sb = slot_of(sl, false);
+    if (sb < 0)
+        return;
+    if (sl.map) {  // batched launch: this slot's buffers
+        r32 += sb * sl.field;
+        x += sb * sl.field;
+        d += sb * sl.field;
+        r += sb * sl.field;
+        partial += sb * sl.part;
+    }
     double            a0 = 0.0;
     for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
         const float v = (float) r[i];
@@ -205,13 +226,25 @@ struct DirRing {
 
 // FULL: all kDirRing slots (count ignored; every load unconditional, so all of
 // them are in flight together)
+// count < 0: each slot's pending count, its iteration count mod kDirRing.
+// gate: skip finished slots.
 template <typename T, bool FULL>
 __global__ __launch_bounds__(256) void k_ring_update(T *__restrict__ u, DirRing<T> ring,
                                                     const double *__restrict__ alphas, int count, int64_t n,
-                                                    const int *__restrict__ done)
+                                                    Slots sl, int gate)
 {
-    if (done && *done)
+    const int sb = slot_of(sl, gate != 0);
+    if (sb < 0)
         return;
+    if (sl.map) {  // batched launch: this slot's buffers
+        u += sb * sl.field;
+#pragma unroll
+        for (int m = 0; m < kDirRing; ++m)
+            ring.d[m] += sb * sl.field;
+        alphas += (int64_t) sb * SC_COUNT;
+    }
+    if (count < 0)
+        count = sl.is[sb * IS_COUNT + IS_ITER] % kDirRing;
     T a[kDirRing];
 #pragma unroll
     for (int m = 0; m < kDirRing; ++m)
@@ -241,10 +274,18 @@ __global__ __launch_bounds__(256) void k_stream_copy(double2 *__restrict__ dst, 
         dst[i] = src[i];
 }
 
-// y = a * x + b * y  (a, b from host)
+// y = a * x + b * y  (a, b from host), n per slot
 template <typename T, typename TX>
-__global__ __launch_bounds__(256) void k_axpby(T *__restrict__ y, const TX *__restrict__ x, double a, double b, int64_t n)
+__global__ __launch_bounds__(256) void k_axpby(T *__restrict__ y, const TX *__restrict__ x, double a, double b, int64_t n,
+                                               Slots sl)
 {
+    const int sb = slot_of(sl, false);
+    if (sb < 0)
+        return;
+    if (sl.map) {  // batched launch: this slot's buffers
+        y += sb * sl.field;
+        x += sb * sl.field;
+    }
     for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
         y[i] = (T) (a * (double) x[i] + b * (double) y[i]);
 }

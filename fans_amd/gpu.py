@@ -64,7 +64,8 @@ EXPORTS = [
     "fans_gpu_get_green", "fans_gpu_get_reference_stiffness", "fans_gpu_fft_r2c", "fans_gpu_fft_c2r",
     "fans_gpu_fft_r2c_noalloc", "fans_gpu_stream_bandwidth",
     "fans_gpu_set_option", "fans_gpu_kernel_times", "fans_gpu_kernel_family_name", "fans_gpu_reset_state",
-    "fans_gpu_set_voxel_moduli", "fans_gpu_sensitivity",
+    "fans_gpu_set_voxel_moduli", "fans_gpu_sensitivity", "fans_gpu_create_batch", "fans_gpu_select_slot",
+    "fans_gpu_batch_results", "fans_gpu_batch_homogenized_stress",
 ]
 
 
@@ -78,6 +79,11 @@ def lib():
         L.fans_gpu_last_error.restype = C.c_char_p
         L.fans_gpu_last_error.argtypes = [vp]
         L.fans_gpu_create.argtypes = [C.POINTER(vp), C.POINTER(Desc)]
+        if hasattr(L, "fans_gpu_create_batch"):  # (FANS_GPU_LIB may name an older build)
+            L.fans_gpu_create_batch.argtypes = [C.POINTER(vp), C.POINTER(Desc), C.c_int32]
+            L.fans_gpu_select_slot.argtypes = [vp, C.c_int32]
+            L.fans_gpu_batch_results.argtypes = [vp, ip, ip, dp, C.c_int32]
+            L.fans_gpu_batch_homogenized_stress.argtypes = [vp, dp, C.c_int32, C.c_int32]
         L.fans_gpu_destroy.argtypes = [vp]
         L.fans_gpu_set_microstructure.argtypes = [vp, C.POINTER(C.c_uint16), i64]
         L.fans_gpu_set_materials.argtypes = [vp, C.c_int32, ip, ip, ip, dp, i64, dp]
@@ -200,7 +206,7 @@ def context_key(problem: Problem):
 class GpuSolver:
     """One load case on the GPU: the reference's (MaterialManager, Solver) pair."""
 
-    def __init__(self, problem: Problem, precision="fp64", device=0, timing=False):
+    def __init__(self, problem: Problem, precision="fp64", device=0, timing=False, slots=1):
         self.p = problem
         self.L = lib()
         self.h = problem.howmany
@@ -224,7 +230,10 @@ class GpuSolver:
         d.precision = PRECISION[precision]
         d.device = device
         self.ptr = C.c_void_p()
-        rc = self.L.fans_gpu_create(C.byref(self.ptr), C.byref(d))
+        if slots > 1:
+            rc = self.L.fans_gpu_create_batch(C.byref(self.ptr), C.byref(d), int(slots))
+        else:
+            rc = self.L.fans_gpu_create(C.byref(self.ptr), C.byref(d))
         if rc:
             msg = self._err()
             self.close()
@@ -465,3 +474,49 @@ class GpuSolver:
             name = self.L.fans_gpu_kernel_family_name(k).decode()
             res[name] = (float(buf[2 * k]), int(buf[2 * k + 1]))
         return res
+
+
+class GpuBatch(GpuSolver):
+    """Several RVEs of one creation setting (context_key) in one batched
+    context: solve() runs every RVE at once, each kernel launch covering every
+    RVE still iterating, with per-RVE results equal to solving each alone.
+    The GpuSolver methods address the selected RVE (select())."""
+
+    def __init__(self, problems, precision="mixed", device=0):
+        problems = list(problems)
+        if not problems:
+            raise ValueError("GpuBatch needs at least one problem")
+        self.slots = len(problems)
+        self.problems = list(problems)
+        super().__init__(problems[0], precision=precision, device=device, slots=self.slots)
+        for b in range(1, self.slots):
+            self.load_slot(b, problems[b])
+        self.select(0)
+
+    def select(self, slot):
+        self._check(self.L.fans_gpu_select_slot(self.ptr, int(slot)))
+        self.p = self.problems[slot]
+
+    def load_slot(self, slot, problem, g0=None):
+        """Microstructure, materials (fresh state) and optionally the gradient of
+        RVE `slot`; leaves it selected."""
+        self.select(slot)
+        self.load(problem)
+        self.problems[slot] = problem
+        if g0 is not None:
+            self.set_gradient(g0)
+
+    def results(self):
+        """Per RVE of the last solve: iterations (mixed form: FP32 inner
+        iterations), converged flags, final errors."""
+        it = np.zeros(self.slots, dtype=np.int32)
+        cv = np.zeros(self.slots, dtype=np.int32)
+        fe = np.zeros(self.slots)
+        self._check(self.L.fans_gpu_batch_results(self.ptr, _ip(it), _ip(cv), _dp(fe), self.slots))
+        return it, cv.astype(bool), fe
+
+    def stresses(self):
+        """Homogenized stress (flux) of every RVE, [slots, n_str]."""
+        out = np.zeros((self.slots, self.nstr))
+        self._check(self.L.fans_gpu_batch_homogenized_stress(self.ptr, _dp(out), self.nstr, self.slots))
+        return out

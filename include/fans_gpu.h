@@ -231,6 +231,29 @@ int fans_gpu_fft_c2r(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t
 int fans_gpu_fft_r2c_noalloc(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t h, const double *in,
                              double *out_cplx, void *stream);
 
+/* Batched contexts (no reference counterpart: the reference solves one RVE per
+ * run, main.cpp:10-41, and a batch of them is that run repeated).  A batched
+ * context holds `slots` RVEs (1..4096) of one creation setting (grid, physics,
+ * element, CG method, tolerances), each with its own microstructure, phase
+ * table (at most 256 phases, linear models), Green operator, gradient and
+ * solution.  The single-RVE entry points above address the selected slot
+ * (fans_gpu_select_slot; slot 0 after creation), except fans_gpu_solve, which
+ * solves every slot at once: each kernel launch covers every slot still
+ * running, and each slot takes the iterations and gets the results of a
+ * solve of that RVE alone.  Mixed boundary conditions, the tangent, per-voxel
+ * moduli, the sensitivity and per-kernel timing are refused on such a context.
+ * fans_gpu_solve's statistics are then the largest iteration counts and final
+ * error over the slots (converged: all slots). */
+int fans_gpu_create_batch(fans_gpu_ctx **out, const fans_gpu_desc *desc, int32_t slots);
+int fans_gpu_select_slot(fans_gpu_ctx *ctx, int32_t slot);
+/* per slot of the last solve (count = slots; any output may be NULL): CG
+ * iterations (mixed form: FP32 inner iterations), converged flag, final error */
+int fans_gpu_batch_results(fans_gpu_ctx *ctx, int32_t *iterations, int32_t *converged, double *final_error,
+                           int32_t count);
+/* Solver::get_homogenized_stress (solver.h:707-737) of every slot in one
+ * launch: out [count][n_str] */
+int fans_gpu_batch_homogenized_stress(fans_gpu_ctx *ctx, double *out, int32_t n_str, int32_t count);
+
 /* measurement helper (no reference counterpart): average milliseconds of one
  * device-to-device copy of `bytes` (>= 1 MiB) by the library's own streaming
  * kernel (16 bytes per lane, contiguous chunks per workgroup), over `reps`

@@ -66,3 +66,49 @@ def test_load_rejects_other_context_settings():
             g.load(replace(p, L=(2.0, 1.0, 1.0)))
     finally:
         g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [32, 48])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_context_matches_single_solves(precision, n):
+    """One batched context solving 5 RVEs at once gives every RVE the iteration
+    count, homogenized response and displacement field of solving it alone,
+    bit for bit (same kernels and summation order; the RVEs finish at different
+    iterations, so the launch grid shrinks on the way).  n = 32 runs the
+    compile-time z and register y / x kernels, n = 48 the generic ones."""
+    from fans_amd.gpu import FansGpuError, GpuBatch, GpuSolver
+    probs = config5_rves(count=10, seed=11, edges=(n,))
+    for phys in (0, 1):
+        ps = probs[phys::2]
+        g0 = ps[0].load_cases[0].g0_path[0]
+        print(f"[gpu-case] batched context {precision} n={n} howmany={ps[0].howmany} slots={len(ps)}", flush=True)
+        single = []
+        for p in ps:
+            g = GpuSolver(p, precision=precision)
+            try:
+                g.set_gradient(g0)
+                g.solve()
+                single.append((int(g.stats.inner_iterations), g.homogenized_stress(), g.get_u()))
+            finally:
+                g.close()
+        bt = GpuBatch(ps, precision=precision)
+        try:
+            for rep in range(2):  # the second solve replays the captured graphs
+                for b, p in enumerate(ps):
+                    bt.load_slot(b, p, g0)
+                bt.solve()
+                its, conv, _ = bt.results()
+                st = bt.stresses()
+                for b in range(len(ps)):
+                    assert conv[b]
+                    assert its[b] == single[b][0], (rep, b, its, [s[0] for s in single])
+                    assert np.array_equal(st[b], single[b][1]), (rep, b)
+                    bt.select(b)
+                    assert np.array_equal(bt.homogenized_stress(), single[b][1])
+                    assert np.array_equal(bt.get_u(), single[b][2])
+            assert len(set(int(v) for v in its)) > 1  # the slots finished apart
+            with pytest.raises(FansGpuError, match="batched"):
+                bt.set_voxel_moduli(np.ones(int(np.prod(ps[0].dims))))
+        finally:
+            bt.close()
