@@ -200,8 +200,14 @@ def run_batch_workload(args):
     from fans_amd.batch import config5_rves, solve_batch
     print("[bench] generating the 384-RVE batch", file=sys.stderr, flush=True)
     problems = config5_rves()
+    workers = None
+    if args.batch_group:
+        # one (edge, howmany) group alone, in the calling thread
+        e, h = (int(v) for v in args.batch_group.split(","))
+        problems = [p for p in problems if p.dims[0] == e and p.howmany == h]
+        workers = 1
     print("[bench] batch: setup + autotune, then the timed solves", file=sys.stderr, flush=True)
-    rep = solve_batch(problems, precision="mixed")
+    rep = solve_batch(problems, precision="mixed", workers=workers)
     its = [r.iterations for r in rep.results]
     scaling = None
     if args.batch_workers:
@@ -219,7 +225,9 @@ def run_batch_workload(args):
             scaling.append({"workers": w, "wall_ms": round(1000 * rw.wall_s, 1), "rves_per_s": round(rw.rves_per_s, 1),
                             "sum_solve_ms": round(sm, 1),
                             "solve_ms_per_rve": {k: round(float(np.mean(v)), 3) for k, v in sorted(by.items())}})
-    out = {"metric": "RVEs/s (384 uneven RVEs, mixed form, tol 1e-8)", "value": round(rep.rves_per_s, 2),
+    out = {"metric": "RVEs/s (384 uneven RVEs, mixed form, tol 1e-8)" if not args.batch_group else
+           f"RVEs/s ({len(problems)} RVEs of the 384-RVE batch: edge,howmany {args.batch_group})",
+           "value": round(rep.rves_per_s, 2),
            "unit": "RVEs/s", "n_gpus": 1, "steps": 1, "warmup": 0,
            "ms_per_step": round(1000 * rep.wall_s, 1), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
@@ -229,7 +237,8 @@ def run_batch_workload(args):
            "voxel_iters_per_s": round(rep.voxel_iters_per_s, 1),
            "iterations": {"min": int(min(its)), "mean": round(float(np.mean(its)), 1), "max": int(max(its))},
            "converged": int(sum(r.converged for r in rep.results)),
-           "autotuned_asm_planes": {f"{k[0][0]}^3 h={k[1]}": v for k, v in rep.tuned.items()}}
+           "autotuned_tiles": {f"{k[0][0]}^3 h={k[1]}": {"asm_planes": v[0], "fft_lds_kib": v[1]}
+                               for k, v in rep.tuned.items()}}
     out["units"] = sorted(rep.units, key=lambda u: u["group"])
     if args.batch_single:
         # the same batch with one single-RVE context per group (RVEs in turn)
@@ -499,6 +508,8 @@ def main():
                     help="batch: the 384-RVE uneven-size batch (RVEs/s); j2: the J2 load-step ramp; accuracy: "
                          "per-configuration errors, mixed vs FP64 and FP64 vs oracle; none is the headline metric")
     ap.add_argument("--j2-steps", type=int, default=20, help="load steps of the j2 workload (of 20)")
+    ap.add_argument("--batch-group", default="",
+                    help="batch workload: only the RVEs of edge E and howmany H ('E,H'), solved in the calling thread")
     ap.add_argument("--batch-single", action="store_true",
                     help="batch workload: also run it with one single-RVE context per group (RVEs in turn)")
     ap.add_argument("--batch-workers", default="",

@@ -9,8 +9,9 @@ every kernel launch covering each RVE still iterating (the captured CG
 iterations replay for the whole group; finished RVEs leave the launch grid).
 Groups run concurrently, one host thread and one HIP stream each (the C ABI
 calls release the interpreter lock).  Before the timed batch, a small
-autotuner times the stencil tile depth (x planes per assembly workgroup) on
-each group's first RVE and keeps the fastest.  With batched=False each group
+autotuner times the FFT tile width of the generic passes (their LDS budget per
+workgroup) and the stencil tile depth (x planes per assembly workgroup) on
+each group's context and keeps the fastest.  With batched=False each group
 is one single-RVE context that loads and solves its RVEs in turn.
 
 The synthetic batch of the differentiation-free throughput benchmark (384 RVEs,
@@ -100,24 +101,37 @@ def config5_rves(count=384, seed=0, tol=1e-8, edges=EDGES):
     return out
 
 
-def _autotune(solver: GpuSolver, g0, candidates=(4, 8, 16), repeats=1):
-    """Pick the assembly tile depth with the smallest solve time."""
-    best, best_ms = None, float("inf")
-    for cx in candidates:
-        if cx > solver.dims[0]:
-            continue
-        solver.set_asm_planes(cx)
+# generic FFT passes: lengths other than the powers of two 16..256
+def _generic_fft(dims):
+    return any(n not in (16, 32, 64, 128, 256) for n in (dims[0], dims[1], dims[2] // 2))
+
+
+def _autotune(solver: GpuSolver, fresh, planes=(4, 8, 16), fft_kib=(16, 24, 40, 60), repeats=1):
+    """Tile settings with the smallest solve time (the library's device time
+    of the solve, after `fresh()` has set up a new one): the FFT LDS budget of
+    the generic passes (when the grid has such lengths), then the assembly tile
+    depth.  Returns (planes, fft_kib)."""
+    def timed():
         ms = []
         for _ in range(repeats):
-            solver.reset_state()
-            solver.set_gradient(g0)
+            fresh()
             solver.solve()
             ms.append(solver.stats.solve_ms)
-        t = min(ms)
-        if t < best_ms:
-            best, best_ms = cx, t
-    solver.set_asm_planes(best)
-    return best
+        return min(ms)
+
+    def pick(values, setter):
+        best, best_ms = None, float("inf")
+        for v in values:
+            setter(v)
+            t = timed()
+            if t < best_ms:
+                best, best_ms = v, t
+        setter(best)
+        return best
+
+    kib = pick(fft_kib, solver.set_fft_lds_kib) if _generic_fft(solver.dims) else 40
+    cx = pick([c for c in planes if c <= solver.dims[0]], solver.set_asm_planes)
+    return cx, kib
 
 
 def solve_batch(problems, precision="mixed", workers=None, autotune=True, batched=True, max_slots=64):
@@ -135,31 +149,47 @@ def solve_batch(problems, precision="mixed", workers=None, autotune=True, batche
     report = BatchReport()
 
     # setup, outside the timed region like the reference's plan creation:
-    # the contexts, and the tile depth autotuned on each group's first RVE
+    # the contexts and their autotuned tiles
     solvers = {}
 
     def setup(unit):
         key, idx = unit
         first = problems[idx[0]]
-        if autotune and key not in report.tuned:
-            t = GpuSolver(first, precision=precision)
-            try:
-                report.tuned[key] = _autotune(t, first.load_cases[0].g0_path[0])
-            finally:
-                t.close()
-        s = GpuBatch([problems[i] for i in idx], precision=precision) if batched else \
-            GpuSolver(first, precision=precision)
+        g0 = [problems[i].load_cases[0].g0_path[0] for i in idx]
+        if batched:
+            s = GpuBatch([problems[i] for i in idx], precision=precision)
+
+            def fresh():
+                for b, i in enumerate(idx):
+                    s.load_slot(b, problems[i], g0[b])
+        else:
+            s = GpuSolver(first, precision=precision)
+
+            def fresh():
+                s.reset_state()
+                s.set_gradient(g0[0])
         if autotune:
-            s.set_asm_planes(report.tuned[key])
+            # tuned on the context itself (a batched one: its whole chunk)
+            if key not in report.tuned:
+                report.tuned[key] = _autotune(s, fresh)
+            cx, kib = report.tuned[key]
+            s.set_asm_planes(cx)
+            s.set_fft_lds_kib(kib)
         solvers[unit] = s
+
+    loaded = {}
+
+    def load_unit(unit):
+        s, (_, idx) = solvers[unit], unit
+        t0 = time.perf_counter()
+        for b, i in enumerate(idx):
+            s.load_slot(b, problems[i], problems[i].load_cases[0].g0_path[0])
+        loaded[unit] = (t0, time.perf_counter())
 
     def run_unit(unit):
         s, (_, idx) = solvers[unit], unit
         res = []
         if batched:
-            t0 = time.perf_counter()
-            for b, i in enumerate(idx):
-                s.load_slot(b, problems[i], problems[i].load_cases[0].g0_path[0])
             t1 = time.perf_counter()
             s.solve()
             t2 = time.perf_counter()
@@ -167,8 +197,9 @@ def solve_batch(problems, precision="mixed", workers=None, autotune=True, batche
             stress = s.stresses()
             t3 = time.perf_counter()
             p0 = problems[idx[0]]
+            l0, l1 = loaded[unit]
             report.units.append({"group": f"{p0.dims[0]}^3 h={p0.howmany}", "rves": len(idx),
-                                 "load_ms": round(1e3 * (t1 - t0), 2), "solve_ms": round(1e3 * (t2 - t1), 2),
+                                 "load_ms": round(1e3 * (l1 - l0), 2), "solve_ms": round(1e3 * (t2 - t1), 2),
                                  "device_solve_ms": round(float(s.stats.solve_ms), 2),
                                  "results_ms": round(1e3 * (t3 - t2), 2), "max_iterations": int(max(its))})
             for b, i in enumerate(idx):
@@ -192,9 +223,21 @@ def solve_batch(problems, precision="mixed", workers=None, autotune=True, batche
             setup(unit)
         t0 = time.perf_counter()
         nw = workers if workers else min(len(order), 16)
-        with ThreadPoolExecutor(max_workers=max(1, nw)) as ex:
-            for res in ex.map(run_unit, order):
+        # batched: every context loads its RVEs (host-bound table and Green
+        # operator setup), then all solve; loads that ran beside other
+        # contexts' solves waited on the busy GPU at every synchronisation.
+        # One worker runs in the calling thread.
+        if nw <= 1:
+            if batched:
+                list(map(load_unit, order))
+            for res in map(run_unit, order):
                 report.results.extend(res)
+        else:
+            with ThreadPoolExecutor(max_workers=nw) as ex:
+                if batched:
+                    list(ex.map(load_unit, order))
+                for res in ex.map(run_unit, order):
+                    report.results.extend(res)
         report.wall_s = time.perf_counter() - t0
     finally:
         for s in solvers.values():

@@ -135,6 +135,9 @@ struct Fft3D {
     cplx<T>      *twz = nullptr, *twy = nullptr, *twx = nullptr;
     int           lpb = 1, bc = 8, bk = 2, bcx = 8;
     size_t        smz = 0, smy = 0, smx = 0, smx_plain = 0;
+    // LDS budget of one workgroup of the generic (LDS-tiled) passes; it sets
+    // their lines / columns per workgroup (FANS_OPT_FFT_LDS_KIB)
+    int           lds_kib = 40;
 
     // largest power of two <= min(limit, cap), at least 1
     static int pow2_cols(int limit, int cap)
@@ -183,8 +186,9 @@ struct Fft3D {
         const size_t cb = sizeof(cplx<T>);
         if ((int64_t) M * h > FFT_MAX_ELEMS || ny > FFT_MAX_ELEMS || (int64_t) nx * h > FFT_MAX_ELEMS)
             return hipErrorInvalidValue;
-        // ~40 KiB of LDS per workgroup keeps four workgroups resident per CU
-        const int lds_elems = (int) (40 * 1024 / cb);
+        // the default 40 KiB of LDS per workgroup keeps four workgroups
+        // resident per CU
+        const int lds_elems = (int) ((int64_t) lds_kib * 1024 / cb);
         const int lim       = std::min(FFT_MAX_ELEMS, lds_elems);
         lpb                 = std::max(1, std::min(16, lim / (M * h)));
         smz                 = cb * M * (lpb * h + 1);
@@ -304,6 +308,7 @@ struct fans_gpu_ctx {
     double       *partial = nullptr;
     int64_t       partial_len = 0;
     double       *partial_e   = nullptr;  // [nblk_z] of the linear CG's inverse z pass
+    int64_t       partial_e_len = 0;
     double       *sc = nullptr;
     int          *is = nullptr;
     double       *hist = nullptr;
@@ -576,13 +581,10 @@ Slots launch_slots(const fans_gpu_ctx *c, bool gated)
     return s;
 }
 
-// launch the given slots from now on: grid z is the next power of two (graphs
-// are captured per grid), the map entries past the slots are -1
+// launch the given slots from now on, one grid z plane each
 int set_map(fans_gpu_ctx *c, const std::vector<int> &slots)
 {
-    int gz = 1;
-    while (gz < (int) slots.size())
-        gz *= 2;
+    const int gz = std::max<int>(1, (int) slots.size());
     c->map_h.assign(gz, -1);
     std::copy(slots.begin(), slots.end(), c->map_h.begin());
     c->map_slots = slots;
@@ -1232,7 +1234,7 @@ int read_is_all(fans_gpu_ctx *c, std::vector<int> &is)
 // selected RVE, or batched (c->use_map) of every slot of the map.  A batched
 // loop runs its slots in lockstep, each stopping on its own test (its
 // launches return at entry from then on); the finished slots leave the map
-// whenever the remaining ones fit a grid half as deep.
+// once at most half of its slots still run.
 template <typename T>
 int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *const *ring, T *rnew, T *u, cplx<T> *spec, const T *Gp,
                    const Phase<T> *ph, int *iters_out)
@@ -1277,11 +1279,9 @@ int cg_linear_loop(fans_gpu_ctx *c, Fft3D<T> &P, T *r, T *const *ring, T *rnew, 
         for (int b : c->map_slots)
             if (!isall[(size_t) b * IS_COUNT + IS_DONE])
                 act.push_back(b);
-        stop   = act.empty();
-        int gz = 1;
-        while (gz < (int) act.size())
-            gz *= 2;
-        return (!stop && gz < c->gz) ? set_map(c, act) : 0;
+        stop = act.empty();
+        // a new grid depth means a new captured graph: shrink by halves
+        return (!stop && 2 * (int) act.size() <= c->gz) ? set_map(c, act) : 0;
     };
     // iterations 0 .. k-1 have run; those after the last full ring are pending
     auto finish = [&]() -> int {
@@ -1901,6 +1901,49 @@ SolveResult slot_result(const fans_gpu_ctx *c, int b)
     return b == c->cur ? solve_result(c, *c) : solve_result(c, c->parked[b]);
 }
 
+// partial sums: the largest grid of any reducing kernel times its quantities;
+// any_tiles: for every assembly tile depth and FFT LDS budget (one line or
+// frequency per FFT workgroup, one x plane per assembly workgroup)
+int64_t partial_need(const fans_gpu_ctx *c, bool any_tiles)
+{
+    const int64_t asm_blocks = any_tiles ? (int64_t) c->nx * c->nty * c->ntz : c->nblk_asm;
+    int64_t       fft_blocks = std::max<int64_t>({c->fft64.nblk_z, c->fft32.nblk_z, x_green_blocks(c->fft64, c->h, c->ny),
+                                                  x_green_blocks(c->fft32, c->h, c->ny)});
+    if (any_tiles)
+        fft_blocks = std::max<int64_t>({fft_blocks, (int64_t) c->nx * c->ny, (int64_t) (c->nz / 2 + 1) * c->ny});
+    int64_t need = 2 * std::max<int64_t>({asm_blocks, (int64_t) c->nblk_vec, fft_blocks});
+    need         = std::max<int64_t>(need, (int64_t) c->nstr * c->nblk_el);
+    return std::max<int64_t>(need, 6 * 1024);  // k_moduli_stats: 6 quantities, <= 1024 blocks
+}
+
+// single-RVE contexts: grow the partial buffers to a new launch shape
+int ensure_partials(fans_gpu_ctx *c)
+{
+    const int64_t need   = partial_need(c, false);
+    const int64_t need_e = std::max<int64_t>(c->fft64.nblk_z, c->fft32.nblk_z);
+    if (c->nb > 1)
+        return (need > c->partial_len || need_e > c->partial_e_len)
+                   ? fail(c, FANS_GPU_EINVAL, "launch shape outside the batched context's partial buffers")
+                   : 0;
+    if (need > c->partial_len || need_e > c->partial_e_len) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (need > c->partial_len) {
+        HIPCHK(c, hipFree(c->partial));
+        c->partial = nullptr;
+        HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * need));
+        c->partial_len   = need;
+        c->strides.part  = need;
+    }
+    if (need_e > c->partial_e_len) {
+        HIPCHK(c, hipFree(c->partial_e));
+        c->partial_e = nullptr;
+        HIPCHK(c, hipMalloc(&c->partial_e, sizeof(double) * need_e));
+        c->partial_e_len = need_e;
+    }
+    return 0;
+}
+
 // every per-RVE buffer nb times (batched contexts: slot b at b * stride);
 // a batched context also allocates now what a single one allocates on first
 // use (Green operators, phase tables, the CG direction ring), so that no slot
@@ -1940,21 +1983,15 @@ int alloc_fields(fans_gpu_ctx *c)
         HIPCHK(c, hipMalloc(&c->x32, fb));
         HIPCHK(c, hipMalloc(&c->spec32, sizeof(cplx<float>) * c->Nf * c->h * nb));
     }
-    // partial sums: the largest grid of any reducing kernel times its
-    // quantities (batched: for any assembly tile depth, so that the per-slot
-    // stride never changes)
-    const int64_t asm_blocks = nb > 1 ? (int64_t) c->nx * c->nty * c->ntz : c->nblk_asm;
-    int64_t       need = 2 * std::max<int64_t>({asm_blocks, (int64_t) c->nblk_vec, c->fft64.nblk_z, c->fft32.nblk_z,
-                                                x_green_blocks(c->fft64, c->h, c->ny),
-                                                x_green_blocks(c->fft32, c->h, c->ny)});
-    need         = std::max<int64_t>(need, (int64_t) c->nstr * c->nblk_el);
-    need         = std::max<int64_t>(need, 6 * 1024);  // k_moduli_stats: 6 quantities, <= 1024 blocks
-    c->partial_len = need;
+    // partial sums (batched: for any tile setting, so that the per-slot
+    // stride never changes); partial_e holds the per-block partials of the
+    // linear CG's inverse z pass, kept apart because the matvec that follows
+    // writes `partial` (batched: at the same stride)
+    const int64_t need = partial_need(c, nb > 1);
+    c->partial_len     = need;
     HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * need * nb));
-    // per-block partials of the linear CG's inverse z pass, kept apart because
-    // the matvec that follows writes `partial` (batched: at the same stride)
-    HIPCHK(c, hipMalloc(&c->partial_e, sizeof(double) *
-                                           (nb > 1 ? need * nb : std::max<int64_t>(c->fft64.nblk_z, c->fft32.nblk_z))));
+    c->partial_e_len = nb > 1 ? need : std::max<int64_t>(c->fft64.nblk_z, c->fft32.nblk_z);
+    HIPCHK(c, hipMalloc(&c->partial_e, sizeof(double) * c->partial_e_len * nb));
     const int npk      = c->h * (c->h + 1) / 2;
     c->strides.field   = c->Nh;
     c->strides.spec    = c->Nf * c->h;
@@ -3662,19 +3699,24 @@ int fans_gpu_set_option(fans_gpu_ctx *c, int32_t opt, int64_t value)
         // x planes per assembly workgroup (tile shape of the stencil pass)
         if (value < 1 || value > c->nx)
             return fail(c, FANS_GPU_EINVAL, "assembly planes per block must be in [1, n_x]");
-        c->cx               = (int) value;
-        const int     nbx   = (c->nx + c->cx - 1) / c->cx;
-        c->nblk_asm         = nbx * c->nty * c->ntz;
-        const int64_t need  = 2 * (int64_t) c->nblk_asm;
-        if (need > c->partial_len) {
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            HIPCHK(c, hipFree(c->partial));
-            c->partial = nullptr;
-            HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * need));
-            c->partial_len = need;
-        }
+        c->cx           = (int) value;
+        const int nbx   = (c->nx + c->cx - 1) / c->cx;
+        c->nblk_asm     = nbx * c->nty * c->ntz;
         drop_graphs(c);
-        return 0;
+        return ensure_partials(c);
+    }
+    if (opt == FANS_OPT_FFT_LDS_KIB) {
+        // LDS budget of the generic FFT passes' workgroups (lengths other
+        // than the powers of two 16..256, whose kernels fix their tiles)
+        if (value < 8 || value > 64)
+            return fail(c, FANS_GPU_EINVAL, "FFT LDS budget must be in [8, 64] KiB");
+        c->fft64.lds_kib = (int) value;
+        c->fft32.lds_kib = (int) value;
+        if (c->fft64.plan(c->nx, c->ny, c->nz, c->h) != hipSuccess ||
+            (c->fft32.twz && c->fft32.plan(c->nx, c->ny, c->nz, c->h) != hipSuccess))
+            return fail(c, FANS_GPU_EINVAL, "FFT plan failed");
+        drop_graphs(c);
+        return ensure_partials(c);
     }
     if (opt == FANS_OPT_OPERATOR_FP32) {
         if (value && !c->r32)

@@ -29,6 +29,7 @@ OPT_FFT_CHUNKS = 2
 OPT_GRAPHS = 3
 OPT_ASM_PLANES = 4
 OPT_OPERATOR_FP32 = 5
+OPT_FFT_LDS_KIB = 6
 ERRORS = {-1: "EINVAL", -2: "ENODEV", -3: "EHIP", -4: "ESTATE", -5: "EUNSUP"}
 
 
@@ -340,6 +341,11 @@ class GpuSolver:
     def set_asm_planes(self, p):
         """x planes per assembly workgroup (stencil tile depth)."""
         self._check(self.L.fans_gpu_set_option(self.ptr, OPT_ASM_PLANES, int(p)))
+
+    def set_fft_lds_kib(self, kib):
+        """LDS budget (KiB) of one workgroup of the generic FFT passes (tile
+        width of lengths other than the powers of two 16..256)."""
+        self._check(self.L.fans_gpu_set_option(self.ptr, OPT_FFT_LDS_KIB, int(kib)))
 
     def set_operator_fp32(self, on=True):
         """convolve() and residual(linear_kd=True) run the mixed form's FP32

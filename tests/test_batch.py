@@ -112,3 +112,31 @@ def test_batched_context_matches_single_solves(precision, n):
                 bt.set_voxel_moduli(np.ones(int(np.prod(ps[0].dims))))
         finally:
             bt.close()
+
+
+@pytest.mark.gpu
+def test_fft_tile_budget_keeps_results():
+    """The FFT tile width of the generic passes (LDS budget per workgroup, the
+    autotuner's knob) moves only the order of the partial sums: iterations and
+    homogenized response stay put across the budgets."""
+    from fans_amd.gpu import FansGpuError, GpuSolver
+    p = config5_rves(count=2, seed=4, edges=(48,))[1]
+    g0 = p.load_cases[0].g0_path[0]
+    print(f"[gpu-case] FFT LDS budgets on {p.dims} howmany={p.howmany}", flush=True)
+    g = GpuSolver(p, precision="fp64")
+    try:
+        res = {}
+        for kib in (16, 24, 40, 60):
+            g.set_fft_lds_kib(kib)
+            g.reset_state()
+            g.set_gradient(g0)
+            g.solve()
+            res[kib] = (g.stats.iterations, g.homogenized_stress())
+        it0, s0 = res[40]
+        for kib, (it, st) in res.items():
+            assert abs(it - it0) <= 1, (kib, it, it0)
+            assert np.max(np.abs(st - s0)) <= 1e-10 * np.max(np.abs(s0)), kib
+        with pytest.raises(FansGpuError, match="LDS"):
+            g.set_fft_lds_kib(200)
+    finally:
+        g.close()
