@@ -35,7 +35,8 @@ enum {
     RED_DELTA    = 1,  // q0 = <r,s> partials     -> delta, beta
     RED_ALPHA    = 2,  // q0 = <d,rnew>, q1 = <d,rnew_prev> (nq 2) -> alpha (and deltamid)
     RED_ERR_NEXT = 4,  // q0 = norm partials      -> err, iter++
-    RED_SUM      = 5   // q0..q(nq-1) sums        -> SC_RES0 + q
+    RED_SUM      = 5,  // q0..q(nq-1) sums        -> SC_RES0 + q
+    RED_ERR_SET  = 6   // q0 = norm partials      -> err, stop test (iteration and CG scalars kept)
 };
 
 __device__ __forceinline__ double finish_norm(double v, int measure) { return measure == 1 ? sqrt(v) : v; }
@@ -77,14 +78,14 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double *__restr
         hist += sb * sl.hist;
     }
     const int measure = is[IS_MEASURE];
-    if (stage != RED_ERR_INIT && stage != RED_SUM && is[IS_DONE])
+    if (stage != RED_ERR_INIT && stage != RED_SUM && stage != RED_ERR_SET && is[IS_DONE])
         return;
     double v[2] = {0.0, 0.0};
     bool   mx[2];
     for (int q = 0; q < nq; q += 2) {
         const int nqq = (nq - q) < 2 ? 1 : 2;
         for (int k = 0; k < 2; ++k) {
-            const bool norm_slot = (k + q == 0) && (stage == RED_ERR_INIT || stage == RED_ERR_NEXT);
+            const bool norm_slot = (k + q == 0) && (stage == RED_ERR_INIT || stage == RED_ERR_NEXT || stage == RED_ERR_SET);
             mx[k]                = norm_slot && measure == 2;
             double a[4] = {0.0, 0.0, 0.0, 0.0};
             if (k < nqq) {
@@ -129,6 +130,8 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double *__restr
                     sc[SC_DELTAMID] = sc[SC_DELTA] - sc[SC_ALPHA] * (v[0] - sc[SC_BETA] * v[1]);
             } else if (stage == RED_ERR_NEXT) {
                 is[IS_ITER] += 1;
+                record_error(sc, is, hist, finish_norm(v[0], measure));
+            } else if (stage == RED_ERR_SET) {
                 record_error(sc, is, hist, finish_norm(v[0], measure));
             }
         }
@@ -183,8 +186,9 @@ __global__ __launch_bounds__(256) void k_dot(const T *__restrict__ a, const T *_
 }
 
 // start of an inner FP32 solve of the mixed form in one pass: r32 = (float) r,
-// x = 0, d = 0 (solverCG's zero search direction) and per block the norm
-// partial of r32 (slot 0); same grid and accumulation order as k_dot's norm
+// x = 0, d = 0 (solverCG's zero search direction; d null: kept) and per block
+// the norm partial of r32 (slot 0); same grid and accumulation order as
+// k_dot's norm
 __global__ __launch_bounds__(256) void k_inner_start(float *__restrict__ r32, float *__restrict__ x,
                                                      float *__restrict__ d, const double *__restrict__ r, int64_t n,
                                                      int measure, double *__restrict__ partial, Slots sl)
@@ -196,7 +200,7 @@ __global__ __launch_bounds__(256) void k_inner_start(float *__restrict__ r32, fl
     if (sl.map) {  // batched launch: this slot's buffers
         r32 += sb * sl.field;
         x += sb * sl.field;
-        d += sb * sl.field;
+        d = slot_ptr(d, sb * sl.field);
         r += sb * sl.field;
         partial += sb * sl.part;
     }
@@ -205,7 +209,8 @@ __global__ __launch_bounds__(256) void k_inner_start(float *__restrict__ r32, fl
         const float v = (float) r[i];
         r32[i]        = v;
         x[i]          = 0.f;
-        d[i]          = 0.f;
+        if (d)
+            d[i] = 0.f;
         acc_norm(a0, v, measure);
     }
     double     v[1]  = {a0};

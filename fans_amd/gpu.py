@@ -30,6 +30,7 @@ OPT_GRAPHS = 3
 OPT_ASM_PLANES = 4
 OPT_OPERATOR_FP32 = 5
 OPT_FFT_LDS_KIB = 6
+OPT_MIXED_RESTART = 7
 ERRORS = {-1: "EINVAL", -2: "ENODEV", -3: "EHIP", -4: "ESTATE", -5: "EUNSUP"}
 
 
@@ -346,6 +347,11 @@ class GpuSolver:
         """LDS budget (KiB) of one workgroup of the generic FFT passes (tile
         width of lengths other than the powers of two 16..256)."""
         self._check(self.L.fans_gpu_set_option(self.ptr, OPT_FFT_LDS_KIB, int(kib)))
+
+    def set_mixed_restart(self, on=True):
+        """Mixed form: restart the FP32 CG in every outer round (the batched
+        contexts' algorithm) instead of replacing its residual."""
+        self._check(self.L.fans_gpu_set_option(self.ptr, OPT_MIXED_RESTART, int(bool(on))))
 
     def set_operator_fp32(self, on=True):
         """convolve() and residual(linear_kd=True) run the mixed form's FP32

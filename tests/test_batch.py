@@ -73,8 +73,9 @@ def test_load_rejects_other_context_settings():
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batched_context_matches_single_solves(precision, n):
     """One batched context solving 5 RVEs at once gives every RVE the iteration
-    count, homogenized response and displacement field of solving it alone,
-    bit for bit (same kernels and summation order; the RVEs finish at different
+    count, homogenized response and displacement field of solving it alone
+    (mixed form: with restarted inner solves, the batched algorithm), bit for
+    bit (same kernels and summation order; the RVEs finish at different
     iterations, so the launch grid shrinks on the way).  n = 32 runs the
     compile-time z and register y / x kernels, n = 48 the generic ones."""
     from fans_amd.gpu import FansGpuError, GpuBatch, GpuSolver
@@ -87,6 +88,8 @@ def test_batched_context_matches_single_solves(precision, n):
         for p in ps:
             g = GpuSolver(p, precision=precision)
             try:
+                if precision == "mixed":
+                    g.set_mixed_restart(True)  # the batched contexts' inner solves
                 g.set_gradient(g0)
                 g.solve()
                 single.append((int(g.stats.inner_iterations), g.homogenized_stress(), g.get_u()))
