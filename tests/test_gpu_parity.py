@@ -301,6 +301,35 @@ def test_mixed_precision_elastic():
     assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-8
 
 
+@pytest.mark.parametrize("restart", [False, True])
+@pytest.mark.parametrize("err_type", ["absolute", "relative"])
+def test_mixed_forms_reach_the_fp64_solution(restart, err_type):
+    """Both inner-solve schedules of the mixed form (one FP32 CG whose residual
+    is replaced by the true FP64 one at each outer check, or a new FP32 CG per
+    outer round) stop on the true FP64 residual and agree with the oracle, for
+    absolute and relative error types (the inner floor follows the target)."""
+    from dataclasses import replace
+    p = replace(cases.elastic(n=32, seed=5, count=10, rmin=3, rmax=6), err_type=err_type,
+                tol=1e-10 if err_type == "absolute" else 1e-9)
+    say(f"mixed form {err_type} restart={restart} 32^3")
+    g0 = np.array([1.0, 0, 0, 0, 0, 0])
+    o = oracle.Oracle(p)
+    o.set_gradient(g0)
+    it_o = o.solve()
+    g = gpu.GpuSolver(p, precision="mixed")
+    g.set_mixed_restart(restart)
+    g.set_gradient(g0)
+    g.solve()
+    h = g.error_history()  # the true FP64 errors: initial, then one per outer check
+    assert len(h) == g.stats.outer_iterations + 1 and g.stats.outer_iterations >= 1
+    assert (h[-1] if err_type == "absolute" else h[-1] / h[0]) <= p.tol
+    assert g.stats.converged
+    # one FP32 CG tracks the FP64 one; restarts lose a few iterations
+    assert g.stats.inner_iterations <= it_o + (6 if restart else 3)
+    assert relerr(g.homogenized_stress(), o.homogenized_stress()) < 1e-8
+    g.close()
+
+
 def test_j2_mixed_bc_steps():
     """J2 matrix + elastic spheres, uniaxial strain with zero lateral stress (config-3 control law)."""
     p = cases.j2(n=12, seed=1)
