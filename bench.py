@@ -463,10 +463,10 @@ def run_accuracy_workload(args):
         "mixed_vs_fp64_fields_adjoint_state": _field_errs(res4["mixed"][2], res4["fp64"][2])}
     del res4
 
-    # config 5: every RVE of the batch, its mixed form (restarted inner
-    # solves, bit for bit the batched contexts' results) against FP64 at
-    # 1e-10 and against FP64 at the batch's own tolerance; the first RVE of
-    # each (edge, physics) group against the oracle
+    # config 5: every RVE of the batch, its mixed form (bit for bit the
+    # batched contexts' results) against FP64 at 1e-10 and against FP64 at the
+    # batch's own tolerance; the first RVE of each (edge, physics) group
+    # against the oracle
     print("[bench] accuracy: config 5 (384 RVEs)", file=sys.stderr, flush=True)
     probs = config5_rves()
     worst, worst_same, worst64, worst_o, seen = 0.0, 0.0, 0.0, 0.0, set()
@@ -475,8 +475,6 @@ def run_accuracy_workload(args):
         sig = {}
         for tag, prec, tol in (("fp64", "fp64", 1e-10), ("fp64_tol", "fp64", q.tol), ("mixed", "mixed", q.tol)):
             g = GpuSolver(replace(q, tol=tol), precision=prec)
-            if prec == "mixed":
-                g.set_mixed_restart(True)
             g.set_gradient(g0)
             g.solve()
             sig[tag] = g.homogenized_stress().copy()

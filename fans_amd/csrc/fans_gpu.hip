@@ -1593,23 +1593,17 @@ int solve_fp_fp64(fans_gpu_ctx *c)
 }
 
 // The mixed form's FP32 inner solves.
-// Residual replacement (single RVE): one FP32 CG whose recursive residual is
+// Residual replacement (the default): one FP32 CG whose recursive residual is
 // replaced by the true FP64 one, after u += x, whenever it has dropped by
 // kReplaceEta against the last true error, or below kReplaceFloor times the
 // error the outer test asks for; the directions and CG scalars carry over, so
-// no Krylov information is lost.  Restarted inner solves (batched contexts,
-// or FANS_OPT_MIXED_RESTART): each outer round starts a new CG from x = 0 to
+// no Krylov information is lost.  Restarted inner solves (single RVE with
+// FANS_OPT_MIXED_RESTART): each outer round starts a new CG from x = 0 to
 // kRestartEta times the outer error, not below kRestartFloor times the
 // target.  A restart loses the smooth modes' convergence, so at an equal
 // final residual its homogenized stress is less accurate: the lower floor
 // keeps it within 5e-9 of the FP64 solution on the 384-RVE batch.
-#ifndef FANS_REPLACE_FLOOR
-#define FANS_REPLACE_FLOOR 0.9
-#endif
-#ifndef FANS_REPLACE_ETA
-#define FANS_REPLACE_ETA 1e-4
-#endif
-constexpr double kReplaceFloor = FANS_REPLACE_FLOOR, kReplaceEta = FANS_REPLACE_ETA;
+constexpr double kReplaceFloor = 0.9, kReplaceEta = 1e-4;
 constexpr double kRestartFloor = 0.5, kRestartEta = 1e-4;
 
 // Mixed form with restarted inner solves: FP64 defect correction around FP32
@@ -1844,8 +1838,14 @@ int solve_batch_cg64(fans_gpu_ctx *c, std::vector<SlotHost *> &H)
     return 0;
 }
 
-// solve_cg_mixed for every slot: the slots still above the tolerance run the
-// next inner FP32 solve together
+// solve_cg_mixed for every slot: one FP32 CG per slot with FP64 residual
+// replacement.  A phase runs until every slot has met its phase tolerance;
+// then each slot's true residual replaces its recursive one.  The slots that
+// continue resume together at the largest iteration index J reached: a slot
+// that stopped at k < J takes index J, its last direction moves to ring slot
+// (J - 1) mod 8 and its iteration budget and error-history room follow.  The
+// arithmetic of each slot, the order of its solution updates included, is
+// the single-RVE solve's.
 int solve_batch_mixed(fans_gpu_ctx *c, std::vector<SlotHost *> &H)
 {
     const int        nb = c->nb;
@@ -1863,63 +1863,92 @@ int solve_batch_mixed(fans_gpu_ctx *c, std::vector<SlotHost *> &H)
     if ((rc = set_map(c, all)) || (rc = residual64(c, c->u, c->r, nullptr)) ||
         (rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_INIT)) || (rc = read_all_scalars(c, v)))
         return rc;
-    std::vector<double>              err0(nb);
+    auto scal = [&](int b, int k) -> double & { return v.sc[(size_t) b * SC_COUNT + k]; };
+    auto ints = [&](int b, int k) -> int & { return v.is[(size_t) b * IS_COUNT + k]; };
+    std::vector<double>              err0(nb), tol_abs(nb);
     std::vector<std::vector<double>> ohist(nb);
-    std::vector<int>                 inner(nb, 0), outer(nb, 0), it(nb, 0);
-    std::vector<char>                fin(nb);
+    std::vector<int>                 inner(nb, 0), outer(nb, 0), kstart(nb, 0);
+    std::vector<int>                 act;
     for (int b = 0; b < nb; ++b) {
-        err0[b]  = v.sc[(size_t) b * SC_COUNT + SC_ERR];
-        ohist[b] = {err0[b]};
-        fin[b]   = (char) v.is[(size_t) b * IS_COUNT + IS_DONE];
+        err0[b]    = scal(b, SC_ERR);
+        tol_abs[b] = c->tol_type == FANS_ERR_ABSOLUTE ? c->tol : c->tol * err0[b];
+        ohist[b]   = {err0[b]};
+        if (!ints(b, IS_DONE))
+            act.push_back(b);
     }
-    const double eta = kRestartEta;
-    for (;;) {
-        std::vector<int> act;
-        for (int b = 0; b < nb; ++b)
-            if (!fin[b] && inner[b] < c->d.n_it)
-                act.push_back(b);
-        if (act.empty())
-            break;
-        // inner FP32 CG on K x = r, x = 0, per slot to eta times its outer error
-        for (int b : act) {
-            const double err_out = v.sc[(size_t) b * SC_COUNT + SC_ERR];
-            const double tol_abs = c->tol_type == FANS_ERR_ABSOLUTE ? c->tol : c->tol * err0[b];
-            init_slot_scalars(c, v, b, std::max(eta * err_out, kRestartFloor * tol_abs), FANS_ERR_ABSOLUTE,
-                              c->d.n_it - inner[b]);
-        }
+    auto phase_tol = [&](int b, double err) { return std::max(kReplaceEta * err, kReplaceFloor * tol_abs[b]); };
+    int  J         = 0;
+    if (!act.empty()) {
+        for (int b : act)
+            init_slot_scalars(c, v, b, phase_tol(b, err0[b]), FANS_ERR_ABSOLUTE, c->d.n_it);
         if ((rc = write_all_scalars(c, v)) || (rc = set_map(c, act)))
             return rc;
+        // x = 0 and d_(-1) = 0 in the ring's last slot
         hipLaunchKernelGGL(k_inner_start, dim3(c->nblk_vec, 1, c->gz), dim3(kThreads), 0, c->stream, c->r32, c->x32,
                            c->ring32[kDirRing - 1], c->r, c->Nh, c->d.measure, c->partial, launch_slots(c, false));
         HIPCHK(c, hipGetLastError());
         if ((rc = reduce(c, c->partial, c->nblk_vec, 1, RED_ERR_INIT)))
             return rc;
+    }
+    while (!act.empty()) {
         if ((rc = cg_linear_loop<float>(c, c->fft32, c->r32, c->ring32, c->rnew32, c->x32, c->spec32, c->G32,
-                                        c->dph32, nullptr)))
+                                        c->dph32, nullptr, J)))
             return rc;
         if ((rc = read_all_scalars(c, v)))
             return rc;
-        for (int b : act)
-            it[b] = v.is[(size_t) b * IS_COUNT + IS_ITER];
-        // u += x and the true residual of the slots that ran
+        std::vector<int> kend(nb, 0);
+        for (int b : act) {
+            kend[b] = ints(b, IS_ITER);
+            inner[b] += kend[b] - kstart[b];
+            outer[b] += 1;
+        }
+        // u += x and the true residual of the slots that ran, tested against
+        // the outer tolerance
         hipLaunchKernelGGL((k_axpby<double, float>), dim3(c->nblk_vec, 1, c->gz), dim3(kThreads), 0, c->stream, c->u,
                            c->x32, 1.0, 1.0, c->Nh, launch_slots(c, false));
         HIPCHK(c, hipGetLastError());
         if ((rc = residual64(c, c->u, c->r, nullptr)))
             return rc;
         for (int b : act)
-            init_slot_scalars(c, v, b, c->tol, c->tol_type, 1 << 30);
-        if ((rc = write_all_scalars(c, v)) || (rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_INIT)) ||
+            scal(b, SC_TOL) = tol_abs[b];
+        if ((rc = write_all_scalars(c, v)) || (rc = reduce(c, c->partial, c->nblk_asm, 1, RED_ERR_SET)) ||
             (rc = read_all_scalars(c, v)))
             return rc;
+        std::vector<int> next;
         for (int b : act) {
-            const double err = v.sc[(size_t) b * SC_COUNT + SC_ERR];
-            inner[b] += it[b];
-            outer[b] += 1;
+            const double err = scal(b, SC_ERR);
             ohist[b].push_back(err);
             const double e = c->tol_type == FANS_ERR_ABSOLUTE ? err : err / err0[b];
-            fin[b]         = !(e > c->tol) || it[b] == 0;  // it == 0: FP32 floor reached without progress
+            if (e > c->tol && inner[b] < c->d.n_it && kend[b] > kstart[b])
+                next.push_back(b);
         }
+        act = next;
+        if (act.empty())
+            break;
+        // resume together at the largest iteration index reached
+        J = 0;
+        for (int b : act)
+            J = std::max(J, kend[b]);
+        for (int b : act) {
+            const int from = (kend[b] + kDirRing - 1) % kDirRing, to = (J + kDirRing - 1) % kDirRing;
+            if (from != to)
+                HIPCHK(c, hipMemcpyAsync(c->ring32[to] + (size_t) b * c->strides.field,
+                                         c->ring32[from] + (size_t) b * c->strides.field,
+                                         sizeof(float) * c->Nh, hipMemcpyDeviceToDevice, c->stream));
+            ints(b, IS_ITER) = J;
+            ints(b, IS_NIT)  = J + (c->d.n_it - inner[b]);
+            ints(b, IS_DONE) = 0;
+            scal(b, SC_TOL)  = phase_tol(b, scal(b, SC_ERR));
+            for (int m = 0; m < kDirRing; ++m)
+                scal(b, SC_ARING + m) = 0.0;  // every direction is in x (flushed) or was moved
+            kstart[b] = J;
+        }
+        if ((rc = write_all_scalars(c, v)) || (rc = set_map(c, act)))
+            return rc;
+        // r32 = (float) r, x = 0; the directions stay
+        hipLaunchKernelGGL(k_inner_start, dim3(c->nblk_vec, 1, c->gz), dim3(kThreads), 0, c->stream, c->r32, c->x32,
+                           (float *) nullptr, c->r, c->Nh, c->d.measure, c->partial, launch_slots(c, false));
+        HIPCHK(c, hipGetLastError());
     }
     for (int b = 0; b < nb; ++b) {
         H[b]->iter        = inner[b];
@@ -2089,7 +2118,10 @@ int alloc_fields(fans_gpu_ctx *c)
     HIPCHK(c, hipMalloc(&c->sc, sizeof(double) * SC_COUNT * nb));
     HIPCHK(c, hipMalloc(&c->is, sizeof(int) * IS_COUNT * nb));
     HIPCHK(c, hipMemsetAsync(c->is, 0, sizeof(int) * IS_COUNT * nb, c->stream));
-    HIPCHK(c, hipMalloc(&c->hist, sizeof(double) * (c->d.n_it + 2) * nb));
+    // error history; a batched mixed solve moves a slot's iteration index
+    // ahead by up to n_it (solve_batch_mixed)
+    const int64_t hist_len = nb > 1 ? 2 * (int64_t) c->d.n_it + 2 : c->d.n_it + 2;
+    HIPCHK(c, hipMalloc(&c->hist, sizeof(double) * hist_len * nb));
     // phase ids: bytes, room for 16-bit ids per slot of a batch
     c->ms_bytes = nb > 1 ? 2 * c->N : c->N;
     HIPCHK(c, hipMalloc(&c->ms, c->ms_bytes * nb));
@@ -2119,7 +2151,7 @@ int alloc_fields(fans_gpu_ctx *c)
     c->strides.green   = c->Nf * npk;
     c->strides.part    = need;
     c->strides.ms      = c->ms_bytes;
-    c->strides.hist    = c->d.n_it + 2;
+    c->strides.hist    = hist_len;
     if (nb > 1) {
         c->nph_cap       = 256;
         c->strides.ph    = c->nph_cap;

@@ -275,9 +275,9 @@ int fans_gpu_stream_bandwidth(int32_t device, int64_t bytes, int32_t reps, doubl
  * KiB, default 40) of one workgroup of the generic FFT passes, i.e. their
  * lines or columns per workgroup (lengths other than the powers of two
  * 16..256, whose kernels have fixed tiles); FANS_OPT_MIXED_RESTART=1 makes a
- * single-RVE context's mixed form restart its FP32 CG in every outer round,
- * as batched contexts do (default 0: one FP32 CG whose residual is replaced
- * by the true FP64 one at each outer check). */
+ * single-RVE context's mixed form restart its FP32 CG in every outer round
+ * (default 0: one FP32 CG whose residual is replaced by the true FP64 one at
+ * each outer check, as batched contexts also do). */
 enum { FANS_OPT_TIMING = 1, FANS_OPT_FFT_CHUNKS = 2, FANS_OPT_GRAPHS = 3, FANS_OPT_ASM_PLANES = 4,
        FANS_OPT_OPERATOR_FP32 = 5, FANS_OPT_FFT_LDS_KIB = 6, FANS_OPT_MIXED_RESTART = 7 };
 int fans_gpu_set_option(fans_gpu_ctx *ctx, int32_t opt, int64_t value);
