@@ -98,3 +98,24 @@ def test_grid_factors_are_refused(sphere32_ms):
     p = Problem.from_json(os.path.join(GOLDEN, "test_LinearThermal.json"), ms=ms)
     with pytest.raises(gpu.FansGpuError, match="products of 2, 3 and 5"):
         gpu.GpuSolver(p)
+
+
+def test_batch_slot_count_is_checked_before_the_device():
+    """fans_gpu_create_batch refuses slot counts outside 1..4096 and methods
+    other than CG with FANS_GPU_EINVAL, before it looks for a device (so also
+    on a host without one)."""
+    import ctypes as C
+    L = gpu.lib()
+    d = gpu.Desc()
+    d.n_x = d.n_y = d.n_z = 16
+    d.howmany = 1
+    d.method = gpu.METHOD["cg"]
+    d.n_it = 10
+    d.L[:] = [1.0, 1.0, 1.0]
+    d.tol = 1e-8
+    ptr = C.c_void_p()
+    for slots in (0, -3, 4097):
+        assert L.fans_gpu_create_batch(C.byref(ptr), C.byref(d), slots) == -1
+        assert not ptr.value
+    d.method = gpu.METHOD["fp"]
+    assert L.fans_gpu_create_batch(C.byref(ptr), C.byref(d), 4) == -1
