@@ -69,26 +69,29 @@ def test_load_rejects_other_context_settings():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [32, 48])
+@pytest.mark.parametrize("n", [32, 48, 80])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batched_context_matches_single_solves(precision, n):
     """One batched context solving 5 RVEs at once gives every RVE the iteration
     count, homogenized response and displacement field of solving it alone,
-    bit for bit (mixed form: the slots that continue after a residual
-    replacement resume at a common iteration index, their last direction moved
-    in the ring) (same kernels and summation order; the RVEs finish at different
-    iterations, so the launch grid shrinks on the way).  n = 32 runs the
-    compile-time z and register y / x kernels, n = 48 the generic ones."""
+    bit for bit (same kernels and summation order).  The RVEs finish at
+    different iterations, so the launch grid shrinks on the way; one slot has
+    a zero load and is finished before the first iteration.  Mixed form: the
+    slots that continue after a residual replacement resume at a common
+    iteration index, their last direction moved in the ring.  n = 32 runs the
+    compile-time z and register y / x kernels, n = 48 and 80 (radix 3 and 5)
+    the generic ones."""
     from fans_amd.gpu import FansGpuError, GpuBatch, GpuSolver
     # at 1e-10 the mixed form takes two or more residual replacements, after
     # which the slots resume together from different iteration counts
     probs = config5_rves(count=10, seed=11, edges=(n,), tol=1e-10)
     for phys in (0, 1):
         ps = probs[phys::2]
-        g0 = ps[0].load_cases[0].g0_path[0]
+        g0s = [ps[0].load_cases[0].g0_path[0]] * len(ps)
+        g0s[2] = np.zeros_like(g0s[2])
         print(f"[gpu-case] batched context {precision} n={n} howmany={ps[0].howmany} slots={len(ps)}", flush=True)
         single = []
-        for p in ps:
+        for p, g0 in zip(ps, g0s):
             g = GpuSolver(p, precision=precision)
             try:
                 g.set_gradient(g0)
@@ -100,10 +103,11 @@ def test_batched_context_matches_single_solves(precision, n):
         try:
             for rep in range(2):  # the second solve replays the captured graphs
                 for b, p in enumerate(ps):
-                    bt.load_slot(b, p, g0)
+                    bt.load_slot(b, p, g0s[b])
                 bt.solve()
                 its, conv, _ = bt.results()
                 st = bt.stresses()
+                assert its[2] == 0 and not np.any(st[2])
                 for b in range(len(ps)):
                     assert conv[b]
                     assert its[b] == single[b][0], (rep, b, its, [s[0] for s in single])
