@@ -542,8 +542,8 @@ def main():
         print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 8}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is a design "
-              "described in DESIGN.md and is not executed", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); sharding one RVE over GPUs "
+              "(x-slabs with one-plane halos, all-to-all FFT transposes) is not built", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch":
         run_batch_workload(args)
