@@ -136,6 +136,20 @@ struct ProArgs {
     double       *partial;  // [gridDim.x]
 };
 
+// Loads a thread issues together before the stores that follow them in the
+// fused prologues / epilogues of the z passes.  Element by element, a store may
+// alias the next element's loads, so the compiler keeps program order: one
+// memory round trip per element.  kZGroup for the compile-time passes,
+// kZGroupRt for the runtime-length ones (bounds-checked groups).
+#ifndef FANS_Z_GROUP
+#define FANS_Z_GROUP 16
+#endif
+constexpr int kZGroup = FANS_Z_GROUP;
+#ifndef FANS_Z_GROUP_RT
+#define FANS_Z_GROUP_RT 8
+#endif
+constexpr int kZGroupRt = FANS_Z_GROUP_RT;
+
 template <typename T, int PRO>
 __device__ __forceinline__ T z_load(T *__restrict__ in, int64_t gi, const ProArgs<T> &pa, T alpha, double &a0)
 {
@@ -206,10 +220,40 @@ __global__ __launch_bounds__(256) void k_fft_z_r2c(T *__restrict__ in, cplx<T> *
     double        a0    = 0.0;
     if constexpr (PRO == PRO_CG)
         alpha = (T) *pa.alpha;
-    for (int e = threadIdx.x; e < nl * nzh; e += blockDim.x) {
-        const int l = e / nzh, rem = e - l * nzh, z = rem / H, c = rem - z * H;
-        T        *d = reinterpret_cast<T *>(&A[(z >> 1) * ld + l * H + c]);
-        d[z & 1]    = z_load<T, PRO>(in, g0 + e, pa, alpha, a0);
+    if constexpr (PRO == PRO_CG && kZGroupRt > 0) {
+        // r -= alpha K d in groups (same arithmetic and order as z_load)
+        constexpr int CH = kZGroupRt;
+        const int     tot = nl * nzh, stp = blockDim.x;
+        for (int e0 = threadIdx.x; e0 < tot; e0 += CH * stp) {
+            T rv[CH], qv[CH];
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                const int e = e0 + i * stp;
+                rv[i] = qv[i] = (T) 0;
+                if (e < tot) {
+                    rv[i] = __builtin_nontemporal_load(&in[g0 + e]);
+                    qv[i] = __builtin_nontemporal_load(&pa.rnew[g0 + e]);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                const int e = e0 + i * stp;
+                if (e < tot) {
+                    const int l = e / nzh, rem = e - l * nzh, z = rem / H, c = rem - z * H;
+                    const T   r = rv[i] - alpha * qv[i];
+                    __builtin_nontemporal_store(r, &in[g0 + e]);
+                    acc_norm(a0, r, pa.measure);
+                    T *d     = reinterpret_cast<T *>(&A[(z >> 1) * ld + l * H + c]);
+                    d[z & 1] = r;
+                }
+            }
+        }
+    } else {
+        for (int e = threadIdx.x; e < nl * nzh; e += blockDim.x) {
+            const int l = e / nzh, rem = e - l * nzh, z = rem / H, c = rem - z * H;
+            T        *d = reinterpret_cast<T *>(&A[(z >> 1) * ld + l * H + c]);
+            d[z & 1]    = z_load<T, PRO>(in, g0 + e, pa, alpha, a0);
+        }
     }
     __syncthreads();
     lds_fft<T, false>(A, LM, tw, ncols, ld);
@@ -300,6 +344,61 @@ __device__ __forceinline__ void z_epilogue(const EpiArgs<T> &ep, int64_t gi, T v
     }
 }
 
+// z_epilogue over this thread's elements e = threadIdx.x + k stp < tot in
+// groups of CH, each group's global operands requested before its stores (the
+// stores never alias the operands of another element: d_next and d_prev, s
+// and r are distinct buffers and u is updated per element); same arithmetic
+// and accumulation order as z_epilogue.  val_of(e) is element e's result.
+template <typename T, int EPI, int CH, typename ValOf>
+__device__ __forceinline__ void z_epilogue_grouped(const EpiArgs<T> &ep, int64_t g0, int tot, int stp, ValOf val_of,
+                                                   double &acc, T beta)
+{
+    for (int e0 = threadIdx.x; e0 < tot; e0 += CH * stp) {
+        T      pa[CH], pb[CH];
+        double pu[CH];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int     e  = e0 + i * stp;
+            const int64_t gi = g0 + e;
+            pa[i] = pb[i] = (T) 0;
+            pu[i]         = 0.0;
+            if (e < tot) {
+                if constexpr (EPI == EPI_CGD) {
+                    pa[i] = __builtin_nontemporal_load(&ep.dprev[gi]);
+                    if (ep.qprev)
+                        pb[i] = __builtin_nontemporal_load(&ep.qprev[gi]);
+                } else if constexpr (EPI == EPI_CG) {
+                    pb[i] = __builtin_nontemporal_load(&ep.r[gi]);
+                } else if constexpr (EPI == EPI_FP) {
+                    pu[i] = ep.u[gi];
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int     e  = e0 + i * stp;
+            const int64_t gi = g0 + e;
+            if (e < tot) {
+                const T val = val_of(e);
+                if constexpr (EPI == EPI_STORE) {
+                    ep.out[gi] = val;
+                } else if constexpr (EPI == EPI_CGD) {
+                    const T dn   = beta * pa[i] - val;
+                    ep.dnext[gi] = dn;
+                    if (ep.qprev)
+                        acc += (double) dn * (double) pb[i];
+                } else if constexpr (EPI == EPI_CG) {
+                    const T sv = -val;
+                    ep.out[gi] = sv;
+                    acc += (double) pb[i] * (double) sv;
+                } else {
+                    ep.u[gi] = pu[i] - (double) val;
+                }
+            }
+        }
+    }
+}
+
 template <int EPI, typename T>
 __device__ __forceinline__ void z_epi_partials(const EpiArgs<T> &ep, double acc, double *red)
 {
@@ -344,10 +443,16 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const cplx<T> *__restrict__ i
     T             beta = (T) 0;
     if constexpr (EPI == EPI_CGD)
         beta = (T) ep.scal[0];
-    for (int e = threadIdx.x; e < nl * nzh; e += blockDim.x) {
+    auto val_of = [&](int e) {
         const int     l = e / nzh, rem = e - l * nzh, z = rem / H, c = rem - z * H;
         const cplx<T> v = A[(z >> 1) * ld + l * H + c];
-        z_epilogue<T, EPI>(ep, g0 + e, (z & 1) ? v.y : v.x, acc, beta);
+        return (z & 1) ? v.y : v.x;
+    };
+    if constexpr (kZGroupRt > 0 && EPI != EPI_STORE) {
+        z_epilogue_grouped<T, EPI, kZGroupRt>(ep, g0, nl * nzh, blockDim.x, val_of, acc, beta);
+    } else {
+        for (int e = threadIdx.x; e < nl * nzh; e += blockDim.x)
+            z_epilogue<T, EPI>(ep, g0 + e, val_of(e), acc, beta);
     }
     z_epi_partials<EPI>(ep, acc, red);
 }
@@ -468,11 +573,37 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_r2c_ct(T *__res
     double             a0    = 0.0;
     if constexpr (PRO == PRO_CG)
         alpha = (T) *pa.alpha;
+    constexpr int PER = (LPB * NZH) / NT;
+    constexpr int CH  = kZGroup < PER ? kZGroup : PER;
+    if constexpr (PRO == PRO_CG && kZGroup > 0 && (LPB * NZH) % NT == 0 && PER % CH == 0) {
+        // r -= alpha K d in groups of CH elements: the loads of a group are in
+        // flight together (same arithmetic and accumulation order as z_load)
+        for (int i0 = 0; i0 < PER; i0 += CH) {
+            T rv[CH], qv[CH];
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                const int64_t gi = g0 + threadIdx.x + (i0 + i) * NT;
+                rv[i]            = __builtin_nontemporal_load(&in[gi]);
+                qv[i]            = __builtin_nontemporal_load(&pa.rnew[gi]);
+            }
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                const int e = threadIdx.x + (i0 + i) * NT;
+                const int l = e / NZH, rem = e - l * NZH, z = rem / H, c = rem - z * H;
+                const T   r = rv[i] - alpha * qv[i];
+                __builtin_nontemporal_store(r, &in[g0 + e]);
+                acc_norm(a0, r, pa.measure);
+                T *d     = reinterpret_cast<T *>(&A[(z >> 1) * LD + l * H + c]);
+                d[z & 1] = r;
+            }
+        }
+    } else {
 #pragma unroll 4
-    for (int e = threadIdx.x; e < LPB * NZH; e += NT) {
-        const int l = e / NZH, rem = e - l * NZH, z = rem / H, c = rem - z * H;
-        T        *d = reinterpret_cast<T *>(&A[(z >> 1) * LD + l * H + c]);
-        d[z & 1]    = z_load<T, PRO>(in, g0 + e, pa, alpha, a0);
+        for (int e = threadIdx.x; e < LPB * NZH; e += NT) {
+            const int l = e / NZH, rem = e - l * NZH, z = rem / H, c = rem - z * H;
+            T        *d = reinterpret_cast<T *>(&A[(z >> 1) * LD + l * H + c]);
+            d[z & 1]    = z_load<T, PRO>(in, g0 + e, pa, alpha, a0);
+        }
     }
     __syncthreads();
     lds_fft_ct<T, false, M, NC, LD, NT>(A, tw);
@@ -537,6 +668,15 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cp
         beta = (T) ep.scal[0];
     if constexpr (PRE) {
         T *__restrict__ dn = ep.dnext;
+        // K d_prev when it was not requested ahead: all of this thread's values
+        // are requested before the first store of d
+        constexpr bool LATEQ = !PREQ && kZGroup > 0;
+        T              qlv[LATEQ ? PER : 1];
+        if constexpr (LATEQ)
+            if (ep.qprev)
+#pragma unroll
+                for (int i = 0; i < PER; ++i)
+                    qlv[i] = __builtin_nontemporal_load(&ep.qprev[g0 + threadIdx.x + i * NT]);
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int     e = threadIdx.x + i * NT;
@@ -546,15 +686,25 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cp
             dn[g0 + e]      = d;
             if constexpr (PREQ)
                 acc += (double) d * (double) qpv[i];
-            else if (ep.qprev)
+            else if constexpr (LATEQ) {
+                if (ep.qprev)
+                    acc += (double) d * (double) qlv[i];
+            } else if (ep.qprev)
                 acc += (double) d * (double) __builtin_nontemporal_load(&ep.qprev[g0 + e]);
         }
     } else {
-#pragma unroll 4
-        for (int e = threadIdx.x; e < LPB * NZH; e += NT) {
+        auto val_of = [&](int e) {
             const int     l = e / NZH, rem = e - l * NZH, z = rem / H, c = rem - z * H;
             const cplx<T> v = A[(z >> 1) * LD + l * H + c];
-            z_epilogue<T, EPI>(ep, g0 + e, (z & 1) ? v.y : v.x, acc, beta);
+            return (z & 1) ? v.y : v.x;
+        };
+        if constexpr (kZGroup > 0 && EPI != EPI_STORE) {
+            constexpr int CH = kZGroup < 8 ? kZGroup : 8;
+            z_epilogue_grouped<T, EPI, CH>(ep, g0, LPB * NZH, NT, val_of, acc, beta);
+        } else {
+#pragma unroll 4
+            for (int e = threadIdx.x; e < LPB * NZH; e += NT)
+                z_epilogue<T, EPI>(ep, g0 + e, val_of(e), acc, beta);
         }
     }
     z_epi_partials<EPI>(ep, acc, red);
