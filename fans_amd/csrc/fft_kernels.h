@@ -458,14 +458,19 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const cplx<T> *__restrict__ i
 }
 
 // ---- z passes with compile-time shape ---------------------------------------
-#ifndef FANS_C2R_PREFETCH
-#define FANS_C2R_PREFETCH 1
+// operands of the linear CG's inverse z epilogue requested before the
+// transform (0 none, 1 d_prev, 2 d_prev and K d_prev), per precision; the
+// others are requested together right after it.  At 256^3: FP64 0.294 ms
+// with d_prev ahead, 0.295 with none, 0.306 with both; FP32 0.134 ms with
+// none, 0.136 with d_prev, 0.223 with both.
+#ifndef FANS_C2R_AHEAD64
+#define FANS_C2R_AHEAD64 1
 #endif
-#ifndef FANS_C2R_PREFETCH32
-#define FANS_C2R_PREFETCH32 1
+#ifndef FANS_C2R_AHEAD32
+#define FANS_C2R_AHEAD32 0
 #endif
-constexpr bool kC2rPrefetch   = FANS_C2R_PREFETCH != 0;
-constexpr int  kC2rPrefetch32 = FANS_C2R_PREFETCH32;
+constexpr int kC2rAhead64 = FANS_C2R_AHEAD64;
+constexpr int kC2rAhead32 = FANS_C2R_AHEAD32;
 
 // For M = nz/2 in {16, 32, 64, 128} the line count per block, the thread count
 // and every index are compile-time constants (no runtime divisions), and the
@@ -635,17 +640,15 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cp
     const int64_t      line0 = (int64_t) blockIdx.x * LPB;
     const cplx<T>     *src   = in + line0 * MC * H;
     const int64_t      g0    = line0 * NZH;
-    // EPI_CGD: the per-thread values of d_prev and (FP64) K d_prev, requested
-    // before the transform so that their loads overlap it (FP64 0.354 ->
-    // 0.306 ms at 256^3; FP32 0.200 -> 0.192 ms with d_prev alone, 0.208 ->
-    // 0.220 ms with both, whose registers cost more occupancy than they gain)
-    constexpr int PER = (LPB * NZH) / NT;
-    // FP64: both arrays ahead; FP32: kC2rPrefetch32 = 1 d_prev only, 2 both
-    constexpr bool PRE  = EPI == EPI_CGD && kC2rPrefetch && (sizeof(T) == 8 || kC2rPrefetch32 > 0) &&
-                         (LPB * NZH) % NT == 0;
-    constexpr bool PREQ = PRE && (sizeof(T) == 8 || kC2rPrefetch32 > 1);
-    T              dpv[PRE ? PER : 1], qpv[PREQ ? PER : 1];
-    if constexpr (PRE) {
+    // EPI_CGD: this thread's values of d_prev and K d_prev, the ones of
+    // kC2rAhead* requested before the transform so that their loads overlap it,
+    // at the cost of registers during it
+    constexpr int  PER   = (LPB * NZH) / NT;
+    constexpr int  AHEAD = sizeof(T) == 8 ? kC2rAhead64 : kC2rAhead32;
+    constexpr bool PRE   = EPI == EPI_CGD && (LPB * NZH) % NT == 0;
+    constexpr bool PRED = PRE && AHEAD >= 1, PREQ = PRE && AHEAD >= 2;
+    T              dpv[PRE ? PER : 1], qpv[PRE ? PER : 1];
+    if constexpr (PRED) {
         const T *__restrict__ dp = ep.dprev;
         const T *__restrict__ qp = ep.qprev;
 #pragma unroll
@@ -668,15 +671,17 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cp
         beta = (T) ep.scal[0];
     if constexpr (PRE) {
         T *__restrict__ dn = ep.dnext;
-        // K d_prev when it was not requested ahead: all of this thread's values
-        // are requested before the first store of d
-        constexpr bool LATEQ = !PREQ && kZGroup > 0;
-        T              qlv[LATEQ ? PER : 1];
-        if constexpr (LATEQ)
+        // the operands not requested ahead: all of this thread's values are
+        // requested before the first store of d
+        if constexpr (!PRED)
+#pragma unroll
+            for (int i = 0; i < PER; ++i)
+                dpv[i] = __builtin_nontemporal_load(&ep.dprev[g0 + threadIdx.x + i * NT]);
+        if constexpr (!PREQ)
             if (ep.qprev)
 #pragma unroll
                 for (int i = 0; i < PER; ++i)
-                    qlv[i] = __builtin_nontemporal_load(&ep.qprev[g0 + threadIdx.x + i * NT]);
+                    qpv[i] = __builtin_nontemporal_load(&ep.qprev[g0 + threadIdx.x + i * NT]);
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int     e = threadIdx.x + i * NT;
@@ -684,13 +689,8 @@ __global__ __launch_bounds__((zct_threads<H, M>())) void k_fft_z_c2r_ct(const cp
             const cplx<T> v = A[(z >> 1) * LD + l * H + c];
             const T       d = beta * dpv[i] - ((z & 1) ? v.y : v.x);
             dn[g0 + e]      = d;
-            if constexpr (PREQ)
+            if (ep.qprev)
                 acc += (double) d * (double) qpv[i];
-            else if constexpr (LATEQ) {
-                if (ep.qprev)
-                    acc += (double) d * (double) qlv[i];
-            } else if (ep.qprev)
-                acc += (double) d * (double) __builtin_nontemporal_load(&ep.qprev[g0 + e]);
         }
     } else {
         auto val_of = [&](int e) {

@@ -41,17 +41,31 @@ enum {
 
 __device__ __forceinline__ double finish_norm(double v, int measure) { return measure == 1 ? sqrt(v) : v; }
 
-// sets done if the loop condition of solverCG.h:84 / solverFP.h:221 fails
-__device__ __forceinline__ void record_error(double *sc, int *is, double *hist, double err)
+// The scalars a reduction stage reads, loaded before its first store (the
+// compiler cannot tell the stores to sc / is / hist from those loads, so it
+// would keep each load behind the stores before it: one round trip each).
+struct StageScalars {
+    int    iter, errtype, nit;
+    double delta, deltamid, beta, err0, tol;
+};
+__device__ __forceinline__ StageScalars load_stage_scalars(const double *sc, const int *is)
 {
-    const int it = is[IS_ITER];
-    hist[it]     = err;
+    return StageScalars{is[IS_ITER],     is[IS_ERRTYPE], is[IS_NIT], sc[SC_DELTA], sc[SC_DELTAMID],
+                        sc[SC_BETA],     sc[SC_ERR0],    sc[SC_TOL]};
+}
+
+// records the error of iteration it and sets done if the loop condition of
+// solverCG.h:84 / solverFP.h:221 fails
+__device__ __forceinline__ void record_error(double *sc, int *is, double *hist, const StageScalars &S, int it,
+                                             double err)
+{
+    hist[it] = err;
     if (it == 0)
         sc[SC_ERR0] = err;
     sc[SC_ERR]           = err;
-    const double err_rel = it == 0 ? 100.0 : err / sc[SC_ERR0];
-    const double e       = is[IS_ERRTYPE] == 0 ? err : err_rel;
-    is[IS_DONE]          = !(it < is[IS_NIT] && e > sc[SC_TOL]);
+    const double err_rel = it == 0 ? 100.0 : err / S.err0;
+    const double e       = S.errtype == 0 ? err : err_rel;
+    is[IS_DONE]          = !(it < S.nit && e > S.tol);
 }
 
 // one block of kReduceThreads; each thread folds its strided partials into
@@ -110,29 +124,33 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double *__restr
                 sc[SC_RES0 + q] = v[0];
                 if (nqq == 2)
                     sc[SC_RES0 + q + 1] = v[1];
-            } else if (stage == RED_ERR_INIT) {
-                is[IS_ITER]     = 0;
-                sc[SC_DELTA]    = 1.0;
-                sc[SC_DELTAMID] = 0.0;
-                record_error(sc, is, hist, finish_norm(v[0], measure));
-            } else if (stage == RED_DELTA) {
-                sc[SC_DELTA0] = sc[SC_DELTA];
-                sc[SC_DELTA]  = v[0];
-                sc[SC_BETA]   = fmax(0.0, (sc[SC_DELTA] - sc[SC_DELTAMID]) / sc[SC_DELTA0]);
-            } else if (stage == RED_ALPHA) {
-                sc[SC_DN]    = v[0];
-                sc[SC_ALPHA] = sc[SC_DELTA] / v[0];
-                sc[SC_ARING + is[IS_ITER] % kDirRing] = sc[SC_ALPHA];
-                // deltamid of the next iteration, <r - alpha K d, s> with
-                // s = d - beta d_prev (solverCG.h:87, 91 and 104):
-                // delta - alpha (<d, K d> - beta <K d_prev, d>), K symmetric
-                if (nq == 2)
-                    sc[SC_DELTAMID] = sc[SC_DELTA] - sc[SC_ALPHA] * (v[0] - sc[SC_BETA] * v[1]);
-            } else if (stage == RED_ERR_NEXT) {
-                is[IS_ITER] += 1;
-                record_error(sc, is, hist, finish_norm(v[0], measure));
-            } else if (stage == RED_ERR_SET) {
-                record_error(sc, is, hist, finish_norm(v[0], measure));
+            } else {
+                const StageScalars S = load_stage_scalars(sc, is);
+                if (stage == RED_ERR_INIT) {
+                    is[IS_ITER]     = 0;
+                    sc[SC_DELTA]    = 1.0;
+                    sc[SC_DELTAMID] = 0.0;
+                    record_error(sc, is, hist, S, 0, finish_norm(v[0], measure));
+                } else if (stage == RED_DELTA) {
+                    sc[SC_DELTA0] = S.delta;
+                    sc[SC_DELTA]  = v[0];
+                    sc[SC_BETA]   = fmax(0.0, (v[0] - S.deltamid) / S.delta);
+                } else if (stage == RED_ALPHA) {
+                    const double alpha                = S.delta / v[0];
+                    sc[SC_DN]                         = v[0];
+                    sc[SC_ALPHA]                      = alpha;
+                    sc[SC_ARING + S.iter % kDirRing] = alpha;
+                    // deltamid of the next iteration, <r - alpha K d, s> with
+                    // s = d - beta d_prev (solverCG.h:87, 91 and 104):
+                    // delta - alpha (<d, K d> - beta <K d_prev, d>), K symmetric
+                    if (nq == 2)
+                        sc[SC_DELTAMID] = S.delta - alpha * (v[0] - S.beta * v[1]);
+                } else if (stage == RED_ERR_NEXT) {
+                    is[IS_ITER] = S.iter + 1;
+                    record_error(sc, is, hist, S, S.iter + 1, finish_norm(v[0], measure));
+                } else if (stage == RED_ERR_SET) {
+                    record_error(sc, is, hist, S, S.iter, finish_norm(v[0], measure));
+                }
             }
         }
         __syncthreads();
