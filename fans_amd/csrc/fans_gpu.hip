@@ -607,6 +607,8 @@ void clear_map(fans_gpu_ctx *c)
 // power-of-two lengths 16..256 use the register four-step kernels
 // ---------------------------------------------------------------------------
 bool reg_len(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256; }
+// mixed-radix lengths of the register y pass (24 x 4, 20 x 4, 12 x 4)
+bool reg_len_y(int n) { return n == 96 || n == 80 || n == 48; }
 
 // columns per block: a power of two (>= 4) within ~72 KiB of LDS (two
 // workgroups per CU) and at most MAXT threads; at 256^3 FP64 this gives 16
@@ -644,7 +646,10 @@ template <typename T, bool INV>
 void launch_strided(fans_gpu_ctx *c, const FftLen &L, const cplx<T> *tw, int bc, size_t smem, cplx<T> *spec,
                     int64_t sn, int c0, int c1, int64_t ostride, int nouter, const int *done)
 {
-    switch (reg_len(L.n) ? L.n : 0) {
+    switch (reg_len(L.n) || reg_len_y(L.n) ? L.n : 0) {
+    case 96: launch_strided_reg<T, INV, 24, 4>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;
+    case 80: launch_strided_reg<T, INV, 20, 4>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;
+    case 48: launch_strided_reg<T, INV, 12, 4>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;
     case 256: launch_strided_reg<T, INV, 16, 16>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;
     case 128: launch_strided_reg<T, INV, 16, 8>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;
     case 64: launch_strided_reg<T, INV, 8, 8>(c, spec, tw, sn, c0, c1, ostride, nouter, done); break;

@@ -1,4 +1,5 @@
-// Register four-step FFT kernels for power-of-two transform lengths.
+// Register four-step FFT kernels for the power-of-two transform lengths
+// 16..256 and, along y, for 48 = 12 x 4, 80 = 20 x 4 and 96 = 24 x 4.
 //
 // A length N = N1 * N2 transform of one column is done by N2 cooperating
 // threads, thread j holding the N1 values x[j + N2 m]:
@@ -55,17 +56,95 @@ __device__ __forceinline__ void dft16(cplx<T> (&v)[16])
     }
 }
 
+// exp(-2 pi i k / n) for compile-time k, n: the angle reduced to a quadrant
+// and a Taylor series there (terms below 1e-20 of the result)
+constexpr double ct_taylor(double x, bool sine)
+{
+    // sin: x - x^3/3! + ...; cos: 1 - x^2/2! + ...
+    double t = sine ? x : 1.0, s = t;
+    for (int m = 1; m < 14; ++m) {
+        const double a = sine ? 2.0 * m : 2.0 * m - 1.0;
+        t *= -x * x / (a * (a + 1.0));
+        s += t;
+    }
+    return s;
+}
+struct CtW {
+    double re, im;
+};
+constexpr CtW ct_twiddle(int k, int n)
+{
+    k %= n;
+    const int    q  = (4 * k) / n;  // quadrant of 2 pi k / n
+    const double r  = 1.57079632679489661923 * (double) (4 * k - q * n) / (double) n;
+    const double cr = ct_taylor(r, false), sr = ct_taylor(r, true);
+    // angle q pi/2 + r: rotate (cos r, sin r) by q quarter turns
+    const double c = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
+    const double s = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
+    return CtW{c, -s};
+}
+
+template <typename T, int N, bool INV>
+__device__ __forceinline__ void dft_reg(cplx<T> (&v)[N]);
+
+// in-register DFT of length P Q by Cooley-Tukey: P-point DFTs over the
+// stride-Q subsequences, the twiddles W_PQ^(i2 k1), Q-point DFTs; input
+// index i2 + Q m, output k1 + P k2 (dft16's order for P = Q = 4)
+template <typename T, int P, int Q, bool INV>
+__device__ __forceinline__ void dft_pq(cplx<T> (&v)[P * Q])
+{
+    cplx<T> y[Q][P];
+#pragma unroll
+    for (int i2 = 0; i2 < Q; ++i2) {
+        cplx<T> a[P];
+#pragma unroll
+        for (int m = 0; m < P; ++m)
+            a[m] = v[i2 + Q * m];
+        dft_reg<T, P, INV>(a);
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) {
+            if (i2 * k1 == 0) {
+                y[i2][k1] = a[k1];
+            } else {
+                constexpr int n = P * Q;
+                const CtW     w = ct_twiddle(i2 * k1, n);
+                const T       wr = (T) w.re, wi = INV ? (T) -w.im : (T) w.im;
+                y[i2][k1]        = cplx<T>{a[k1].x * wr - a[k1].y * wi, a[k1].x * wi + a[k1].y * wr};
+            }
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+        cplx<T> b[Q];
+#pragma unroll
+        for (int i2 = 0; i2 < Q; ++i2)
+            b[i2] = y[i2][k1];
+        dft_reg<T, Q, INV>(b);
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2)
+            v[k1 + P * k2] = b[k2];
+    }
+}
+
 template <typename T, int N, bool INV>
 __device__ __forceinline__ void dft_reg(cplx<T> (&v)[N])
 {
-    if constexpr (N == 2)
-        dft<T, 2, INV>(v);
+    if constexpr (N == 2 || N == 3 || N == 5)
+        dft<T, N, INV>(v);
     else if constexpr (N == 4)
         dft4<T, INV>(v);
     else if constexpr (N == 8)
         dft8<T, INV>(v);
-    else
+    else if constexpr (N == 16)
         dft16<T, INV>(v);
+    else if constexpr (N == 12)
+        dft_pq<T, 3, 4, INV>(v);
+    else if constexpr (N == 20)
+        dft_pq<T, 5, 4, INV>(v);
+    else {
+        static_assert(N == 24, "unsupported register DFT length");
+        dft_pq<T, 3, 8, INV>(v);
+    }
 }
 
 // Four-step transform of one column (see the file comment).  `tile` is the

@@ -34,11 +34,13 @@ def say(msg):
 
 
 @pytest.mark.parametrize("shape", [(8, 64, 64, 3), (4, 128, 128, 3), (4, 256, 256, 3), (2, 256, 256, 1),
-                                   (256, 4, 64, 3), (128, 8, 256, 1), (6, 96, 80, 3), (4, 48, 96, 1)])
+                                   (256, 4, 64, 3), (128, 8, 256, 1), (6, 96, 80, 3), (4, 48, 96, 1),
+                                   (4, 80, 48, 3)])
 def test_fft_lengths(shape):
     """Standalone r2c / c2r against numpy for every power-of-two length up to
     256 along z and y (x through the strided register kernel) and the batch's
-    radix-3/5 lengths."""
+    radix-3/5 lengths (along y 48, 80 and 96 through the mixed-radix register
+    kernels)."""
     say(f"fft shape={shape}")
     rng = np.random.default_rng(11)
     x = rng.standard_normal(shape)
