@@ -607,7 +607,8 @@ void clear_map(fans_gpu_ctx *c)
 // power-of-two lengths 16..256 use the register four-step kernels
 // ---------------------------------------------------------------------------
 bool reg_len(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256; }
-// mixed-radix lengths of the register y pass (24 x 4, 20 x 4, 12 x 4)
+// mixed-radix lengths of the register y pass (24 x 4, 20 x 4, 12 x 4); the
+// FP32 x+Green pass takes them too
 bool reg_len_y(int n) { return n == 96 || n == 80 || n == 48; }
 
 // columns per block: a power of two (>= 4) within ~72 KiB of LDS (two
@@ -688,6 +689,14 @@ int launch_x_green(fans_gpu_ctx *c, Fft3D<T> &P, cplx<T> *spec, const T *G, int 
                    const int *done)
 {
     const GreenView<T> gv{G, c->nx, c->ny, c->gxq, c->gyq, P.Mc};
+    if constexpr (sizeof(T) == 4) {  // FP64 at 20 x 4 and 24 x 4 would spill (one wave)
+        switch (P.Lx.n) {
+        case 96: return launch_x_green_reg<T, H, 24, 4>(c, spec, P.twx, gv, kz_lo, kz_hi, partial, done);
+        case 80: return launch_x_green_reg<T, H, 20, 4>(c, spec, P.twx, gv, kz_lo, kz_hi, partial, done);
+        case 48: return launch_x_green_reg<T, H, 12, 4>(c, spec, P.twx, gv, kz_lo, kz_hi, partial, done);
+        default: break;
+        }
+    }
     switch (reg_len(P.Lx.n) ? P.Lx.n : 0) {
     case 256: return launch_x_green_reg<T, H, 16, 16>(c, spec, P.twx, gv, kz_lo, kz_hi, partial, done);
     case 128: return launch_x_green_reg<T, H, 16, 8>(c, spec, P.twx, gv, kz_lo, kz_hi, partial, done);
@@ -707,8 +716,16 @@ int launch_x_green(fans_gpu_ctx *c, Fft3D<T> &P, cplx<T> *spec, const T *G, int 
 template <typename T>
 int64_t x_green_blocks(const Fft3D<T> &P, int h, int ny)
 {
-    int kb;
-    switch (reg_len(P.Lx.n) ? P.Lx.n : 0) {
+    int kb = 0;
+    if constexpr (sizeof(T) == 4)  // the same choice as launch_x_green
+        switch (P.Lx.n) {
+        case 96: kb = reg_cols<T, 24, 4>(); break;
+        case 80: kb = reg_cols<T, 20, 4>(); break;
+        case 48: kb = reg_cols<T, 12, 4>(); break;
+        default: break;
+        }
+    switch (kb ? -1 : (reg_len(P.Lx.n) ? P.Lx.n : 0)) {
+    case -1: break;
     case 256: kb = reg_cols<T, 16, 16>(); break;
     case 128: kb = reg_cols<T, 16, 8>(); break;
     case 64: kb = reg_cols<T, 8, 8>(); break;
