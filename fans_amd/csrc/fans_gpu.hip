@@ -122,7 +122,10 @@ int zct_lines(int h, int M)
     switch (M) {
     case 128: return h == 1 ? zct_lpb<1, 128>() : zct_lpb<3, 128>();
     case 64: return h == 1 ? zct_lpb<1, 64>() : zct_lpb<3, 64>();
+    case 48: return h == 1 ? zct_lpb<1, 48>() : zct_lpb<3, 48>();
+    case 40: return h == 1 ? zct_lpb<1, 40>() : zct_lpb<3, 40>();
     case 32: return h == 1 ? zct_lpb<1, 32>() : zct_lpb<3, 32>();
+    case 24: return h == 1 ? zct_lpb<1, 24>() : zct_lpb<3, 24>();
     case 16: return h == 1 ? zct_lpb<1, 16>() : zct_lpb<3, 16>();
     default: return 0;
     }
@@ -741,8 +744,8 @@ int64_t x_green_blocks(const Fft3D<T> &P, int h, int ny)
 int reduce(fans_gpu_ctx *c, const double *partial, int nblk, int nq, int stage, const double *partial_b = nullptr,
            int nblk_b = 0);
 
-// z passes: compile-time kernels where M = nz/2 is a power of two in 16..128
-// and the line count divides evenly, else the generic ones
+// z passes: compile-time kernels where M = nz/2 is in {16, 24, 32, 40, 48, 64,
+// 128} and the line count divides evenly, else the generic ones
 template <typename T, int H, int PRO>
 void launch_z_r2c(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const ProArgs<T> &pa, const int *done)
 {
@@ -754,7 +757,10 @@ void launch_z_r2c(fans_gpu_ctx *c, Fft3D<T> &P, T *rin, cplx<T> *spec, const Pro
     switch (P.zct ? P.M : 0) {
     case 128: ZF(128); break;
     case 64: ZF(64); break;
+    case 48: ZF(48); break;
+    case 40: ZF(40); break;
     case 32: ZF(32); break;
+    case 24: ZF(24); break;
     case 16: ZF(16); break;
     default:
         hipLaunchKernelGGL((k_fft_z_r2c<T, H, PRO>), dim3(P.nblk_z, 1, c->gz), dim3(kThreads), P.smz, c->stream, rin,
@@ -774,7 +780,10 @@ void launch_z_c2r(fans_gpu_ctx *c, Fft3D<T> &P, const cplx<T> *spec, const EpiAr
     switch (P.zct ? P.M : 0) {
     case 128: ZI(128); break;
     case 64: ZI(64); break;
+    case 48: ZI(48); break;
+    case 40: ZI(40); break;
     case 32: ZI(32); break;
+    case 24: ZI(24); break;
     case 16: ZI(16); break;
     default:
         hipLaunchKernelGGL((k_fft_z_c2r<T, H, EPI>), dim3(P.nblk_z, 1, c->gz), dim3(kThreads), P.smz, c->stream, spec,

@@ -472,8 +472,8 @@ __global__ __launch_bounds__(256) void k_fft_z_c2r(const cplx<T> *__restrict__ i
 constexpr int kC2rAhead64 = FANS_C2R_AHEAD64;
 constexpr int kC2rAhead32 = FANS_C2R_AHEAD32;
 
-// For M = nz/2 in {16, 32, 64, 128} the line count per block, the thread count
-// and every index are compile-time constants (no runtime divisions), and the
+// For M = nz/2 in {16, 24, 32, 40, 48, 64, 128} the line count per block, the
+// thread count and every index are compile-time constants (no runtime divisions), and the
 // thread count equals the butterfly count of the first radix stage.
 template <int M>
 constexpr int zct_radix1()
@@ -533,11 +533,23 @@ __device__ __forceinline__ void stockham_stage_ct(cplx<T> *__restrict__ buf, con
     __syncthreads();
 }
 
-// radix schedules: 128 = 8*4*4, 64 = 8*8, 32 = 8*4, 16 = 4*4
+// radix schedules: 128 = 8*4*4, 64 = 8*8, 48 = 8*3*2, 40 = 8*5, 32 = 8*4,
+// 24 = 4*3*2, 16 = 4*4 (the first radix is zct_radix1)
 template <typename T, bool INV, int N, int NC, int LD, int NT>
 __device__ __forceinline__ void lds_fft_ct(cplx<T> *buf, const cplx<T> *__restrict__ tw)
 {
-    if constexpr (N == 128) {
+    if constexpr (N == 48) {
+        stockham_stage_ct<T, 8, INV, N, NC, LD, 1, NT>(buf, tw);
+        stockham_stage_ct<T, 3, INV, N, NC, LD, 8, NT>(buf, tw);
+        stockham_stage_ct<T, 2, INV, N, NC, LD, 24, NT>(buf, tw);
+    } else if constexpr (N == 40) {
+        stockham_stage_ct<T, 8, INV, N, NC, LD, 1, NT>(buf, tw);
+        stockham_stage_ct<T, 5, INV, N, NC, LD, 8, NT>(buf, tw);
+    } else if constexpr (N == 24) {
+        stockham_stage_ct<T, 4, INV, N, NC, LD, 1, NT>(buf, tw);
+        stockham_stage_ct<T, 3, INV, N, NC, LD, 4, NT>(buf, tw);
+        stockham_stage_ct<T, 2, INV, N, NC, LD, 12, NT>(buf, tw);
+    } else if constexpr (N == 128) {
         stockham_stage_ct<T, 8, INV, N, NC, LD, 1, NT>(buf, tw);
         stockham_stage_ct<T, 4, INV, N, NC, LD, 8, NT>(buf, tw);
         stockham_stage_ct<T, 4, INV, N, NC, LD, 32, NT>(buf, tw);
