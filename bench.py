@@ -562,7 +562,7 @@ def main():
           flush=True)
     r64 = run_gpu(p, "fp64", args.steps, args.warmup, g0, all_load_cases=not args.no_load_cases)
     tot_it = sum(r64["iters"])
-    # median step (median of the timed solves); the mean is
+    # the median of the timed solves is the reported step; the mean is
     # reported beside it
     med = int(np.argsort(r64["step_s"])[len(r64["step_s"]) // 2])
     value = N * r64["iters"][med] / r64["step_s"][med]
