@@ -242,8 +242,9 @@ __global__ __launch_bounds__(256) void k_inner_start(float *__restrict__ r32, fl
 // ring slots 0 .. count-1 in order (iteration order, each step rounded as the
 // reference's per-iteration update v_u_real -= alpha * d_real,
 // solverCG.h:105), alpha_j from the device scalars
-// elements per step of the ring update (256^3: FP64 0.095 -> 0.090 ms per
-// iteration with 2, 0.095 with 4; FP32 0.050 -> 0.046)
+// elements per step of the ring update (256^3, interleaved runs: FP64 solve
+// 70.06 -> 69.64 ms with 2, 69.92 with 4; mixed 38.48 -> 38.09 ms; the
+// traced duration of the update itself stays at 0.70 ms)
 #ifndef FANS_RING_GROUP
 #define FANS_RING_GROUP 2
 #endif
